@@ -1,0 +1,248 @@
+"""MI355X-native falling-sand tick: Python host mirror of the C-ABI.
+
+The product is ``lib/libfsg.so`` (hand-written HIP kernels for gfx950 behind
+the C-ABI in ``include/fsg.h``).  This module is a thin ctypes layer with the
+reference's ``ME::world`` method names (source/engine/world.hpp:148-192):
+``World.tick``, ``World.tickTemperature``, ``World.getTile``, ``World.setTile``
+plus bulk upload/download, statistics, rigid-body stamping and batching.
+
+Cells are 8-byte values, exposed to numpy as ``uint64``:
+bits 0-7 material id, bit 8 MOVED, bits 16-31 temperature (int16, 1/16 degree),
+bits 32-63 RGBA colour.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import (AIR, GAS, MOVED, SAND, SOLID, SOUP, Body, Cell, CellEdit, Config,  # noqa: F401
+                   FsgError, Stats, StampResult, check, lib)
+
+__all__ = ["Context", "World", "default_config", "generate", "make_cell", "cell_mat",
+           "cell_temp", "cell_rgba", "batch_step", "FsgError", "Body", "Config"]
+
+MAT_AIR, MAT_STONE, MAT_SAND, MAT_WATER, MAT_OIL, MAT_STEAM, MAT_LAVA, MAT_WOOD = range(8)
+MATERIAL_NAMES = ["air", "stone", "sand", "water", "oil", "steam", "lava", "wood"]
+
+
+def default_config(**overrides) -> Config:
+    """The 8-material benchmark table (fsg_default_config) with field overrides."""
+    cfg = Config()
+    check(lib().fsg_default_config(C.byref(cfg)), "fsg_default_config")
+    for k, v in overrides.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def make_cell(mat, temp=320, rgba=0):
+    return (int(mat) & 0xFFFF) | ((int(temp) & 0xFFFF) << 16) | ((int(rgba) & 0xFFFFFFFF) << 32)
+
+
+def cell_mat(a):
+    return (np.asarray(a, dtype=np.uint64) & np.uint64(0xFF)).astype(np.uint8)
+
+
+def cell_temp(a):
+    return ((np.asarray(a, dtype=np.uint64) >> np.uint64(16)) & np.uint64(0xFFFF)).astype(np.uint16).view(np.int16)
+
+
+def cell_rgba(a):
+    return (np.asarray(a, dtype=np.uint64) >> np.uint64(32)).astype(np.uint32)
+
+
+def generate(kind: int, width: int, height: int, seed: int) -> np.ndarray:
+    """Benchmark world generator (host code in libfsg): returns (H, W) uint64."""
+    out = np.empty((height, width), dtype=np.uint64)
+    check(lib().fsg_generate(kind, width, height, seed, out.ctypes.data), "fsg_generate")
+    return out
+
+
+class Context:
+    """One device, one HIP stream, an HBM budget (replaces WorldSystem's pools)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        cfg = _abi.CtxConfig(device, hbm_budget)
+        h = C.c_void_p()
+        check(lib().fsg_ctx_create(C.byref(cfg), C.byref(h)), "fsg_ctx_create")
+        self.handle = h
+
+    def usage(self) -> int:
+        v = C.c_uint64()
+        check(lib().fsg_ctx_usage(self.handle, C.byref(v)), "fsg_ctx_usage")
+        return v.value
+
+    def inject_alloc_failures(self, n: int):
+        check(lib().fsg_ctx_inject_alloc_failures(self.handle, n))
+
+    def close(self):
+        if self.handle:
+            check(lib().fsg_ctx_destroy(self.handle), "fsg_ctx_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class World:
+    """Device-resident world with the reference's world API (world.hpp:63-193)."""
+
+    def __init__(self, ctx: Context, width: int, height: int, seed: int = 0, config: Config = None):
+        self.ctx = ctx
+        self.width, self.height, self.seed = width, height, seed
+        self.config = config if config is not None else default_config()
+        h = C.c_void_p()
+        check(lib().fsg_world_create(ctx.handle, width, height, seed, C.byref(self.config), C.byref(h)),
+              "fsg_world_create")
+        self.handle = h
+        self._edits = []
+
+    # --- reference method names -------------------------------------------
+    def tick(self, n: int = 1):
+        """world::tick (world.cpp:1036) n times, temperature by cadence."""
+        self._flush()
+        check(lib().fsg_world_step(self.handle, n), "fsg_world_step")
+
+    def tickTemperature(self):  # noqa: N802 (reference name)
+        """world::tickTemperature (world.cpp:1950)."""
+        self._flush()
+        check(lib().fsg_world_tick_temperature(self.handle), "fsg_world_tick_temperature")
+
+    def getTile(self, x: int, y: int) -> int:  # noqa: N802
+        """world::getTile (world.cpp:999): out of bounds reads as a solid wall."""
+        if x < 0 or y < 0 or x >= self.width or y >= self.height:
+            return 15
+        self._flush()
+        return int(self.download(x, y, 1, 1)[0, 0])
+
+    def setTile(self, x: int, y: int, cell: int):  # noqa: N802
+        """world::setTile (world.cpp:1004): queued, applied before the next tick."""
+        self._edits.append((x, y, int(cell)))
+
+    # --- bulk / state -----------------------------------------------------------
+    def _flush(self):
+        if not self._edits:
+            return
+        arr = (CellEdit * len(self._edits))()
+        for i, (x, y, v) in enumerate(self._edits):
+            arr[i].x, arr[i].y = x, y
+            arr[i].cell.mat = v & 0xFFFF
+            arr[i].cell.temp = C.c_int16(v >> 16 & 0xFFFF).value
+            arr[i].cell.rgba = (v >> 32) & 0xFFFFFFFF
+        self._edits = []
+        check(lib().fsg_world_set_cells(self.handle, arr, len(arr)), "fsg_world_set_cells")
+
+    def upload(self, cells: np.ndarray):
+        a = np.ascontiguousarray(cells, dtype=np.uint64)
+        assert a.size == self.width * self.height
+        check(lib().fsg_world_upload(self.handle, a.ctypes.data, a.size), "fsg_world_upload")
+
+    def download(self, x0=0, y0=0, w=None, h=None) -> np.ndarray:
+        self._flush()
+        w = self.width if w is None else w
+        h = self.height if h is None else h
+        out = np.empty((h, w), dtype=np.uint64)
+        check(lib().fsg_world_download(self.handle, out.ctypes.data, x0, y0, w, h), "fsg_world_download")
+        return out
+
+    def step_timed(self, n: int) -> float:
+        self._flush()
+        ms = C.c_double()
+        check(lib().fsg_world_step_timed(self.handle, n, C.byref(ms)), "fsg_world_step_timed")
+        return ms.value
+
+    def sync(self):
+        check(lib().fsg_world_sync(self.handle))
+
+    @property
+    def tick_count(self) -> int:
+        t = C.c_uint64()
+        check(lib().fsg_world_info(self.handle, None, None, C.byref(t)))
+        return t.value
+
+    def set_tick(self, t: int):
+        check(lib().fsg_world_set_tick(self.handle, t))
+
+    def stats(self) -> Stats:
+        s = Stats()
+        check(lib().fsg_world_stats(self.handle, C.byref(s)), "fsg_world_stats")
+        return s
+
+    def reset_activity(self):
+        check(lib().fsg_world_reset_activity(self.handle))
+
+    # --- rigid bodies -----------------------------------------------------------
+    def stamp_bodies(self, bodies, pixels: np.ndarray, displaced_cap: int = 0):
+        nb = len(bodies)
+        barr = (Body * max(nb, 1))(*bodies)
+        px = np.ascontiguousarray(pixels, dtype=np.uint64)
+        res = StampResult()
+        disp = (CellEdit * max(displaced_cap, 1))()
+        hs = (C.c_uint32 * max(nb, 1))()
+        hw = (C.c_uint32 * max(nb, 1))()
+        res.displaced_cap = displaced_cap
+        res.displaced_out = C.cast(disp, C.POINTER(CellEdit))
+        res.hits_sand = C.cast(hs, C.POINTER(C.c_uint32))
+        res.hits_soup = C.cast(hw, C.POINTER(C.c_uint32))
+        check(lib().fsg_world_stamp_bodies(self.handle, barr, nb, px.ctypes.data, px.size, C.byref(res)),
+              "fsg_world_stamp_bodies")
+        n = min(res.displaced, displaced_cap)
+        displaced = [(disp[i].x, disp[i].y,
+                      make_cell(disp[i].cell.mat, disp[i].cell.temp, disp[i].cell.rgba)) for i in range(n)]
+        return {"placed": res.placed, "unplaced": res.unplaced, "displaced": res.displaced,
+                "displaced_cells": displaced, "hits_sand": list(hs)[:nb], "hits_soup": list(hw)[:nb]}
+
+    def unstamp_bodies(self, npixels: int) -> np.ndarray:
+        out = np.empty(npixels, dtype=np.uint64)
+        check(lib().fsg_world_unstamp_bodies(self.handle, out.ctypes.data, npixels), "fsg_world_unstamp_bodies")
+        return out
+
+    def close(self):
+        if self.handle:
+            check(lib().fsg_world_destroy(self.handle), "fsg_world_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def _handles(worlds):
+    arr = (C.c_void_p * len(worlds))()
+    for i, w in enumerate(worlds):
+        w._flush()
+        arr[i] = w.handle
+    return arr
+
+
+def batch_step(ctx: Context, worlds, ticks: int):
+    """All worlds advance `ticks` ticks in shared launches (bit-identical to
+    stepping each alone)."""
+    check(lib().fsg_batch_step(ctx.handle, _handles(worlds), len(worlds), ticks), "fsg_batch_step")
+
+
+def batch_step_timed(ctx: Context, worlds, ticks: int) -> float:
+    ms = C.c_double()
+    check(lib().fsg_batch_step_timed(ctx.handle, _handles(worlds), len(worlds), ticks, C.byref(ms)),
+          "fsg_batch_step_timed")
+    return ms.value
+
+
+def perworld_step_timed(ctx: Context, worlds, ticks: int) -> float:
+    ms = C.c_double()
+    check(lib().fsg_perworld_step_timed(ctx.handle, _handles(worlds), len(worlds), ticks, C.byref(ms)),
+          "fsg_perworld_step_timed")
+    return ms.value
+
+
+def body_from_pose(x: float, y: float, angle: float, mat_w: int, mat_h: int, pixel_offset: int,
+                   enabled: bool = True) -> Body:
+    """Fixed-point pose from a float pose (b2Body::GetPosition/GetAngle), done
+    once per body per tick on the host."""
+    import math
+    return Body(int(round(x * 65536)), int(round(y * 65536)), int(round(math.cos(angle) * 65536)),
+                int(round(math.sin(angle) * 65536)), mat_w, mat_h, pixel_offset, 1 if enabled else 0)

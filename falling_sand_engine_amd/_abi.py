@@ -1,0 +1,147 @@
+"""ctypes mirror of include/fsg.h (structs, status codes, function signatures)."""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfsg.so")
+
+FSG_OK = 0
+STATUS = {
+    0: "FSG_OK", -1: "FSG_ERR_INVALID", -2: "FSG_ERR_NOMEM", -3: "FSG_ERR_DEVICE",
+    -4: "FSG_ERR_BUDGET", -5: "FSG_ERR_STATE", -6: "FSG_ERR_CHECKSUM", -7: "FSG_ERR_TIMEOUT",
+    -8: "FSG_ERR_CANCELLED", -9: "FSG_ERR_NOTFOUND", -10: "FSG_ERR_AGAIN",
+}
+FSG_ERR_INVALID, FSG_ERR_NOMEM, FSG_ERR_DEVICE, FSG_ERR_BUDGET, FSG_ERR_STATE = -1, -2, -3, -4, -5
+FSG_ERR_CHECKSUM, FSG_ERR_TIMEOUT, FSG_ERR_CANCELLED, FSG_ERR_NOTFOUND, FSG_ERR_AGAIN = -6, -7, -8, -9, -10
+
+AIR, SOLID, SAND, SOUP, GAS = 0, 1, 2, 3, 4
+MOVED = 0x100
+
+
+class Material(C.Structure):
+    _fields_ = [("name", C.c_char * 16), ("type", C.c_uint8), ("rank", C.c_uint8),
+                ("iterations", C.c_uint8), ("slipperyness", C.c_uint8), ("add_temp", C.c_int16),
+                ("reserved", C.c_uint16), ("rgba", C.c_uint32)]
+
+
+class Config(C.Structure):
+    _fields_ = [("cell_iter", C.c_uint32), ("reactions", C.c_uint32), ("temp_every", C.c_uint32),
+                ("temp_phase", C.c_uint32), ("chunk_skip", C.c_uint32), ("wood_burn_temp", C.c_int32),
+                ("mat_air", C.c_uint8), ("mat_water", C.c_uint8), ("mat_lava", C.c_uint8),
+                ("mat_steam", C.c_uint8), ("mat_stone", C.c_uint8), ("mat_wood", C.c_uint8),
+                ("pad", C.c_uint8 * 2), ("n_materials", C.c_uint32), ("materials", Material * 15)]
+
+
+class CtxConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("hbm_budget", C.c_uint64)]
+
+
+class Cell(C.Structure):
+    _fields_ = [("mat", C.c_uint16), ("temp", C.c_int16), ("rgba", C.c_uint32)]
+
+
+class CellEdit(C.Structure):
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("cell", Cell)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("tick", C.c_uint64), ("count", C.c_uint64 * 16), ("checksum", C.c_uint64),
+                ("tiles_total", C.c_uint64), ("tiles_processed", C.c_uint64),
+                ("tiles_offered", C.c_uint64)]
+
+
+class Body(C.Structure):
+    _fields_ = [("pos_x_q16", C.c_int32), ("pos_y_q16", C.c_int32), ("cos_q16", C.c_int32),
+                ("sin_q16", C.c_int32), ("mat_w", C.c_int32), ("mat_h", C.c_int32),
+                ("pixel_offset", C.c_uint32), ("enabled", C.c_uint32)]
+
+
+class StampResult(C.Structure):
+    _fields_ = [("placed", C.c_uint32), ("unplaced", C.c_uint32), ("displaced", C.c_uint32),
+                ("displaced_cap", C.c_uint32), ("displaced_out", C.POINTER(CellEdit)),
+                ("hits_sand", C.POINTER(C.c_uint32)), ("hits_soup", C.POINTER(C.c_uint32))]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("max_batch_worlds", C.c_int32), ("max_inflight_ticks", C.c_int32),
+                ("block_ticks", C.c_int32), ("reserved", C.c_int32)]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("world_id", C.c_uint64), ("create_kind", C.c_int32), ("width", C.c_int32),
+                ("height", C.c_int32), ("ticks", C.c_int32), ("seed", C.c_uint64),
+                ("deadline_ns", C.c_uint64), ("n_bodies", C.c_int32), ("body_seed", C.c_int32)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("request_id", C.c_uint64), ("world_id", C.c_uint64), ("status", C.c_int32),
+                ("ticks_done", C.c_int32), ("tick", C.c_uint64), ("checksum", C.c_uint64),
+                ("t_submit_ns", C.c_uint64), ("t_admit_ns", C.c_uint64), ("t_done_ns", C.c_uint64)]
+
+
+P = C.c_void_p
+u64p = C.POINTER(C.c_uint64)
+
+# name -> (restype, argtypes); every symbol the header declares
+SIGNATURES = {
+    "fsg_status_string": (C.c_char_p, [C.c_int]),
+    "fsg_default_config": (C.c_int, [C.POINTER(Config)]),
+    "fsg_ctx_create": (C.c_int, [C.POINTER(CtxConfig), C.POINTER(P)]),
+    "fsg_ctx_destroy": (C.c_int, [P]),
+    "fsg_ctx_usage": (C.c_int, [P, u64p]),
+    "fsg_ctx_inject_alloc_failures": (C.c_int, [P, C.c_int]),
+    "fsg_world_create": (C.c_int, [P, C.c_int, C.c_int, C.c_uint64, C.POINTER(Config), C.POINTER(P)]),
+    "fsg_world_destroy": (C.c_int, [P]),
+    "fsg_world_info": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), u64p]),
+    "fsg_world_set_tick": (C.c_int, [P, C.c_uint64]),
+    "fsg_world_upload": (C.c_int, [P, P, C.c_size_t]),
+    "fsg_world_download": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "fsg_world_set_cells": (C.c_int, [P, P, C.c_size_t]),
+    "fsg_world_step": (C.c_int, [P, C.c_int]),
+    "fsg_world_step_timed": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
+    "fsg_world_tick_temperature": (C.c_int, [P]),
+    "fsg_world_sync": (C.c_int, [P]),
+    "fsg_world_stats": (C.c_int, [P, C.POINTER(Stats)]),
+    "fsg_world_reset_activity": (C.c_int, [P]),
+    "fsg_world_stamp_bodies": (C.c_int, [P, P, C.c_int, P, C.c_size_t, C.POINTER(StampResult)]),
+    "fsg_world_unstamp_bodies": (C.c_int, [P, P, C.c_size_t]),
+    "fsg_batch_step": (C.c_int, [P, P, C.c_int, C.c_int]),
+    "fsg_batch_step_timed": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "fsg_perworld_step_timed": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "fsg_generate": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, P]),
+    "fsg_splitmix64": (C.c_uint64, [C.c_uint64]),
+    "fsg_phase_key": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+}
+
+_LIB = None
+
+
+def lib():
+    """The loaded libfsg.so.  Raises if it has not been built: there is no
+    fallback implementation of the tick."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m falling_sand_engine_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            if not hasattr(L, name):
+                continue
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+class FsgError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        super().__init__(f"{what}: {STATUS.get(status, status)}")
+
+
+def check(status, what=""):
+    if status != FSG_OK:
+        raise FsgError(status, what)
+    return status

@@ -1,0 +1,44 @@
+"""Build libfsg.so (HIP kernels + C-ABI) for gfx950 in-tree.
+
+Run as ``python -m falling_sand_engine_amd.build`` or through
+``__graft_entry__.build()``.  The library lands in
+``falling_sand_engine_amd/lib/libfsg.so``; nothing is installed elsewhere.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIBDIR, "libfsg.so")
+
+SOURCES = ["fsg_kernels.hip", "fsg_api.hip", "fsg_host.cpp"]
+HEADERS = ["fsg_device.hpp", "fsg_kernels.hpp"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("FSG_ARCH", "gfx950")
+
+# Packed fp32 instructions are disabled for all device code.
+FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+    "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+]
+
+
+def build(verbose: bool = True) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC] + FLAGS + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-q" not in sys.argv)

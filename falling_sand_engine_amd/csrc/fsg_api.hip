@@ -1,0 +1,778 @@
+// fsg_api.hip -- the C-ABI (include/fsg.h) over the HIP kernels.
+//
+// Ownership follows ME::world (world.cpp:65, 110-133, 3532-3636): the world
+// owns its device buffers and frees them on destroy.  Host buffers are
+// caller-owned and copied.  Every entry point returns a status and never
+// throws.  All work of a context goes to one HIP stream of default priority.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <unordered_map>
+#include <vector>
+
+#include "fsg.h"
+#include "fsg_device.hpp"
+#include "fsg_kernels.hpp"
+
+using fsg::DevWorld;
+using fsg::MatTab;
+
+#define HIPCHK(call)                                   \
+    do {                                               \
+        if ((call) != hipSuccess) return FSG_ERR_DEVICE; \
+    } while (0)
+
+struct fsg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    uint64_t budget = 0, used = 0;
+    int inject = 0;
+    std::mutex mu;
+    // batch launch state: device world table + chunk prefix sums + graph
+    DevWorld *d_worlds = nullptr;
+    uint32_t *d_tile_start = nullptr;
+    int batch_cap = 0;
+    std::vector<fsg_world *> graph_key;
+    std::vector<uint32_t> graph_tiles;
+    hipGraphExec_t graph = nullptr;
+    uint32_t total_tiles = 0;
+};
+
+struct fsg_world {
+    fsg_ctx *ctx = nullptr;
+    fsg_config cfg{};
+    DevWorld d{};
+    size_t bytes = 0;
+    uint64_t offered = 0;
+    unsigned long long *d_stats = nullptr;  // [17] census scratch + [1] processed counter
+    // rigid-body stamp state
+    uint64_t *d_pixels = nullptr;
+    uint32_t *d_pixbody = nullptr;
+    fsg_body *d_bodies = nullptr;
+    uint8_t *d_state = nullptr;
+    uint64_t *d_disp = nullptr, *d_dispxy = nullptr;
+    size_t pix_cap = 0;
+    int body_cap = 0;
+    size_t npix = 0;
+    int nbodies = 0;
+    bool stamped = false;
+    std::vector<fsg_body> bodies;
+};
+
+namespace {
+
+constexpr int kGraphTicks = 16;
+
+int dev_alloc(fsg_ctx *c, void **p, size_t bytes) {
+    *p = nullptr;
+    if (c->inject > 0) {
+        c->inject--;
+        return FSG_ERR_NOMEM;
+    }
+    if (bytes == 0) bytes = 8;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return FSG_ERR_NOMEM;
+    }
+    return FSG_OK;
+}
+
+void dev_free(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+int validate_config(const fsg_config &c) {
+    if (c.cell_iter < 1 || c.cell_iter > (uint32_t)fsg::MAX_IT) return FSG_ERR_INVALID;
+    if (c.n_materials < 1 || c.n_materials > FSG_MAX_MATERIALS) return FSG_ERR_INVALID;
+    if (c.temp_every > 0 && c.temp_phase >= c.temp_every) return FSG_ERR_INVALID;
+    const uint8_t roles[6] = {c.mat_air, c.mat_water, c.mat_lava, c.mat_steam, c.mat_stone, c.mat_wood};
+    for (uint8_t r : roles)
+        if (r >= c.n_materials) return FSG_ERR_INVALID;
+    for (uint32_t i = 0; i < c.n_materials; i++) {
+        if (c.materials[i].type > FSG_GAS) return FSG_ERR_INVALID;
+        if (c.materials[i].rank > 14) return FSG_ERR_INVALID;
+    }
+    if (c.materials[c.mat_air].type != FSG_AIR) return FSG_ERR_INVALID;
+    return FSG_OK;
+}
+
+// Material nibble tables and masks (Material fields, game_datastruct.hpp:130-168).
+MatTab build_mattab(const fsg_config &c) {
+    MatTab M;
+    std::memset(&M, 0, sizeof(M));
+    for (int id = 0; id < 16; id++) {
+        int type = FSG_SOLID, rank = 15, iters = 0, add = 0;
+        uint32_t rgba = 0;
+        if ((uint32_t)id < c.n_materials) {
+            type = c.materials[id].type;
+            rank = c.materials[id].rank;
+            iters = c.materials[id].iterations;
+            add = c.materials[id].add_temp;
+            rgba = c.materials[id].rgba;
+        }
+        const uint64_t sh = (uint64_t)id * 4;
+        M.pass |= (uint64_t)(type == FSG_SOLID ? 15 : rank) << sh;
+        for (int it = 0; it < fsg::MAX_IT; it++) {
+            const bool act = it < iters;
+            if ((type == FSG_SAND || type == FSG_SOUP) && act) M.sink[it] |= (uint64_t)rank << sh;
+            if (type == FSG_SOUP && act) M.flow[it] |= (uint64_t)rank << sh;
+            if (type == FSG_GAS && act) M.gas[it] |= 1u << id;
+        }
+        if (type == FSG_AIR) M.air |= 1u << id;
+        if (type == FSG_SAND) M.sand |= 1u << id;
+        if (type == FSG_SOUP) M.soup |= 1u << id;
+        M.rgba[id] = rgba;
+        M.add_temp[id] = add;
+        if (add) M.any_add_temp = 1;
+    }
+    M.burn = c.wood_burn_temp;
+    M.water = c.mat_water;
+    M.lava = c.mat_lava;
+    M.steam = c.mat_steam;
+    M.stone = c.mat_stone;
+    M.wood = c.mat_wood;
+    M.air_id = c.mat_air;
+    M.reactions = c.reactions ? 1 : 0;
+    M.cell_iter = c.cell_iter;
+    M.temp_every = c.temp_every;
+    M.temp_phase = c.temp_phase;
+    return M;
+}
+
+bool cell_ok(const fsg_config &c, uint16_t mat) {
+    return (mat & 0xFE00u) == 0 && (uint32_t)(mat & 0xFFu) < c.n_materials;
+}
+
+inline uint64_t as_u64(const fsg_cell &c) {
+    uint64_t v;
+    std::memcpy(&v, &c, 8);
+    return v;
+}
+
+unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+int launch_tick(fsg_world *w) {
+    const unsigned tiles = (unsigned)(w->d.tiles_x * w->d.tiles_y);
+    fsg::k_tick_world<<<tiles, fsg::NT, 0, w->ctx->stream>>>(w->d, 0u);
+    HIPCHK(hipGetLastError());
+    w->d.tick++;
+    w->offered += tiles;
+    return FSG_OK;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+extern "C" int fsg_ctx_create(const fsg_ctx_config *cfg, fsg_ctx **out) {
+    if (!out) return FSG_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return FSG_ERR_DEVICE;
+    }
+    const int dev = cfg ? cfg->device : 0;
+    if (dev < 0 || dev >= ndev) return FSG_ERR_INVALID;
+    fsg_ctx *c = new (std::nothrow) fsg_ctx();
+    if (!c) return FSG_ERR_NOMEM;
+    c->device = dev;
+    c->budget = cfg ? cfg->hbm_budget : 0;
+    if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
+        delete c;
+        return FSG_ERR_DEVICE;
+    }
+    *out = c;
+    return FSG_OK;
+}
+
+extern "C" int fsg_ctx_destroy(fsg_ctx *c) {
+    if (!c) return FSG_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    dev_free(c->d_worlds);
+    dev_free(c->d_tile_start);
+    (void)hipEventDestroy(c->e0);
+    (void)hipEventDestroy(c->e1);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FSG_OK;
+}
+
+extern "C" int fsg_ctx_usage(fsg_ctx *c, uint64_t *bytes) {
+    if (!c || !bytes) return FSG_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    *bytes = c->used;
+    return FSG_OK;
+}
+
+extern "C" int fsg_ctx_inject_alloc_failures(fsg_ctx *c, int n) {
+    if (!c || n < 0) return FSG_ERR_INVALID;
+    c->inject = n;
+    return FSG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// World
+// ---------------------------------------------------------------------------
+extern "C" int fsg_world_create(fsg_ctx *c, int W, int H, uint64_t seed, const fsg_config *cfg,
+                                fsg_world **out) {
+    if (!c || !out || W <= 0 || H <= 0 || W > FSG_MAX_SIDE || H > FSG_MAX_SIDE) return FSG_ERR_INVALID;
+    *out = nullptr;
+    fsg_config conf;
+    if (cfg) conf = *cfg;
+    else fsg_default_config(&conf);
+    if (int rc = validate_config(conf)) return rc;
+
+    const int tx = (W + fsg::TW - 1) / fsg::TW, ty = (H + fsg::TH - 1) / fsg::TH;
+    const size_t cells = (size_t)W * H;
+    const size_t bytes = 2 * cells * 8 + 2 * (size_t)tx * ty * 4 + 18 * 8;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (c->budget && c->used + bytes > c->budget) return FSG_ERR_BUDGET;
+        c->used += bytes;  // reserve
+    }
+    fsg_world *w = new (std::nothrow) fsg_world();
+    auto fail = [&](int rc) {
+        if (w) {
+            dev_free(w->d.buf[0]);
+            dev_free(w->d.buf[1]);
+            dev_free(w->d.flags[0]);
+            dev_free(w->d.flags[1]);
+            dev_free(w->d_stats);
+            delete w;
+        }
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->used -= bytes;
+        return rc;
+    };
+    if (!w) return fail(FSG_ERR_NOMEM);
+    w->ctx = c;
+    w->cfg = conf;
+    w->bytes = bytes;
+    if (hipSetDevice(c->device) != hipSuccess) return fail(FSG_ERR_DEVICE);
+    void *p;
+    if (dev_alloc(c, &p, cells * 8)) return fail(FSG_ERR_NOMEM);
+    w->d.buf[0] = (uint64_t *)p;
+    if (dev_alloc(c, &p, cells * 8)) return fail(FSG_ERR_NOMEM);
+    w->d.buf[1] = (uint64_t *)p;
+    if (dev_alloc(c, &p, (size_t)tx * ty * 4)) return fail(FSG_ERR_NOMEM);
+    w->d.flags[0] = (uint32_t *)p;
+    if (dev_alloc(c, &p, (size_t)tx * ty * 4)) return fail(FSG_ERR_NOMEM);
+    w->d.flags[1] = (uint32_t *)p;
+    if (dev_alloc(c, &p, 18 * 8)) return fail(FSG_ERR_NOMEM);
+    w->d_stats = (unsigned long long *)p;
+    w->d.W = W;
+    w->d.H = H;
+    w->d.tiles_x = tx;
+    w->d.tiles_y = ty;
+    w->d.seed = seed;
+    w->d.tick = 0;
+    w->d.force_tick = 0;
+    w->d.skip = conf.chunk_skip ? 1 : 0;
+    w->d.processed = w->d_stats + 17;
+    w->d.mat = build_mattab(conf);
+    const uint64_t air = conf.mat_air;
+    fsg::k_fill<<<grid_for(cells, 256), 256, 0, c->stream>>>(w->d.buf[0], cells, air);
+    fsg::k_fill<<<grid_for(cells, 256), 256, 0, c->stream>>>(w->d.buf[1], cells, air);
+    if (hipMemsetAsync(w->d.flags[0], 0, (size_t)tx * ty * 4, c->stream) != hipSuccess ||
+        hipMemsetAsync(w->d.flags[1], 0, (size_t)tx * ty * 4, c->stream) != hipSuccess ||
+        hipMemsetAsync(w->d_stats, 0, 18 * 8, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return fail(FSG_ERR_DEVICE);
+    *out = w;
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_destroy(fsg_world *w) {
+    if (!w) return FSG_ERR_INVALID;
+    fsg_ctx *c = w->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->used -= w->bytes;
+        // a graph captured over this world must not be replayed
+        if (std::find(c->graph_key.begin(), c->graph_key.end(), w) != c->graph_key.end()) {
+            if (c->graph) (void)hipGraphExecDestroy(c->graph);
+            c->graph = nullptr;
+            c->graph_key.clear();
+        }
+    }
+    dev_free(w->d.buf[0]);
+    dev_free(w->d.buf[1]);
+    dev_free(w->d.flags[0]);
+    dev_free(w->d.flags[1]);
+    dev_free(w->d_stats);
+    dev_free(w->d_pixels);
+    dev_free(w->d_pixbody);
+    dev_free(w->d_bodies);
+    dev_free(w->d_state);
+    dev_free(w->d_disp);
+    dev_free(w->d_dispxy);
+    delete w;
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_info(fsg_world *w, int *width, int *height, uint64_t *tick) {
+    if (!w) return FSG_ERR_INVALID;
+    if (width) *width = w->d.W;
+    if (height) *height = w->d.H;
+    if (tick) *tick = w->d.tick;
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_set_tick(fsg_world *w, uint64_t tick) {
+    if (!w) return FSG_ERR_INVALID;
+    HIPCHK(hipStreamSynchronize(w->ctx->stream));
+    if ((tick & 1) != (w->d.tick & 1)) std::swap(w->d.buf[0], w->d.buf[1]);
+    w->d.tick = tick;
+    w->d.force_tick = tick;
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_upload(fsg_world *w, const fsg_cell *cells, size_t n) {
+    if (!w || !cells || n != (size_t)w->d.W * w->d.H) return FSG_ERR_INVALID;
+    for (size_t i = 0; i < n; i++)
+        if (!cell_ok(w->cfg, cells[i].mat)) return FSG_ERR_INVALID;
+    HIPCHK(hipSetDevice(w->ctx->device));
+    HIPCHK(hipMemcpyAsync(w->d.buf[w->d.tick & 1], cells, n * 8, hipMemcpyHostToDevice, w->ctx->stream));
+    HIPCHK(hipStreamSynchronize(w->ctx->stream));
+    w->d.force_tick = w->d.tick;
+    w->stamped = false;
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_download(fsg_world *w, fsg_cell *out, int x0, int y0, int rw, int rh) {
+    if (!w || !out || rw <= 0 || rh <= 0 || x0 < 0 || y0 < 0 || x0 + rw > w->d.W || y0 + rh > w->d.H)
+        return FSG_ERR_INVALID;
+    HIPCHK(hipSetDevice(w->ctx->device));
+    const uint64_t *src = w->d.buf[w->d.tick & 1] + (size_t)y0 * w->d.W + x0;
+    HIPCHK(hipMemcpy2DAsync(out, (size_t)rw * 8, src, (size_t)w->d.W * 8, (size_t)rw * 8, (size_t)rh,
+                            hipMemcpyDeviceToHost, w->ctx->stream));
+    HIPCHK(hipStreamSynchronize(w->ctx->stream));
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_set_cells(fsg_world *w, const fsg_cell_edit *edits, size_t n) {
+    if (!w || (!edits && n)) return FSG_ERR_INVALID;
+    if (n == 0) return FSG_OK;
+    for (size_t i = 0; i < n; i++)
+        if (!cell_ok(w->cfg, edits[i].cell.mat)) return FSG_ERR_INVALID;
+    // last edit of a position wins (array order), like consecutive setTile calls
+    std::unordered_map<uint64_t, size_t> last;
+    last.reserve(n * 2);
+    for (size_t i = 0; i < n; i++) {
+        const fsg_cell_edit &e = edits[i];
+        if (e.x < 0 || e.y < 0 || e.x >= w->d.W || e.y >= w->d.H) continue;
+        last[(uint64_t)(uint32_t)e.y << 32 | (uint32_t)e.x] = i;
+    }
+    std::vector<int32_t> xy;
+    std::vector<uint64_t> vals;
+    xy.reserve(2 * last.size());
+    vals.reserve(last.size());
+    for (auto &kv : last) {
+        xy.push_back(edits[kv.second].x);
+        xy.push_back(edits[kv.second].y);
+        vals.push_back(as_u64(edits[kv.second].cell));
+    }
+    if (vals.empty()) return FSG_OK;
+    fsg_ctx *c = w->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    void *dxy = nullptr, *dv = nullptr;
+    if (hipMalloc(&dxy, xy.size() * 4) != hipSuccess) return FSG_ERR_NOMEM;
+    if (hipMalloc(&dv, vals.size() * 8) != hipSuccess) {
+        dev_free(dxy);
+        return FSG_ERR_NOMEM;
+    }
+    int rc = FSG_OK;
+    if (hipMemcpyAsync(dxy, xy.data(), xy.size() * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(dv, vals.data(), vals.size() * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = FSG_ERR_DEVICE;
+    if (!rc) {
+        fsg::k_set_cells<<<grid_for(vals.size(), 256), 256, 0, c->stream>>>(
+            w->d.buf[w->d.tick & 1], w->d.W, w->d.H, (const int32_t *)dxy, (const uint64_t *)dv, vals.size());
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) rc = FSG_ERR_DEVICE;
+    }
+    dev_free(dxy);
+    dev_free(dv);
+    w->d.force_tick = w->d.tick;
+    return rc;
+}
+
+extern "C" int fsg_world_step(fsg_world *w, int ticks) {
+    if (!w || ticks < 0) return FSG_ERR_INVALID;
+    HIPCHK(hipSetDevice(w->ctx->device));
+    for (int t = 0; t < ticks; t++)
+        if (int rc = launch_tick(w)) return rc;
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_step_timed(fsg_world *w, int ticks, double *ms) {
+    if (!w || ticks < 0) return FSG_ERR_INVALID;
+    fsg_ctx *c = w->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipEventRecord(c->e0, c->stream));
+    for (int t = 0; t < ticks; t++)
+        if (int rc = launch_tick(w)) return rc;
+    HIPCHK(hipEventRecord(c->e1, c->stream));
+    HIPCHK(hipEventSynchronize(c->e1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, c->e0, c->e1));
+    if (ms) *ms = f;
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_tick_temperature(fsg_world *w) {
+    if (!w) return FSG_ERR_INVALID;
+    fsg_ctx *c = w->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n = (size_t)w->d.W * w->d.H;
+    uint64_t *cur = w->d.buf[w->d.tick & 1], *spare = w->d.buf[(w->d.tick + 1) & 1];
+    fsg::k_temperature<<<grid_for(n, fsg::NT), fsg::NT, 0, c->stream>>>(cur, spare, w->d.W, w->d.H, w->d.mat);
+    HIPCHK(hipGetLastError());
+    // the result becomes the current state without advancing the tick
+    std::swap(w->d.buf[0], w->d.buf[1]);
+    w->d.force_tick = w->d.tick;
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_sync(fsg_world *w) {
+    if (!w) return FSG_ERR_INVALID;
+    HIPCHK(hipStreamSynchronize(w->ctx->stream));
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_stats(fsg_world *w, fsg_stats *out) {
+    if (!w || !out) return FSG_ERR_INVALID;
+    fsg_ctx *c = w->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n = (size_t)w->d.W * w->d.H;
+    HIPCHK(hipMemsetAsync(w->d_stats, 0, 17 * 8, c->stream));
+    const unsigned blocks = (unsigned)std::min<size_t>(grid_for(n, fsg::NT), 4096);
+    fsg::k_stats<<<blocks, fsg::NT, 0, c->stream>>>(w->d.buf[w->d.tick & 1], n, w->d_stats);
+    HIPCHK(hipGetLastError());
+    unsigned long long h[18];
+    HIPCHK(hipMemcpyAsync(h, w->d_stats, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::memset(out, 0, sizeof(*out));
+    out->tick = w->d.tick;
+    for (int i = 0; i < 16; i++) out->count[i] = h[i];
+    out->checksum = h[16];
+    out->tiles_total = (uint64_t)w->d.tiles_x * w->d.tiles_y;
+    out->tiles_processed = w->d.skip ? h[17] : w->offered;
+    out->tiles_offered = w->offered;
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_reset_activity(fsg_world *w) {
+    if (!w) return FSG_ERR_INVALID;
+    HIPCHK(hipMemsetAsync(w->d_stats + 17, 0, 8, w->ctx->stream));
+    HIPCHK(hipStreamSynchronize(w->ctx->stream));
+    w->offered = 0;
+    return FSG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Rigid-body pixels
+// ---------------------------------------------------------------------------
+namespace {
+
+fsg::StampArgs stamp_args(fsg_world *w) {
+    fsg::StampArgs a;
+    const uint64_t t = w->d.tick;
+    a.cells = w->d.buf[t & 1];
+    a.claims = w->d.buf[(t + 1) & 1];
+    a.pixels = w->d_pixels;
+    a.pix_body = w->d_pixbody;
+    a.bodies = w->d_bodies;
+    a.state = w->d_state;
+    a.disp = w->d_disp;
+    a.disp_xy = w->d_dispxy;
+    a.flags = w->d.skip ? w->d.flags[(t + 1) & 1] : nullptr;  // flags the next tick reads
+    a.W = w->d.W;
+    a.H = w->d.H;
+    a.tiles_x = w->d.tiles_x;
+    a.npix = (uint32_t)w->npix;
+    a.mat = w->d.mat;
+    return a;
+}
+
+int ensure_stamp_buffers(fsg_world *w, size_t npix, int nb) {
+    fsg_ctx *c = w->ctx;
+    void *p;
+    if (npix > w->pix_cap) {
+        dev_free(w->d_pixels);
+        dev_free(w->d_pixbody);
+        dev_free(w->d_state);
+        dev_free(w->d_disp);
+        dev_free(w->d_dispxy);
+        w->d_pixels = nullptr;
+        w->d_pixbody = nullptr;
+        w->d_state = nullptr;
+        w->d_disp = w->d_dispxy = nullptr;
+        w->pix_cap = 0;
+        if (dev_alloc(c, &p, npix * 8)) return FSG_ERR_NOMEM;
+        w->d_pixels = (uint64_t *)p;
+        if (dev_alloc(c, &p, npix * 4)) return FSG_ERR_NOMEM;
+        w->d_pixbody = (uint32_t *)p;
+        if (dev_alloc(c, &p, npix)) return FSG_ERR_NOMEM;
+        w->d_state = (uint8_t *)p;
+        if (dev_alloc(c, &p, npix * 8)) return FSG_ERR_NOMEM;
+        w->d_disp = (uint64_t *)p;
+        if (dev_alloc(c, &p, npix * 8)) return FSG_ERR_NOMEM;
+        w->d_dispxy = (uint64_t *)p;
+        w->pix_cap = npix;
+    }
+    if (nb > w->body_cap) {
+        dev_free(w->d_bodies);
+        w->d_bodies = nullptr;
+        w->body_cap = 0;
+        if (dev_alloc(c, &p, (size_t)nb * sizeof(fsg_body))) return FSG_ERR_NOMEM;
+        w->d_bodies = (fsg_body *)p;
+        w->body_cap = nb;
+    }
+    return FSG_OK;
+}
+
+} // namespace
+
+extern "C" int fsg_world_stamp_bodies(fsg_world *w, const fsg_body *bodies, int nb, const fsg_cell *pixels,
+                                      size_t npix, fsg_stamp_result *res) {
+    if (!w || nb < 0 || (nb && !bodies) || (npix && !pixels)) return FSG_ERR_INVALID;
+    if (npix >= 0xFFFFFFFFull) return FSG_ERR_INVALID;
+    if (w->stamped) return FSG_ERR_STATE;
+    std::vector<uint32_t> pix_body(npix, 0xFFFFFFFFu);
+    for (int b = 0; b < nb; b++) {
+        const fsg_body &B = bodies[b];
+        if (B.mat_w <= 0 || B.mat_h <= 0 || B.mat_w > 4096 || B.mat_h > 4096) return FSG_ERR_INVALID;
+        const size_t cnt = (size_t)B.mat_w * B.mat_h;
+        if ((size_t)B.pixel_offset + cnt > npix) return FSG_ERR_INVALID;
+        for (size_t k = 0; k < cnt; k++) {
+            if (pix_body[B.pixel_offset + k] != 0xFFFFFFFFu) return FSG_ERR_INVALID;  // overlapping bodies
+            pix_body[B.pixel_offset + k] = (uint32_t)b;
+        }
+    }
+    for (size_t p = 0; p < npix; p++)
+        if (!cell_ok(w->cfg, pixels[p].mat)) return FSG_ERR_INVALID;
+    fsg_ctx *c = w->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    if (int rc = ensure_stamp_buffers(w, npix ? npix : 1, nb ? nb : 1)) return rc;
+    w->npix = npix;
+    w->nbodies = nb;
+    w->bodies.assign(bodies, bodies + nb);
+    if (npix) {
+        HIPCHK(hipMemcpyAsync(w->d_pixels, pixels, npix * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(w->d_pixbody, pix_body.data(), npix * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    if (nb) HIPCHK(hipMemcpyAsync(w->d_bodies, bodies, (size_t)nb * sizeof(fsg_body), hipMemcpyHostToDevice, c->stream));
+    const fsg::StampArgs a = stamp_args(w);
+    if (npix) {
+        const unsigned g = grid_for(npix, 256);
+        fsg::k_claims_init<<<g, 256, 0, c->stream>>>(a);
+        for (int d = 0; d < 5; d++) {
+            fsg::k_stamp_propose<<<g, 256, 0, c->stream>>>(a, d);
+            fsg::k_stamp_resolve<<<g, 256, 0, c->stream>>>(a, d);
+        }
+        HIPCHK(hipGetLastError());
+    }
+    std::vector<uint8_t> state(npix);
+    std::vector<uint64_t> disp(npix), dispxy(npix);
+    if (npix) {
+        HIPCHK(hipMemcpyAsync(state.data(), w->d_state, npix, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(disp.data(), w->d_disp, npix * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(dispxy.data(), w->d_dispxy, npix * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    w->stamped = true;
+    if (res) {
+        uint32_t placed = 0, nd = 0, unplaced = 0;
+        if (res->hits_sand) std::memset(res->hits_sand, 0, sizeof(uint32_t) * (size_t)nb);
+        if (res->hits_soup) std::memset(res->hits_soup, 0, sizeof(uint32_t) * (size_t)nb);
+        for (size_t p = 0; p < npix; p++) {
+            const uint32_t b = pix_body[p];
+            const bool live = b != 0xFFFFFFFFu && bodies[b].enabled &&
+                              !((w->d.mat.air >> (pixels[p].mat & 15)) & 1u);
+            if (!live) continue;
+            if (!state[p]) {
+                unplaced++;
+                continue;
+            }
+            placed++;
+            if (state[p] >= 2) {
+                if (res->displaced_out && nd < res->displaced_cap) {
+                    res->displaced_out[nd].x = (int32_t)(uint32_t)dispxy[p];
+                    res->displaced_out[nd].y = (int32_t)(uint32_t)(dispxy[p] >> 32);
+                    std::memcpy(&res->displaced_out[nd].cell, &disp[p], 8);
+                }
+                nd++;
+                if (state[p] == 2 && res->hits_sand) res->hits_sand[b]++;
+                if (state[p] == 3 && res->hits_soup) res->hits_soup[b]++;
+            }
+        }
+        res->placed = placed;
+        res->unplaced = unplaced;
+        res->displaced = nd;
+    }
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_unstamp_bodies(fsg_world *w, fsg_cell *pixels_out, size_t npix) {
+    if (!w || (npix && !pixels_out)) return FSG_ERR_INVALID;
+    if (!w->stamped) return FSG_ERR_STATE;
+    if (npix != w->npix) return FSG_ERR_INVALID;
+    fsg_ctx *c = w->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    const fsg::StampArgs a = stamp_args(w);
+    if (npix) {
+        const unsigned g = grid_for(npix, 256);
+        fsg::k_claims_init<<<g, 256, 0, c->stream>>>(a);
+        for (int d = 0; d < 5; d++) {
+            fsg::k_unstamp_propose<<<g, 256, 0, c->stream>>>(a, d);
+            fsg::k_unstamp_resolve<<<g, 256, 0, c->stream>>>(a, d);
+        }
+        fsg::k_unstamp_finish<<<g, 256, 0, c->stream>>>(a);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(pixels_out, w->d_pixels, npix * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    w->stamped = false;
+    return FSG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batching
+// ---------------------------------------------------------------------------
+namespace {
+
+int batch_prepare(fsg_ctx *c, fsg_world *const *ws, int n, bool *graph_ok) {
+    for (int i = 0; i < n; i++)
+        if (!ws[i] || ws[i]->ctx != c) return FSG_ERR_INVALID;
+    if (n > c->batch_cap) {
+        dev_free(c->d_worlds);
+        dev_free(c->d_tile_start);
+        c->d_worlds = nullptr;
+        c->d_tile_start = nullptr;
+        c->batch_cap = 0;
+        void *p;
+        if (dev_alloc(c, &p, (size_t)n * sizeof(DevWorld))) return FSG_ERR_NOMEM;
+        c->d_worlds = (DevWorld *)p;
+        if (dev_alloc(c, &p, (size_t)(n + 1) * 4)) return FSG_ERR_NOMEM;
+        c->d_tile_start = (uint32_t *)p;
+        c->batch_cap = n;
+        if (c->graph) (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+        c->graph_key.clear();
+    }
+    std::vector<DevWorld> dw(n);
+    std::vector<uint32_t> start(n + 1);
+    uint64_t total = 0;
+    for (int i = 0; i < n; i++) {
+        dw[i] = ws[i]->d;
+        start[i] = (uint32_t)total;
+        total += (uint64_t)ws[i]->d.tiles_x * ws[i]->d.tiles_y;
+    }
+    if (total >= 0x7FFFFFFFull) return FSG_ERR_INVALID;
+    start[n] = (uint32_t)total;
+    HIPCHK(hipMemcpyAsync(c->d_worlds, dw.data(), (size_t)n * sizeof(DevWorld), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_tile_start, start.data(), (size_t)(n + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    // host staging buffers must outlive the async copies
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const std::vector<fsg_world *> key(ws, ws + n);
+    *graph_ok = c->graph && key == c->graph_key && start == c->graph_tiles;
+    if (!*graph_ok) {
+        c->graph_key = key;
+        c->graph_tiles = start;
+    }
+    c->total_tiles = (uint32_t)total;
+    return FSG_OK;
+}
+
+int batch_run(fsg_ctx *c, fsg_world *const *ws, int n, int ticks, bool graph_ok) {
+    const unsigned tiles = c->total_tiles;
+    int done = 0;
+    if (ticks >= kGraphTicks) {
+        if (!graph_ok) {
+            if (c->graph) (void)hipGraphExecDestroy(c->graph);
+            c->graph = nullptr;
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < kGraphTicks; k++)
+                fsg::k_tick_batch<<<tiles, fsg::NT, 0, c->stream>>>(c->d_worlds, c->d_tile_start, n, (uint32_t)k);
+            fsg::k_advance<<<grid_for(n, 256), 256, 0, c->stream>>>(c->d_worlds, n, kGraphTicks);
+            HIPCHK(hipStreamEndCapture(c->stream, &g));
+            const hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                c->graph = nullptr;
+                return FSG_ERR_DEVICE;
+            }
+        }
+        for (; done + kGraphTicks <= ticks; done += kGraphTicks) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    }
+    const int rem = ticks - done;
+    if (rem > 0) {
+        for (int k = 0; k < rem; k++)
+            fsg::k_tick_batch<<<tiles, fsg::NT, 0, c->stream>>>(c->d_worlds, c->d_tile_start, n, (uint32_t)k);
+        fsg::k_advance<<<grid_for(n, 256), 256, 0, c->stream>>>(c->d_worlds, n, (uint32_t)rem);
+        HIPCHK(hipGetLastError());
+    }
+    for (int i = 0; i < n; i++) {
+        ws[i]->d.tick += (uint64_t)ticks;
+        ws[i]->offered += (uint64_t)ticks * ws[i]->d.tiles_x * ws[i]->d.tiles_y;
+    }
+    return FSG_OK;
+}
+
+} // namespace
+
+extern "C" int fsg_batch_step(fsg_ctx *c, fsg_world *const *ws, int n, int ticks) {
+    if (!c || (n && !ws) || n < 0 || ticks < 0) return FSG_ERR_INVALID;
+    if (n == 0 || ticks == 0) return FSG_OK;
+    HIPCHK(hipSetDevice(c->device));
+    bool graph_ok = false;
+    if (int rc = batch_prepare(c, ws, n, &graph_ok)) return rc;
+    return batch_run(c, ws, n, ticks, graph_ok);
+}
+
+extern "C" int fsg_batch_step_timed(fsg_ctx *c, fsg_world *const *ws, int n, int ticks, double *ms) {
+    if (!c || (n && !ws) || n < 0 || ticks < 0) return FSG_ERR_INVALID;
+    HIPCHK(hipSetDevice(c->device));
+    bool graph_ok = false;
+    if (n > 0)
+        if (int rc = batch_prepare(c, ws, n, &graph_ok)) return rc;
+    HIPCHK(hipEventRecord(c->e0, c->stream));
+    if (n > 0 && ticks > 0)
+        if (int rc = batch_run(c, ws, n, ticks, graph_ok)) return rc;
+    HIPCHK(hipEventRecord(c->e1, c->stream));
+    HIPCHK(hipEventSynchronize(c->e1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, c->e0, c->e1));
+    if (ms) *ms = f;
+    return FSG_OK;
+}
+
+extern "C" int fsg_perworld_step_timed(fsg_ctx *c, fsg_world *const *ws, int n, int ticks, double *ms) {
+    if (!c || (n && !ws) || n < 0 || ticks < 0) return FSG_ERR_INVALID;
+    for (int i = 0; i < n; i++)
+        if (!ws[i] || ws[i]->ctx != c) return FSG_ERR_INVALID;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipEventRecord(c->e0, c->stream));
+    for (int t = 0; t < ticks; t++)
+        for (int i = 0; i < n; i++)
+            if (int rc = launch_tick(ws[i])) return rc;
+    HIPCHK(hipEventRecord(c->e1, c->stream));
+    HIPCHK(hipEventSynchronize(c->e1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, c->e0, c->e1));
+    if (ms) *ms = f;
+    return FSG_OK;
+}

@@ -1,0 +1,272 @@
+// fsg_device.hpp -- device-side data layout and the per-block update rule of
+// the deterministic falling-sand tick, written for gfx950 (CDNA4).
+//
+// Material properties live in 64-bit nibble tables and 32-bit bit masks so a
+// lookup is one shift and one mask on values that sit in SGPRs: no dependent
+// loads through a Material* per neighbour (the reference chases tile.mat->...
+// for every neighbour, world.cpp:1097-1211).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fsg {
+
+constexpr int TW = 64;               // chunk (tile) width in cells
+constexpr int TH = 64;               // chunk height
+constexpr int HALO = 4;              // cell_iter (<= 3) Margolus phases + 1 temperature step
+constexpr int LW = TW + 2 * HALO;    // LDS tile width  (72)
+constexpr int LH = TH + 2 * HALO;    // LDS tile height (72)
+constexpr int NT = 256;              // threads per workgroup (4 wave64)
+constexpr int MAX_IT = 3;            // largest cell_iter the fused kernel supports
+constexpr uint64_t MOVED = 0x100;
+constexpr uint64_t WALL = 15;        // out-of-world cell: inert solid (getTile OOB, world.cpp:1000)
+constexpr uint64_t GOLDEN = 0x9E3779B97F4A7C15ull;
+
+struct MatTab {
+    uint64_t pass;          // nibble/id: SOLID -> 15, else density rank
+    uint64_t sink[MAX_IT];  // nibble/id: SAND or SOUP active in iteration it -> rank, else 0
+    uint64_t flow[MAX_IT];  // nibble/id: SOUP active in it -> rank, else 0
+    uint32_t gas[MAX_IT];   // bit/id: GAS active in it
+    uint32_t air, sand, soup; // bit/id
+    uint32_t rgba[16];      // colour of cells created by reactions
+    int32_t add_temp[16];
+    uint32_t any_add_temp;
+    int32_t burn;           // wood burn temperature
+    uint32_t water, lava, steam, stone, wood, air_id;
+    uint32_t reactions, cell_iter, temp_every, temp_phase;
+};
+
+struct DevWorld {
+    uint64_t *buf[2];       // state before tick t lives in buf[t & 1]
+    uint32_t *flags[2];     // per-chunk activity; tick t reads flags[(t+1)&1], writes flags[t&1]
+    int32_t W, H, tiles_x, tiles_y;
+    uint64_t seed;
+    uint64_t tick;          // tick of launch offset k = 0
+    uint64_t force_tick;    // every chunk is processed at this tick
+    uint32_t skip;          // exact dirty-rect skipping on
+    uint32_t pad;
+    unsigned long long *processed; // chunk updates run (skip mode)
+    MatTab mat;
+};
+
+__host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    uint64_t z = x + GOLDEN;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint32_t lowbias32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ int nib(uint64_t tab, uint32_t m) { return (int)((tab >> (m << 2)) & 15u); }
+__device__ __forceinline__ bool bit(uint32_t mask, uint32_t m) { return (mask >> m) & 1u; }
+__device__ __forceinline__ uint32_t cid(uint64_t c) { return (uint32_t)c & 15u; }
+__device__ __forceinline__ uint32_t cbyte(uint64_t c) { return (uint32_t)c & 0xFFu; }
+__device__ __forceinline__ int ctemp(uint64_t c) { return (int)(int16_t)(uint16_t)(c >> 16); }
+
+// Reaction product: material m, same temperature, base colour, flags clear.
+__device__ __forceinline__ uint64_t xform(const MatTab &M, uint64_t c, uint32_t m) {
+    return (uint64_t)m | (c & 0xFFFF0000ull) | ((uint64_t)M.rgba[m & 15] << 32);
+}
+
+// Block random words for the block whose top-left world cell is (x0, y0).
+__device__ __forceinline__ void block_hash(uint64_t key, int x0, int y0, uint32_t &r, uint32_t &s) {
+    const uint32_t pos = (uint32_t)(x0 + 1) | ((uint32_t)(y0 + 1) << 16);
+    r = lowbias32(pos ^ (uint32_t)key);
+    s = lowbias32(r ^ (uint32_t)(key >> 32));
+}
+
+// One Margolus block, q = {TL, TR, BL, BR}.  Returns true if a cell changed.
+// Stages: R reactions, V vertical, D diagonal, F friction stop, H horizontal;
+// each stage restates one group of reference rules (world.cpp:1148-1889).
+__device__ __forceinline__ bool block_rule(const MatTab &M, int it, uint32_t r, uint32_t s,
+                                           uint64_t q[4]) {
+    const uint64_t o0 = q[0], o1 = q[1], o2 = q[2], o3 = q[3];
+    const uint64_t pass = M.pass, sinkt = M.sink[it], flowt = M.flow[it];
+    const uint32_t gasm = M.gas[it], airm = M.air, sandm = M.sand;
+
+    // R: water above lava -> steam, lava of the block -> stone; hot wood -> steam
+    if (M.reactions) {
+        bool fired = false;
+        const bool wl = cbyte(q[0]) == M.water && cbyte(q[2]) == M.lava;
+        const bool wr = cbyte(q[1]) == M.water && cbyte(q[3]) == M.lava;
+        if (wl || wr) {
+            if (wl) q[0] = xform(M, q[0], M.steam);
+            if (wr) q[1] = xform(M, q[1], M.steam);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (cbyte(q[i]) == M.lava) q[i] = xform(M, q[i], M.stone);
+            fired = true;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (cbyte(q[i]) == M.wood && ctemp(q[i]) > M.burn) {
+                q[i] = xform(M, q[i], M.steam);
+                bool done = false;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (!done && bit(airm, cid(q[j]))) {
+                        const uint64_t t = q[i];
+                        q[i] = q[j];
+                        q[j] = t;
+                        done = true;
+                    }
+                }
+                fired = true;
+            }
+        }
+        if (fired) return q[0] != o0 || q[1] != o1 || q[2] != o2 || q[3] != o3;
+    }
+
+    // V: per column, left then right
+    bool vm = false;
+#pragma unroll
+    for (int col = 0; col < 2; col++) {
+        const int o = 1 - col;
+        uint64_t t = q[col];
+        const uint64_t u = q[col + 2];
+        const uint32_t mt = cid(t), mu = cid(u);
+        const int sr = nib(sinkt, mt);
+        if (nib(pass, mu) < sr) {
+            const bool sand = bit(sandm, mt);
+            const bool skip = sand && nib(pass, cid(q[o + 2])) < sr && ((r >> (4 * col)) & 15u) == 0;
+            if (!skip) {
+                if (sand && ((r >> (8 + col)) & 1u)) t |= MOVED;
+                q[col] = u;
+                q[col + 2] = t;
+                vm = true;
+                if (sand && ((r >> (10 + 2 * col)) & 3u) == 0 && bit(sandm, cid(q[o + 2])))
+                    q[o + 2] |= MOVED;
+            }
+        } else if (bit(gasm, mu) && bit(airm, mt)) {
+            if (!(bit(airm, cid(q[o])) && ((r >> (14 + col)) & 1u))) {
+                q[col] = u;
+                q[col + 2] = t;
+                vm = true;
+            }
+        }
+    }
+    if (vm) return true;
+
+    // D: TL->BR or TR->BL for sinking cells (sand needs MOVED and shouldMove),
+    // else gas BL->TR or BR->TL into air
+    {
+        const uint32_t ma = cid(q[0]), mb = cid(q[1]);
+        bool cl = nib(pass, cid(q[3])) < nib(sinkt, ma);
+        bool cr = nib(pass, cid(q[2])) < nib(sinkt, mb);
+        if (cl && bit(sandm, ma)) {
+            if (!(q[0] & MOVED)) cl = false;
+            else if ((s & 31u) == 0) { cl = false; q[0] &= ~MOVED; }
+        }
+        if (cr && bit(sandm, mb)) {
+            if (!(q[1] & MOVED)) cr = false;
+            else if (((s >> 5) & 31u) == 0) { cr = false; q[1] &= ~MOVED; }
+        }
+        if (cl && cr) {
+            if ((s >> 10) & 1u) cl = false;
+            else cr = false;
+        }
+        if (cl) {
+            uint64_t a2 = q[0];
+            const bool sand = bit(sandm, ma);
+            if (sand && ((s >> 11) & 511u) == 0) a2 &= ~MOVED;
+            if (sand && bit(sandm, cid(q[2])) && ((s >> 20) & 3u) == 0) q[2] |= MOVED;
+            if (bit(airm, cid(q[1]))) {
+                const uint64_t side = q[1];
+                q[1] = q[3];
+                q[0] = side;
+            } else {
+                q[0] = q[3];
+            }
+            q[3] = a2;
+            return true;
+        }
+        if (cr) {
+            uint64_t b2 = q[1];
+            const bool sand = bit(sandm, mb);
+            if (sand && ((s >> 11) & 511u) == 0) b2 &= ~MOVED;
+            if (sand && bit(sandm, cid(q[3])) && ((s >> 20) & 3u) == 0) q[3] |= MOVED;
+            if (bit(airm, cid(q[0]))) {
+                const uint64_t side = q[0];
+                q[0] = q[2];
+                q[1] = side;
+            } else {
+                q[1] = q[2];
+            }
+            q[2] = b2;
+            return true;
+        }
+        bool gl = bit(gasm, cid(q[2])) && bit(airm, cid(q[1]));
+        bool gr = bit(gasm, cid(q[3])) && bit(airm, cid(q[0]));
+        if (gl && gr) {
+            if ((s >> 22) & 1u) gl = false;
+            else gr = false;
+        }
+        if (gl) {
+            const uint64_t t = q[2];
+            q[2] = q[1];
+            q[1] = t;
+            return true;
+        }
+        if (gr) {
+            const uint64_t t = q[3];
+            q[3] = q[0];
+            q[0] = t;
+            return true;
+        }
+    }
+
+    // F: an active sand cell with neither its straight nor its in-block
+    // diagonal move open stops (MOVED cleared)
+#pragma unroll
+    for (int col = 0; col < 2; col++) {
+        const int o = 1 - col;
+        const uint32_t mt = cid(q[col]);
+        const int sr = nib(sinkt, mt);
+        if (bit(sandm, mt) && sr > 0 && nib(pass, cid(q[col + 2])) >= sr && nib(pass, cid(q[o + 2])) >= sr)
+            q[col] &= ~MOVED;
+    }
+
+    // H: per row, top then bottom; soup into lighter non-solid, gas into air, p=1/2
+#pragma unroll
+    for (int row = 0; row < 2; row++) {
+        const int i = 2 * row, j = i + 1;
+        const uint32_t ml = cid(q[i]), mr = cid(q[j]);
+        const bool f = nib(pass, mr) < nib(flowt, ml) || nib(pass, ml) < nib(flowt, mr) ||
+                       (bit(gasm, ml) && bit(airm, mr)) || (bit(gasm, mr) && bit(airm, ml));
+        if (f && ((s >> (23 + row)) & 1u)) {
+            const uint64_t t = q[i];
+            q[i] = q[j];
+            q[j] = t;
+        }
+    }
+    return q[0] != o0 || q[1] != o1 || q[2] != o2 || q[3] != o3;
+}
+
+// Pair tests of the dirty-rect predicate (iteration gate taken at it = 0).
+__device__ __forceinline__ bool pair_vert(const MatTab &M, uint64_t t, uint64_t u) {
+    const uint32_t mt = cid(t), mu = cid(u);
+    return nib(M.pass, mu) < nib(M.sink[0], mt) || (bit(M.gas[0], mu) && bit(M.air, mt)) ||
+           (M.reactions && cbyte(t) == M.water && cbyte(u) == M.lava);
+}
+__device__ __forceinline__ bool pair_diag(const MatTab &M, uint64_t t, uint64_t u) {
+    const uint32_t mt = cid(t), mu = cid(u);
+    return nib(M.pass, mu) < nib(M.sink[0], mt) || (bit(M.gas[0], mu) && bit(M.air, mt));
+}
+__device__ __forceinline__ bool pair_horz(const MatTab &M, uint64_t l, uint64_t r) {
+    const uint32_t ml = cid(l), mr = cid(r);
+    return nib(M.pass, mr) < nib(M.flow[0], ml) || nib(M.pass, ml) < nib(M.flow[0], mr) ||
+           (bit(M.gas[0], ml) && bit(M.air, mr)) || (bit(M.gas[0], mr) && bit(M.air, ml));
+}
+__device__ __forceinline__ bool single_unstable(const MatTab &M, uint64_t c) {
+    return (c & MOVED) || (M.reactions && cbyte(c) == M.wood && ctemp(c) > M.burn);
+}
+
+} // namespace fsg

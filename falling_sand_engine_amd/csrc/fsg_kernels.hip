@@ -1,0 +1,337 @@
+// fsg_kernels.hip -- HIP kernels of the falling-sand tick for gfx950.
+//
+// k_tick_world / k_tick_batch: one workgroup updates one 64x64 chunk for one
+// whole tick.  It stages the chunk plus a 4-cell halo (72x72 cells, 41 KB) in
+// LDS with coalesced 8-byte loads, runs every Margolus phase of the tick and
+// the temperature step on the LDS copy (blocks are disjoint inside a phase,
+// so no atomics are needed; halo cells are recomputed redundantly and shrink
+// by one ring per phase), and writes the chunk interior once.  HBM traffic is
+// one read and one write of each cell per tick plus the halo re-read that the
+// L2/MALL mostly absorbs.  This replaces the reference's 16-thread chunk pool
+// (world.cpp:1057-1077) and its 3 sequential passes per chunk.
+#include "fsg_device.hpp"
+#include "fsg_kernels.hpp"
+
+namespace fsg {
+
+__constant__ int kOX[4] = {0, 1, 1, 0};
+__constant__ int kOY[4] = {0, 1, 0, 1};
+
+__device__ __forceinline__ uint64_t temp_step(const MatTab &M, uint64_t (*L)[LW], int ly, int lx) {
+    const uint64_t c = L[ly][lx];
+    const int T = ctemp(c);
+    const uint64_t nu = L[ly - 1][lx], nd = L[ly + 1][lx], nl = L[ly][lx - 1], nr = L[ly][lx + 1];
+    const int tu = cid(nu) == WALL ? T : ctemp(nu);
+    const int td = cid(nd) == WALL ? T : ctemp(nd);
+    const int tl = cid(nl) == WALL ? T : ctemp(nl);
+    const int tr = cid(nr) == WALL ? T : ctemp(nr);
+    int v = T + ((tu + td + tl + tr - 4 * T) >> 3);
+    if (M.any_add_temp) v += M.add_temp[cid(c)];
+    v = v > 32767 ? 32767 : (v < -32768 ? -32768 : v);
+    return (c & 0xFFFFFFFF0000FFFFull) | ((uint64_t)(uint16_t)v << 16);
+}
+
+// Dirty-rect predicate over the staged tile: can any block of the LDS region
+// change under some random draw, or any temperature stencil of the interior?
+__device__ bool tile_unstable(const MatTab &M, uint64_t (*L)[LW], bool temp_enabled) {
+    bool u = false;
+    for (int i = threadIdx.x; i < LW * LH; i += NT) {
+        const int ly = i / LW, lx = i - ly * LW;
+        const uint64_t c = L[ly][lx];
+        u |= single_unstable(M, c);
+        if (lx + 1 < LW) u |= pair_horz(M, c, L[ly][lx + 1]);
+        if (ly + 1 < LH) {
+            u |= pair_vert(M, c, L[ly + 1][lx]);
+            if (lx + 1 < LW) u |= pair_diag(M, c, L[ly + 1][lx + 1]);
+            if (lx > 0) u |= pair_diag(M, c, L[ly + 1][lx - 1]);
+        }
+    }
+    if (temp_enabled) {
+        for (int i = threadIdx.x; i < TW * TH; i += NT) {
+            const int ly = HALO + i / TW, lx = HALO + (i & (TW - 1));
+            const uint64_t c = L[ly][lx];
+            if (cid(c) == WALL) continue;
+            u |= temp_step(M, L, ly, lx) != c;
+        }
+    }
+    return u;
+}
+
+__device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, int tile,
+                                            uint64_t (*L)[LW]) {
+    const int tid = threadIdx.x;
+    const MatTab &M = w.mat;
+    const int tx = tile % w.tiles_x, ty = tile / w.tiles_x;
+    const int gx0 = tx * TW - HALO, gy0 = ty * TH - HALO;  // world coords of L[0][0]
+    const uint64_t *__restrict__ in = w.buf[tick & 1];
+    uint64_t *__restrict__ out = w.buf[(tick + 1) & 1];
+    const bool skip = w.skip != 0;
+
+    if (skip) {
+        // Exact skip: a chunk sleeps when its last update found no possible
+        // change in its region and no chunk of its 3x3 neighbourhood changed.
+        const uint32_t *fr = w.flags[(tick + 1) & 1];
+        bool awake = tick == w.force_tick || (fr[tile] & 1u);
+        for (int dy = -1; dy <= 1 && !awake; dy++)
+            for (int dx = -1; dx <= 1 && !awake; dx++) {
+                const int nx = tx + dx, ny = ty + dy;
+                if (nx < 0 || ny < 0 || nx >= w.tiles_x || ny >= w.tiles_y) continue;
+                awake = (fr[ny * w.tiles_x + nx] & 2u) != 0;
+            }
+        if (!awake) {
+            if (tid == 0) w.flags[tick & 1][tile] = 0;
+            return;
+        }
+    }
+
+    for (int i = tid; i < LW * LH; i += NT) {
+        const int ly = i / LW, lx = i - ly * LW;
+        const int gx = gx0 + lx, gy = gy0 + ly;
+        L[ly][lx] = (gx >= 0 && gy >= 0 && gx < w.W && gy < w.H) ? in[(size_t)gy * w.W + gx] : WALL;
+    }
+    __syncthreads();
+
+    bool pot = true;
+    if (skip) pot = __syncthreads_or(tile_unstable(M, L, M.temp_every != 0)) != 0;
+
+    if (pot) {
+        const int ci = (int)M.cell_iter;
+        for (int it = 0; it < ci; it++) {
+            const uint64_t g = tick * (uint64_t)M.cell_iter + (uint64_t)it;
+            const uint64_t key = splitmix64(splitmix64(w.seed) ^ g);
+            const int ox = kOX[g & 3], oy = kOY[g & 3];
+            const int nbx = (LW - ox) >> 1, nby = (LH - oy) >> 1;
+            for (int b = tid; b < nbx * nby; b += NT) {
+                const int by = b / nbx, bx = b - by * nbx;
+                const int lx = ox + 2 * bx, ly = oy + 2 * by;
+                uint64_t q[4] = {L[ly][lx], L[ly][lx + 1], L[ly + 1][lx], L[ly + 1][lx + 1]};
+                uint32_t r, s;
+                block_hash(key, gx0 + lx, gy0 + ly, r, s);
+                if (block_rule(M, it, r, s, q)) {
+                    L[ly][lx] = q[0];
+                    L[ly][lx + 1] = q[1];
+                    L[ly + 1][lx] = q[2];
+                    L[ly + 1][lx + 1] = q[3];
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    const bool temp_now = M.temp_every != 0 && (tick % M.temp_every) == M.temp_phase;
+    bool changed = false;
+    for (int i = tid; i < TW * TH; i += NT) {
+        const int ly = HALO + i / TW, lx = HALO + (i & (TW - 1));
+        const int gx = gx0 + lx, gy = gy0 + ly;
+        if (gx >= w.W || gy >= w.H) continue;
+        const uint64_t c = temp_now ? temp_step(M, L, ly, lx) : L[ly][lx];
+        const size_t gi = (size_t)gy * w.W + gx;
+        out[gi] = c;
+        if (skip) changed |= c != in[gi];
+    }
+    if (skip) {
+        changed = __syncthreads_or(changed) != 0;
+        if (tid == 0) {
+            w.flags[tick & 1][tile] = (pot ? 1u : 0u) | (changed ? 2u : 0u);
+            atomicAdd(w.processed, 1ull);
+        }
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_tick_world(DevWorld w, uint32_t k) {
+    __shared__ uint64_t L[LH][LW];
+    tile_update(w, w.tick + k, blockIdx.x, L);
+}
+
+// Many worlds in one launch: tile_start[i] is the first global chunk index of
+// world i (prefix sums, tile_start[n] = total).
+__global__ __launch_bounds__(NT) void k_tick_batch(const DevWorld *__restrict__ worlds,
+                                                   const uint32_t *__restrict__ tile_start,
+                                                   int nworlds, uint32_t k) {
+    __shared__ uint64_t L[LH][LW];
+    const uint32_t gt = blockIdx.x;
+    int lo = 0, hi = nworlds - 1;
+    while (lo < hi) {  // last world with tile_start <= gt
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= gt) lo = mid;
+        else hi = mid - 1;
+    }
+    const DevWorld &w = worlds[lo];
+    tile_update(w, w.tick + k, (int)(gt - tile_start[lo]), L);
+}
+
+__global__ void k_advance(DevWorld *worlds, int nworlds, uint32_t ticks) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nworlds) worlds[i].tick += ticks;
+}
+
+// Standalone temperature step (world::tickTemperature): reads buf[t&1], writes
+// buf[(t+1)&1]; the caller advances the buffer parity by swapping roles.
+__global__ __launch_bounds__(NT) void k_temperature(const uint64_t *__restrict__ in,
+                                                     uint64_t *__restrict__ out, int W, int H,
+                                                     MatTab M) {
+    const size_t i = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= (size_t)W * H) return;
+    const int x = (int)(i % (size_t)W), y = (int)(i / (size_t)W);
+    const uint64_t c = in[i];
+    const int T = ctemp(c);
+    const int tu = y > 0 ? ctemp(in[i - W]) : T;
+    const int td = y < H - 1 ? ctemp(in[i + W]) : T;
+    const int tl = x > 0 ? ctemp(in[i - 1]) : T;
+    const int tr = x < W - 1 ? ctemp(in[i + 1]) : T;
+    int v = T + ((tu + td + tl + tr - 4 * T) >> 3);
+    if (M.any_add_temp) v += M.add_temp[cid(c)];
+    v = v > 32767 ? 32767 : (v < -32768 ? -32768 : v);
+    out[i] = (c & 0xFFFFFFFF0000FFFFull) | ((uint64_t)(uint16_t)v << 16);
+}
+
+// Census: per-material counts and an order-sensitive checksum
+// (sum of splitmix64(cell + index*GOLDEN)); out[0..15] counts, out[16] checksum.
+__global__ __launch_bounds__(NT) void k_stats(const uint64_t *__restrict__ cells, size_t n,
+                                              unsigned long long *out) {
+    __shared__ unsigned int hist[16];
+    __shared__ unsigned long long wsum[NT / 64];
+    if (threadIdx.x < 16) hist[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned long long sum = 0;
+    for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) {
+        const uint64_t c = cells[i];
+        atomicAdd(&hist[c & 15u], 1u);
+        sum += splitmix64(c + (uint64_t)i * GOLDEN);
+    }
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x < 16 && hist[threadIdx.x]) atomicAdd(&out[threadIdx.x], (unsigned long long)hist[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < NT / 64; k++) t += wsum[k];
+        atomicAdd(&out[16], t);
+    }
+}
+
+// setTile batch (edits deduplicated on the host, so order does not matter).
+__global__ void k_fill(uint64_t *cells, size_t n, uint64_t v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cells[i] = v;
+}
+
+__global__ void k_set_cells(uint64_t *cells, int W, int H, const int32_t *xy, const uint64_t *vals,
+                            size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = xy[2 * i], y = xy[2 * i + 1];
+    if (x < 0 || y < 0 || x >= W || y >= H) return;  // setTile ignores OOB (world.cpp:1005)
+    cells[(size_t)y * W + x] = vals[i];
+}
+
+// ---------------------------------------------------------------------------
+// Rigid-body stamp / unstamp (game.cpp:1760-1815, 1896-1983) in 5 claim
+// rounds.  A claim is the 64-bit value (round << 32 | pixel); atomicMin on
+// the target's slot in the spare state buffer makes the lowest pixel of the
+// earliest round win, which the CPU mirrors by scanning pixels in order.
+// ---------------------------------------------------------------------------
+__constant__ int kDX[5] = {0, 1, -1, 0, 0};
+__constant__ int kDY[5] = {0, 0, 0, 1, -1};
+
+__device__ __forceinline__ bool pixel_target(const fsg_body *bodies, const uint32_t *pix_body,
+                                             uint32_t p, int d, int W, int H, int &wx, int &wy) {
+    const uint32_t b = pix_body[p];
+    if (b == 0xFFFFFFFFu) return false;
+    const fsg_body B = bodies[b];
+    if (!B.enabled) return false;
+    const uint32_t k = p - B.pixel_offset;
+    const int tx = (int)(k % (uint32_t)B.mat_w), ty = (int)(k / (uint32_t)B.mat_w);
+    const int64_t fx = (int64_t)tx * B.cos_q16 - (int64_t)(ty + 1) * B.sin_q16 + B.pos_x_q16;
+    const int64_t fy = (int64_t)tx * B.sin_q16 + (int64_t)(ty + 1) * B.cos_q16 + B.pos_y_q16;
+    wx = (int)(fx >> 16);
+    wy = (int)(fy >> 16);
+    if (d >= 0) {
+        wx += kDX[d];
+        wy += kDY[d];
+    }
+    return wx >= 0 && wy >= 0 && wx < W && wy < H;
+}
+
+__device__ __forceinline__ void mark_tile(uint32_t *flags, int tiles_x, int x, int y) {
+    if (flags) atomicOr(&flags[(y / TH) * tiles_x + (x / TW)], 2u);
+}
+
+// Initialise the claim slots of every possible target; mark touched chunks.
+__global__ void k_claims_init(StampArgs a) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.npix) return;
+    a.state[p] = 0;
+    if (bit(a.mat.air, cid(a.pixels[p]))) return;
+    for (int d = 0; d < 5; d++) {
+        int wx, wy;
+        if (!pixel_target(a.bodies, a.pix_body, p, d, a.W, a.H, wx, wy)) continue;
+        a.claims[(size_t)wy * a.W + wx] = ~0ull;
+        mark_tile(a.flags, a.tiles_x, wx, wy);
+    }
+}
+
+__global__ void k_stamp_propose(StampArgs a, int d) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.npix || a.state[p] || bit(a.mat.air, cid(a.pixels[p]))) return;
+    int wx, wy;
+    if (!pixel_target(a.bodies, a.pix_body, p, d, a.W, a.H, wx, wy)) return;
+    const size_t ci = (size_t)wy * a.W + wx;
+    const uint32_t m = cid(a.cells[ci]);
+    if (!(bit(a.mat.air, m) || bit(a.mat.sand, m) || bit(a.mat.soup, m))) return;
+    atomicMin(reinterpret_cast<unsigned long long *>(&a.claims[ci]),
+              ((unsigned long long)d << 32) | p);
+}
+
+__global__ void k_stamp_resolve(StampArgs a, int d) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.npix || a.state[p] || bit(a.mat.air, cid(a.pixels[p]))) return;
+    int wx, wy;
+    if (!pixel_target(a.bodies, a.pix_body, p, d, a.W, a.H, wx, wy)) return;
+    const size_t ci = (size_t)wy * a.W + wx;
+    if (a.claims[ci] != (((uint64_t)d << 32) | p)) return;
+    const uint64_t old = a.cells[ci];
+    if (!bit(a.mat.air, cid(old))) {
+        a.disp[p] = old;
+        a.disp_xy[p] = (uint64_t)(uint32_t)wx | ((uint64_t)(uint32_t)wy << 32);
+        a.state[p] = bit(a.mat.sand, cid(old)) ? 2 : 3;  // placed, displaced sand / soup
+    } else {
+        a.state[p] = 1;  // placed
+    }
+    a.cells[ci] = a.pixels[p];
+}
+
+__global__ void k_unstamp_propose(StampArgs a, int d) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.npix || a.state[p] || bit(a.mat.air, cid(a.pixels[p]))) return;
+    int wx, wy;
+    if (!pixel_target(a.bodies, a.pix_body, p, d, a.W, a.H, wx, wy)) return;
+    const size_t ci = (size_t)wy * a.W + wx;
+    if (cbyte(a.cells[ci]) != cbyte(a.pixels[p])) return;
+    atomicMin(reinterpret_cast<unsigned long long *>(&a.claims[ci]),
+              ((unsigned long long)d << 32) | p);
+}
+
+__global__ void k_unstamp_resolve(StampArgs a, int d) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.npix || a.state[p] || bit(a.mat.air, cid(a.pixels[p]))) return;
+    int wx, wy;
+    if (!pixel_target(a.bodies, a.pix_body, p, d, a.W, a.H, wx, wy)) return;
+    const size_t ci = (size_t)wy * a.W + wx;
+    if (a.claims[ci] != (((uint64_t)d << 32) | p)) return;
+    const uint64_t c = a.cells[ci];
+    a.pixels[p] = c;
+    a.cells[ci] = (uint64_t)a.mat.air_id | (c & 0xFFFF0000ull);
+    a.state[p] = 1;
+}
+
+// A pixel that found nothing is lost when the cell under it is air.
+__global__ void k_unstamp_finish(StampArgs a) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.npix || a.state[p] || bit(a.mat.air, cid(a.pixels[p]))) return;
+    int wx, wy;
+    if (!pixel_target(a.bodies, a.pix_body, p, -1, a.W, a.H, wx, wy)) return;
+    if (bit(a.mat.air, cid(a.cells[(size_t)wy * a.W + wx]))) a.pixels[p] = (uint64_t)a.mat.air_id;
+}
+
+} // namespace fsg

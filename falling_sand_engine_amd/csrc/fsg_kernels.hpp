@@ -1,0 +1,40 @@
+// fsg_kernels.hpp -- kernel entry points shared by the kernels and the C-ABI.
+#pragma once
+#include "fsg.h"
+#include "fsg_device.hpp"
+
+namespace fsg {
+
+// Arguments of the rigid-body claim-round kernels.
+struct StampArgs {
+    uint64_t *cells;           // current state buffer
+    uint64_t *claims;          // spare state buffer used as claim slots
+    uint64_t *pixels;          // body pixels (cells), body-major, row-major per body
+    const uint32_t *pix_body;  // body index per pixel (0xFFFFFFFF = none)
+    const fsg_body *bodies;
+    uint8_t *state;            // per pixel: 0 open, 1 placed/found, 2/3 placed over sand/soup
+    uint64_t *disp;            // displaced cell per pixel
+    uint64_t *disp_xy;         // its position (x | y << 32)
+    uint32_t *flags;           // chunk flags read by the next tick (null: no skipping)
+    int W, H, tiles_x;
+    uint32_t npix;
+    MatTab mat;
+};
+
+__global__ void k_tick_world(DevWorld w, uint32_t k);
+__global__ void k_tick_batch(const DevWorld *__restrict__ worlds, const uint32_t *__restrict__ tile_start,
+                             int nworlds, uint32_t k);
+__global__ void k_advance(DevWorld *worlds, int nworlds, uint32_t ticks);
+__global__ void k_temperature(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int W, int H,
+                              MatTab M);
+__global__ void k_stats(const uint64_t *__restrict__ cells, size_t n, unsigned long long *out);
+__global__ void k_fill(uint64_t *cells, size_t n, uint64_t v);
+__global__ void k_set_cells(uint64_t *cells, int W, int H, const int32_t *xy, const uint64_t *vals, size_t n);
+__global__ void k_claims_init(StampArgs a);
+__global__ void k_stamp_propose(StampArgs a, int d);
+__global__ void k_stamp_resolve(StampArgs a, int d);
+__global__ void k_unstamp_propose(StampArgs a, int d);
+__global__ void k_unstamp_resolve(StampArgs a, int d);
+__global__ void k_unstamp_finish(StampArgs a);
+
+} // namespace fsg

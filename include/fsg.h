@@ -1,0 +1,249 @@
+/*
+ * fsg.h -- C-ABI of the MI355X-native falling-sand tick.
+ *
+ * This is the drop-in boundary for the per-cell simulation path of
+ * cstom4994/falling_sand_engine.  The reference has no FFI on this path: its
+ * boundary is the C++ class ME::world (source/engine/world.hpp:63-193) plus
+ * the rigid-body stamp/unstamp loops in game::tick
+ * (source/engine/game.cpp:1760-1815, 1896-1983) that index world::real_tiles
+ * directly.  Every entry point below names the reference member or loop it
+ * replaces.  A C++ facade with the reference's method names (init, tick,
+ * tickTemperature, getTile, setTile) sits on top of this ABI in
+ * falling_sand_engine_amd/csrc/fsg_world.hpp.
+ *
+ * Conventions (all entry points):
+ *   - extern "C", plain pointers and sizes, no C++ or torch types;
+ *   - every function returns an int status: FSG_OK (0) or a negative
+ *     fsg_status; nothing throws across the ABI;
+ *   - handles are opaque; the library owns all device memory; host buffers
+ *     are caller-owned and copied;
+ *   - one HIP stream per context; calls on one world are serialised by the
+ *     caller; different worlds may be driven from different threads (the
+ *     serving queue has its own mutex).
+ */
+#ifndef FSG_H
+#define FSG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSG_ABI_VERSION 1
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------ */
+typedef enum fsg_status {
+    FSG_OK = 0,
+    FSG_ERR_INVALID = -1,   /* bad argument (null handle, size, id, ...)    */
+    FSG_ERR_NOMEM = -2,     /* device or host allocation failed             */
+    FSG_ERR_DEVICE = -3,    /* a HIP runtime call failed / no device        */
+    FSG_ERR_BUDGET = -4,    /* admission refused: HBM state budget exceeded */
+    FSG_ERR_STATE = -5,     /* call not valid in the object's current state */
+    FSG_ERR_CHECKSUM = -6,  /* resident state checksum mismatch             */
+    FSG_ERR_TIMEOUT = -7,   /* request deadline passed                      */
+    FSG_ERR_CANCELLED = -8, /* request cancelled by the caller              */
+    FSG_ERR_NOTFOUND = -9,  /* unknown request / world id                   */
+    FSG_ERR_AGAIN = -10     /* nothing ready yet (poll)                     */
+} fsg_status;
+
+const char *fsg_status_string(int status);
+
+/* ------------------------------------------------------------------------ */
+/* Cell: 8 bytes, replaces MaterialInstance (game_datastruct.hpp:207-225).   */
+/*   mat  bits 0-7  material id (Material::id)                               */
+/*        bit  8    MOVED flag (MaterialInstance::moved, sand friction)      */
+/*        bits 9-15 zero                                                     */
+/*   temp temperature in 1/16 degree (MaterialInstance::temperature, i16)    */
+/*   rgba colour, byte order R,G,B,A in memory (MaterialInstance::color +    */
+/*        Material::alpha)                                                   */
+/* Grid is row-major, index x + y*width, y grows downward (world.cpp:1001).  */
+/* ------------------------------------------------------------------------ */
+typedef struct fsg_cell {
+    uint16_t mat;
+    int16_t temp;
+    uint32_t rgba;
+} fsg_cell;
+
+#define FSG_MAT_ID_MASK 0x00FFu
+#define FSG_MOVED 0x0100u
+#define FSG_MAX_MATERIALS 15 /* ids 0..14; id 15 is reserved for the world wall */
+#define FSG_MAX_SIDE 32768   /* worlds are at most 32768 cells on a side       */
+
+/* PhysicsType (game_datastruct.hpp:87-95); PASSABLE/OBJECT are not used. */
+typedef enum fsg_phys {
+    FSG_AIR = 0,
+    FSG_SOLID = 1,
+    FSG_SAND = 2,
+    FSG_SOUP = 3,
+    FSG_GAS = 4
+} fsg_phys;
+
+/* Material (game_datastruct.hpp:130-168).  density is replaced by an integer
+ * rank (0 lightest); a heavier SAND/SOUP cell sinks into any non-SOLID cell
+ * of lower rank.  iterations gates the cell_iter loop exactly as
+ * world.cpp:1093-1096 does (a cell acts in iteration it iff it < iterations). */
+typedef struct fsg_material {
+    char name[16];
+    uint8_t type;        /* fsg_phys                                       */
+    uint8_t rank;        /* density rank 0..14                             */
+    uint8_t iterations;  /* Material::iterations                           */
+    uint8_t slipperyness;/* Material::slipperyness (documentation only)    */
+    int16_t add_temp;    /* Material::addTemp, 1/16 degree per temp step   */
+    uint16_t reserved;
+    uint32_t rgba;       /* base colour of cells created by reactions      */
+} fsg_material;
+
+/* Simulation config: the GlobalDEF fields on this path (cvar.hpp:38-87,
+ * data/scripts/global.lua:34-47) plus the build-defined reaction roles. */
+typedef struct fsg_config {
+    uint32_t cell_iter;      /* GlobalDEF::cell_iter (3)                    */
+    uint32_t reactions;      /* 0/1: water+lava, wood burn                  */
+    uint32_t temp_every;     /* 0 = temperature off; run when               */
+    uint32_t temp_phase;     /*   tick % temp_every == temp_phase           */
+    uint32_t chunk_skip;     /* 0/1: exact dirty-rect skipping, 64x64 chunks */
+    int32_t wood_burn_temp;  /* wood turns to steam above this (1/16 deg)   */
+    uint8_t mat_air, mat_water, mat_lava, mat_steam, mat_stone, mat_wood;
+    uint8_t pad[2];
+    uint32_t n_materials;    /* entries used in materials[]                 */
+    fsg_material materials[FSG_MAX_MATERIALS];
+} fsg_config;
+
+/* Fills *cfg with the 8-material table of the benchmark worlds
+ * (air, stone, sand, water, oil, steam, lava, wood), cell_iter 3,
+ * reactions on, temperature every tick, chunk skipping off. */
+int fsg_default_config(fsg_config *cfg);
+
+/* ------------------------------------------------------------------------ */
+/* Context: one device + one stream + an HBM state budget.                   */
+/* Replaces WorldSystem's thread pools (world.hpp:56-61, world.cpp:59-61).   */
+/* ------------------------------------------------------------------------ */
+typedef struct fsg_ctx fsg_ctx;
+typedef struct fsg_world fsg_world;
+
+typedef struct fsg_ctx_config {
+    int device;               /* HIP device ordinal                          */
+    uint64_t hbm_budget;      /* bytes of world state this ctx may hold; 0 = unlimited */
+} fsg_ctx_config;
+
+int fsg_ctx_create(const fsg_ctx_config *cfg, fsg_ctx **out);
+int fsg_ctx_destroy(fsg_ctx *ctx);
+/* Bytes of world state currently held by the context. */
+int fsg_ctx_usage(fsg_ctx *ctx, uint64_t *bytes_used);
+/* Test hook: make the next n device allocations of this ctx fail. */
+int fsg_ctx_inject_alloc_failures(fsg_ctx *ctx, int n);
+
+/* ------------------------------------------------------------------------ */
+/* World: replaces ME::world's grid and tick (world.hpp:63-193).             */
+/* ------------------------------------------------------------------------ */
+
+/* world::init (world.cpp:43-172): allocates the double-buffered device grid.
+ * cfg may be NULL (fsg_default_config).  seed keys the counter RNG. */
+int fsg_world_create(fsg_ctx *ctx, int width, int height, uint64_t seed,
+                     const fsg_config *cfg, fsg_world **out);
+int fsg_world_destroy(fsg_world *w);
+int fsg_world_info(fsg_world *w, int *width, int *height, uint64_t *tick);
+int fsg_world_set_tick(fsg_world *w, uint64_t tick); /* resume at tick */
+
+/* Whole-grid host->device copy (the chunk load path writes real_tiles,
+ * world.cpp:2653-2665).  n must equal width*height. */
+int fsg_world_upload(fsg_world *w, const fsg_cell *cells, size_t n);
+/* Rectangle device->host copy (getTile over a region, world.cpp:999-1002). */
+int fsg_world_download(fsg_world *w, fsg_cell *out, int x0, int y0, int rw, int rh);
+
+/* setTile (world.cpp:1004-1008) batched: edits are applied in array order
+ * before the next tick; out-of-bounds edits are ignored like setTile's. */
+typedef struct fsg_cell_edit {
+    int32_t x, y;
+    fsg_cell cell;
+} fsg_cell_edit;
+int fsg_world_set_cells(fsg_world *w, const fsg_cell_edit *edits, size_t n);
+
+/* world::tick (world.cpp:1036-1948) ticks times, each followed by the
+ * temperature step when the cadence says so (game.cpp:2157-2159).  Enqueued
+ * on the ctx stream; returns when the launches are queued. */
+int fsg_world_step(fsg_world *w, int ticks);
+/* Same, synchronous, and reports elapsed device time measured with HIP
+ * events on the ctx stream around the launches. */
+int fsg_world_step_timed(fsg_world *w, int ticks, double *elapsed_ms);
+/* world::tickTemperature (world.cpp:1950-2004): one standalone temperature
+ * step for callers that drive the cadence themselves (temp_every = 0). */
+int fsg_world_tick_temperature(fsg_world *w);
+int fsg_world_sync(fsg_world *w);
+
+/* Per-material counts and activity (the movingTiles[] census,
+ * game.cpp:1991-2000, and the active chunk fraction). */
+typedef struct fsg_stats {
+    uint64_t tick;
+    uint64_t count[16];       /* cells per material id                   */
+    uint64_t checksum;        /* order-dependent 64-bit hash of the grid */
+    uint64_t tiles_total;     /* 64x64 chunks in the world               */
+    uint64_t tiles_processed; /* chunk updates run since the last reset  */
+    uint64_t tiles_offered;   /* chunk slots offered since the last reset */
+} fsg_stats;
+int fsg_world_stats(fsg_world *w, fsg_stats *out);
+int fsg_world_reset_activity(fsg_world *w);
+
+/* ------------------------------------------------------------------------ */
+/* Rigid-body pixels (game.cpp:1760-1815 stamp, 1896-1983 unstamp;           */
+/* RigidBody::tiles, player.hpp:15-65).  Poses are fixed point, converted    */
+/* from the host physics engine's float pose once per body per tick.         */
+/* ------------------------------------------------------------------------ */
+typedef struct fsg_body {
+    int32_t pos_x_q16, pos_y_q16; /* b2Body::GetPosition, Q16.16 cells        */
+    int32_t cos_q16, sin_q16;     /* cos/sin of b2Body::GetAngle, Q16.16      */
+    int32_t mat_w, mat_h;         /* RigidBody::matWidth/matHeight            */
+    uint32_t pixel_offset;        /* first pixel in the pixel array           */
+    uint32_t enabled;             /* b2Body::IsEnabled                        */
+} fsg_body;
+
+typedef struct fsg_stamp_result {
+    uint32_t placed;          /* pixels written into the grid               */
+    uint32_t unplaced;        /* pixels with no free target in 5 offsets    */
+    uint32_t displaced;       /* SAND/SOUP cells pushed out (CellData analogue) */
+    uint32_t displaced_cap;   /* in: capacity of displaced_out              */
+    fsg_cell_edit *displaced_out; /* in: optional host buffer for displaced cells */
+    uint32_t *hits_sand;      /* in: optional [nbodies] damping counters    */
+    uint32_t *hits_soup;      /* in: optional [nbodies]                     */
+} fsg_stamp_result;
+
+/* Stamps every enabled body's non-air pixels into the grid.  The world keeps
+ * the bodies and pixels until fsg_world_unstamp_bodies. */
+int fsg_world_stamp_bodies(fsg_world *w, const fsg_body *bodies, int nbodies,
+                           const fsg_cell *pixels, size_t npixels,
+                           fsg_stamp_result *res);
+/* Pulls the pixels back out (same poses) and writes the updated pixel array
+ * (npixels cells, same layout as stamped) to pixels_out. */
+int fsg_world_unstamp_bodies(fsg_world *w, fsg_cell *pixels_out, size_t npixels);
+
+/* ------------------------------------------------------------------------ */
+/* Batching: many worlds, one launch per tick over a multi-world chunk list, */
+/* ticks captured as HIP graphs.  Results are bit-identical to stepping each */
+/* world alone.                                                              */
+/* ------------------------------------------------------------------------ */
+int fsg_batch_step(fsg_ctx *ctx, fsg_world *const *worlds, int n, int ticks);
+int fsg_batch_step_timed(fsg_ctx *ctx, fsg_world *const *worlds, int n, int ticks,
+                         double *elapsed_ms);
+/* Per-world launches of the same work, for comparison. */
+int fsg_perworld_step_timed(fsg_ctx *ctx, fsg_world *const *worlds, int n, int ticks,
+                            double *elapsed_ms);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side generator of the benchmark worlds (not on the device path).     */
+/* kind: 1 = basin terrain + sand discs (512^2 world), 2 = kind 1 + lava and  */
+/* oil discs, 3 = all-active striped world, 4 = kind 2 with disc counts       */
+/* scaled to the world's area.                                               */
+/* ------------------------------------------------------------------------ */
+int fsg_generate(int kind, int width, int height, uint64_t seed, fsg_cell *out);
+
+/* splitmix64 output for state x (Vigna), and the per-phase RNG key. */
+uint64_t fsg_splitmix64(uint64_t x);
+uint64_t fsg_phase_key(uint64_t seed, uint64_t phase_index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSG_H */

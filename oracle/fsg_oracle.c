@@ -1,0 +1,681 @@
+/*
+ * fsg_oracle.c -- deterministic CPU restatement of the falling-sand tick.
+ *
+ * TEST INFRASTRUCTURE ONLY (see fsg_oracle.h): the checker for the HIP path,
+ * never linked into the product.
+ *
+ * The reference tick (ME::world::tick, source/engine/world.cpp:1036-1948) is
+ * a sequential bottom-to-top scan per 128x128 chunk with global rand() on 16
+ * threads, so its trajectories are not reproducible.  This file defines the
+ * deterministic tick that replaces it and that the GPU matches bit for bit:
+ *
+ *   tick t = cell_iter phases (GlobalDEF::cell_iter, global.lua:47), phase
+ *   `it` being a Margolus pass over disjoint 2x2 blocks whose offset comes
+ *   from the global phase index g = t*cell_iter + it; then, if the cadence
+ *   says so, one integer temperature step (game.cpp:2157-2159).
+ *
+ * Each block is updated by orc_block_rule from its own four cells and two
+ * 32-bit random words keyed on (seed, g, block position).  Blocks never
+ * overlap inside a phase, so the result is independent of block order.
+ * The rules restate the reference's per-cell SAND/SOUP/GAS rules within the
+ * block; every rule cites the lines it follows.
+ */
+#include "fsg_oracle.h"
+#include <stdlib.h>
+#include <string.h>
+
+#define GOLDEN 0x9E3779B97F4A7C15ull
+#define WALL_ID 15
+#define MOVED ((uint64_t)FSG_MOVED)
+
+/* ----------------------------------------------------------------------- */
+/* RNG                                                                      */
+/* ----------------------------------------------------------------------- */
+
+/* splitmix64 (S. Vigna, public-domain reference implementation): the output
+ * of next() when the state before the call is `state`. */
+uint64_t orc_splitmix64(uint64_t state) {
+    uint64_t z = state + GOLDEN;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* lowbias32 (C. Wellons, hash-prospector): a 32-bit bijective mixer. */
+uint32_t orc_lowbias32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+/* Its published inverse. */
+uint32_t orc_lowbias32_inv(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x43021123u;
+    x ^= x >> 15 ^ x >> 30;
+    x *= 0x1d69e2a5u;
+    x ^= x >> 16;
+    return x;
+}
+
+/* Key of global phase g for a world seeded with `seed`.  Replaces the global
+ * rand() of world.cpp (passim) and RNG_Create's time(NULL) seed
+ * (game_utils/rng.cpp:8-19). */
+uint64_t orc_phase_key(uint64_t seed, uint64_t g) {
+    return orc_splitmix64(orc_splitmix64(seed) ^ g);
+}
+
+/* Random words of the block whose top-left cell is (x0, y0); x0, y0 >= -1. */
+void orc_block_hash(uint64_t key, int x0, int y0, uint32_t *r, uint32_t *s) {
+    uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+    uint32_t pos = (uint32_t)(x0 + 1) | ((uint32_t)(y0 + 1) << 16);
+    uint32_t a = orc_lowbias32(pos ^ k0);
+    *r = a;
+    *s = orc_lowbias32(a ^ k1);
+}
+
+/* ----------------------------------------------------------------------- */
+/* Material properties (Material, game_datastruct.hpp:130-168)              */
+/* ----------------------------------------------------------------------- */
+typedef struct props {
+    uint8_t type[16], rank[16], iters[16];
+    int16_t add_temp[16];
+    uint32_t rgba[16];
+} props;
+
+static void build_props(const fsg_config *cfg, props *P) {
+    for (int i = 0; i < 16; i++) {
+        /* ids without a table entry behave like the wall: inert solid */
+        P->type[i] = FSG_SOLID;
+        P->rank[i] = 15;
+        P->iters[i] = 0;
+        P->add_temp[i] = 0;
+        P->rgba[i] = 0;
+    }
+    for (uint32_t i = 0; i < cfg->n_materials && i < FSG_MAX_MATERIALS; i++) {
+        P->type[i] = cfg->materials[i].type;
+        P->rank[i] = cfg->materials[i].rank;
+        P->iters[i] = cfg->materials[i].iterations;
+        P->add_temp[i] = cfg->materials[i].add_temp;
+        P->rgba[i] = cfg->materials[i].rgba;
+    }
+}
+
+static inline int mid(uint64_t c) { return (int)(c & 0xFF); }
+static inline int temp_of(uint64_t c) { return (int)(int16_t)(uint16_t)(c >> 16); }
+
+/* Rank a cell offers to a sinking neighbour: SOLID never yields
+ * (canMoveBelow's "below != SOLID", world.cpp:1206). */
+static inline int pass_rank(const props *P, int m) {
+    return P->type[m] == FSG_SOLID ? 15 : P->rank[m];
+}
+/* Rank with which an active SAND/SOUP cell sinks; 0 = cannot sink.  The
+ * iteration gate is world.cpp:1093-1096. */
+static inline int sink_rank(const props *P, int m, int it) {
+    int t = P->type[m];
+    return ((t == FSG_SAND || t == FSG_SOUP) && it < P->iters[m]) ? P->rank[m] : 0;
+}
+/* Rank with which an active SOUP cell flows sideways; 0 = does not. */
+static inline int flow_rank(const props *P, int m, int it) {
+    return (P->type[m] == FSG_SOUP && it < P->iters[m]) ? P->rank[m] : 0;
+}
+static inline int gas_act(const props *P, int m, int it) {
+    return P->type[m] == FSG_GAS && it < P->iters[m];
+}
+static inline int is_air(const props *P, int m) { return P->type[m] == FSG_AIR; }
+static inline int is_sand(const props *P, int m) { return P->type[m] == FSG_SAND; }
+
+/* New cell of material m (TilesCreate*, game_datastruct.cpp:304-574):
+ * keeps the temperature (world.cpp:1186-1187), takes the material's base
+ * colour, clears the flags. */
+static inline uint64_t transform(const props *P, uint64_t c, int m) {
+    return (uint64_t)(uint32_t)m | (c & 0xFFFF0000ull) | ((uint64_t)P->rgba[m] << 32);
+}
+
+#define SWAP(a, b) do { uint64_t _t = (a); (a) = (b); (b) = _t; } while (0)
+
+/* ----------------------------------------------------------------------- */
+/* The block rule                                                           */
+/* ----------------------------------------------------------------------- */
+/* q[0]=TL (x0,y0)  q[1]=TR (x0+1,y0)  q[2]=BL (x0,y0+1)  q[3]=BR (x0+1,y0+1)
+ * Random bits: r[0:3]/r[4:7] sand vertical skip L/R, r[8]/r[9] MOVED on fall
+ * L/R, r[10:11]/r[12:13] nudge L/R, r[14]/r[15] gas rise skip L/R;
+ * s[0:4]/s[5:9] slide refused L/R, s[10] slide side choice, s[11:19] MOVED
+ * cleared after a slide, s[20:21] slide nudge, s[22] gas diagonal choice,
+ * s[23]/s[24] horizontal swap top/bottom row. */
+static int block_rule(const props *P, const fsg_config *cfg, int it, uint32_t r, uint32_t s,
+                      uint64_t q[4]) {
+    const uint64_t o0 = q[0], o1 = q[1], o2 = q[2], o3 = q[3];
+#define CHANGED() (q[0] != o0 || q[1] != o1 || q[2] != o2 || q[3] != o3)
+
+    /* --- R: reactions ------------------------------------------------- */
+    if (cfg->reactions) {
+        int fired = 0;
+        const int W = cfg->mat_water, L = cfg->mat_lava;
+        /* Water directly above lava: the water becomes steam and every lava
+         * cell of the block becomes stone (world.cpp:1519-1537: water ->
+         * steam, lava below and lava in x-1..x+1, y..y+2 -> obsidian; stone
+         * plays obsidian's role and the neighbourhood is the block). */
+        int wl = mid(q[0]) == W && mid(q[2]) == L;
+        int wr = mid(q[1]) == W && mid(q[3]) == L;
+        if (wl || wr) {
+            if (wl) q[0] = transform(P, q[0], cfg->mat_steam);
+            if (wr) q[1] = transform(P, q[1], cfg->mat_steam);
+            for (int i = 0; i < 4; i++)
+                if (mid(q[i]) == L) q[i] = transform(P, q[i], cfg->mat_stone);
+            fired = 1;
+        }
+        /* Wood above the burn temperature becomes steam and trades places
+         * with the first air cell of the block, leaving air where the wood
+         * was (REACT_TEMPERATURE_ABOVE form, world.cpp:1193-1200). */
+        for (int i = 0; i < 4; i++) {
+            if (mid(q[i]) == cfg->mat_wood && temp_of(q[i]) > cfg->wood_burn_temp) {
+                q[i] = transform(P, q[i], cfg->mat_steam);
+                for (int j = 0; j < 4; j++)
+                    if (is_air(P, mid(q[j]))) { SWAP(q[i], q[j]); break; }
+                fired = 1;
+            }
+        }
+        if (fired) return CHANGED(); /* reacted cells do not also move */
+    }
+
+    /* --- V: vertical moves, left column then right -------------------- */
+    int vm = 0;
+    for (int col = 0; col < 2; col++) {
+        const int o = 1 - col;
+        uint64_t t = q[col], u = q[col + 2];
+        const int mt = mid(t), mu = mid(u);
+        const int sr = sink_rank(P, mt, it);
+        if (pass_rank(P, mu) < sr) {
+            /* SAND/SOUP falls or sinks through a lighter non-solid cell
+             * (canMoveBelow, world.cpp:1206; swap, 1227-1239). */
+            const int sand = is_sand(P, mt);
+            /* Sand with a free diagonal skips the straight fall 1/16 of the
+             * time (rand()%20 in world.cpp:1217). */
+            if (sand && pass_rank(P, mid(q[o + 2])) < sr && ((r >> (4 * col)) & 15u) == 0)
+                continue;
+            /* The faller is marked moving with p=1/2 (world.cpp:1231-1233). */
+            if (sand && ((r >> (8 + col)) & 1u)) t |= MOVED;
+            q[col] = u;
+            q[col + 2] = t;
+            vm = 1;
+            /* A SAND cell beside the landing spot is nudged into motion with
+             * p=1/4 (world.cpp:1242-1266). */
+            if (sand && ((r >> (10 + 2 * col)) & 3u) == 0 && is_sand(P, mid(q[o + 2])))
+                q[o + 2] |= MOVED;
+        } else if (gas_act(P, mu, it) && is_air(P, mt)) {
+            /* GAS rises into air, skipping with p=1/2 when the other upper
+             * cell is air too (world.cpp:1569-1585). */
+            if (is_air(P, mid(q[o])) && ((r >> (14 + col)) & 1u)) continue;
+            q[col] = u;
+            q[col + 2] = t;
+            vm = 1;
+        }
+    }
+    if (vm) return 1;
+
+    /* --- D: diagonal moves ------------------------------------------- */
+    {
+        const int ma = mid(q[0]), mb = mid(q[1]);
+        int cl = pass_rank(P, mid(q[3])) < sink_rank(P, ma, it); /* TL -> BR */
+        int cr = pass_rank(P, mid(q[2])) < sink_rank(P, mb, it); /* TR -> BL */
+        /* Sand friction (world.cpp:1612-1700): a sand cell slides only while
+         * MOVED is set, and refuses with p=1/32 (shouldMove, rand()%(2*slip)
+         * at 1656), which also stops it (moved = false, 1695-1700). */
+        if (cl && is_sand(P, ma)) {
+            if (!(q[0] & MOVED)) cl = 0;
+            else if ((s & 31u) == 0) { cl = 0; q[0] &= ~MOVED; }
+        }
+        if (cr && is_sand(P, mb)) {
+            if (!(q[1] & MOVED)) cr = 0;
+            else if (((s >> 5) & 31u) == 0) { cr = 0; q[1] &= ~MOVED; }
+        }
+        if (cl && cr) { if ((s >> 10) & 1u) cl = 0; else cr = 0; }
+        if (cl) {
+            uint64_t a2 = q[0];
+            const int sand = is_sand(P, ma);
+            /* moved cleared with p=1/512 after a slide (1/(20*slip), 1688). */
+            if (sand && ((s >> 11) & 511u) == 0) a2 &= ~MOVED;
+            /* the cell below the slider is nudged with p=1/4 (1658-1671). */
+            if (sand && is_sand(P, mid(q[2])) && ((s >> 20) & 3u) == 0) q[2] |= MOVED;
+            /* Side cell air: the displaced diagonal cell moves up beside and
+             * the slider's old place becomes air; else a plain swap
+             * (world.cpp:1676-1686). */
+            if (is_air(P, mid(q[1]))) {
+                uint64_t side = q[1];
+                q[1] = q[3];
+                q[3] = a2;
+                q[0] = side;
+            } else {
+                q[0] = q[3];
+                q[3] = a2;
+            }
+            return 1;
+        }
+        if (cr) {
+            uint64_t b2 = q[1];
+            const int sand = is_sand(P, mb);
+            if (sand && ((s >> 11) & 511u) == 0) b2 &= ~MOVED;
+            if (sand && is_sand(P, mid(q[3])) && ((s >> 20) & 3u) == 0) q[3] |= MOVED;
+            if (is_air(P, mid(q[0]))) {
+                uint64_t side = q[0];
+                q[0] = q[2];
+                q[2] = b2;
+                q[1] = side;
+            } else {
+                q[1] = q[2];
+                q[2] = b2;
+            }
+            return 1;
+        }
+        /* GAS rises diagonally into air (world.cpp:1799-1819). */
+        int gl = gas_act(P, mid(q[2]), it) && is_air(P, mid(q[1])); /* BL -> TR */
+        int gr = gas_act(P, mid(q[3]), it) && is_air(P, mid(q[0])); /* BR -> TL */
+        if (gl && gr) { if ((s >> 22) & 1u) gl = 0; else gr = 0; }
+        if (gl) { SWAP(q[2], q[1]); return 1; }
+        if (gr) { SWAP(q[3], q[0]); return 1; }
+    }
+
+    /* --- F: a sand cell with no move left in the block stops -----------
+     * (stoppedByFriction || !(canMoveBelowL || canMoveBelowR) -> moved =
+     * false, world.cpp:1646-1652). */
+    for (int col = 0; col < 2; col++) {
+        const int o = 1 - col;
+        const int mt = mid(q[col]);
+        const int sr = sink_rank(P, mt, it);
+        if (is_sand(P, mt) && sr > 0 && !(pass_rank(P, mid(q[col + 2])) < sr) &&
+            !(pass_rank(P, mid(q[o + 2])) < sr))
+            q[col] &= ~MOVED;
+    }
+
+    /* --- H: horizontal moves, top row then bottom row -----------------
+     * SOUP flows sideways into a lighter non-solid cell (the discrete SOUP
+     * rule, world.cpp:1839-1861) and GAS spreads into air (1862-1882), each
+     * with p=1/2 so that the side is unbiased. */
+    for (int row = 0; row < 2; row++) {
+        const int i = 2 * row, j = i + 1;
+        const int ml = mid(q[i]), mr = mid(q[j]);
+        const int f = pass_rank(P, mr) < flow_rank(P, ml, it) ||
+                      pass_rank(P, ml) < flow_rank(P, mr, it) ||
+                      (gas_act(P, ml, it) && is_air(P, mr)) ||
+                      (gas_act(P, mr, it) && is_air(P, ml));
+        if (f && ((s >> (23 + row)) & 1u)) SWAP(q[i], q[j]);
+    }
+    return CHANGED();
+#undef CHANGED
+}
+
+int orc_block_rule(const fsg_config *cfg, int it, uint32_t r, uint32_t s, uint64_t q[4]) {
+    props P;
+    build_props(cfg, &P);
+    return block_rule(&P, cfg, it, r, s, q);
+}
+
+/* ----------------------------------------------------------------------- */
+/* Tick                                                                     */
+/* ----------------------------------------------------------------------- */
+/* Block offset of global phase g: (0,0), (1,1), (1,0), (0,1) repeating, so
+ * the row offset alternates every phase. */
+static inline void phase_offset(uint64_t g, int *ox, int *oy) {
+    static const int OX[4] = {0, 1, 1, 0}, OY[4] = {0, 1, 0, 1};
+    *ox = OX[g & 3];
+    *oy = OY[g & 3];
+}
+
+static const uint64_t WALL_CELL = (uint64_t)WALL_ID;
+
+static inline uint64_t cell_at(const uint64_t *c, int W, int H, int x, int y) {
+    if (x < 0 || y < 0 || x >= W || y >= H) return WALL_CELL; /* getTile OOB, world.cpp:1000 */
+    return c[(size_t)y * W + x];
+}
+
+static void run_phase(uint64_t *cells, int W, int H, const props *P, const fsg_config *cfg,
+                      uint64_t seed, uint64_t tick, int it) {
+    const uint64_t g = tick * (uint64_t)cfg->cell_iter + (uint64_t)it;
+    const uint64_t key = orc_phase_key(seed, g);
+    int ox, oy;
+    phase_offset(g, &ox, &oy);
+    for (int y0 = oy ? -1 : 0; y0 < H; y0 += 2) {
+        for (int x0 = ox ? -1 : 0; x0 < W; x0 += 2) {
+            uint64_t q[4] = {cell_at(cells, W, H, x0, y0), cell_at(cells, W, H, x0 + 1, y0),
+                             cell_at(cells, W, H, x0, y0 + 1),
+                             cell_at(cells, W, H, x0 + 1, y0 + 1)};
+            uint32_t r, s;
+            orc_block_hash(key, x0, y0, &r, &s);
+            if (!block_rule(P, cfg, it, r, s, q)) continue;
+            /* setTile semantics: out-of-world writes are dropped
+             * (world.cpp:1004-1008); walls never change anyway. */
+            const int xs[4] = {x0, x0 + 1, x0, x0 + 1}, ys[4] = {y0, y0, y0 + 1, y0 + 1};
+            for (int k = 0; k < 4; k++)
+                if (xs[k] >= 0 && ys[k] >= 0 && xs[k] < W && ys[k] < H)
+                    cells[(size_t)ys[k] * W + xs[k]] = q[k];
+        }
+    }
+}
+
+static void temperature_step(uint64_t *cells, int W, int H, const props *P) {
+    /* Integer 5-point diffusion with coefficient 1/8, floor rounding
+     * (arithmetic shift); an out-of-world neighbour counts as the cell
+     * itself.  Replaces world::tickTemperature's float 9-point mean
+     * (world.cpp:1950-2004); add_temp is Material::addTemp (1988). */
+    int16_t *nt = (int16_t *)malloc((size_t)W * H * sizeof(int16_t));
+    for (int y = 0; y < H; y++) {
+        for (int x = 0; x < W; x++) {
+            const uint64_t c = cells[(size_t)y * W + x];
+            const int T = temp_of(c);
+            const int tu = y > 0 ? temp_of(cells[(size_t)(y - 1) * W + x]) : T;
+            const int td = y < H - 1 ? temp_of(cells[(size_t)(y + 1) * W + x]) : T;
+            const int tl = x > 0 ? temp_of(cells[(size_t)y * W + x - 1]) : T;
+            const int tr = x < W - 1 ? temp_of(cells[(size_t)y * W + x + 1]) : T;
+            int v = T + ((tu + td + tl + tr - 4 * T) >> 3) + P->add_temp[mid(c)];
+            if (v > 32767) v = 32767;
+            if (v < -32768) v = -32768;
+            nt[(size_t)y * W + x] = (int16_t)v;
+        }
+    }
+    for (size_t i = 0; i < (size_t)W * H; i++)
+        cells[i] = (cells[i] & 0xFFFFFFFF0000FFFFull) | ((uint64_t)(uint16_t)nt[i] << 16);
+    free(nt);
+}
+
+static int temp_runs(const fsg_config *cfg, uint64_t tick) {
+    return cfg->temp_every > 0 && (tick % cfg->temp_every) == cfg->temp_phase;
+}
+
+int orc_temperature(uint64_t *cells, int W, int H, const fsg_config *cfg) {
+    props P;
+    build_props(cfg, &P);
+    temperature_step(cells, W, H, &P);
+    return 0;
+}
+
+int orc_tick(uint64_t *cells, int W, int H, const fsg_config *cfg, uint64_t seed, uint64_t tick) {
+    if (!cells || W <= 0 || H <= 0 || cfg->cell_iter == 0) return FSG_ERR_INVALID;
+    props P;
+    build_props(cfg, &P);
+    for (int it = 0; it < (int)cfg->cell_iter; it++) run_phase(cells, W, H, &P, cfg, seed, tick, it);
+    if (temp_runs(cfg, tick)) temperature_step(cells, W, H, &P);
+    return 0;
+}
+
+int orc_run(uint64_t *cells, int W, int H, const fsg_config *cfg, uint64_t seed, uint64_t tick0,
+            int nticks) {
+    for (int t = 0; t < nticks; t++) {
+        int rc = orc_tick(cells, W, H, cfg, seed, tick0 + (uint64_t)t);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+/* ----------------------------------------------------------------------- */
+/* Dirty-rect predicate                                                     */
+/* ----------------------------------------------------------------------- */
+/* A block can only change if it holds a pair of cells that some rule could
+ * act on (vertical, diagonal or horizontal) or a single cell that can change
+ * by itself (a set MOVED flag, burning wood).  The iteration gate is taken
+ * at it = 0, where every material with iterations > 0 is active. */
+static int pair_v(const props *P, const fsg_config *cfg, uint64_t t, uint64_t u) {
+    const int mt = mid(t), mu = mid(u);
+    return pass_rank(P, mu) < sink_rank(P, mt, 0) || (gas_act(P, mu, 0) && is_air(P, mt)) ||
+           (cfg->reactions && mt == cfg->mat_water && mu == cfg->mat_lava);
+}
+static int pair_d(const props *P, uint64_t t, uint64_t u) {
+    const int mt = mid(t), mu = mid(u);
+    return pass_rank(P, mu) < sink_rank(P, mt, 0) || (gas_act(P, mu, 0) && is_air(P, mt));
+}
+static int pair_h(const props *P, uint64_t l, uint64_t r) {
+    const int ml = mid(l), mr = mid(r);
+    return pass_rank(P, mr) < flow_rank(P, ml, 0) || pass_rank(P, ml) < flow_rank(P, mr, 0) ||
+           (gas_act(P, ml, 0) && is_air(P, mr)) || (gas_act(P, mr, 0) && is_air(P, ml));
+}
+
+int orc_region_unstable(const uint64_t *cells, int W, int H, const fsg_config *cfg, int x0,
+                        int y0, int x1, int y1, int cx0, int cy0, int cx1, int cy1) {
+    props P;
+    build_props(cfg, &P);
+    if (x0 < 0) x0 = 0;
+    if (y0 < 0) y0 = 0;
+    if (x1 > W) x1 = W;
+    if (y1 > H) y1 = H;
+    for (int y = y0; y < y1; y++) {
+        for (int x = x0; x < x1; x++) {
+            const uint64_t c = cells[(size_t)y * W + x];
+            if (c & MOVED) return 1;
+            if (cfg->reactions && mid(c) == cfg->mat_wood && temp_of(c) > cfg->wood_burn_temp)
+                return 1;
+            if (x + 1 < x1 && pair_h(&P, c, cells[(size_t)y * W + x + 1])) return 1;
+            if (y + 1 < y1) {
+                if (pair_v(&P, cfg, c, cells[(size_t)(y + 1) * W + x])) return 1;
+                if (x + 1 < x1 && pair_d(&P, c, cells[(size_t)(y + 1) * W + x + 1])) return 1;
+                if (x - 1 >= x0 && pair_d(&P, c, cells[(size_t)(y + 1) * W + x - 1])) return 1;
+            }
+        }
+    }
+    if (cfg->temp_every > 0) {
+        if (cx0 < 0) cx0 = 0;
+        if (cy0 < 0) cy0 = 0;
+        if (cx1 > W) cx1 = W;
+        if (cy1 > H) cy1 = H;
+        for (int y = cy0; y < cy1; y++) {
+            for (int x = cx0; x < cx1; x++) {
+                const uint64_t c = cells[(size_t)y * W + x];
+                const int T = temp_of(c);
+                const int tu = y > 0 ? temp_of(cells[(size_t)(y - 1) * W + x]) : T;
+                const int td = y < H - 1 ? temp_of(cells[(size_t)(y + 1) * W + x]) : T;
+                const int tl = x > 0 ? temp_of(cells[(size_t)y * W + x - 1]) : T;
+                const int tr = x < W - 1 ? temp_of(cells[(size_t)y * W + x + 1]) : T;
+                if (((tu + td + tl + tr - 4 * T) >> 3) + P.add_temp[mid(c)] != 0) return 1;
+            }
+        }
+    }
+    return 0;
+}
+
+/* ----------------------------------------------------------------------- */
+/* Rigid-body pixels                                                        */
+/* ----------------------------------------------------------------------- */
+/* Pixel (tx, ty) of a body lands on
+ *   wx = floor((tx*c - (ty+1)*s + x) / 2^16),  wy = floor((tx*s + (ty+1)*c + y) / 2^16)
+ * in Q16.16 integer arithmetic (the float rotate of game.cpp:1774-1775,
+ * without its FMA sensitivity). */
+int orc_body_pixel_xy(const fsg_body *b, int tx, int ty, int *wx, int *wy) {
+    const int64_t X = b->pos_x_q16, Y = b->pos_y_q16, C = b->cos_q16, S = b->sin_q16;
+    const int64_t fx = (int64_t)tx * C - (int64_t)(ty + 1) * S + X;
+    const int64_t fy = (int64_t)tx * S + (int64_t)(ty + 1) * C + Y;
+    *wx = (int)(fx >> 16);
+    *wy = (int)(fy >> 16);
+    return 0;
+}
+
+/* Target offsets tried in order (checkDirs, game.cpp:1766, 1907). */
+static const int DIRX[5] = {0, 1, -1, 0, 0};
+static const int DIRY[5] = {0, 0, 0, 1, -1};
+
+static int body_of_pixel(const fsg_body *bodies, int nbodies, size_t p, int *tx, int *ty) {
+    for (int b = 0; b < nbodies; b++) {
+        const size_t n = (size_t)bodies[b].mat_w * (size_t)bodies[b].mat_h;
+        if (p >= bodies[b].pixel_offset && p < bodies[b].pixel_offset + n) {
+            const size_t k = p - bodies[b].pixel_offset;
+            *tx = (int)(k % (size_t)bodies[b].mat_w);
+            *ty = (int)(k / (size_t)bodies[b].mat_w);
+            return b;
+        }
+    }
+    return -1;
+}
+
+/* Stamp (game.cpp:1760-1815) in 5 claim rounds: in round d every unplaced
+ * non-air pixel of an enabled body proposes its d-th offset; a target is
+ * available if no pixel claimed it before and its cell is AIR, SAND or SOUP;
+ * among proposers of one target the lowest pixel index wins.  A SAND/SOUP
+ * occupant is displaced (the reference ejects it as a CellData particle) and
+ * reported in pixel order. */
+int orc_stamp(uint64_t *cells, int W, int H, const fsg_config *cfg, const fsg_body *bodies,
+              int nbodies, const uint64_t *pixels, size_t npixels, uint8_t *placed,
+              uint32_t *n_placed, uint32_t *n_displaced, fsg_cell_edit *displaced_out,
+              uint32_t displaced_cap, uint32_t *hits_sand, uint32_t *hits_soup) {
+    props P;
+    build_props(cfg, &P);
+    uint8_t *claimed = (uint8_t *)calloc((size_t)W * H, 1);
+    uint64_t *disp_cell = (uint64_t *)calloc(npixels ? npixels : 1, sizeof(uint64_t));
+    int32_t *disp_xy = (int32_t *)calloc(npixels ? 2 * npixels : 1, sizeof(int32_t));
+    uint8_t *disp = (uint8_t *)calloc(npixels ? npixels : 1, 1);
+    memset(placed, 0, npixels);
+    if (hits_sand) memset(hits_sand, 0, sizeof(uint32_t) * (size_t)nbodies);
+    if (hits_soup) memset(hits_soup, 0, sizeof(uint32_t) * (size_t)nbodies);
+    uint32_t np = 0, nd = 0;
+    for (int d = 0; d < 5; d++) {
+        for (size_t p = 0; p < npixels; p++) {
+            int tx, ty;
+            const int b = body_of_pixel(bodies, nbodies, p, &tx, &ty);
+            if (b < 0 || !bodies[b].enabled || placed[p]) continue;
+            if (is_air(&P, mid(pixels[p]))) continue;
+            int wx, wy;
+            orc_body_pixel_xy(&bodies[b], tx, ty, &wx, &wy);
+            wx += DIRX[d];
+            wy += DIRY[d];
+            if (wx < 0 || wy < 0 || wx >= W || wy >= H) continue;
+            const size_t ci = (size_t)wy * W + wx;
+            if (claimed[ci]) continue;
+            const int tt = P.type[mid(cells[ci])];
+            if (tt != FSG_AIR && tt != FSG_SAND && tt != FSG_SOUP) continue;
+            claimed[ci] = 1;
+            if (tt != FSG_AIR) {
+                disp[p] = 1;
+                disp_cell[p] = cells[ci];
+                disp_xy[2 * p] = wx;
+                disp_xy[2 * p + 1] = wy;
+                if (tt == FSG_SAND && hits_sand) hits_sand[b]++;
+                if (tt == FSG_SOUP && hits_soup) hits_soup[b]++;
+            }
+            cells[ci] = pixels[p];
+            placed[p] = 1;
+            np++;
+        }
+    }
+    for (size_t p = 0; p < npixels; p++) {
+        if (!disp[p]) continue;
+        if (displaced_out && nd < displaced_cap) {
+            displaced_out[nd].x = disp_xy[2 * p];
+            displaced_out[nd].y = disp_xy[2 * p + 1];
+            memcpy(&displaced_out[nd].cell, &disp_cell[p], 8);
+        }
+        nd++;
+    }
+    if (n_placed) *n_placed = np;
+    if (n_displaced) *n_displaced = nd;
+    free(claimed);
+    free(disp_cell);
+    free(disp_xy);
+    free(disp);
+    return 0;
+}
+
+/* Unstamp (game.cpp:1896-1983) in 5 claim rounds: a pixel takes back the
+ * cell at its d-th offset if that cell has the pixel's material id and was
+ * not claimed before; lowest pixel index wins.  The cell becomes air that
+ * keeps its temperature.  A pixel that found nothing is lost (becomes air)
+ * when the cell under it is air (game.cpp:1958-1964). */
+int orc_unstamp(uint64_t *cells, int W, int H, const fsg_config *cfg, const fsg_body *bodies,
+                int nbodies, uint64_t *pixels, size_t npixels) {
+    props P;
+    build_props(cfg, &P);
+    const uint64_t air_id = cfg->mat_air;
+    uint8_t *claimed = (uint8_t *)calloc((size_t)W * H, 1);
+    uint8_t *found = (uint8_t *)calloc(npixels ? npixels : 1, 1);
+    for (int d = 0; d < 5; d++) {
+        for (size_t p = 0; p < npixels; p++) {
+            int tx, ty;
+            const int b = body_of_pixel(bodies, nbodies, p, &tx, &ty);
+            if (b < 0 || !bodies[b].enabled || found[p]) continue;
+            if (is_air(&P, mid(pixels[p]))) continue;
+            int wx, wy;
+            orc_body_pixel_xy(&bodies[b], tx, ty, &wx, &wy);
+            wx += DIRX[d];
+            wy += DIRY[d];
+            if (wx < 0 || wy < 0 || wx >= W || wy >= H) continue;
+            const size_t ci = (size_t)wy * W + wx;
+            if (claimed[ci]) continue;
+            if (mid(cells[ci]) != mid(pixels[p])) continue;
+            claimed[ci] = 1;
+            pixels[p] = cells[ci];
+            cells[ci] = air_id | (cells[ci] & 0xFFFF0000ull);
+            found[p] = 1;
+        }
+    }
+    for (size_t p = 0; p < npixels; p++) {
+        int tx, ty;
+        const int b = body_of_pixel(bodies, nbodies, p, &tx, &ty);
+        if (b < 0 || !bodies[b].enabled || found[p]) continue;
+        if (is_air(&P, mid(pixels[p]))) continue;
+        int wx, wy;
+        orc_body_pixel_xy(&bodies[b], tx, ty, &wx, &wy);
+        if (wx < 0 || wy < 0 || wx >= W || wy >= H) continue;
+        if (is_air(&P, mid(cells[(size_t)wy * W + wx]))) pixels[p] = air_id;
+    }
+    free(claimed);
+    free(found);
+    return 0;
+}
+
+/* ----------------------------------------------------------------------- */
+/* Census                                                                   */
+/* ----------------------------------------------------------------------- */
+void orc_counts(const uint64_t *cells, size_t n, uint64_t counts[16]) {
+    memset(counts, 0, 16 * sizeof(uint64_t));
+    for (size_t i = 0; i < n; i++) counts[cells[i] & 15]++;
+}
+
+/* Sum over cells of splitmix64(cell + index*GOLDEN): position-sensitive,
+ * and the same in any summation order. */
+uint64_t orc_checksum(const uint64_t *cells, size_t n) {
+    uint64_t h = 0;
+    for (size_t i = 0; i < n; i++) h += orc_splitmix64(cells[i] + (uint64_t)i * GOLDEN);
+    return h;
+}
+
+/* ----------------------------------------------------------------------- */
+/* The benchmark material table                                             */
+/* ----------------------------------------------------------------------- */
+/* Eight materials: the reference's GENERIC_AIR, STONE, GENERIC_SAND, WATER,
+ * STEAM and LAVA (game_datastruct.cpp:72-110, densities 0, solid, 10, 1.5,
+ * -1, 2 turned into ranks) plus oil (a liquid lighter than water) and wood
+ * (a solid that burns), which the reference does not have. */
+static void set_mat(fsg_material *m, const char *name, int type, int rank, int iters, int slip,
+                    uint32_t rgba) {
+    memset(m, 0, sizeof(*m));
+    strncpy(m->name, name, sizeof(m->name) - 1);
+    m->type = (uint8_t)type;
+    m->rank = (uint8_t)rank;
+    m->iterations = (uint8_t)iters;
+    m->slipperyness = (uint8_t)slip;
+    m->rgba = rgba;
+}
+
+void orc_default_config(fsg_config *cfg) {
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->cell_iter = 3;
+    cfg->reactions = 1;
+    cfg->temp_every = 1;
+    cfg->temp_phase = 0;
+    cfg->chunk_skip = 0;
+    cfg->wood_burn_temp = 300 * 16;
+    cfg->mat_air = 0;
+    cfg->mat_stone = 1;
+    cfg->mat_water = 3;
+    cfg->mat_steam = 5;
+    cfg->mat_lava = 6;
+    cfg->mat_wood = 7;
+    cfg->n_materials = 8;
+    set_mat(&cfg->materials[0], "air", FSG_AIR, 1, 0, 0, 0x00000000u);
+    set_mat(&cfg->materials[1], "stone", FSG_SOLID, 14, 0, 0, 0xFF808080u);
+    set_mat(&cfg->materials[2], "sand", FSG_SAND, 5, 2, 20, 0xFF50C8E6u);
+    set_mat(&cfg->materials[3], "water", FSG_SOUP, 3, 6, 0, 0x80E06020u);
+    set_mat(&cfg->materials[4], "oil", FSG_SOUP, 2, 3, 0, 0xE0104050u);
+    set_mat(&cfg->materials[5], "steam", FSG_GAS, 0, 1, 0, 0xFF666666u);
+    set_mat(&cfg->materials[6], "lava", FSG_SOUP, 4, 1, 0, 0xC00069FFu);
+    set_mat(&cfg->materials[7], "wood", FSG_SOLID, 14, 0, 0, 0xFF2D5280u);
+}

@@ -1,0 +1,73 @@
+/*
+ * fsg_oracle.h -- CPU oracle for the deterministic falling-sand tick.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product library (libfsg.so) never links or calls it.
+ *
+ * Two CPU modes live here:
+ *   det   (fsg_oracle.c)  the deterministic phase-schedule + counter-RNG tick
+ *                         that the GPU must match bit-exactly;
+ *   refseq (refseq.c)     a restatement of the reference's sequential,
+ *                         order-dependent world::tick on a thread pool; the
+ *                         timed CPU baseline and the statistical comparator.
+ */
+#ifndef FSG_ORACLE_H
+#define FSG_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+#include "../include/fsg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The benchmark material table, written independently of the product's. */
+void orc_default_config(fsg_config *cfg);
+
+uint64_t orc_splitmix64(uint64_t state);          /* next() of splitmix64 at `state` */
+uint32_t orc_lowbias32(uint32_t x);
+uint32_t orc_lowbias32_inv(uint32_t x);
+uint64_t orc_phase_key(uint64_t seed, uint64_t g); /* key of global phase g          */
+void orc_block_hash(uint64_t key, int x0, int y0, uint32_t *r, uint32_t *s);
+
+/* One Margolus block update.  q[4] = TL, TR, BL, BR (as u64 cells);
+ * returns 1 if any cell changed. */
+int orc_block_rule(const fsg_config *cfg, int it, uint32_t r, uint32_t s, uint64_t q[4]);
+
+/* One det tick (cell_iter phases + temperature if the cadence says so),
+ * in place on a W*H row-major grid of u64 cells. */
+int orc_tick(uint64_t *cells, int W, int H, const fsg_config *cfg, uint64_t seed, uint64_t tick);
+int orc_run(uint64_t *cells, int W, int H, const fsg_config *cfg, uint64_t seed,
+            uint64_t tick0, int nticks);
+/* One temperature step (5-point, 1/8, floor; out-of-world neighbour = self). */
+int orc_temperature(uint64_t *cells, int W, int H, const fsg_config *cfg);
+
+/* Dirty-rect predicate: 1 if any cell of the rectangle [x0,x1)x[y0,y1) or
+ * any pair of cells within it could change under some RNG draw, or any
+ * temperature stencil centred in [cx0,cx1)x[cy0,cy1) is non-zero. */
+int orc_region_unstable(const uint64_t *cells, int W, int H, const fsg_config *cfg,
+                        int x0, int y0, int x1, int y1,
+                        int cx0, int cy0, int cx1, int cy1);
+
+/* Rigid-body stamp / unstamp in deterministic claim rounds. */
+int orc_stamp(uint64_t *cells, int W, int H, const fsg_config *cfg, const fsg_body *bodies,
+              int nbodies, const uint64_t *pixels, size_t npixels, uint8_t *placed,
+              uint32_t *n_placed, uint32_t *n_displaced,
+              fsg_cell_edit *displaced_out, uint32_t displaced_cap,
+              uint32_t *hits_sand, uint32_t *hits_soup);
+int orc_unstamp(uint64_t *cells, int W, int H, const fsg_config *cfg, const fsg_body *bodies,
+                int nbodies, uint64_t *pixels, size_t npixels);
+int orc_body_pixel_xy(const fsg_body *b, int tx, int ty, int *wx, int *wy);
+
+void orc_counts(const uint64_t *cells, size_t n, uint64_t counts[16]);
+uint64_t orc_checksum(const uint64_t *cells, size_t n);
+
+/* refseq: the reference's sequential chunk-checkerboard tick. */
+int refseq_run(uint64_t *cells, int W, int H, const fsg_config *cfg, uint64_t seed,
+               int nticks, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,171 @@
+"""ctypes binding of the CPU oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from falling_sand_engine_amd._abi import Body, CellEdit, Config
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+
+_L = None
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        u64, u32, i32, P = C.c_uint64, C.c_uint32, C.c_int, C.c_void_p
+        cfgp = C.POINTER(Config)
+        sig = {
+            "orc_default_config": (None, [cfgp]),
+            "orc_splitmix64": (u64, [u64]),
+            "orc_lowbias32": (u32, [u32]),
+            "orc_lowbias32_inv": (u32, [u32]),
+            "orc_phase_key": (u64, [u64, u64]),
+            "orc_block_hash": (None, [u64, i32, i32, C.POINTER(u32), C.POINTER(u32)]),
+            "orc_block_rule": (i32, [cfgp, i32, u32, u32, P]),
+            "orc_tick": (i32, [P, i32, i32, cfgp, u64, u64]),
+            "orc_run": (i32, [P, i32, i32, cfgp, u64, u64, i32]),
+            "orc_temperature": (i32, [P, i32, i32, cfgp]),
+            "orc_region_unstable": (i32, [P, i32, i32, cfgp] + [i32] * 8),
+            "orc_stamp": (i32, [P, i32, i32, cfgp, P, i32, P, C.c_size_t, P, C.POINTER(u32),
+                                C.POINTER(u32), P, u32, P, P]),
+            "orc_unstamp": (i32, [P, i32, i32, cfgp, P, i32, P, C.c_size_t]),
+            "orc_body_pixel_xy": (i32, [P, i32, i32, C.POINTER(i32), C.POINTER(i32)]),
+            "orc_counts": (None, [P, C.c_size_t, P]),
+            "orc_checksum": (u64, [P, C.c_size_t]),
+            "refseq_run": (i32, [P, i32, i32, cfgp, u64, i32, i32]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _L = L
+    return _L
+
+
+def default_config(**overrides) -> Config:
+    cfg = Config()
+    lib().orc_default_config(C.byref(cfg))
+    for k, v in overrides.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def _grid(cells):
+    assert cells.dtype == np.uint64 and cells.flags.c_contiguous and cells.ndim == 2
+    return cells.ctypes.data, cells.shape[1], cells.shape[0]
+
+
+def run(cells: np.ndarray, cfg: Config, seed: int, tick0: int, nticks: int) -> np.ndarray:
+    """Det-mode ticks tick0 .. tick0+nticks-1, in place on an (H, W) uint64 grid."""
+    p, W, H = _grid(cells)
+    rc = lib().orc_run(p, W, H, C.byref(cfg), seed, tick0, nticks)
+    assert rc == 0, rc
+    return cells
+
+
+def temperature(cells: np.ndarray, cfg: Config) -> np.ndarray:
+    p, W, H = _grid(cells)
+    lib().orc_temperature(p, W, H, C.byref(cfg))
+    return cells
+
+
+def refseq(cells: np.ndarray, cfg: Config, seed: int, nticks: int, threads: int = 1) -> np.ndarray:
+    p, W, H = _grid(cells)
+    rc = lib().refseq_run(p, W, H, C.byref(cfg), seed, nticks, threads)
+    assert rc == 0, rc
+    return cells
+
+
+def region_unstable(cells, cfg, x0, y0, x1, y1, cx0, cy0, cx1, cy1) -> bool:
+    p, W, H = _grid(cells)
+    return bool(lib().orc_region_unstable(p, W, H, C.byref(cfg), x0, y0, x1, y1, cx0, cy0, cx1, cy1))
+
+
+def block_rule(cfg, it, r, s, q):
+    arr = (C.c_uint64 * 4)(*q)
+    ch = lib().orc_block_rule(C.byref(cfg), it, r, s, arr)
+    return bool(ch), list(arr)
+
+
+def splitmix64(x):
+    return lib().orc_splitmix64(x)
+
+
+def lowbias32(x):
+    return lib().orc_lowbias32(x)
+
+
+def lowbias32_inv(x):
+    return lib().orc_lowbias32_inv(x)
+
+
+def phase_key(seed, g):
+    return lib().orc_phase_key(seed, g)
+
+
+def block_hash(key, x0, y0):
+    r, s = C.c_uint32(), C.c_uint32()
+    lib().orc_block_hash(key, x0, y0, C.byref(r), C.byref(s))
+    return r.value, s.value
+
+
+def counts(cells) -> np.ndarray:
+    out = np.zeros(16, dtype=np.uint64)
+    a = np.ascontiguousarray(cells, dtype=np.uint64)
+    lib().orc_counts(a.ctypes.data, a.size, out.ctypes.data)
+    return out
+
+
+def checksum(cells) -> int:
+    a = np.ascontiguousarray(cells, dtype=np.uint64)
+    return lib().orc_checksum(a.ctypes.data, a.size)
+
+
+def body_pixel_xy(body: Body, tx, ty):
+    x, y = C.c_int(), C.c_int()
+    lib().orc_body_pixel_xy(C.byref(body), tx, ty, C.byref(x), C.byref(y))
+    return x.value, y.value
+
+
+def stamp(cells, cfg, bodies, pixels, displaced_cap=0):
+    p, W, H = _grid(cells)
+    nb = len(bodies)
+    barr = (Body * max(nb, 1))(*bodies)
+    px = np.ascontiguousarray(pixels, dtype=np.uint64)
+    placed = np.zeros(max(px.size, 1), dtype=np.uint8)
+    npl, ndp = C.c_uint32(), C.c_uint32()
+    disp = (CellEdit * max(displaced_cap, 1))()
+    hs = (C.c_uint32 * max(nb, 1))()
+    hw = (C.c_uint32 * max(nb, 1))()
+    lib().orc_stamp(p, W, H, C.byref(cfg), barr, nb, px.ctypes.data, px.size, placed.ctypes.data,
+                    C.byref(npl), C.byref(ndp), disp, displaced_cap, hs, hw)
+    n = min(ndp.value, displaced_cap)
+    displaced = [(disp[i].x, disp[i].y,
+                  (disp[i].cell.mat & 0xFFFF) | ((disp[i].cell.temp & 0xFFFF) << 16) |
+                  ((disp[i].cell.rgba & 0xFFFFFFFF) << 32)) for i in range(n)]
+    return {"placed": npl.value, "displaced": ndp.value, "displaced_cells": displaced,
+            "hits_sand": list(hs)[:nb], "hits_soup": list(hw)[:nb], "placed_mask": placed[:px.size]}
+
+
+def unstamp(cells, cfg, bodies, pixels):
+    p, W, H = _grid(cells)
+    nb = len(bodies)
+    barr = (Body * max(nb, 1))(*bodies)
+    px = np.array(pixels, dtype=np.uint64, copy=True)
+    lib().orc_unstamp(p, W, H, C.byref(cfg), barr, nb, px.ctypes.data, px.size)
+    return px

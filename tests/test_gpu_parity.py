@@ -1,0 +1,130 @@
+"""GPU tick vs the CPU oracle, bit-exact on every cell (det mode)."""
+import numpy as np
+import pytest
+
+import falling_sand_engine_amd as fsg
+
+pytestmark = pytest.mark.gpu
+
+
+def run_gpu(ctx, cells, cfg, seed, ticks, tick0=0):
+    H, W = cells.shape
+    with fsg.World(ctx, W, H, seed, cfg) as w:
+        if tick0:
+            w.set_tick(tick0)
+        w.upload(cells)
+        w.tick(ticks)
+        return w.download()
+
+
+def first_diff(a, b):
+    idx = np.argwhere(a != b)
+    if idx.size == 0:
+        return None
+    y, x = idx[0]
+    return f"{len(idx)} cells differ; first at (x={x}, y={y}): gpu={int(a[y, x]):#x} cpu={int(b[y, x]):#x}"
+
+
+@pytest.mark.parametrize("W,H", [(8, 8), (64, 64), (70, 50), (130, 66), (128, 192)])
+@pytest.mark.parametrize("seed", [0, 1, 7])
+def test_small_worlds_random_mix(ctx, orc, W, H, seed):
+    rng = np.random.default_rng(seed * 1000 + W * 7 + H)
+    mats = rng.choice([0, 1, 2, 3, 4, 5, 6, 7], size=(H, W), p=[.35, .1, .2, .12, .08, .07, .05, .03])
+    temps = rng.integers(0, 700 * 16, size=(H, W))
+    moved = rng.integers(0, 2, size=(H, W)) * (mats == 2)
+    cells = (mats.astype(np.uint64) | (moved.astype(np.uint64) << np.uint64(8)) |
+             ((temps.astype(np.uint64) & np.uint64(0xFFFF)) << np.uint64(16)) |
+             (rng.integers(0, 2**32, size=(H, W), dtype=np.uint64) << np.uint64(32)))
+    cfg = orc.default_config()
+    ref = orc.run(cells.copy(), cfg, seed, 0, 25)
+    got = run_gpu(ctx, cells, cfg, seed, 25)
+    assert first_diff(got, ref) is None, first_diff(got, ref)
+
+
+@pytest.mark.parametrize("kind,W,H,seed,ticks", [(1, 512, 512, 1, 1000), (2, 256, 256, 2, 200),
+                                                 (3, 512, 256, 3, 100)])
+def test_generated_worlds(ctx, orc, kind, W, H, seed, ticks):
+    cells = fsg.generate(kind, W, H, seed)
+    cfg = orc.default_config(reactions=0 if kind == 3 else 1)
+    ref = orc.run(cells.copy(), cfg, seed, 0, ticks)
+    got = run_gpu(ctx, cells, cfg, seed, ticks)
+    assert first_diff(got, ref) is None, first_diff(got, ref)
+    # sand/stone are never created or destroyed without reactions involving them
+    assert np.count_nonzero(fsg.cell_mat(got) == 2) == np.count_nonzero(fsg.cell_mat(cells) == 2)
+
+
+def test_resume_at_tick(ctx, orc):
+    cells = fsg.generate(2, 192, 128, 5)
+    cfg = orc.default_config(temp_every=4, temp_phase=2)
+    ref = orc.run(cells.copy(), cfg, 5, 37, 23)
+    got = run_gpu(ctx, cells, cfg, 5, 23, tick0=37)
+    assert first_diff(got, ref) is None, first_diff(got, ref)
+
+
+def test_cell_iter_1_and_2(ctx, orc):
+    cells = fsg.generate(2, 128, 128, 9)
+    for ci in (1, 2):
+        cfg = orc.default_config(cell_iter=ci)
+        ref = orc.run(cells.copy(), cfg, 9, 0, 40)
+        got = run_gpu(ctx, cells, cfg, 9, 40)
+        assert first_diff(got, ref) is None, (ci, first_diff(got, ref))
+
+
+@pytest.mark.parametrize("kind,W,H,seed,ticks", [(1, 384, 320, 11, 600), (4, 640, 384, 12, 500)])
+def test_chunk_skip_is_exact(ctx, orc, kind, W, H, seed, ticks):
+    cells = fsg.generate(kind, W, H, seed)
+    cfg_on = orc.default_config(chunk_skip=1)
+    cfg_off = orc.default_config(chunk_skip=0)
+    ref = orc.run(cells.copy(), cfg_off, seed, 0, ticks)
+    with fsg.World(ctx, W, H, seed, cfg_on) as w:
+        w.upload(cells)
+        w.tick(ticks)
+        got = w.download()
+        st = w.stats()
+    assert first_diff(got, ref) is None, first_diff(got, ref)
+    print(f"processed {st.tiles_processed} of {st.tiles_offered} chunk updates")
+    assert st.tiles_processed < st.tiles_offered  # something was actually skipped
+
+
+def test_tick_temperature_standalone(ctx, orc):
+    cells = fsg.generate(2, 200, 100, 4)
+    cfg = orc.default_config(temp_every=0)
+    ref = orc.run(cells.copy(), cfg, 4, 0, 3)
+    orc.temperature(ref, cfg)
+    ref = orc.run(ref, cfg, 4, 3, 2)
+    with fsg.World(ctx, 200, 100, 4, cfg) as w:
+        w.upload(cells)
+        w.tick(3)
+        w.tickTemperature()
+        w.tick(2)
+        got = w.download()
+    assert first_diff(got, ref) is None, first_diff(got, ref)
+
+
+def test_set_tile_and_get_tile(ctx, orc):
+    cells = fsg.generate(1, 128, 128, 3)
+    cfg = orc.default_config()
+    ref = cells.copy()
+    with fsg.World(ctx, 128, 128, 3, cfg) as w:
+        w.upload(cells)
+        w.setTile(5, 5, fsg.make_cell(2, 320, 0xFF00FFFF))
+        w.setTile(5, 5, fsg.make_cell(3, 320, 0x80E06020))   # last edit wins
+        w.setTile(-1, 3, fsg.make_cell(2))                    # ignored, like setTile
+        w.tick(10)
+        got = w.download()
+        assert w.getTile(-5, 0) == 15
+    ref[5, 5] = fsg.make_cell(3, 320, 0x80E06020)
+    orc.run(ref, cfg, 3, 0, 10)
+    assert first_diff(got, ref) is None, first_diff(got, ref)
+
+
+def test_stats_match_oracle(ctx, orc):
+    cells = fsg.generate(2, 320, 192, 8)
+    cfg = orc.default_config()
+    ref = orc.run(cells.copy(), cfg, 8, 0, 30)
+    with fsg.World(ctx, 320, 192, 8, cfg) as w:
+        w.upload(cells)
+        w.tick(30)
+        st = w.stats()
+    assert list(st.count) == [int(v) for v in orc.counts(ref)]
+    assert st.checksum == orc.checksum(ref)
