@@ -74,6 +74,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--side", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", default="all", choices=["all", "c3", "copy"],
+                    help="all: the JSON line; c3: only the timed tick loop (for profiling); "
+                         "copy: the HBM copy calibration kernel")
+    ap.add_argument("--copy-bytes", type=int, default=2 << 30)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 "
@@ -82,6 +86,17 @@ def main():
     if int(os.environ.get("RANK", "0")) != 0:
         return
     import falling_sand_engine_amd as fsg
+
+    if args.mode == "copy":
+        with fsg.Context(0) as ctx:
+            ms = ctx.copy_timed(args.copy_bytes, max(1, args.steps))
+        gbs = 2 * args.copy_bytes / (ms / 1e3) / 1e9
+        print(json.dumps({"mode": "copy", "bytes": args.copy_bytes, "ms": ms, "GBs_read_plus_write": gbs}))
+        return
+    if args.mode == "c3":
+        r = bench_c3(fsg, args.steps, args.warmup, args.side)
+        print(json.dumps({"mode": "c3", "steps": args.steps, "dev_ms": r["dev_ms"], "wall_s": r["wall_s"]}))
+        return
 
     log(f"[bench] C3 {args.side}^2 warmup={args.warmup} steps={args.steps}")
     r = bench_c3(fsg, args.steps, args.warmup, args.side)

@@ -74,6 +74,12 @@ class Context:
     def inject_alloc_failures(self, n: int):
         check(lib().fsg_ctx_inject_alloc_failures(self.handle, n))
 
+    def copy_timed(self, nbytes: int, reps: int = 10) -> float:
+        """Mean ms of a plain device copy of nbytes (HBM calibration)."""
+        ms = C.c_double()
+        check(lib().fsg_ctx_copy_timed(self.handle, nbytes, reps, C.byref(ms)), "fsg_ctx_copy_timed")
+        return ms.value
+
     def close(self):
         if self.handle:
             check(lib().fsg_ctx_destroy(self.handle), "fsg_ctx_destroy")

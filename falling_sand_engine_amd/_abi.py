@@ -90,6 +90,7 @@ SIGNATURES = {
     "fsg_ctx_destroy": (C.c_int, [P]),
     "fsg_ctx_usage": (C.c_int, [P, u64p]),
     "fsg_ctx_inject_alloc_failures": (C.c_int, [P, C.c_int]),
+    "fsg_ctx_copy_timed": (C.c_int, [P, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "fsg_world_create": (C.c_int, [P, C.c_int, C.c_int, C.c_uint64, C.POINTER(Config), C.POINTER(P)]),
     "fsg_world_destroy": (C.c_int, [P]),
     "fsg_world_info": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), u64p]),
@@ -109,6 +110,7 @@ SIGNATURES = {
     "fsg_batch_step_timed": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "fsg_perworld_step_timed": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "fsg_generate": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, P]),
+    "fsg_rule_eval": (C.c_int, [C.POINTER(Config), C.c_int, C.c_uint32, C.c_uint32, P, C.c_int]),
     "fsg_splitmix64": (C.c_uint64, [C.c_uint64]),
     "fsg_phase_key": (C.c_uint64, [C.c_uint64, C.c_uint64]),
 }

@@ -167,6 +167,25 @@ int launch_tick(fsg_world *w) {
 
 } // namespace
 
+extern "C" int fsg_rule_eval(const fsg_config *cfg, int it, uint32_t r, uint32_t s, uint64_t q[4], int form) {
+    if (!cfg || !q || validate_config(*cfg) || it < 0 || it >= (int)cfg->cell_iter) return FSG_ERR_INVALID;
+    const MatTab M = build_mattab(*cfg);
+    if (form == 0) return fsg::block_rule(M, it, r, s, q) ? 1 : 0;
+    if (form != 1) return FSG_ERR_INVALID;
+    auto code = [&](uint64_t t, uint64_t u) { return fsg::pair_code(M, it, fsg::cid(t), fsg::cid(u)); };
+    const uint32_t VL = code(q[0], q[2]), VR = code(q[1], q[3]), DL = code(q[0], q[3]), DR = code(q[1], q[2]);
+    const uint32_t HT = code(q[0], q[1]), HTR = code(q[1], q[0]), HB = code(q[2], q[3]);
+    bool react = ((VL | VR) & fsg::K_REACT) != 0;
+    for (int i = 0; i < 4; i++)
+        react = react || (M.reactions && fsg::cbyte(q[i]) == M.wood && fsg::ctemp(q[i]) > M.burn);
+    if (react) return fsg::block_rule(M, it, r, s, q) ? 1 : 0;
+    const uint32_t movA = (q[0] >> 8) & 1u, movB = (q[1] >> 8) & 1u;
+    if (fsg::block_inert(VL, VR, DL, DR, HT, HB, movA, movB)) return 0;
+    const uint64_t o0 = q[0], o1 = q[1], o2 = q[2], o3 = q[3];
+    fsg::apply_decision(fsg::block_decide(VL, VR, DL, DR, HT, HTR, HB, movA, movB, r, s), q);
+    return (q[0] != o0 || q[1] != o1 || q[2] != o2 || q[3] != o3) ? 1 : 0;
+}
+
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
@@ -218,6 +237,34 @@ extern "C" int fsg_ctx_inject_alloc_failures(fsg_ctx *c, int n) {
     if (!c || n < 0) return FSG_ERR_INVALID;
     c->inject = n;
     return FSG_OK;
+}
+
+extern "C" int fsg_ctx_copy_timed(fsg_ctx *c, size_t bytes, int reps, double *ms) {
+    if (!c || bytes < 16 || reps < 1) return FSG_ERR_INVALID;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n16 = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    if (dev_alloc(c, &a, n16 * 16)) return FSG_ERR_NOMEM;
+    if (dev_alloc(c, &b, n16 * 16)) {
+        dev_free(a);
+        return FSG_ERR_NOMEM;
+    }
+    int rc = FSG_OK;
+    const unsigned grid = 256 * 16;  // 16 workgroups per CU, grid-stride
+    (void)hipMemsetAsync(a, 1, n16 * 16, c->stream);
+    fsg::k_copy<<<grid, 256, 0, c->stream>>>((const uint4 *)a, (uint4 *)b, n16);  // warm-up
+    if (hipEventRecord(c->e0, c->stream) != hipSuccess) rc = FSG_ERR_DEVICE;
+    for (int r = 0; r < reps && !rc; r++)
+        fsg::k_copy<<<grid, 256, 0, c->stream>>>((const uint4 *)a, (uint4 *)b, n16);
+    float f = 0.f;
+    if (!rc && (hipEventRecord(c->e1, c->stream) != hipSuccess || hipEventSynchronize(c->e1) != hipSuccess ||
+                hipEventElapsedTime(&f, c->e0, c->e1) != hipSuccess))
+        rc = FSG_ERR_DEVICE;
+    (void)hipStreamSynchronize(c->stream);
+    dev_free(a);
+    dev_free(b);
+    if (!rc && ms) *ms = f / reps;
+    return rc;
 }
 
 // ---------------------------------------------------------------------------

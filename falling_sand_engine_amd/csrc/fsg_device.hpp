@@ -56,7 +56,7 @@ __host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     return z ^ (z >> 31);
 }
 
-__device__ __forceinline__ uint32_t lowbias32(uint32_t x) {
+__host__ __device__ __forceinline__ uint32_t lowbias32(uint32_t x) {
     x ^= x >> 16;
     x *= 0x7feb352du;
     x ^= x >> 15;
@@ -65,19 +65,19 @@ __device__ __forceinline__ uint32_t lowbias32(uint32_t x) {
     return x;
 }
 
-__device__ __forceinline__ int nib(uint64_t tab, uint32_t m) { return (int)((tab >> (m << 2)) & 15u); }
-__device__ __forceinline__ bool bit(uint32_t mask, uint32_t m) { return (mask >> m) & 1u; }
-__device__ __forceinline__ uint32_t cid(uint64_t c) { return (uint32_t)c & 15u; }
-__device__ __forceinline__ uint32_t cbyte(uint64_t c) { return (uint32_t)c & 0xFFu; }
-__device__ __forceinline__ int ctemp(uint64_t c) { return (int)(int16_t)(uint16_t)(c >> 16); }
+__host__ __device__ __forceinline__ int nib(uint64_t tab, uint32_t m) { return (int)((tab >> (m << 2)) & 15u); }
+__host__ __device__ __forceinline__ bool bit(uint32_t mask, uint32_t m) { return (mask >> m) & 1u; }
+__host__ __device__ __forceinline__ uint32_t cid(uint64_t c) { return (uint32_t)c & 15u; }
+__host__ __device__ __forceinline__ uint32_t cbyte(uint64_t c) { return (uint32_t)c & 0xFFu; }
+__host__ __device__ __forceinline__ int ctemp(uint64_t c) { return (int)(int16_t)(uint16_t)(c >> 16); }
 
 // Reaction product: material m, same temperature, base colour, flags clear.
-__device__ __forceinline__ uint64_t xform(const MatTab &M, uint64_t c, uint32_t m) {
+__host__ __device__ __forceinline__ uint64_t xform(const MatTab &M, uint64_t c, uint32_t m) {
     return (uint64_t)m | (c & 0xFFFF0000ull) | ((uint64_t)M.rgba[m & 15] << 32);
 }
 
 // Block random words for the block whose top-left world cell is (x0, y0).
-__device__ __forceinline__ void block_hash(uint64_t key, int x0, int y0, uint32_t &r, uint32_t &s) {
+__host__ __device__ __forceinline__ void block_hash(uint64_t key, int x0, int y0, uint32_t &r, uint32_t &s) {
     const uint32_t pos = (uint32_t)(x0 + 1) | ((uint32_t)(y0 + 1) << 16);
     r = lowbias32(pos ^ (uint32_t)key);
     s = lowbias32(r ^ (uint32_t)(key >> 32));
@@ -86,7 +86,7 @@ __device__ __forceinline__ void block_hash(uint64_t key, int x0, int y0, uint32_
 // One Margolus block, q = {TL, TR, BL, BR}.  Returns true if a cell changed.
 // Stages: R reactions, V vertical, D diagonal, F friction stop, H horizontal;
 // each stage restates one group of reference rules (world.cpp:1148-1889).
-__device__ __forceinline__ bool block_rule(const MatTab &M, int it, uint32_t r, uint32_t s,
+__host__ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uint32_t r, uint32_t s,
                                            uint64_t q[4]) {
     const uint64_t o0 = q[0], o1 = q[1], o2 = q[2], o3 = q[3];
     const uint64_t pass = M.pass, sinkt = M.sink[it], flowt = M.flow[it];
@@ -248,6 +248,134 @@ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uint32_t r, 
         }
     }
     return q[0] != o0 || q[1] != o1 || q[2] != o2 || q[3] != o3;
+}
+
+// ---------------------------------------------------------------------------
+// Table-driven form of block_rule for the hot loop.
+//
+// Every decision of block_rule is a relation between two cells of the block
+// (or a property of one cell).  pair_code(t, u) packs those relations for an
+// ordered pair of material ids at iteration `it` into 9 bits; a workgroup
+// builds the 3 x 256-entry table in LDS once, and a block then needs seven
+// table reads instead of dozens of nibble lookups.  The outcome is expressed
+// as an XOR mask on the identity permutation of the block (2 bits per
+// destination) plus per-cell MOVED set/clear masks, applied once, so the
+// divergent stages only combine lane masks.  block_rule (above) stays the
+// readable statement of the same rule and handles reacting blocks.
+// ---------------------------------------------------------------------------
+constexpr uint32_t K_SINK = 1u << 0;     // t sinks into u
+constexpr uint32_t K_GRISE = 1u << 1;    // u is active gas and t is air
+constexpr uint32_t K_HSWAP = 1u << 2;    // the horizontal pair (t, u) may swap
+constexpr uint32_t K_REACT = 1u << 3;    // t water above u lava, reactions on
+constexpr uint32_t K_TSAND = 1u << 4;
+constexpr uint32_t K_USAND = 1u << 5;
+constexpr uint32_t K_TAIR = 1u << 6;
+constexpr uint32_t K_UAIR = 1u << 7;
+constexpr uint32_t K_TSANDACT = 1u << 8; // t is sand active in this iteration
+
+__host__ __device__ __forceinline__ uint32_t pair_code(const MatTab &M, int it, uint32_t t, uint32_t u) {
+    const int pt = nib(M.pass, t), pu = nib(M.pass, u);
+    const int st = nib(M.sink[it], t);
+    const int ft = nib(M.flow[it], t), fu = nib(M.flow[it], u);
+    const bool gt = bit(M.gas[it], t), gu = bit(M.gas[it], u);
+    const bool at = bit(M.air, t), au = bit(M.air, u);
+    uint32_t c = 0;
+    if (pu < st) c |= K_SINK;
+    if (gu && at) c |= K_GRISE;
+    if (pu < ft || pt < fu || (gt && au) || (gu && at)) c |= K_HSWAP;
+    if (M.reactions && t == M.water && u == M.lava) c |= K_REACT;
+    if (bit(M.sand, t)) c |= K_TSAND;
+    if (bit(M.sand, u)) c |= K_USAND;
+    if (at) c |= K_TAIR;
+    if (au) c |= K_UAIR;
+    if (bit(M.sand, t) && st > 0) c |= K_TSANDACT;
+    return c;
+}
+
+// Decision of a non-reacting block, as one word:
+//   bits 0-7   scatter mask: cell k moves to position k ^ ((y >> 2k) & 3)
+//   bits 8-11  MOVED set on original cell k (bit 8 + k)
+//   bits 12-15 MOVED cleared on original cell k (bit 12 + k)
+// All intermediate values are 0/1 integers, so the stages combine with plain
+// bit operations instead of divergent branches.  Same result as block_rule
+// for blocks where no reaction fires.
+__host__ __device__ __forceinline__ uint32_t block_decide(uint32_t VL, uint32_t VR, uint32_t DL, uint32_t DR,
+                                                          uint32_t HT, uint32_t HTR, uint32_t HB,
+                                                          uint32_t movA, uint32_t movB, uint32_t r,
+                                                          uint32_t s) {
+    const uint32_t sL = VL & 1u, sR = VR & 1u, dL = DL & 1u, dR = DR & 1u, htr = HTR & 1u;
+    const uint32_t gL = (VL >> 1) & 1u, gR = (VR >> 1) & 1u, gdl = (DL >> 1) & 1u, gdr = (DR >> 1) & 1u;
+    const uint32_t sandA = (VL >> 4) & 1u, sandB = (VR >> 4) & 1u;
+    const uint32_t sandC = (VL >> 5) & 1u, sandD = (VR >> 5) & 1u;
+    const uint32_t airA = (VL >> 6) & 1u, airB = (DR >> 6) & 1u, airC = (VL >> 7) & 1u;
+    const uint32_t actA = (VL >> 8) & 1u, actB = (VR >> 8) & 1u;
+    const uint32_t hT = (HT >> 2) & 1u, hB = (HB >> 2) & 1u;
+    const uint32_t T1 = (r & 0xFu) == 0, T2 = (r & 0xF0u) == 0, T5 = (r & 0xC00u) == 0, T6 = (r & 0x3000u) == 0;
+    const uint32_t T3 = (r >> 8) & 1u, T4 = (r >> 9) & 1u, T7 = (r >> 14) & 1u, T8 = (r >> 15) & 1u;
+    const uint32_t S1 = (s & 0x1Fu) == 0, S2 = (s & 0x3E0u) == 0, S4 = (s & 0xFF800u) == 0;
+    const uint32_t S5 = (s & 0x300000u) == 0;
+    const uint32_t S3 = (s >> 10) & 1u, S6 = (s >> 22) & 1u, S7 = (s >> 23) & 1u, S8 = (s >> 24) & 1u;
+
+    // V: left column, then the right column seeing the left column's result
+    const uint32_t vLs = sL & ((sandA & dL & T1) ^ 1u);
+    const uint32_t vLg = (sL ^ 1u) & gL & ((airB & T7) ^ 1u);
+    const uint32_t vL = vLs | vLg;
+    const uint32_t sinksBnewC = vL ? htr : dR;           // after a left swap, new BL is old TL
+    const uint32_t vRs = sR & ((sandB & sinksBnewC & T2) ^ 1u);
+    const uint32_t airNewA = vL ? airC : airA;           // after a left swap, new TL is old BL
+    const uint32_t vRg = (sR ^ 1u) & gR & ((airNewA & T8) ^ 1u);
+    const uint32_t vR = vRs | vRg;
+    const uint32_t vm = vL | vR;
+    uint32_t set = (vLs & sandA & T3) | ((vRs & sandB & T4) << 1) | ((vLs & sandA & T5 & sandD) << 3);
+    const uint32_t nudR = vRs & sandB & T6 & (vL ? sandA : sandC);
+    set |= vL ? nudR : (nudR << 2);
+    uint32_t y = (vL * 0x22u) | (vR * 0x88u);
+
+    // D: sinking diagonals with sand friction, then gas diagonals
+    const uint32_t nvm = vm ^ 1u;
+    const uint32_t cl0 = nvm & dL, cr0 = nvm & dR;
+    const uint32_t failA = cl0 & sandA & movA & S1, failB = cr0 & sandB & movB & S2;
+    const uint32_t cl1 = cl0 & ((sandA ^ 1u) | (movA & (failA ^ 1u)));
+    const uint32_t cr1 = cr0 & ((sandB ^ 1u) | (movB & (failB ^ 1u)));
+    const uint32_t both = cl1 & cr1;
+    const uint32_t cl = cl1 & ((both & S3) ^ 1u);
+    const uint32_t cr = cr1 & ((both & (S3 ^ 1u)) ^ 1u);
+    uint32_t clr = failA | (failB << 1) | (cl & sandA & S4) | ((cr & sandB & S4) << 1);
+    set |= ((cl & sandA & sandC & S5) << 2) | ((cr & sandB & sandD & S5) << 3);
+    y |= cl * (0xC3u ^ (airB * 0x44u));  // swap TL/BR, or 3-cycle TL->BR->TR->TL when TR is air
+    y |= cr * (0x3Cu ^ (airA * 0x11u));  // swap TR/BL, or 3-cycle TR->BL->TL->TR when TL is air
+    const uint32_t none = nvm & (cl ^ 1u) & (cr ^ 1u);
+    const uint32_t gl0 = none & gdr, gr0 = none & gdl;
+    const uint32_t gb = gl0 & gr0;
+    const uint32_t gl = gl0 & ((gb & S6) ^ 1u), gr = gr0 & ((gb & (S6 ^ 1u)) ^ 1u);
+    y |= (gl * 0x3Cu) | (gr * 0xC3u);
+    const uint32_t rest = none & (gl ^ 1u) & (gr ^ 1u);
+
+    // F: an active sand top cell with no straight or in-block diagonal move stops
+    clr |= (rest & actA & (sL ^ 1u) & (dL ^ 1u)) | ((rest & actB & (sR ^ 1u) & (dR ^ 1u)) << 1);
+    // H: horizontal swaps, top row and bottom row
+    y ^= (rest & hT & S7) * 0x05u;
+    y ^= (rest & hB & S8) * 0x50u;
+    return y | (set << 8) | (clr << 12);
+}
+
+// True if no rule can change the block for any random draw: no pair
+// relation, no active sand top cell whose MOVED flag could be cleared.
+__host__ __device__ __forceinline__ bool block_inert(uint32_t VL, uint32_t VR, uint32_t DL, uint32_t DR,
+                                                     uint32_t HT, uint32_t HB, uint32_t movA, uint32_t movB) {
+    const uint32_t rel = (VL | VR | DL | DR | HT | HB) & (K_SINK | K_GRISE | K_HSWAP | K_REACT);
+    return rel == 0 && ((((VL >> 8) & movA) | ((VR >> 8) & movB)) & 1u) == 0;
+}
+
+// Host form of the decision's effect on q (the kernel scatters to LDS).
+__host__ __device__ __forceinline__ void apply_decision(uint32_t dec, uint64_t q[4]) {
+    uint64_t v[4];
+    for (int k = 0; k < 4; k++) {
+        v[k] = q[k];
+        if ((dec >> (8 + k)) & 1u) v[k] |= MOVED;
+        if ((dec >> (12 + k)) & 1u) v[k] &= ~MOVED;
+    }
+    for (int k = 0; k < 4; k++) q[k ^ ((dec >> (2 * k)) & 3u)] = v[k];
 }
 
 // Pair tests of the dirty-rect predicate (iteration gate taken at it = 0).

@@ -57,8 +57,66 @@ __device__ bool tile_unstable(const MatTab &M, uint64_t (*L)[LW], bool temp_enab
     return u;
 }
 
+__device__ __forceinline__ bool wood_hot(const MatTab &M, uint64_t c) {
+    return cbyte(c) == M.wood && ctemp(c) > M.burn;
+}
+
+// One Margolus phase over the blocks whose cells the rest of the tick still
+// reads: after this phase the tick needs `need` more rings around the chunk
+// (need = cell_iter - it, so the region shrinks by one ring per phase and the
+// last phase leaves the one ring the temperature stencil reads).
+__device__ __forceinline__ void run_phase(const MatTab &M, uint64_t (*L)[LW], const uint16_t *T, int it,
+                                          uint64_t key, int ox, int oy, int need, int gx0, int gy0,
+                                          bool has_wood) {
+    // first/last block origin (chunk coordinates) with the phase's parity
+    const int x0f = -need - ((need ^ ox) & 1), y0f = -need - ((need ^ oy) & 1);
+    const int x0l = (TW - 1 + need) - (((TW - 1 + need) ^ ox) & 1);
+    const int y0l = (TH - 1 + need) - (((TH - 1 + need) ^ oy) & 1);
+    const int nbx = ((x0l - x0f) >> 1) + 1, nby = ((y0l - y0f) >> 1) + 1;
+    const uint32_t magic = (65536u + (uint32_t)nbx - 1u) / (uint32_t)nbx;  // b / nbx for b < 2^16/nbx
+    const int lx0 = HALO + x0f, ly0 = HALO + y0f;
+    for (int b = threadIdx.x; b < nbx * nby; b += NT) {
+        const int by = (int)(((uint32_t)b * magic) >> 16), bx = b - by * nbx;
+        const int lx = lx0 + 2 * bx, ly = ly0 + 2 * by;
+        uint64_t *p0 = &L[ly][lx];
+        uint64_t q[4] = {p0[0], p0[1], p0[LW], p0[LW + 1]};
+        const uint32_t i0 = (uint32_t)q[0] & 15u, i1 = (uint32_t)q[1] & 15u;
+        const uint32_t i2 = (uint32_t)q[2] & 15u, i3 = (uint32_t)q[3] & 15u;
+        const uint32_t VL = T[(i0 << 4) | i2], VR = T[(i1 << 4) | i3];
+        const uint32_t DL = T[(i0 << 4) | i3], DR = T[(i1 << 4) | i2];
+        const uint32_t HT = T[(i0 << 4) | i1], HB = T[(i2 << 4) | i3];
+        const uint32_t movA = ((uint32_t)q[0] >> 8) & 1u, movB = ((uint32_t)q[1] >> 8) & 1u;
+        bool hot = false;
+        if (has_wood)
+            hot = wood_hot(M, q[0]) || wood_hot(M, q[1]) || wood_hot(M, q[2]) || wood_hot(M, q[3]);
+        if (!hot && block_inert(VL, VR, DL, DR, HT, HB, movA, movB)) continue;
+        uint32_t r, s;
+        block_hash(key, gx0 + lx, gy0 + ly, r, s);
+        if (hot || ((VL | VR) & K_REACT)) {
+            if (block_rule(M, it, r, s, q)) {
+                p0[0] = q[0];
+                p0[1] = q[1];
+                p0[LW] = q[2];
+                p0[LW + 1] = q[3];
+            }
+            continue;
+        }
+        const uint32_t HTR = T[(i1 << 4) | i0];
+        const uint32_t dec = block_decide(VL, VR, DL, DR, HT, HTR, HB, movA, movB, r, s);
+        if (dec == 0) continue;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint64_t v = q[k];
+            if ((dec >> (8 + k)) & 1u) v |= MOVED;
+            if ((dec >> (12 + k)) & 1u) v &= ~MOVED;
+            const uint32_t d = (uint32_t)k ^ ((dec >> (2 * k)) & 3u);
+            p0[(d & 1u) + (d >> 1) * LW] = v;
+        }
+    }
+}
+
 __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, int tile,
-                                            uint64_t (*L)[LW]) {
+                                            uint64_t (*L)[LW], uint16_t (*lut)[256]) {
     const int tid = threadIdx.x;
     const MatTab &M = w.mat;
     const int tx = tile % w.tiles_x, ty = tile / w.tiles_x;
@@ -84,36 +142,30 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         }
     }
 
+    const int ci = (int)M.cell_iter;
+    for (int i = tid; i < ci * 256; i += NT)
+        lut[i >> 8][i & 255] = (uint16_t)pair_code(M, i >> 8, (uint32_t)(i >> 4) & 15u, (uint32_t)i & 15u);
+    bool wood = false;
     for (int i = tid; i < LW * LH; i += NT) {
         const int ly = i / LW, lx = i - ly * LW;
         const int gx = gx0 + lx, gy = gy0 + ly;
-        L[ly][lx] = (gx >= 0 && gy >= 0 && gx < w.W && gy < w.H) ? in[(size_t)gy * w.W + gx] : WALL;
+        const uint64_t c = (gx >= 0 && gy >= 0 && gx < w.W && gy < w.H) ? in[(size_t)gy * w.W + gx] : WALL;
+        L[ly][lx] = c;
+        wood |= cbyte(c) == M.wood;
     }
-    __syncthreads();
+    // burning wood is the one reaction the pair table cannot see
+    const bool has_wood = __syncthreads_or(wood && M.reactions) != 0;
 
     bool pot = true;
     if (skip) pot = __syncthreads_or(tile_unstable(M, L, M.temp_every != 0)) != 0;
 
     if (pot) {
-        const int ci = (int)M.cell_iter;
+        const uint64_t seedmix = splitmix64(w.seed);
         for (int it = 0; it < ci; it++) {
             const uint64_t g = tick * (uint64_t)M.cell_iter + (uint64_t)it;
-            const uint64_t key = splitmix64(splitmix64(w.seed) ^ g);
+            const uint64_t key = splitmix64(seedmix ^ g);
             const int ox = kOX[g & 3], oy = kOY[g & 3];
-            const int nbx = (LW - ox) >> 1, nby = (LH - oy) >> 1;
-            for (int b = tid; b < nbx * nby; b += NT) {
-                const int by = b / nbx, bx = b - by * nbx;
-                const int lx = ox + 2 * bx, ly = oy + 2 * by;
-                uint64_t q[4] = {L[ly][lx], L[ly][lx + 1], L[ly + 1][lx], L[ly + 1][lx + 1]};
-                uint32_t r, s;
-                block_hash(key, gx0 + lx, gy0 + ly, r, s);
-                if (block_rule(M, it, r, s, q)) {
-                    L[ly][lx] = q[0];
-                    L[ly][lx + 1] = q[1];
-                    L[ly + 1][lx] = q[2];
-                    L[ly + 1][lx + 1] = q[3];
-                }
-            }
+            run_phase(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood);
             __syncthreads();
         }
     }
@@ -140,7 +192,8 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
 
 __global__ __launch_bounds__(NT) void k_tick_world(DevWorld w, uint32_t k) {
     __shared__ uint64_t L[LH][LW];
-    tile_update(w, w.tick + k, blockIdx.x, L);
+    __shared__ uint16_t lut[MAX_IT][256];
+    tile_update(w, w.tick + k, blockIdx.x, L, lut);
 }
 
 // Many worlds in one launch: tile_start[i] is the first global chunk index of
@@ -149,6 +202,7 @@ __global__ __launch_bounds__(NT) void k_tick_batch(const DevWorld *__restrict__ 
                                                    const uint32_t *__restrict__ tile_start,
                                                    int nworlds, uint32_t k) {
     __shared__ uint64_t L[LH][LW];
+    __shared__ uint16_t lut[MAX_IT][256];
     const uint32_t gt = blockIdx.x;
     int lo = 0, hi = nworlds - 1;
     while (lo < hi) {  // last world with tile_start <= gt
@@ -157,7 +211,7 @@ __global__ __launch_bounds__(NT) void k_tick_batch(const DevWorld *__restrict__ 
         else hi = mid - 1;
     }
     const DevWorld &w = worlds[lo];
-    tile_update(w, w.tick + k, (int)(gt - tile_start[lo]), L);
+    tile_update(w, w.tick + k, (int)(gt - tile_start[lo]), L, lut);
 }
 
 __global__ void k_advance(DevWorld *worlds, int nworlds, uint32_t ticks) {
@@ -214,6 +268,14 @@ __global__ __launch_bounds__(NT) void k_stats(const uint64_t *__restrict__ cells
 __global__ void k_fill(uint64_t *cells, size_t n, uint64_t v) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) cells[i] = v;
+}
+
+// Streaming copy (16 B per lane per iteration), the calibration kernel for
+// HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                              size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
 }
 
 __global__ void k_set_cells(uint64_t *cells, int W, int H, const int32_t *xy, const uint64_t *vals,

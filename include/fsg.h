@@ -135,6 +135,10 @@ int fsg_ctx_destroy(fsg_ctx *ctx);
 int fsg_ctx_usage(fsg_ctx *ctx, uint64_t *bytes_used);
 /* Test hook: make the next n device allocations of this ctx fail. */
 int fsg_ctx_inject_alloc_failures(fsg_ctx *ctx, int n);
+/* Plain device-to-device copy of `bytes` bytes by a streaming kernel, timed
+ * with HIP events (ms = mean over `reps`).  It has no counterpart in the
+ * reference: it calibrates HBM counters and the achievable copy bandwidth. */
+int fsg_ctx_copy_timed(fsg_ctx *ctx, size_t bytes, int reps, double *ms);
 
 /* ------------------------------------------------------------------------ */
 /* World: replaces ME::world's grid and tick (world.hpp:63-193).             */
@@ -238,6 +242,14 @@ int fsg_perworld_step_timed(fsg_ctx *ctx, fsg_world *const *worlds, int n, int t
 /* scaled to the world's area.                                               */
 /* ------------------------------------------------------------------------ */
 int fsg_generate(int kind, int width, int height, uint64_t seed, fsg_cell *out);
+
+/* Test hook with no counterpart in the reference: evaluates the device
+ * block rule compiled for the host on one block q[4] = TL, TR, BL, BR at
+ * iteration it with random words r, s.  form 0 is the staged rule, form 1
+ * the table-driven form the tick kernel runs (reacting blocks routed to the
+ * staged rule, as in the kernel).  Returns 1 if the block changed, 0 if not,
+ * or a negative status. */
+int fsg_rule_eval(const fsg_config *cfg, int it, uint32_t r, uint32_t s, uint64_t q[4], int form);
 
 /* splitmix64 output for state x (Vigna), and the per-phase RNG key. */
 uint64_t fsg_splitmix64(uint64_t x);
