@@ -146,12 +146,25 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     for (int i = tid; i < ci * 256; i += NT)
         lut[i >> 8][i & 255] = (uint16_t)pair_code(M, i >> 8, (uint32_t)(i >> 4) & 15u, (uint32_t)i & 15u);
     bool wood = false;
-    for (int i = tid; i < LW * LH; i += NT) {
-        const int ly = i / LW, lx = i - ly * LW;
-        const int gx = gx0 + lx, gy = gy0 + ly;
-        const uint64_t c = (gx >= 0 && gy >= 0 && gx < w.W && gy < w.H) ? in[(size_t)gy * w.W + gx] : WALL;
-        L[ly][lx] = c;
-        wood |= cbyte(c) == M.wood;
+    // A chunk whose halo lies inside an even-width world takes the fast
+    // paths: 16-byte loads/stores with no bounds or wall checks.
+    const bool interior = gx0 >= 0 && gy0 >= 0 && gx0 + LW <= w.W && gy0 + LH <= w.H && (w.W & 1) == 0;
+    if (interior) {
+        const uint64_t *__restrict__ src = in + (size_t)gy0 * w.W + gx0;
+        for (int i = tid; i < LH * (LW / 2); i += NT) {
+            const int ly = i / (LW / 2), c2 = i - ly * (LW / 2);
+            const uint4 v = *reinterpret_cast<const uint4 *>(src + (size_t)__umul24(ly, w.W) + 2 * c2);
+            *reinterpret_cast<uint4 *>(&L[ly][2 * c2]) = v;
+            if (M.reactions) wood |= (v.x & 0xFFu) == M.wood || (v.z & 0xFFu) == M.wood;
+        }
+    } else {
+        for (int i = tid; i < LW * LH; i += NT) {
+            const int ly = i / LW, lx = i - ly * LW;
+            const int gx = gx0 + lx, gy = gy0 + ly;
+            const uint64_t c = (gx >= 0 && gy >= 0 && gx < w.W && gy < w.H) ? in[(size_t)gy * w.W + gx] : WALL;
+            L[ly][lx] = c;
+            wood |= cbyte(c) == M.wood;
+        }
     }
     // burning wood is the one reaction the pair table cannot see
     const bool has_wood = __syncthreads_or(wood && M.reactions) != 0;
@@ -172,14 +185,54 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
 
     const bool temp_now = M.temp_every != 0 && (tick % M.temp_every) == M.temp_phase;
     bool changed = false;
-    for (int i = tid; i < TW * TH; i += NT) {
-        const int ly = HALO + i / TW, lx = HALO + (i & (TW - 1));
-        const int gx = gx0 + lx, gy = gy0 + ly;
-        if (gx >= w.W || gy >= w.H) continue;
-        const uint64_t c = temp_now ? temp_step(M, L, ly, lx) : L[ly][lx];
-        const size_t gi = (size_t)gy * w.W + gx;
-        out[gi] = c;
-        if (skip) changed |= c != in[gi];
+    if (interior && !skip) {
+        // thread = 2-column strip x 8 rows; the stencil slides down the strip
+        const int j = tid & 31, rg = tid >> 5;
+        const int lx = HALO + 2 * j, ly0 = HALO + rg * (TH / 8);
+        uint64_t *__restrict__ dst = out + (size_t)(gy0 + ly0) * w.W + gx0 + lx;
+        if (!temp_now) {
+#pragma unroll
+            for (int r = 0; r < TH / 8; r++)
+                *reinterpret_cast<uint4 *>(dst + (size_t)__umul24(r, w.W)) =
+                    *reinterpret_cast<const uint4 *>(&L[ly0 + r][lx]);
+        } else {
+            uint4 up = *reinterpret_cast<const uint4 *>(&L[ly0 - 1][lx]);
+            uint4 cur = *reinterpret_cast<const uint4 *>(&L[ly0][lx]);
+            int tu0 = (int)up.x >> 16, tu1 = (int)up.z >> 16;
+#pragma unroll
+            for (int r = 0; r < TH / 8; r++) {
+                const uint4 dn = *reinterpret_cast<const uint4 *>(&L[ly0 + r + 1][lx]);
+                const int tl = (int)(uint32_t)L[ly0 + r][lx - 1] >> 16;
+                const int tr = (int)(uint32_t)L[ly0 + r][lx + 2] >> 16;
+                const int t0 = (int)cur.x >> 16, t1 = (int)cur.z >> 16;
+                const int td0 = (int)dn.x >> 16, td1 = (int)dn.z >> 16;
+                int v0 = t0 + ((tu0 + td0 + tl + t1 - 4 * t0) >> 3);
+                int v1 = t1 + ((tu1 + td1 + t0 + tr - 4 * t1) >> 3);
+                if (M.any_add_temp) {
+                    v0 += M.add_temp[cur.x & 15u];
+                    v1 += M.add_temp[cur.z & 15u];
+                }
+                v0 = min(max(v0, -32768), 32767);
+                v1 = min(max(v1, -32768), 32767);
+                uint4 o = cur;
+                o.x = (cur.x & 0xFFFFu) | ((uint32_t)v0 << 16);
+                o.z = (cur.z & 0xFFFFu) | ((uint32_t)v1 << 16);
+                *reinterpret_cast<uint4 *>(dst + (size_t)__umul24(r, w.W)) = o;
+                tu0 = t0;
+                tu1 = t1;
+                cur = dn;
+            }
+        }
+    } else {
+        for (int i = tid; i < TW * TH; i += NT) {
+            const int ly = HALO + i / TW, lx = HALO + (i & (TW - 1));
+            const int gx = gx0 + lx, gy = gy0 + ly;
+            if (gx >= w.W || gy >= w.H) continue;
+            const uint64_t c = temp_now ? temp_step(M, L, ly, lx) : L[ly][lx];
+            const size_t gi = (size_t)gy * w.W + gx;
+            out[gi] = c;
+            if (skip) changed |= c != in[gi];
+        }
     }
     if (skip) {
         changed = __syncthreads_or(changed) != 0;
@@ -191,7 +244,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
 }
 
 __global__ __launch_bounds__(NT) void k_tick_world(DevWorld w, uint32_t k) {
-    __shared__ uint64_t L[LH][LW];
+    __shared__ __align__(16) uint64_t L[LH][LW];
     __shared__ uint16_t lut[MAX_IT][256];
     tile_update(w, w.tick + k, blockIdx.x, L, lut);
 }
@@ -201,7 +254,7 @@ __global__ __launch_bounds__(NT) void k_tick_world(DevWorld w, uint32_t k) {
 __global__ __launch_bounds__(NT) void k_tick_batch(const DevWorld *__restrict__ worlds,
                                                    const uint32_t *__restrict__ tile_start,
                                                    int nworlds, uint32_t k) {
-    __shared__ uint64_t L[LH][LW];
+    __shared__ __align__(16) uint64_t L[LH][LW];
     __shared__ uint16_t lut[MAX_IT][256];
     const uint32_t gt = blockIdx.x;
     int lo = 0, hi = nworlds - 1;

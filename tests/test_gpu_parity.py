@@ -25,7 +25,7 @@ def first_diff(a, b):
     return f"{len(idx)} cells differ; first at (x={x}, y={y}): gpu={int(a[y, x]):#x} cpu={int(b[y, x]):#x}"
 
 
-@pytest.mark.parametrize("W,H", [(8, 8), (64, 64), (70, 50), (130, 66), (128, 192)])
+@pytest.mark.parametrize("W,H", [(8, 8), (64, 64), (70, 50), (130, 66), (128, 192), (65, 33), (201, 140)])
 @pytest.mark.parametrize("seed", [0, 1, 7])
 def test_small_worlds_random_mix(ctx, orc, W, H, seed):
     rng = np.random.default_rng(seed * 1000 + W * 7 + H)
