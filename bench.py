@@ -50,7 +50,88 @@ def bench_c3(fsg, steps, warmup, side=16384, seed=3):
     return {"wall_s": wall, "dev_ms": ms_dev, "cells": side * side, "counts": list(st.count)[:8]}
 
 
-def cpu_baseline(threads=None, side_w=4096, side_h=1024, ticks=4):
+def bench_c2(fsg, ticks=500, side=4096, seed=2):
+    """4096^2 basin world with lava and oil, reactions on, temperature every
+    tick, exact dirty-rect skipping on 64x64 chunks; also run without
+    skipping to show the two are bit-identical."""
+    cells = fsg.generate(2, side, side, seed)
+    out = {}
+    with fsg.Context(0) as ctx:
+        for skip in (1, 0):
+            w = fsg.World(ctx, side, side, seed, fsg.default_config(chunk_skip=skip))
+            w.upload(cells)
+            w.sync()
+            w.reset_activity()
+            t0 = time.perf_counter()
+            ms = w.step_timed(ticks)
+            wall = time.perf_counter() - t0
+            st = w.stats()
+            out[skip] = {"wall_s": wall, "dev_ms": ms, "checksum": st.checksum,
+                         "active_fraction": st.tiles_processed / max(1, st.tiles_offered)}
+            w.close()
+    cu = side * side * ticks
+    return {"workload": f"basin world {side}x{side}, lava+oil, reactions on, {ticks} ticks",
+            "gcell_s_nominal": round(cu / out[1]["wall_s"] / 1e9, 2),
+            "gcell_s_active": round(cu * out[1]["active_fraction"] / out[1]["wall_s"] / 1e9, 2),
+            "active_fraction": round(out[1]["active_fraction"], 4),
+            "gcell_s_no_skip": round(cu / out[0]["wall_s"] / 1e9, 2),
+            "skip_speedup": round(out[0]["wall_s"] / out[1]["wall_s"], 3),
+            "bit_exact_vs_no_skip": out[0]["checksum"] == out[1]["checksum"]}
+
+
+def uneven_batch_sizes(n=256, seed=4):
+    """Sides log-uniform in [256, 2048] rounded to 64, aspect ratio uniform
+    in [0.5, 2]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = []
+    for _ in range(n):
+        side = float(np.exp(rng.uniform(np.log(256), np.log(2048))))
+        a = float(rng.uniform(0.5, 2.0))
+        w = max(64, int(round(side * np.sqrt(a) / 64)) * 64)
+        h = max(64, int(round(side / np.sqrt(a) / 64)) * 64)
+        sizes.append((w, h))
+    return sizes
+
+
+def bench_c4(fsg, ticks=256, n=256):
+    """256 worlds of uneven sizes: one launch per tick over all of their
+    chunks (16-tick HIP graphs) vs one launch per world per tick."""
+    sizes = uneven_batch_sizes(n)
+    with fsg.Context(0) as ctx:
+        sets = []
+        for _ in range(2):
+            ws = []
+            for i, (W, H) in enumerate(sizes):
+                w = fsg.World(ctx, W, H, 1000 + i, fsg.default_config())
+                w.upload(fsg.generate(4, W, H, 1000 + i))
+                ws.append(w)
+            sets.append(ws)
+        fsg.batch_step(ctx, sets[0], 16)        # warm-up + graph capture
+        for w in sets[1]:
+            w.tick(16)
+        sets[1][0].sync()
+        t0 = time.perf_counter()
+        ms_b = fsg.batch_step_timed(ctx, sets[0], ticks)
+        wall_b = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ms_p = fsg.perworld_step_timed(ctx, sets[1], ticks)
+        wall_p = time.perf_counter() - t0
+        exact = all(a.stats().checksum == b.stats().checksum for a, b in zip(*sets))
+        for ws in sets:
+            for w in ws:
+                w.close()
+    cells = sum(W * H for W, H in sizes)
+    return {"workload": f"{n} worlds, sides log-uniform [256,2048], aspect [0.5,2], {ticks} ticks",
+            "cells": cells,
+            "batched_gcell_s": round(cells * ticks / wall_b / 1e9, 2),
+            "perworld_gcell_s": round(cells * ticks / wall_p / 1e9, 2),
+            "ratio": round(wall_p / wall_b, 3),
+            "batched_dev_ms": round(ms_b, 2), "perworld_dev_ms": round(ms_p, 2),
+            "bit_exact": exact}
+
+
+def cpu_baseline(threads=None, side_w=8192, side_h=4096, ticks=64):
     """refseq (the reference's sequential chunk-checkerboard tick, restated)
     on a bounded sample of the all-active world."""
     import falling_sand_engine_amd as fsg
@@ -74,6 +155,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--side", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--mode", default="all", choices=["all", "c3", "copy"],
                     help="all: the JSON line; c3: only the timed tick loop (for profiling); "
                          "copy: the HBM copy calibration kernel")
@@ -106,6 +188,18 @@ def main():
     achieved = BYTES_PER_CELL_UPDATE * r["cells"] / kernel_s / 1e9
     log(f"[bench] wall {r['wall_s']:.4f}s  device {r['dev_ms']:.2f}ms  {value:.1f} Gcell/s")
 
+    secondary = {}
+    if not args.no_secondary:
+        log("[bench] C2 dirty-chunk world")
+        secondary["c2_dirty_chunks"] = bench_c2(fsg)
+        log(f"[bench] {secondary['c2_dirty_chunks']}")
+        log("[bench] C4 uneven batch")
+        secondary["c4_batch"] = bench_c4(fsg)
+        log(f"[bench] {secondary['c4_batch']}")
+        with fsg.Context(0) as ctx:
+            cms = ctx.copy_timed(2 << 30, 10)
+        secondary["copy_kernel_GBs"] = round(2 * (2 << 30) / (cms / 1e3) / 1e9, 1)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
@@ -132,6 +226,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_tick_world", "kernel_avg_ms": round(kernel_s * 1e3, 4)},
         "cpu_baseline": cpu,
+        "secondary": secondary,
     }
     print(json.dumps(out), flush=True)
 

@@ -76,9 +76,10 @@ __device__ __forceinline__ void run_phase(const MatTab &M, uint64_t (*L)[LW], co
     const uint32_t magic = (65536u + (uint32_t)nbx - 1u) / (uint32_t)nbx;  // b / nbx for b < 2^16/nbx
     const int lx0 = HALO + x0f, ly0 = HALO + y0f;
     for (int b = threadIdx.x; b < nbx * nby; b += NT) {
-        const int by = (int)(((uint32_t)b * magic) >> 16), bx = b - by * nbx;
+        const int by = (int)(__umul24((uint32_t)b, magic) >> 16);
+        const int bx = b - (int)__umul24((uint32_t)by, (uint32_t)nbx);
         const int lx = lx0 + 2 * bx, ly = ly0 + 2 * by;
-        uint64_t *p0 = &L[ly][lx];
+        uint64_t *p0 = &L[0][0] + __umul24((uint32_t)ly, (uint32_t)LW) + lx;
         uint64_t q[4] = {p0[0], p0[1], p0[LW], p0[LW + 1]};
         const uint32_t i0 = (uint32_t)q[0] & 15u, i1 = (uint32_t)q[1] & 15u;
         const uint32_t i2 = (uint32_t)q[2] & 15u, i3 = (uint32_t)q[3] & 15u;
