@@ -31,19 +31,46 @@ __device__ __forceinline__ uint64_t temp_step(const MatTab &M, uint64_t (*L)[LW]
     return (c & 0xFFFFFFFF0000FFFFull) | ((uint64_t)(uint16_t)v << 16);
 }
 
-// Dirty-rect predicate over the staged tile: can any block of the LDS region
-// change under some random draw, or any temperature stencil of the interior?
-__device__ bool tile_unstable(const MatTab &M, uint64_t (*L)[LW], bool temp_enabled) {
+__device__ __forceinline__ bool wood_hot(const MatTab &M, uint64_t c) {
+    return cbyte(c) == M.wood && ctemp(c) > M.burn;
+}
+
+// Chunk activity flags (one u32 per chunk per tick parity):
+//   F_POT   the chunk may change next tick even if no neighbour changes;
+//   F_ANY   the chunk changed this tick;
+//   F_N..F_SE  it changed within HALO cells of that edge / corner, the band
+//           a neighbour's region (chunk + HALO ring) overlaps.
+constexpr uint32_t F_POT = 1u, F_ANY = 2u;
+constexpr uint32_t F_N = 4u, F_S = 8u, F_W = 16u, F_E = 32u;
+constexpr uint32_t F_NW = 64u, F_NE = 128u, F_SW = 256u, F_SE = 512u;
+
+// Bands touched by a change covering chunk cells [x0, x1] x [y0, y1]
+// (chunk coordinates, already clipped to the chunk).
+__device__ __forceinline__ uint32_t band_bits(int x0, int x1, int y0, int y1) {
+    const bool n = y0 < HALO, so = y1 >= TH - HALO, we = x0 < HALO, ea = x1 >= TW - HALO;
+    return F_ANY | (n ? F_N : 0u) | (so ? F_S : 0u) | (we ? F_W : 0u) | (ea ? F_E : 0u) |
+           ((n && we) ? F_NW : 0u) | ((n && ea) ? F_NE : 0u) | ((so && we) ? F_SW : 0u) |
+           ((so && ea) ? F_SE : 0u);
+}
+
+// Dirty-rect predicate over the staged tile, from the pair table of
+// iteration 0 (where every material with iterations > 0 is active): can any
+// pair of cells of the LDS region (the chunk and its HALO ring) take part in
+// a rule under some random draw, can any cell change alone, or is any
+// temperature stencil of the chunk non-zero?
+__device__ bool tile_unstable(const MatTab &M, uint64_t (*L)[LW], const uint16_t *T0, bool temp_enabled) {
     bool u = false;
+    constexpr uint32_t KV = K_SINK | K_GRISE | K_REACT, KD = K_SINK | K_GRISE;
     for (int i = threadIdx.x; i < LW * LH; i += NT) {
         const int ly = i / LW, lx = i - ly * LW;
         const uint64_t c = L[ly][lx];
+        const uint32_t t = cid(c) << 4;
         u |= single_unstable(M, c);
-        if (lx + 1 < LW) u |= pair_horz(M, c, L[ly][lx + 1]);
+        if (lx + 1 < LW) u |= (T0[t | cid(L[ly][lx + 1])] & K_HSWAP) != 0;
         if (ly + 1 < LH) {
-            u |= pair_vert(M, c, L[ly + 1][lx]);
-            if (lx + 1 < LW) u |= pair_diag(M, c, L[ly + 1][lx + 1]);
-            if (lx > 0) u |= pair_diag(M, c, L[ly + 1][lx - 1]);
+            u |= (T0[t | cid(L[ly + 1][lx])] & KV) != 0;
+            if (lx + 1 < LW) u |= (T0[t | cid(L[ly + 1][lx + 1])] & KD) != 0;
+            if (lx > 0) u |= (T0[t | cid(L[ly + 1][lx - 1])] & KD) != 0;
         }
     }
     if (temp_enabled) {
@@ -57,17 +84,15 @@ __device__ bool tile_unstable(const MatTab &M, uint64_t (*L)[LW], bool temp_enab
     return u;
 }
 
-__device__ __forceinline__ bool wood_hot(const MatTab &M, uint64_t c) {
-    return cbyte(c) == M.wood && ctemp(c) > M.burn;
-}
-
 // One Margolus phase over the blocks whose cells the rest of the tick still
 // reads: after this phase the tick needs `need` more rings around the chunk
 // (need = cell_iter - it, so the region shrinks by one ring per phase and the
-// last phase leaves the one ring the temperature stencil reads).
-__device__ __forceinline__ void run_phase(const MatTab &M, uint64_t (*L)[LW], const uint16_t *T, int it,
-                                          uint64_t key, int ox, int oy, int need, int gx0, int gy0,
-                                          bool has_wood) {
+// last phase leaves the one ring the temperature stencil reads).  With
+// `track`, returns the band bits of chunk cells that changed.
+template <bool TRACK>
+__device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint64_t (*L)[LW], const uint16_t *T, int it,
+                                              uint64_t key, int ox, int oy, int need, int gx0, int gy0,
+                                              bool has_wood) {
     // first/last block origin (chunk coordinates) with the phase's parity
     const int x0f = -need - ((need ^ ox) & 1), y0f = -need - ((need ^ oy) & 1);
     const int x0l = (TW - 1 + need) - (((TW - 1 + need) ^ ox) & 1);
@@ -75,6 +100,7 @@ __device__ __forceinline__ void run_phase(const MatTab &M, uint64_t (*L)[LW], co
     const int nbx = ((x0l - x0f) >> 1) + 1, nby = ((y0l - y0f) >> 1) + 1;
     const uint32_t magic = (65536u + (uint32_t)nbx - 1u) / (uint32_t)nbx;  // b / nbx for b < 2^16/nbx
     const int lx0 = HALO + x0f, ly0 = HALO + y0f;
+    uint32_t bands = 0;
     for (int b = threadIdx.x; b < nbx * nby; b += NT) {
         const int by = (int)(__umul24((uint32_t)b, magic) >> 16);
         const int bx = b - (int)__umul24((uint32_t)by, (uint32_t)nbx);
@@ -93,31 +119,46 @@ __device__ __forceinline__ void run_phase(const MatTab &M, uint64_t (*L)[LW], co
         if (!hot && block_inert(VL, VR, DL, DR, HT, HB, movA, movB)) continue;
         uint32_t r, s;
         block_hash(key, gx0 + lx, gy0 + ly, r, s);
+        bool wrote = false;
         if (hot || ((VL | VR) & K_REACT)) {
             if (block_rule(M, it, r, s, q)) {
                 p0[0] = q[0];
                 p0[1] = q[1];
                 p0[LW] = q[2];
                 p0[LW + 1] = q[3];
+                wrote = true;
             }
-            continue;
-        }
-        const uint32_t HTR = T[(i1 << 4) | i0];
-        const uint32_t dec = block_decide(VL, VR, DL, DR, HT, HTR, HB, movA, movB, r, s);
-        if (dec == 0) continue;
+        } else {
+            const uint32_t HTR = T[(i1 << 4) | i0];
+            const uint32_t dec = block_decide(VL, VR, DL, DR, HT, HTR, HB, movA, movB, r, s);
+            if (dec != 0) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            uint64_t v = q[k];
-            if ((dec >> (8 + k)) & 1u) v |= MOVED;
-            if ((dec >> (12 + k)) & 1u) v &= ~MOVED;
-            const uint32_t d = (uint32_t)k ^ ((dec >> (2 * k)) & 3u);
-            p0[(d & 1u) + (d >> 1) * LW] = v;
+                for (int k = 0; k < 4; k++) {
+                    uint64_t v = q[k];
+                    if ((dec >> (8 + k)) & 1u) v |= MOVED;
+                    if ((dec >> (12 + k)) & 1u) v &= ~MOVED;
+                    const uint32_t d = (uint32_t)k ^ ((dec >> (2 * k)) & 3u);
+                    p0[(d & 1u) + (d >> 1) * LW] = v;
+                }
+                wrote = true;
+            }
+        }
+        if (TRACK && wrote) {
+            const int cx = lx - HALO, cy = ly - HALO;  // chunk coordinates of the block
+            if (cx + 1 >= 0 && cx <= TW - 1 && cy + 1 >= 0 && cy <= TH - 1)
+                bands |= band_bits(max(cx, 0), min(cx + 1, TW - 1), max(cy, 0), min(cy + 1, TH - 1));
         }
     }
+    return bands;
+}
+
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off, 64);
+    return v;
 }
 
 __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, int tile,
-                                            uint64_t (*L)[LW], uint16_t (*lut)[256]) {
+                                            uint64_t (*L)[LW], uint16_t (*lut)[256], uint32_t *red) {
     const int tid = threadIdx.x;
     const MatTab &M = w.mat;
     const int tx = tile % w.tiles_x, ty = tile / w.tiles_x;
@@ -125,22 +166,27 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     const uint64_t *__restrict__ in = w.buf[tick & 1];
     uint64_t *__restrict__ out = w.buf[(tick + 1) & 1];
     const bool skip = w.skip != 0;
+    bool prev_changed = true;
 
     if (skip) {
-        // Exact skip: a chunk sleeps when its last update found no possible
-        // change in its region and no chunk of its 3x3 neighbourhood changed.
+        // Exact skip: a chunk sleeps unless it may change by itself (F_POT)
+        // or a neighbour changed inside the band its region overlaps.
         const uint32_t *fr = w.flags[(tick + 1) & 1];
-        bool awake = tick == w.force_tick || (fr[tile] & 1u);
-        for (int dy = -1; dy <= 1 && !awake; dy++)
-            for (int dx = -1; dx <= 1 && !awake; dx++) {
-                const int nx = tx + dx, ny = ty + dy;
-                if (nx < 0 || ny < 0 || nx >= w.tiles_x || ny >= w.tiles_y) continue;
-                awake = (fr[ny * w.tiles_x + nx] & 2u) != 0;
-            }
+        auto fl = [&](int dx, int dy) -> uint32_t {
+            const int nx = tx + dx, ny = ty + dy;
+            return (nx < 0 || ny < 0 || nx >= w.tiles_x || ny >= w.tiles_y) ? 0u : fr[ny * w.tiles_x + nx];
+        };
+        const uint32_t self = fl(0, 0);
+        const bool forced = tick == w.force_tick;
+        const bool awake = forced || (self & (F_POT | F_ANY)) || (fl(0, -1) & F_S) || (fl(0, 1) & F_N) ||
+                           (fl(-1, 0) & F_E) || (fl(1, 0) & F_W) || (fl(-1, -1) & F_SE) ||
+                           (fl(1, -1) & F_SW) || (fl(-1, 1) & F_NE) || (fl(1, 1) & F_NW);
         if (!awake) {
             if (tid == 0) w.flags[tick & 1][tile] = 0;
             return;
         }
+        // both state buffers hold this chunk unless it changed last tick
+        prev_changed = forced || (self & F_ANY);
     }
 
     const int ci = (int)M.cell_iter;
@@ -170,23 +216,36 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     // burning wood is the one reaction the pair table cannot see
     const bool has_wood = __syncthreads_or(wood && M.reactions) != 0;
 
-    bool pot = true;
-    if (skip) pot = __syncthreads_or(tile_unstable(M, L, M.temp_every != 0)) != 0;
+    if (skip && !prev_changed) {
+        // the chunk did not change last tick: if nothing in its region can
+        // change now it keeps its state, and both buffers already hold it
+        const bool pot = __syncthreads_or(tile_unstable(M, L, lut[0], M.temp_every != 0)) != 0;
+        if (!pot) {
+            if (tid == 0) {
+                w.flags[tick & 1][tile] = 0;
+                atomicAdd(w.processed, 1ull);
+            }
+            return;
+        }
+    }
 
-    if (pot) {
+    uint32_t bands = 0;
+    {
         const uint64_t seedmix = splitmix64(w.seed);
         for (int it = 0; it < ci; it++) {
             const uint64_t g = tick * (uint64_t)M.cell_iter + (uint64_t)it;
             const uint64_t key = splitmix64(seedmix ^ g);
             const int ox = kOX[g & 3], oy = kOY[g & 3];
-            run_phase(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood);
+            if (skip)
+                bands |= run_phase<true>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood);
+            else
+                run_phase<false>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood);
             __syncthreads();
         }
     }
 
     const bool temp_now = M.temp_every != 0 && (tick % M.temp_every) == M.temp_phase;
-    bool changed = false;
-    if (interior && !skip) {
+    if (interior) {
         // thread = 2-column strip x 8 rows; the stencil slides down the strip
         const int j = tid & 31, rg = tid >> 5;
         const int lx = HALO + 2 * j, ly0 = HALO + rg * (TH / 8);
@@ -200,6 +259,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
             uint4 up = *reinterpret_cast<const uint4 *>(&L[ly0 - 1][lx]);
             uint4 cur = *reinterpret_cast<const uint4 *>(&L[ly0][lx]);
             int tu0 = (int)up.x >> 16, tu1 = (int)up.z >> 16;
+            bool tchg = false;
 #pragma unroll
             for (int r = 0; r < TH / 8; r++) {
                 const uint4 dn = *reinterpret_cast<const uint4 *>(&L[ly0 + r + 1][lx]);
@@ -215,6 +275,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
                 }
                 v0 = min(max(v0, -32768), 32767);
                 v1 = min(max(v1, -32768), 32767);
+                tchg |= (v0 != t0) | (v1 != t1);
                 uint4 o = cur;
                 o.x = (cur.x & 0xFFFFu) | ((uint32_t)v0 << 16);
                 o.z = (cur.z & 0xFFFFu) | ((uint32_t)v1 << 16);
@@ -223,22 +284,28 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
                 tu1 = t1;
                 cur = dn;
             }
+            if (skip && tchg)
+                bands |= band_bits(2 * j, 2 * j + 1, rg * (TH / 8), rg * (TH / 8) + TH / 8 - 1);
         }
     } else {
         for (int i = tid; i < TW * TH; i += NT) {
             const int ly = HALO + i / TW, lx = HALO + (i & (TW - 1));
             const int gx = gx0 + lx, gy = gy0 + ly;
             if (gx >= w.W || gy >= w.H) continue;
-            const uint64_t c = temp_now ? temp_step(M, L, ly, lx) : L[ly][lx];
-            const size_t gi = (size_t)gy * w.W + gx;
-            out[gi] = c;
-            if (skip) changed |= c != in[gi];
+            const uint64_t c0 = L[ly][lx];
+            const uint64_t c = temp_now ? temp_step(M, L, ly, lx) : c0;
+            out[(size_t)gy * w.W + gx] = c;
+            if (skip && c != c0) bands |= band_bits(lx - HALO, lx - HALO, ly - HALO, ly - HALO);
         }
     }
     if (skip) {
-        changed = __syncthreads_or(changed) != 0;
+        bands = wave_or(bands);
+        if ((tid & 63) == 0) red[tid >> 6] = bands;
+        __syncthreads();
         if (tid == 0) {
-            w.flags[tick & 1][tile] = (pot ? 1u : 0u) | (changed ? 2u : 0u);
+            uint32_t all = 0;
+            for (int k = 0; k < NT / 64; k++) all |= red[k];
+            w.flags[tick & 1][tile] = F_POT | all;
             atomicAdd(w.processed, 1ull);
         }
     }
@@ -247,7 +314,8 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
 __global__ __launch_bounds__(NT) void k_tick_world(DevWorld w, uint32_t k) {
     __shared__ __align__(16) uint64_t L[LH][LW];
     __shared__ uint16_t lut[MAX_IT][256];
-    tile_update(w, w.tick + k, blockIdx.x, L, lut);
+    __shared__ uint32_t red[NT / 64];
+    tile_update(w, w.tick + k, blockIdx.x, L, lut, red);
 }
 
 // Many worlds in one launch: tile_start[i] is the first global chunk index of
@@ -257,6 +325,7 @@ __global__ __launch_bounds__(NT) void k_tick_batch(const DevWorld *__restrict__ 
                                                    int nworlds, uint32_t k) {
     __shared__ __align__(16) uint64_t L[LH][LW];
     __shared__ uint16_t lut[MAX_IT][256];
+    __shared__ uint32_t red[NT / 64];
     const uint32_t gt = blockIdx.x;
     int lo = 0, hi = nworlds - 1;
     while (lo < hi) {  // last world with tile_start <= gt
@@ -265,7 +334,7 @@ __global__ __launch_bounds__(NT) void k_tick_batch(const DevWorld *__restrict__ 
         else hi = mid - 1;
     }
     const DevWorld &w = worlds[lo];
-    tile_update(w, w.tick + k, (int)(gt - tile_start[lo]), L, lut);
+    tile_update(w, w.tick + k, (int)(gt - tile_start[lo]), L, lut, red);
 }
 
 __global__ void k_advance(DevWorld *worlds, int nworlds, uint32_t ticks) {
