@@ -131,6 +131,54 @@ def bench_c4(fsg, ticks=256, n=256):
             "bit_exact": exact}
 
 
+def bench_c5(fsg, rates=(30.0, 120.0, 400.0), duration_s=6.0, budget=64 << 30, seed=5):
+    """Serving: Poisson arrivals at each offered rate for duration_s.  Each
+    request creates a world (side uniform in {512..4096 step 64}, config-2
+    mix scaled to its area, 0-200 rigid bodies) or steps a resident one,
+    for k ~ U[1,64] ticks, under a 64 GB HBM state budget."""
+    import numpy as np
+    out = []
+    for rate in rates:
+        rng = np.random.default_rng(seed + int(rate))
+        with fsg.Context(0, hbm_budget=budget) as ctx, fsg.Server(ctx) as srv:
+            t0 = time.perf_counter()
+            created, n, t = [], 0, 0.0
+            while True:
+                t += float(rng.exponential(1.0 / rate))
+                if t > duration_s:
+                    break
+                delay = t0 + t - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                k = int(rng.integers(1, 65))
+                if created and rng.random() < 0.5:
+                    wid, W, H = created[int(rng.integers(len(created)))]
+                    srv.submit(wid, k, kind=4, width=W, height=H, seed=wid)
+                else:
+                    side = int(rng.integers(512 // 64, 4096 // 64 + 1)) * 64
+                    wid = 10_000 + n
+                    created.append((wid, side, side))
+                    srv.submit(wid, k, kind=4, width=side, height=side, seed=wid,
+                               n_bodies=int(rng.integers(0, 201)))
+                n += 1
+            rc = srv.drain(600)
+            wall = time.perf_counter() - t0
+            res = srv.poll(1 << 16)
+            st = srv.stats()
+        ok = [r for r in res if r["status"] == 0]
+        lat = np.array([(r["t_done_ns"] - r["t_submit_ns"]) / 1e6 for r in ok])
+        cu = sum(r["cells"] * r["ticks_done"] for r in ok)
+        out.append({"offered_req_s": rate, "requests": n, "completed": len(ok), "drained": rc == 0,
+                    "completed_req_s": round(len(ok) / wall, 2),
+                    "gcell_s": round(cu / wall / 1e9, 2),
+                    "p50_ms": round(float(np.percentile(lat, 50)), 2) if len(lat) else None,
+                    "p99_ms": round(float(np.percentile(lat, 99)), 2) if len(lat) else None,
+                    "steps": st["steps"], "evictions": st["evictions"]})
+    return {"workload": f"Poisson request stream, sides U{{512..4096}}, k~U[1,64] ticks, 0-200 bodies, "
+                        f"{budget >> 30} GB state budget, {duration_s:.0f} s per rate",
+            "rates": out}
+
+
 def cpu_baseline(threads=None, side_w=8192, side_h=4096, ticks=64):
     """refseq (the reference's sequential chunk-checkerboard tick, restated)
     on a bounded sample of the all-active world."""
@@ -156,7 +204,7 @@ def main():
     ap.add_argument("--side", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
-    ap.add_argument("--mode", default="all", choices=["all", "c3", "copy"],
+    ap.add_argument("--mode", default="all", choices=["all", "c3", "copy", "c5"],
                     help="all: the JSON line; c3: only the timed tick loop (for profiling); "
                          "copy: the HBM copy calibration kernel")
     ap.add_argument("--copy-bytes", type=int, default=2 << 30)
@@ -174,6 +222,9 @@ def main():
             ms = ctx.copy_timed(args.copy_bytes, max(1, args.steps))
         gbs = 2 * args.copy_bytes / (ms / 1e3) / 1e9
         print(json.dumps({"mode": "copy", "bytes": args.copy_bytes, "ms": ms, "GBs_read_plus_write": gbs}))
+        return
+    if args.mode == "c5":
+        print(json.dumps(bench_c5(fsg)))
         return
     if args.mode == "c3":
         r = bench_c3(fsg, args.steps, args.warmup, args.side)
@@ -196,6 +247,9 @@ def main():
         log("[bench] C4 uneven batch")
         secondary["c4_batch"] = bench_c4(fsg)
         log(f"[bench] {secondary['c4_batch']}")
+        log("[bench] C5 serving")
+        secondary["c5_serving"] = bench_c5(fsg)
+        log(f"[bench] {secondary['c5_serving']}")
         with fsg.Context(0) as ctx:
             cms = ctx.copy_timed(2 << 30, 10)
         secondary["copy_kernel_GBs"] = round(2 * (2 << 30) / (cms / 1e3) / 1e9, 1)

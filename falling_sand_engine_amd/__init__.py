@@ -245,6 +245,62 @@ def perworld_step_timed(ctx: Context, worlds, ticks: int) -> float:
     return ms.value
 
 
+class Server:
+    """Continuous-batching request loop (fsg_serve_*): worlds stay resident in
+    HBM between requests, admitted under the context's HBM budget."""
+
+    def __init__(self, ctx: Context, max_active: int = 0, block_ticks: int = 16, verify_resident: bool = False):
+        cfg = _abi.ServeConfig(max_active, block_ticks, 1 if verify_resident else 0, 0)
+        h = C.c_void_p()
+        check(lib().fsg_serve_create(ctx.handle, C.byref(cfg), C.byref(h)), "fsg_serve_create")
+        self.handle = h
+
+    def submit(self, world_id: int, ticks: int, kind: int = 4, width: int = 512, height: int = 512,
+               seed: int = 0, n_bodies: int = 0, deadline_ns: int = 0) -> int:
+        r = _abi.ServeRequest(world_id, kind, width, height, ticks, seed, n_bodies, 0, deadline_ns)
+        rid = C.c_uint64()
+        check(lib().fsg_serve_submit(self.handle, C.byref(r), C.byref(rid)), "fsg_serve_submit")
+        return rid.value
+
+    def cancel(self, request_id: int) -> int:
+        return lib().fsg_serve_cancel(self.handle, request_id)
+
+    def poll(self, cap: int = 4096):
+        arr = (_abi.ServeResult * cap)()
+        n = C.c_int()
+        check(lib().fsg_serve_poll(self.handle, arr, cap, C.byref(n)), "fsg_serve_poll")
+        return [{f: getattr(arr[i], f) for f, _ in _abi.ServeResult._fields_} for i in range(n.value)]
+
+    def drain(self, timeout_s: float = 60.0) -> int:
+        return lib().fsg_serve_drain(self.handle, int(timeout_s * 1e9))
+
+    def release(self, world_id: int):
+        check(lib().fsg_serve_release(self.handle, world_id))
+
+    def corrupt(self, world_id: int):
+        check(lib().fsg_serve_corrupt(self.handle, world_id))
+
+    def stats(self) -> dict:
+        s = _abi.ServeStats()
+        check(lib().fsg_serve_stats_get(self.handle, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _abi.ServeStats._fields_}
+
+    def close(self):
+        if self.handle:
+            check(lib().fsg_serve_destroy(self.handle), "fsg_serve_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def now_ns() -> int:
+    return lib().fsg_now_ns()
+
+
 def body_from_pose(x: float, y: float, angle: float, mat_w: int, mat_h: int, pixel_offset: int,
                    enabled: bool = True) -> Body:
     """Fixed-point pose from a float pose (b2Body::GetPosition/GetAngle), done

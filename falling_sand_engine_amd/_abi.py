@@ -63,20 +63,27 @@ class StampResult(C.Structure):
 
 
 class ServeConfig(C.Structure):
-    _fields_ = [("max_batch_worlds", C.c_int32), ("max_inflight_ticks", C.c_int32),
-                ("block_ticks", C.c_int32), ("reserved", C.c_int32)]
+    _fields_ = [("max_active", C.c_int32), ("block_ticks", C.c_int32), ("verify_resident", C.c_int32),
+                ("gen_threads", C.c_int32)]
 
 
 class ServeRequest(C.Structure):
-    _fields_ = [("world_id", C.c_uint64), ("create_kind", C.c_int32), ("width", C.c_int32),
+    _fields_ = [("world_id", C.c_uint64), ("kind", C.c_int32), ("width", C.c_int32),
                 ("height", C.c_int32), ("ticks", C.c_int32), ("seed", C.c_uint64),
-                ("deadline_ns", C.c_uint64), ("n_bodies", C.c_int32), ("body_seed", C.c_int32)]
+                ("n_bodies", C.c_int32), ("reserved", C.c_int32), ("deadline_ns", C.c_uint64)]
 
 
 class ServeResult(C.Structure):
     _fields_ = [("request_id", C.c_uint64), ("world_id", C.c_uint64), ("status", C.c_int32),
                 ("ticks_done", C.c_int32), ("tick", C.c_uint64), ("checksum", C.c_uint64),
-                ("t_submit_ns", C.c_uint64), ("t_admit_ns", C.c_uint64), ("t_done_ns", C.c_uint64)]
+                ("cells", C.c_uint64), ("t_submit_ns", C.c_uint64), ("t_admit_ns", C.c_uint64),
+                ("t_done_ns", C.c_uint64)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "failed", "queued", "active",
+                                          "resident_worlds", "resident_bytes", "steps", "cell_updates",
+                                          "evictions")]
 
 
 P = C.c_void_p
@@ -111,6 +118,16 @@ SIGNATURES = {
     "fsg_perworld_step_timed": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "fsg_generate": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, P]),
     "fsg_rule_eval": (C.c_int, [C.POINTER(Config), C.c_int, C.c_uint32, C.c_uint32, P, C.c_int]),
+    "fsg_now_ns": (C.c_uint64, []),
+    "fsg_serve_create": (C.c_int, [P, C.POINTER(ServeConfig), C.POINTER(P)]),
+    "fsg_serve_destroy": (C.c_int, [P]),
+    "fsg_serve_submit": (C.c_int, [P, C.POINTER(ServeRequest), C.POINTER(C.c_uint64)]),
+    "fsg_serve_cancel": (C.c_int, [P, C.c_uint64]),
+    "fsg_serve_poll": (C.c_int, [P, C.POINTER(ServeResult), C.c_int, C.POINTER(C.c_int)]),
+    "fsg_serve_drain": (C.c_int, [P, C.c_uint64]),
+    "fsg_serve_release": (C.c_int, [P, C.c_uint64]),
+    "fsg_serve_stats_get": (C.c_int, [P, C.POINTER(ServeStats)]),
+    "fsg_serve_corrupt": (C.c_int, [P, C.c_uint64]),
     "fsg_splitmix64": (C.c_uint64, [C.c_uint64]),
     "fsg_phase_key": (C.c_uint64, [C.c_uint64, C.c_uint64]),
 }

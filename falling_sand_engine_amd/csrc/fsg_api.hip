@@ -17,53 +17,11 @@
 #include "fsg_device.hpp"
 #include "fsg_kernels.hpp"
 
-using fsg::DevWorld;
-using fsg::MatTab;
+#include "fsg_internal.hpp"
 
-#define HIPCHK(call)                                   \
-    do {                                               \
-        if ((call) != hipSuccess) return FSG_ERR_DEVICE; \
-    } while (0)
+using namespace fsg_impl;
 
-struct fsg_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    uint64_t budget = 0, used = 0;
-    int inject = 0;
-    std::mutex mu;
-    // batch launch state: device world table + chunk prefix sums + graph
-    DevWorld *d_worlds = nullptr;
-    uint32_t *d_tile_start = nullptr;
-    int batch_cap = 0;
-    std::vector<fsg_world *> graph_key;
-    std::vector<uint32_t> graph_tiles;
-    hipGraphExec_t graph = nullptr;
-    uint32_t total_tiles = 0;
-};
-
-struct fsg_world {
-    fsg_ctx *ctx = nullptr;
-    fsg_config cfg{};
-    DevWorld d{};
-    size_t bytes = 0;
-    uint64_t offered = 0;
-    unsigned long long *d_stats = nullptr;  // [17] census scratch + [1] processed counter
-    // rigid-body stamp state
-    uint64_t *d_pixels = nullptr;
-    uint32_t *d_pixbody = nullptr;
-    fsg_body *d_bodies = nullptr;
-    uint8_t *d_state = nullptr;
-    uint64_t *d_disp = nullptr, *d_dispxy = nullptr;
-    size_t pix_cap = 0;
-    int body_cap = 0;
-    size_t npix = 0;
-    int nbodies = 0;
-    bool stamped = false;
-    std::vector<fsg_body> bodies;
-};
-
-namespace {
+namespace fsg_impl {
 
 constexpr int kGraphTicks = 16;
 
@@ -365,6 +323,7 @@ extern "C" int fsg_world_destroy(fsg_world *w) {
     dev_free(w->d_state);
     dev_free(w->d_disp);
     dev_free(w->d_dispxy);
+    dev_free(w->d_sw);
     delete w;
     return FSG_OK;
 }
@@ -531,26 +490,66 @@ extern "C" int fsg_world_reset_activity(fsg_world *w) {
 // ---------------------------------------------------------------------------
 // Rigid-body pixels
 // ---------------------------------------------------------------------------
-namespace {
+namespace fsg_impl {
 
-fsg::StampArgs stamp_args(fsg_world *w) {
-    fsg::StampArgs a;
+// The world's entry in a stamp batch (pixels numbered from pix_start).
+fsg::StampWorld stamp_world_of(fsg_world *w, uint32_t pix_start) {
+    fsg::StampWorld sw;
     const uint64_t t = w->d.tick;
-    a.cells = w->d.buf[t & 1];
-    a.claims = w->d.buf[(t + 1) & 1];
-    a.pixels = w->d_pixels;
-    a.pix_body = w->d_pixbody;
-    a.bodies = w->d_bodies;
-    a.state = w->d_state;
-    a.disp = w->d_disp;
-    a.disp_xy = w->d_dispxy;
-    a.flags = w->d.skip ? w->d.flags[(t + 1) & 1] : nullptr;  // flags the next tick reads
-    a.W = w->d.W;
-    a.H = w->d.H;
-    a.tiles_x = w->d.tiles_x;
-    a.npix = (uint32_t)w->npix;
-    a.mat = w->d.mat;
-    return a;
+    sw.cells = w->d.buf[t & 1];
+    sw.claims = w->d.buf[(t + 1) & 1];
+    sw.flags = w->d.skip ? w->d.flags[(t + 1) & 1] : nullptr;  // flags the next tick reads
+    sw.pixels = w->d_pixels;
+    sw.pix_body = w->d_pixbody;
+    sw.bodies = w->d_bodies;
+    sw.state = w->d_state;
+    sw.disp = w->d_disp;
+    sw.disp_xy = w->d_dispxy;
+    sw.W = w->d.W;
+    sw.H = w->d.H;
+    sw.tiles_x = w->d.tiles_x;
+    sw.npix = (uint32_t)w->npix;
+    sw.pix_start = pix_start;
+    sw.pad = 0;
+    return sw;
+}
+
+// Claim rounds over a batch of worlds whose table is already on the device.
+int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t total, const MatTab &M,
+                     bool unstamp) {
+    if (total == 0) return FSG_OK;
+    fsg::StampArgs a;
+    a.worlds = d_table;
+    a.nworlds = n;
+    a.total = total;
+    a.mat = M;
+    const unsigned g = grid_for(total, 256);
+    fsg::k_claims_init<<<g, 256, 0, c->stream>>>(a);
+    for (int d = 0; d < 5; d++) {
+        if (unstamp) {
+            fsg::k_unstamp_propose<<<g, 256, 0, c->stream>>>(a, d);
+            fsg::k_unstamp_resolve<<<g, 256, 0, c->stream>>>(a, d);
+        } else {
+            fsg::k_stamp_propose<<<g, 256, 0, c->stream>>>(a, d);
+            fsg::k_stamp_resolve<<<g, 256, 0, c->stream>>>(a, d);
+        }
+    }
+    if (unstamp) fsg::k_unstamp_finish<<<g, 256, 0, c->stream>>>(a);
+    HIPCHK(hipGetLastError());
+    return FSG_OK;
+}
+
+// One-world batch for the single-world entry points.
+int stamp_single(fsg_world *w, bool unstamp) {
+    fsg_ctx *c = w->ctx;
+    if (!w->d_sw) {
+        void *p;
+        if (dev_alloc(c, &p, sizeof(fsg::StampWorld))) return FSG_ERR_NOMEM;
+        w->d_sw = (fsg::StampWorld *)p;
+    }
+    w->sw_host = stamp_world_of(w, 0);
+    HIPCHK(hipMemcpyAsync(w->d_sw, &w->sw_host, sizeof(fsg::StampWorld), hipMemcpyHostToDevice, c->stream));
+    return stamp_rounds(c, w->d_sw, 1, (uint32_t)w->npix, w->d.mat, unstamp);
 }
 
 int ensure_stamp_buffers(fsg_world *w, size_t npix, int nb) {
@@ -590,14 +589,12 @@ int ensure_stamp_buffers(fsg_world *w, size_t npix, int nb) {
     return FSG_OK;
 }
 
-} // namespace
-
-extern "C" int fsg_world_stamp_bodies(fsg_world *w, const fsg_body *bodies, int nb, const fsg_cell *pixels,
-                                      size_t npix, fsg_stamp_result *res) {
-    if (!w || nb < 0 || (nb && !bodies) || (npix && !pixels)) return FSG_ERR_INVALID;
+// Validates bodies/pixels and uploads them (with the pixel -> body map).
+// The host copies must stay valid until the stream is synchronised.
+int set_bodies(fsg_world *w, const fsg_body *bodies, int nb, const fsg_cell *pixels, size_t npix) {
     if (npix >= 0xFFFFFFFFull) return FSG_ERR_INVALID;
-    if (w->stamped) return FSG_ERR_STATE;
-    std::vector<uint32_t> pix_body(npix, 0xFFFFFFFFu);
+    w->pix_body_host.assign(npix, 0xFFFFFFFFu);
+    std::vector<uint32_t> &pix_body = w->pix_body_host;
     for (int b = 0; b < nb; b++) {
         const fsg_body &B = bodies[b];
         if (B.mat_w <= 0 || B.mat_h <= 0 || B.mat_w > 4096 || B.mat_h > 4096) return FSG_ERR_INVALID;
@@ -620,17 +617,21 @@ extern "C" int fsg_world_stamp_bodies(fsg_world *w, const fsg_body *bodies, int 
         HIPCHK(hipMemcpyAsync(w->d_pixels, pixels, npix * 8, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(w->d_pixbody, pix_body.data(), npix * 4, hipMemcpyHostToDevice, c->stream));
     }
-    if (nb) HIPCHK(hipMemcpyAsync(w->d_bodies, bodies, (size_t)nb * sizeof(fsg_body), hipMemcpyHostToDevice, c->stream));
-    const fsg::StampArgs a = stamp_args(w);
-    if (npix) {
-        const unsigned g = grid_for(npix, 256);
-        fsg::k_claims_init<<<g, 256, 0, c->stream>>>(a);
-        for (int d = 0; d < 5; d++) {
-            fsg::k_stamp_propose<<<g, 256, 0, c->stream>>>(a, d);
-            fsg::k_stamp_resolve<<<g, 256, 0, c->stream>>>(a, d);
-        }
-        HIPCHK(hipGetLastError());
-    }
+    if (nb) HIPCHK(hipMemcpyAsync(w->d_bodies, w->bodies.data(), (size_t)nb * sizeof(fsg_body), hipMemcpyHostToDevice, c->stream));
+    return FSG_OK;
+}
+
+} // namespace
+
+extern "C" int fsg_world_stamp_bodies(fsg_world *w, const fsg_body *bodies, int nb, const fsg_cell *pixels,
+                                      size_t npix, fsg_stamp_result *res) {
+    if (!w || nb < 0 || (nb && !bodies) || (npix && !pixels)) return FSG_ERR_INVALID;
+    if (w->stamped) return FSG_ERR_STATE;
+    if (int rc = set_bodies(w, bodies, nb, pixels, npix)) return rc;
+    const std::vector<uint32_t> &pix_body = w->pix_body_host;
+    fsg_ctx *c = w->ctx;
+    if (npix)
+        if (int rc = stamp_single(w, false)) return rc;
     std::vector<uint8_t> state(npix);
     std::vector<uint64_t> disp(npix), dispxy(npix);
     if (npix) {
@@ -678,16 +679,8 @@ extern "C" int fsg_world_unstamp_bodies(fsg_world *w, fsg_cell *pixels_out, size
     if (npix != w->npix) return FSG_ERR_INVALID;
     fsg_ctx *c = w->ctx;
     HIPCHK(hipSetDevice(c->device));
-    const fsg::StampArgs a = stamp_args(w);
     if (npix) {
-        const unsigned g = grid_for(npix, 256);
-        fsg::k_claims_init<<<g, 256, 0, c->stream>>>(a);
-        for (int d = 0; d < 5; d++) {
-            fsg::k_unstamp_propose<<<g, 256, 0, c->stream>>>(a, d);
-            fsg::k_unstamp_resolve<<<g, 256, 0, c->stream>>>(a, d);
-        }
-        fsg::k_unstamp_finish<<<g, 256, 0, c->stream>>>(a);
-        HIPCHK(hipGetLastError());
+        if (int rc = stamp_single(w, true)) return rc;
         HIPCHK(hipMemcpyAsync(pixels_out, w->d_pixels, npix * 8, hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -698,7 +691,7 @@ extern "C" int fsg_world_unstamp_bodies(fsg_world *w, fsg_cell *pixels_out, size
 // ---------------------------------------------------------------------------
 // Batching
 // ---------------------------------------------------------------------------
-namespace {
+namespace fsg_impl {
 
 int batch_prepare(fsg_ctx *c, fsg_world *const *ws, int n, bool *graph_ok) {
     for (int i = 0; i < n; i++)

@@ -1,0 +1,79 @@
+// fsg_internal.hpp -- library-private state shared by the C-ABI and the
+// serving engine (not part of the public boundary).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "fsg.h"
+#include "fsg_device.hpp"
+#include "fsg_kernels.hpp"
+
+using fsg::DevWorld;
+using fsg::MatTab;
+
+#define HIPCHK(call)                                     \
+    do {                                                 \
+        if ((call) != hipSuccess) return FSG_ERR_DEVICE; \
+    } while (0)
+
+struct fsg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    uint64_t budget = 0, used = 0;
+    int inject = 0;
+    std::mutex mu;
+    // batch launch state: device world table + chunk prefix sums + graph
+    DevWorld *d_worlds = nullptr;
+    uint32_t *d_tile_start = nullptr;
+    int batch_cap = 0;
+    std::vector<fsg_world *> graph_key;
+    std::vector<uint32_t> graph_tiles;
+    hipGraphExec_t graph = nullptr;
+    uint32_t total_tiles = 0;
+};
+
+struct fsg_world {
+    fsg_ctx *ctx = nullptr;
+    fsg_config cfg{};
+    DevWorld d{};
+    size_t bytes = 0;
+    uint64_t offered = 0;
+    unsigned long long *d_stats = nullptr;  // [17] census scratch + [1] processed counter
+    // rigid-body stamp state
+    uint64_t *d_pixels = nullptr;
+    uint32_t *d_pixbody = nullptr;
+    fsg_body *d_bodies = nullptr;
+    uint8_t *d_state = nullptr;
+    uint64_t *d_disp = nullptr, *d_dispxy = nullptr;
+    size_t pix_cap = 0;
+    int body_cap = 0;
+    size_t npix = 0;
+    int nbodies = 0;
+    bool stamped = false;
+    std::vector<fsg_body> bodies;
+    fsg::StampWorld *d_sw = nullptr;  // one-entry stamp table on the device
+    fsg::StampWorld sw_host{};
+    std::vector<uint32_t> pix_body_host;
+};
+
+
+namespace fsg_impl {
+int dev_alloc(fsg_ctx *c, void **p, size_t bytes);
+void dev_free(void *p);
+unsigned grid_for(size_t n, unsigned block);
+// Uploads the world table of a batch; *graph_ok tells whether the cached
+// graph matches this world set.
+int batch_prepare(fsg_ctx *c, fsg_world *const *ws, int n, bool *graph_ok);
+// Enqueues `ticks` ticks of the prepared batch (16-tick graphs + remainder).
+int batch_run(fsg_ctx *c, fsg_world *const *ws, int n, int ticks, bool graph_ok);
+// The world's entry in a stamp batch (pixels numbered from pix_start).
+fsg::StampWorld stamp_world_of(fsg_world *w, uint32_t pix_start);
+// Stamp (or unstamp) claim rounds over a device table of n worlds.
+int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t total, const MatTab &M,
+                 bool unstamp);
+// Uploads pixels/bodies of a world for stamping (validates them).
+int set_bodies(fsg_world *w, const fsg_body *bodies, int nb, const fsg_cell *pixels, size_t npix);
+}  // namespace fsg_impl

@@ -43,6 +43,7 @@ __device__ __forceinline__ bool wood_hot(const MatTab &M, uint64_t c) {
 constexpr uint32_t F_POT = 1u, F_ANY = 2u;
 constexpr uint32_t F_N = 4u, F_S = 8u, F_W = 16u, F_E = 32u;
 constexpr uint32_t F_NW = 64u, F_NE = 128u, F_SW = 256u, F_SE = 512u;
+constexpr uint32_t F_WAKE_ALL = 0x3FFu;  // changed everywhere: wakes the chunk and its 8 neighbours
 
 // Bands touched by a change covering chunk cells [x0, x1] x [y0, y1]
 // (chunk coordinates, already clipped to the chunk).
@@ -419,13 +420,35 @@ __global__ void k_set_cells(uint64_t *cells, int W, int H, const int32_t *xy, co
 __constant__ int kDX[5] = {0, 1, -1, 0, 0};
 __constant__ int kDY[5] = {0, 0, 0, 1, -1};
 
-__device__ __forceinline__ bool pixel_target(const fsg_body *bodies, const uint32_t *pix_body,
-                                             uint32_t p, int d, int W, int H, int &wx, int &wy) {
-    const uint32_t b = pix_body[p];
+// Pixel g of the batch -> world, local pixel, target cell of round d
+// (d = -1: the pixel's own cell).  False if the pixel is not live (no body,
+// body disabled, air pixel) or the target is out of its world.
+struct PixelRef {
+    const StampWorld *w;
+    uint32_t p;
+};
+
+__device__ __forceinline__ bool pixel_ref(const StampArgs &a, uint32_t g, PixelRef &ref) {
+    if (g >= a.total) return false;
+    int lo = 0, hi = a.nworlds - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.worlds[mid].pix_start <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    ref.w = &a.worlds[lo];
+    ref.p = g - ref.w->pix_start;
+    if (ref.p >= ref.w->npix) return false;
+    return !bit(a.mat.air, cid(ref.w->pixels[ref.p]));
+}
+
+__device__ __forceinline__ bool pixel_target(const PixelRef &ref, int d, int &wx, int &wy) {
+    const StampWorld &W = *ref.w;
+    const uint32_t b = W.pix_body[ref.p];
     if (b == 0xFFFFFFFFu) return false;
-    const fsg_body B = bodies[b];
+    const fsg_body B = W.bodies[b];
     if (!B.enabled) return false;
-    const uint32_t k = p - B.pixel_offset;
+    const uint32_t k = ref.p - B.pixel_offset;
     const int tx = (int)(k % (uint32_t)B.mat_w), ty = (int)(k / (uint32_t)B.mat_w);
     const int64_t fx = (int64_t)tx * B.cos_q16 - (int64_t)(ty + 1) * B.sin_q16 + B.pos_x_q16;
     const int64_t fy = (int64_t)tx * B.sin_q16 + (int64_t)(ty + 1) * B.cos_q16 + B.pos_y_q16;
@@ -435,88 +458,104 @@ __device__ __forceinline__ bool pixel_target(const fsg_body *bodies, const uint3
         wx += kDX[d];
         wy += kDY[d];
     }
-    return wx >= 0 && wy >= 0 && wx < W && wy < H;
+    return wx >= 0 && wy >= 0 && wx < W.W && wy < W.H;
 }
 
-__device__ __forceinline__ void mark_tile(uint32_t *flags, int tiles_x, int x, int y) {
-    if (flags) atomicOr(&flags[(y / TH) * tiles_x + (x / TW)], 2u);
+// Chunks whose cells a stamp round touches must run on the next tick (their
+// spare-buffer claim slots were written); flags is null without skipping.
+__device__ __forceinline__ void mark_tile(const StampWorld &W, int x, int y) {
+    if (W.flags) atomicOr(&W.flags[(y / TH) * W.tiles_x + (x / TW)], F_WAKE_ALL);
 }
 
-// Initialise the claim slots of every possible target; mark touched chunks.
 __global__ void k_claims_init(StampArgs a) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.npix) return;
-    a.state[p] = 0;
-    if (bit(a.mat.air, cid(a.pixels[p]))) return;
+    PixelRef ref;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.total) return;
+    {
+        int lo = 0, hi = a.nworlds - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.worlds[mid].pix_start <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t p = g - a.worlds[lo].pix_start;
+        if (p < a.worlds[lo].npix) a.worlds[lo].state[p] = 0;
+    }
+    if (!pixel_ref(a, g, ref)) return;
     for (int d = 0; d < 5; d++) {
         int wx, wy;
-        if (!pixel_target(a.bodies, a.pix_body, p, d, a.W, a.H, wx, wy)) continue;
-        a.claims[(size_t)wy * a.W + wx] = ~0ull;
-        mark_tile(a.flags, a.tiles_x, wx, wy);
+        if (!pixel_target(ref, d, wx, wy)) continue;
+        ref.w->claims[(size_t)wy * ref.w->W + wx] = ~0ull;
+        mark_tile(*ref.w, wx, wy);
     }
 }
 
 __global__ void k_stamp_propose(StampArgs a, int d) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.npix || a.state[p] || bit(a.mat.air, cid(a.pixels[p]))) return;
+    PixelRef ref;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!pixel_ref(a, g, ref) || ref.w->state[ref.p]) return;
     int wx, wy;
-    if (!pixel_target(a.bodies, a.pix_body, p, d, a.W, a.H, wx, wy)) return;
-    const size_t ci = (size_t)wy * a.W + wx;
-    const uint32_t m = cid(a.cells[ci]);
+    if (!pixel_target(ref, d, wx, wy)) return;
+    const size_t ci = (size_t)wy * ref.w->W + wx;
+    const uint32_t m = cid(ref.w->cells[ci]);
     if (!(bit(a.mat.air, m) || bit(a.mat.sand, m) || bit(a.mat.soup, m))) return;
-    atomicMin(reinterpret_cast<unsigned long long *>(&a.claims[ci]),
-              ((unsigned long long)d << 32) | p);
+    atomicMin(reinterpret_cast<unsigned long long *>(&ref.w->claims[ci]), ((unsigned long long)d << 32) | ref.p);
 }
 
 __global__ void k_stamp_resolve(StampArgs a, int d) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.npix || a.state[p] || bit(a.mat.air, cid(a.pixels[p]))) return;
+    PixelRef ref;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!pixel_ref(a, g, ref) || ref.w->state[ref.p]) return;
     int wx, wy;
-    if (!pixel_target(a.bodies, a.pix_body, p, d, a.W, a.H, wx, wy)) return;
-    const size_t ci = (size_t)wy * a.W + wx;
-    if (a.claims[ci] != (((uint64_t)d << 32) | p)) return;
-    const uint64_t old = a.cells[ci];
+    if (!pixel_target(ref, d, wx, wy)) return;
+    const StampWorld &W = *ref.w;
+    const size_t ci = (size_t)wy * W.W + wx;
+    if (W.claims[ci] != (((uint64_t)d << 32) | ref.p)) return;
+    const uint64_t old = W.cells[ci];
     if (!bit(a.mat.air, cid(old))) {
-        a.disp[p] = old;
-        a.disp_xy[p] = (uint64_t)(uint32_t)wx | ((uint64_t)(uint32_t)wy << 32);
-        a.state[p] = bit(a.mat.sand, cid(old)) ? 2 : 3;  // placed, displaced sand / soup
+        W.disp[ref.p] = old;
+        W.disp_xy[ref.p] = (uint64_t)(uint32_t)wx | ((uint64_t)(uint32_t)wy << 32);
+        W.state[ref.p] = bit(a.mat.sand, cid(old)) ? 2 : 3;  // placed over sand / soup
     } else {
-        a.state[p] = 1;  // placed
+        W.state[ref.p] = 1;  // placed
     }
-    a.cells[ci] = a.pixels[p];
+    W.cells[ci] = W.pixels[ref.p];
 }
 
 __global__ void k_unstamp_propose(StampArgs a, int d) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.npix || a.state[p] || bit(a.mat.air, cid(a.pixels[p]))) return;
+    PixelRef ref;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!pixel_ref(a, g, ref) || ref.w->state[ref.p]) return;
     int wx, wy;
-    if (!pixel_target(a.bodies, a.pix_body, p, d, a.W, a.H, wx, wy)) return;
-    const size_t ci = (size_t)wy * a.W + wx;
-    if (cbyte(a.cells[ci]) != cbyte(a.pixels[p])) return;
-    atomicMin(reinterpret_cast<unsigned long long *>(&a.claims[ci]),
-              ((unsigned long long)d << 32) | p);
+    if (!pixel_target(ref, d, wx, wy)) return;
+    const size_t ci = (size_t)wy * ref.w->W + wx;
+    if (cbyte(ref.w->cells[ci]) != cbyte(ref.w->pixels[ref.p])) return;
+    atomicMin(reinterpret_cast<unsigned long long *>(&ref.w->claims[ci]), ((unsigned long long)d << 32) | ref.p);
 }
 
 __global__ void k_unstamp_resolve(StampArgs a, int d) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.npix || a.state[p] || bit(a.mat.air, cid(a.pixels[p]))) return;
+    PixelRef ref;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!pixel_ref(a, g, ref) || ref.w->state[ref.p]) return;
     int wx, wy;
-    if (!pixel_target(a.bodies, a.pix_body, p, d, a.W, a.H, wx, wy)) return;
-    const size_t ci = (size_t)wy * a.W + wx;
-    if (a.claims[ci] != (((uint64_t)d << 32) | p)) return;
-    const uint64_t c = a.cells[ci];
-    a.pixels[p] = c;
-    a.cells[ci] = (uint64_t)a.mat.air_id | (c & 0xFFFF0000ull);
-    a.state[p] = 1;
+    if (!pixel_target(ref, d, wx, wy)) return;
+    const StampWorld &W = *ref.w;
+    const size_t ci = (size_t)wy * W.W + wx;
+    if (W.claims[ci] != (((uint64_t)d << 32) | ref.p)) return;
+    const uint64_t c = W.cells[ci];
+    W.pixels[ref.p] = c;
+    W.cells[ci] = (uint64_t)a.mat.air_id | (c & 0xFFFF0000ull);
+    W.state[ref.p] = 1;
 }
 
 // A pixel that found nothing is lost when the cell under it is air.
 __global__ void k_unstamp_finish(StampArgs a) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.npix || a.state[p] || bit(a.mat.air, cid(a.pixels[p]))) return;
+    PixelRef ref;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!pixel_ref(a, g, ref) || ref.w->state[ref.p]) return;
     int wx, wy;
-    if (!pixel_target(a.bodies, a.pix_body, p, -1, a.W, a.H, wx, wy)) return;
-    if (bit(a.mat.air, cid(a.cells[(size_t)wy * a.W + wx]))) a.pixels[p] = (uint64_t)a.mat.air_id;
+    if (!pixel_target(ref, -1, wx, wy)) return;
+    if (bit(a.mat.air, cid(ref.w->cells[(size_t)wy * ref.w->W + wx]))) ref.w->pixels[ref.p] = (uint64_t)a.mat.air_id;
 }
 
 } // namespace fsg

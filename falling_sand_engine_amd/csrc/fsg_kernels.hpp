@@ -5,20 +5,27 @@
 
 namespace fsg {
 
-// Arguments of the rigid-body claim-round kernels.
-struct StampArgs {
+// One world of a rigid-body stamp batch.  Pixels of all worlds are numbered
+// consecutively; world i owns [pix_start, pix_start + npix).
+struct StampWorld {
     uint64_t *cells;           // current state buffer
-    uint64_t *claims;          // spare state buffer used as claim slots
-    uint64_t *pixels;          // body pixels (cells), body-major, row-major per body
+    uint64_t *claims;          // spare state buffer, used as claim slots
+    uint32_t *flags;           // chunk flags the next tick reads (null: no skipping)
+    uint64_t *pixels;          // body pixels, body-major, row-major per body
     const uint32_t *pix_body;  // body index per pixel (0xFFFFFFFF = none)
     const fsg_body *bodies;
     uint8_t *state;            // per pixel: 0 open, 1 placed/found, 2/3 placed over sand/soup
     uint64_t *disp;            // displaced cell per pixel
     uint64_t *disp_xy;         // its position (x | y << 32)
-    uint32_t *flags;           // chunk flags read by the next tick (null: no skipping)
-    int W, H, tiles_x;
-    uint32_t npix;
-    MatTab mat;
+    int32_t W, H, tiles_x;
+    uint32_t npix, pix_start, pad;
+};
+
+struct StampArgs {
+    const StampWorld *worlds;  // device array
+    int nworlds;
+    uint32_t total;            // pixels over all worlds
+    MatTab mat;                // shared material table
 };
 
 __global__ void k_tick_world(DevWorld w, uint32_t k);

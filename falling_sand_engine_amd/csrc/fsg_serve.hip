@@ -1,0 +1,642 @@
+// fsg_serve.hip -- continuous-batching request loop over resident worlds.
+//
+// One worker thread owns every world of the server.  Each step it
+//   1. admits queued requests in arrival order while the context's HBM state
+//      budget allows (evicting idle resident worlds, least recently used
+//      first); a request whose world is busy waits for the earlier one;
+//   2. finishes the host-side generation of newly created worlds and
+//      uploads them;
+//   3. advances all admitted requests together: one batched tick launch per
+//      tick (16-tick HIP graphs when no world has bodies), with the rigid
+//      bodies of every world stamped before and unstamped after each tick in
+//      one claim-round batch, the way game::tick brackets world::tick with the
+//      stamp and unstamp loops (game.cpp:1760-1815, 1838-1840, 1896-1983);
+//   4. completes requests whose tick count is reached.
+// A world's evolution depends only on its seed, its ticks and its bodies, so
+// its result is the same alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <future>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "fsg_internal.hpp"
+
+using namespace fsg_impl;
+
+extern "C" uint64_t fsg_now_ns(void) {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+namespace {
+
+// Host rigid body standing in for a Box2D body: the server steps a simple
+// kinematic model (gravity, wall bounces) because Box2D is not on the path;
+// only the pose crosses into the tick, as b2Body::GetPosition/GetAngle do.
+struct HostBody {
+    double x, y, vx, vy, ang, w;
+    int mw, mh;
+    uint32_t off;
+};
+
+struct Resident {
+    uint64_t id = 0;
+    fsg_world *w = nullptr;
+    bool busy = false;
+    uint64_t last_used = 0;
+    uint64_t checksum = 0;
+    bool checksum_known = false;
+    std::vector<HostBody> hb;
+    std::vector<fsg_body> poses;
+};
+
+struct Req {
+    uint64_t id = 0;
+    fsg_serve_request r{};
+    uint64_t t_submit = 0, t_admit = 0;
+    int done_ticks = 0;
+    bool cancelled = false;
+    Resident *res = nullptr;
+};
+
+struct Pending {  // a created world whose cells are being generated on the host
+    Req req;
+    std::future<std::vector<fsg_cell>> cells;
+};
+
+constexpr double kDt = 1.0 / 30.3;  // b2world->Step interval (world.cpp:2276 at maxTps 30)
+constexpr double kGravity = 20.0;   // b2World gravity (0, 20), world.cpp:147-148
+
+uint64_t world_bytes(int W, int H) {
+    const uint64_t tiles = (uint64_t)((W + fsg::TW - 1) / fsg::TW) * ((H + fsg::TH - 1) / fsg::TH);
+    return 2ull * W * H * 8 + 2 * tiles * 4 + 18 * 8;
+}
+
+// Deterministic bodies of a served world: boxes 4-64 px and convex polygons
+// of 3-8 vertices (rasterised), stone or wood, random poses in the upper
+// third, small random velocities.
+void make_bodies(uint64_t seed, int n, int W, int H, std::vector<HostBody> &hb, std::vector<fsg_cell> &px) {
+    uint64_t st = fsg_splitmix64(seed ^ 0xB0D1E5ull);
+    auto u01 = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        return (double)(fsg_splitmix64(st) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    hb.clear();
+    px.clear();
+    for (int i = 0; i < n; i++) {
+        HostBody b;
+        const bool box = u01() < 0.5;
+        b.mw = 4 + (int)(u01() * 61);
+        b.mh = box ? 4 + (int)(u01() * 61) : b.mw;
+        b.off = (uint32_t)px.size();
+        const uint16_t mat = u01() < 0.25 ? 7 : 1;  // wood or stone
+        std::vector<double> vx, vy;
+        if (!box) {
+            const int k = 3 + (int)(u01() * 6);
+            for (int v = 0; v < k; v++) {
+                const double a = 2.0 * M_PI * (v + 0.8 * u01()) / k;
+                vx.push_back(0.5 * b.mw * (1.0 + std::cos(a)));
+                vy.push_back(0.5 * b.mh * (1.0 + std::sin(a)));
+            }
+        }
+        for (int ty = 0; ty < b.mh; ty++)
+            for (int tx = 0; tx < b.mw; tx++) {
+                bool inside = true;
+                if (!box) {  // convex: the centre is on the same side of every edge
+                    const double cx = tx + 0.5, cy = ty + 0.5;
+                    const size_t k = vx.size();
+                    for (size_t e = 0; e < k && inside; e++) {
+                        const size_t f = (e + 1) % k;
+                        inside = (vx[f] - vx[e]) * (cy - vy[e]) - (vy[f] - vy[e]) * (cx - vx[e]) >= 0;
+                    }
+                }
+                fsg_cell c;
+                c.mat = inside ? mat : 0;
+                c.temp = 320;
+                c.rgba = inside ? (mat == 7 ? 0xFF2D5280u : 0xFF707070u) : 0u;
+                px.push_back(c);
+            }
+        b.x = u01() * W;
+        b.y = u01() * H / 3.0;
+        b.ang = u01() * 2.0 * M_PI;
+        b.vx = (u01() - 0.5) * 10.0;
+        b.vy = (u01() - 0.5) * 4.0;
+        b.w = (u01() - 0.5) * 2.0;
+        hb.push_back(b);
+    }
+}
+
+void poses_of(const std::vector<HostBody> &hb, std::vector<fsg_body> &out) {
+    out.resize(hb.size());
+    for (size_t i = 0; i < hb.size(); i++) {
+        const HostBody &b = hb[i];
+        fsg_body &p = out[i];
+        p.pos_x_q16 = (int32_t)std::llround(b.x * 65536.0);
+        p.pos_y_q16 = (int32_t)std::llround(b.y * 65536.0);
+        p.cos_q16 = (int32_t)std::llround(std::cos(b.ang) * 65536.0);
+        p.sin_q16 = (int32_t)std::llround(std::sin(b.ang) * 65536.0);
+        p.mat_w = b.mw;
+        p.mat_h = b.mh;
+        p.pixel_offset = b.off;
+        p.enabled = 1;
+    }
+}
+
+void step_bodies(std::vector<HostBody> &hb, int W, int H) {
+    for (HostBody &b : hb) {
+        b.vy += kGravity * kDt;
+        b.x += b.vx * kDt;
+        b.y += b.vy * kDt;
+        b.ang += b.w * kDt;
+        if (b.y > H - 8) {
+            b.y = H - 8;
+            b.vy = -0.5 * std::fabs(b.vy);
+        }
+        if (b.x < 0) {
+            b.x = 0;
+            b.vx = std::fabs(b.vx);
+        }
+        if (b.x > W - 1) {
+            b.x = W - 1;
+            b.vx = -std::fabs(b.vx);
+        }
+    }
+}
+
+}  // namespace
+
+struct fsg_server {
+    fsg_ctx *ctx = nullptr;
+    fsg_serve_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv, cv_done;
+    std::deque<Req> queue;                 // submitted, not admitted   (mu)
+    std::vector<fsg_serve_result> done;    // completed, not polled     (mu)
+    std::vector<uint64_t> releases;        // world ids to drop         (mu)
+    std::vector<uint64_t> corruptions;     // test hook requests        (mu)
+    uint64_t next_id = 1;
+    bool stop = false;
+    fsg_serve_stats st{};
+    // worker-owned
+    std::unordered_map<uint64_t, std::unique_ptr<Resident>> res;
+    std::list<Pending> pending;
+    std::atomic<size_t> n_pending{0};  // pending.size() for fsg_serve_stats_get
+    std::vector<Req> active;
+    fsg::StampWorld *d_table = nullptr;
+    int table_cap = 0;
+    std::vector<fsg::StampWorld> table;
+    std::thread th;
+};
+
+namespace {
+
+void finish(fsg_server *S, Req &q, int status, std::vector<fsg_serve_result> &out) {
+    fsg_serve_result r{};
+    r.request_id = q.id;
+    r.world_id = q.r.world_id;
+    r.status = status;
+    r.ticks_done = q.done_ticks;
+    r.t_submit_ns = q.t_submit;
+    r.t_admit_ns = q.t_admit;
+    r.t_done_ns = fsg_now_ns();
+    if (q.res && q.res->w) {
+        int W = 0, H = 0;
+        uint64_t t = 0;
+        fsg_world_info(q.res->w, &W, &H, &t);
+        r.tick = t;
+        r.cells = (uint64_t)W * H;
+        if (status == FSG_OK) {
+            fsg_stats s;
+            if (fsg_world_stats(q.res->w, &s) == FSG_OK) {
+                r.checksum = s.checksum;
+                q.res->checksum = s.checksum;
+                q.res->checksum_known = true;
+            } else {
+                r.status = FSG_ERR_DEVICE;
+            }
+        }
+        q.res->busy = false;
+        q.res->last_used = r.t_done_ns;
+    }
+    (void)S;
+    out.push_back(r);
+}
+
+void drop_world(fsg_server *S, uint64_t id) {
+    auto it = S->res.find(id);
+    if (it == S->res.end() || it->second->busy) return;
+    fsg_world_destroy(it->second->w);
+    S->res.erase(it);
+}
+
+// Frees idle resident worlds (least recently used first) until `need` bytes
+// fit the budget.  True if they fit.
+bool make_room(fsg_server *S, uint64_t need) {
+    fsg_ctx *c = S->ctx;
+    if (!c->budget) return true;
+    for (;;) {
+        uint64_t used = 0;
+        fsg_ctx_usage(c, &used);
+        if (used + need <= c->budget) return true;
+        Resident *lru = nullptr;
+        for (auto &kv : S->res)
+            if (!kv.second->busy && (!lru || kv.second->last_used < lru->last_used)) lru = kv.second.get();
+        if (!lru) return false;
+        drop_world(S, lru->id);
+        S->st.evictions++;
+    }
+}
+
+// Admission (called with S->mu held).
+void admit(fsg_server *S, std::vector<fsg_serve_result> &out) {
+    const uint64_t now = fsg_now_ns();
+    const size_t cap = (size_t)(S->cfg.max_active > 0 ? S->cfg.max_active : 1024);
+    std::vector<uint64_t> blocked;  // worlds with an earlier queued request still waiting
+    for (auto it = S->queue.begin(); it != S->queue.end();) {
+        Req &q = *it;
+        if (q.cancelled || (q.r.deadline_ns && now > q.r.deadline_ns)) {
+            finish(S, q, q.cancelled ? FSG_ERR_CANCELLED : FSG_ERR_TIMEOUT, out);
+            it = S->queue.erase(it);
+            continue;
+        }
+        const uint64_t wid = q.r.world_id;
+        if (std::find(blocked.begin(), blocked.end(), wid) != blocked.end() ||
+            S->active.size() + S->pending.size() >= cap) {
+            blocked.push_back(wid);
+            ++it;
+            continue;
+        }
+        auto rit = S->res.find(wid);
+        if (rit != S->res.end()) {
+            Resident *r = rit->second.get();
+            if (r->busy) {
+                blocked.push_back(wid);
+                ++it;
+                continue;
+            }
+            q.res = r;
+            r->busy = true;
+            if (S->cfg.verify_resident && r->checksum_known) {
+                fsg_stats s;
+                if (fsg_world_stats(r->w, &s) != FSG_OK || s.checksum != r->checksum) {
+                    // the resident state no longer matches what the last request left
+                    r->busy = false;
+                    q.res = nullptr;
+                    finish(S, q, FSG_ERR_CHECKSUM, out);
+                    drop_world(S, wid);
+                    it = S->queue.erase(it);
+                    continue;
+                }
+            }
+            q.t_admit = now;
+            S->active.push_back(q);
+            it = S->queue.erase(it);
+            continue;
+        }
+        // a new world
+        const fsg_serve_request &rq = q.r;
+        if (rq.width <= 0 || rq.height <= 0) {
+            finish(S, q, FSG_ERR_INVALID, out);
+            it = S->queue.erase(it);
+            continue;
+        }
+        const uint64_t need = world_bytes(rq.width, rq.height);
+        if (!make_room(S, need)) {
+            if (S->active.empty() && S->pending.empty()) {  // can never fit
+                finish(S, q, FSG_ERR_BUDGET, out);
+                it = S->queue.erase(it);
+                continue;
+            }
+            break;  // wait for running requests to free memory; keep arrival order
+        }
+        fsg_config cfg;
+        fsg_default_config(&cfg);
+        fsg_world *w = nullptr;
+        const int rc = fsg_world_create(S->ctx, rq.width, rq.height, rq.seed, &cfg, &w);
+        if (rc != FSG_OK) {
+            if (rc == FSG_ERR_BUDGET && !(S->active.empty() && S->pending.empty())) break;
+            finish(S, q, rc, out);
+            it = S->queue.erase(it);
+            continue;
+        }
+        auto r = std::make_unique<Resident>();
+        r->id = wid;
+        r->w = w;
+        r->busy = true;
+        q.res = r.get();
+        q.t_admit = now;
+        S->res[wid] = std::move(r);
+        Pending p;
+        p.req = q;
+        const int kind = rq.kind, W = rq.width, H = rq.height;
+        const uint64_t seed = rq.seed;
+        p.cells = std::async(std::launch::async, [kind, W, H, seed]() {
+            std::vector<fsg_cell> cells((size_t)W * H);
+            if (fsg_generate(kind, W, H, seed, cells.data()) != FSG_OK) cells.clear();
+            return cells;
+        });
+        S->pending.push_back(std::move(p));
+        S->n_pending++;
+        it = S->queue.erase(it);
+    }
+}
+
+// Uploads generated worlds whose host generation has finished.
+void settle_pending(fsg_server *S, std::vector<fsg_serve_result> &out) {
+    for (auto it = S->pending.begin(); it != S->pending.end();) {
+        if (it->cells.wait_for(std::chrono::seconds(0)) != std::future_status::ready) {
+            ++it;
+            continue;
+        }
+        Req q = it->req;
+        std::vector<fsg_cell> cells = it->cells.get();
+        it = S->pending.erase(it);
+        S->n_pending--;
+        Resident *r = q.res;
+        int rc = cells.empty() ? FSG_ERR_INVALID : fsg_world_upload(r->w, cells.data(), cells.size());
+        if (rc == FSG_OK && q.r.n_bodies > 0) {
+            std::vector<fsg_cell> px;
+            make_bodies(q.r.seed, std::min(q.r.n_bodies, 4096), q.r.width, q.r.height, r->hb, px);
+            poses_of(r->hb, r->poses);
+            rc = set_bodies(r->w, r->poses.data(), (int)r->poses.size(), px.data(), px.size());
+            if (rc == FSG_OK) rc = fsg_world_sync(r->w);  // px is a local
+        }
+        std::lock_guard<std::mutex> lk(S->mu);  // active is read by fsg_serve_cancel
+        if (rc != FSG_OK) {
+            finish(S, q, rc, out);
+            r->busy = false;
+            drop_world(S, r->id);
+            continue;
+        }
+        S->active.push_back(q);
+    }
+}
+
+int stamp_batch(fsg_server *S, const std::vector<Resident *> &bw, bool unstamp) {
+    fsg_ctx *c = S->ctx;
+    S->table.clear();
+    uint32_t total = 0;
+    for (Resident *r : bw) {
+        S->table.push_back(stamp_world_of(r->w, total));
+        total += (uint32_t)r->w->npix;
+    }
+    if ((int)S->table.size() > S->table_cap) {
+        dev_free(S->d_table);
+        S->d_table = nullptr;
+        void *p;
+        if (dev_alloc(c, &p, S->table.size() * sizeof(fsg::StampWorld))) return FSG_ERR_NOMEM;
+        S->d_table = (fsg::StampWorld *)p;
+        S->table_cap = (int)S->table.size();
+    }
+    HIPCHK(hipMemcpyAsync(S->d_table, S->table.data(), S->table.size() * sizeof(fsg::StampWorld),
+                          hipMemcpyHostToDevice, c->stream));
+    const int rc = stamp_rounds(c, S->d_table, (int)S->table.size(), total, bw[0]->w->d.mat, unstamp);
+    HIPCHK(hipStreamSynchronize(c->stream));  // the host table is reused next call
+    return rc;
+}
+
+// Advances every active request by T ticks.
+int run_step(fsg_server *S, int T) {
+    std::vector<fsg_world *> ws;
+    std::vector<Resident *> bw;  // worlds with bodies
+    for (Req &q : S->active) {
+        ws.push_back(q.res->w);
+        if (!q.res->hb.empty()) bw.push_back(q.res);
+    }
+    fsg_ctx *c = S->ctx;
+    if (!bw.empty()) {
+        for (Resident *r : bw) {
+            poses_of(r->hb, r->poses);
+            HIPCHK(hipMemcpyAsync(r->w->d_bodies, r->poses.data(), r->poses.size() * sizeof(fsg_body),
+                                  hipMemcpyHostToDevice, c->stream));
+        }
+        if (int rc = stamp_batch(S, bw, false)) return rc;
+    }
+    bool graph_ok = false;
+    if (int rc = batch_prepare(c, ws.data(), (int)ws.size(), &graph_ok)) return rc;
+    if (int rc = batch_run(c, ws.data(), (int)ws.size(), T, graph_ok)) return rc;
+    if (!bw.empty()) {
+        if (int rc = stamp_batch(S, bw, true)) return rc;
+        for (Resident *r : bw) step_bodies(r->hb, r->w->d.W, r->w->d.H);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FSG_OK;
+}
+
+void worker(fsg_server *S) {
+    for (;;) {
+        std::vector<fsg_serve_result> out;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (S->stop) break;
+            for (uint64_t id : S->releases) drop_world(S, id);
+            S->releases.clear();
+            for (uint64_t id : S->corruptions) {
+                auto it = S->res.find(id);
+                if (it != S->res.end() && !it->second->busy) {
+                    fsg_cell_edit e{};
+                    e.cell.mat = 2;
+                    e.cell.temp = 12345;
+                    fsg_world_set_cells(it->second->w, &e, 1);
+                }
+            }
+            S->corruptions.clear();
+            admit(S, out);
+            if (S->active.empty() && S->n_pending.load() == 0 && out.empty()) {
+                S->cv.wait_for(lk, std::chrono::milliseconds(2));
+                continue;
+            }
+        }
+        settle_pending(S, out);
+        if (!S->active.empty()) {
+            // cancelled or late active requests leave before the step
+            const uint64_t now = fsg_now_ns();
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                for (auto it = S->active.begin(); it != S->active.end();) {
+                    if (it->cancelled || (it->r.deadline_ns && now > it->r.deadline_ns)) {
+                        finish(S, *it, it->cancelled ? FSG_ERR_CANCELLED : FSG_ERR_TIMEOUT, out);
+                        it = S->active.erase(it);
+                    } else {
+                        ++it;
+                    }
+                }
+            }
+        }
+        if (!S->active.empty()) {
+            bool bodies = false;
+            int T = S->cfg.block_ticks > 0 ? S->cfg.block_ticks : 16;
+            for (Req &q : S->active) {
+                T = std::min(T, q.r.ticks - q.done_ticks);
+                bodies = bodies || !q.res->hb.empty();
+            }
+            if (bodies) T = 1;
+            const int rc = run_step(S, T);
+            uint64_t cu = 0;
+            std::lock_guard<std::mutex> lk(S->mu);
+            for (auto it = S->active.begin(); it != S->active.end();) {
+                if (rc != FSG_OK) {
+                    finish(S, *it, rc, out);
+                    it = S->active.erase(it);
+                    continue;
+                }
+                it->done_ticks += T;
+                cu += (uint64_t)T * it->res->w->d.W * it->res->w->d.H;
+                if (it->done_ticks >= it->r.ticks) {
+                    finish(S, *it, FSG_OK, out);
+                    it = S->active.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            S->st.steps++;
+            S->st.cell_updates += cu;
+        }
+        if (!out.empty()) {
+            std::lock_guard<std::mutex> lk(S->mu);
+            for (auto &r : out) {
+                if (r.status == FSG_OK) S->st.completed++;
+                else S->st.failed++;
+                S->done.push_back(r);
+            }
+            S->cv_done.notify_all();
+        }
+    }
+    // shutdown: fail what is left
+    std::vector<fsg_serve_result> out;
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (Req &q : S->active) finish(S, q, FSG_ERR_CANCELLED, out);
+    for (Req &q : S->queue) finish(S, q, FSG_ERR_CANCELLED, out);
+    S->active.clear();
+    S->queue.clear();
+    for (auto &p : S->pending) p.cells.wait();
+    S->pending.clear();
+    for (auto &r : out) S->done.push_back(r);
+}
+
+}  // namespace
+
+extern "C" int fsg_serve_create(fsg_ctx *ctx, const fsg_serve_config *cfg, fsg_server **out) {
+    if (!ctx || !out) return FSG_ERR_INVALID;
+    *out = nullptr;
+    fsg_server *S = new (std::nothrow) fsg_server();
+    if (!S) return FSG_ERR_NOMEM;
+    S->ctx = ctx;
+    if (cfg) S->cfg = *cfg;
+    if (S->cfg.block_ticks <= 0 || S->cfg.block_ticks > 16) S->cfg.block_ticks = 16;
+    S->th = std::thread(worker, S);
+    *out = S;
+    return FSG_OK;
+}
+
+extern "C" int fsg_serve_destroy(fsg_server *S) {
+    if (!S) return FSG_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv.notify_all();
+    S->th.join();
+    for (auto &kv : S->res) fsg_world_destroy(kv.second->w);
+    S->res.clear();
+    dev_free(S->d_table);
+    delete S;
+    return FSG_OK;
+}
+
+extern "C" int fsg_serve_submit(fsg_server *S, const fsg_serve_request *req, uint64_t *id) {
+    if (!S || !req || req->ticks < 1) return FSG_ERR_INVALID;
+    Req q;
+    q.r = *req;
+    q.t_submit = fsg_now_ns();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stop) return FSG_ERR_STATE;
+        q.id = S->next_id++;
+        S->queue.push_back(q);
+        S->st.submitted++;
+    }
+    S->cv.notify_all();
+    if (id) *id = q.id;
+    return FSG_OK;
+}
+
+extern "C" int fsg_serve_cancel(fsg_server *S, uint64_t id) {
+    if (!S) return FSG_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (Req &q : S->queue)
+        if (q.id == id) {
+            q.cancelled = true;
+            S->cv.notify_all();
+            return FSG_OK;
+        }
+    for (Req &q : S->active)  // read by the worker between steps under mu
+        if (q.id == id) {
+            q.cancelled = true;
+            return FSG_OK;
+        }
+    return FSG_ERR_NOTFOUND;
+}
+
+extern "C" int fsg_serve_poll(fsg_server *S, fsg_serve_result *out, int cap, int *n) {
+    if (!S || !n || cap < 0 || (cap && !out)) return FSG_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    const int k = std::min<int>(cap, (int)S->done.size());
+    for (int i = 0; i < k; i++) out[i] = S->done[i];
+    S->done.erase(S->done.begin(), S->done.begin() + k);
+    *n = k;
+    return FSG_OK;
+}
+
+extern "C" int fsg_serve_drain(fsg_server *S, uint64_t timeout_ns) {
+    if (!S) return FSG_ERR_INVALID;
+    const uint64_t end = fsg_now_ns() + timeout_ns;
+    std::unique_lock<std::mutex> lk(S->mu);
+    while (S->st.completed + S->st.failed < S->st.submitted) {
+        if (fsg_now_ns() > end) return FSG_ERR_TIMEOUT;
+        S->cv_done.wait_for(lk, std::chrono::milliseconds(5));
+    }
+    return FSG_OK;
+}
+
+extern "C" int fsg_serve_release(fsg_server *S, uint64_t world_id) {
+    if (!S) return FSG_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->releases.push_back(world_id);
+    S->cv.notify_all();
+    return FSG_OK;
+}
+
+extern "C" int fsg_serve_corrupt(fsg_server *S, uint64_t world_id) {
+    if (!S) return FSG_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->corruptions.push_back(world_id);
+    S->cv.notify_all();
+    return FSG_OK;
+}
+
+extern "C" int fsg_serve_stats_get(fsg_server *S, fsg_serve_stats *out) {
+    if (!S || !out) return FSG_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    *out = S->st;
+    out->queued = S->queue.size();
+    out->active = S->active.size() + S->n_pending.load();
+    out->resident_worlds = S->res.size();
+    uint64_t used = 0;
+    fsg_ctx_usage(S->ctx, &used);
+    out->resident_bytes = used;
+    return FSG_OK;
+}

@@ -236,6 +236,72 @@ int fsg_perworld_step_timed(fsg_ctx *ctx, fsg_world *const *worlds, int n, int t
                             double *elapsed_ms);
 
 /* ------------------------------------------------------------------------ */
+/* Serving: a request loop that keeps worlds resident in HBM and steps them  */
+/* in continuous batches.  The reference runs one world at a fixed 30 TPS    */
+/* from game::run (game.cpp:1009-1018); there is no request API to mirror,   */
+/* so these entry points have no reference counterpart.                      */
+/*                                                                           */
+/* A request names a world by a caller-chosen id.  An unknown id creates the */
+/* world (fsg_generate kind/size/seed, plus n_bodies rigid bodies whose      */
+/* poses the server steps on the host and stamps/unstamps every tick); a     */
+/* known id steps the resident world.  Requests are admitted while the HBM   */
+/* state budget of the context allows (idle resident worlds are evicted,     */
+/* least recently used first); all admitted requests advance together in one */
+/* batched launch per step, and new requests join between steps.  A world's */
+/* result does not depend on which other requests share its batch.          */
+/* ------------------------------------------------------------------------ */
+typedef struct fsg_server fsg_server;
+
+typedef struct fsg_serve_config {
+    int32_t max_active;      /* requests stepped together (0 = 1024)            */
+    int32_t block_ticks;     /* ticks per step when no active world has bodies  */
+    int32_t verify_resident; /* 1: check a resident world's checksum on reuse   */
+    int32_t gen_threads;     /* host threads generating new worlds (0 = 4)      */
+} fsg_serve_config;
+
+typedef struct fsg_serve_request {
+    uint64_t world_id;
+    int32_t kind;            /* fsg_generate kind for a new world               */
+    int32_t width, height;   /* new world size                                  */
+    int32_t ticks;           /* ticks to advance (>= 1)                         */
+    uint64_t seed;           /* new world seed                                  */
+    int32_t n_bodies;        /* new world: rigid bodies, 0..4096                */
+    int32_t reserved;
+    uint64_t deadline_ns;    /* fsg_now_ns() deadline; 0 = none                 */
+} fsg_serve_request;
+
+typedef struct fsg_serve_result {
+    uint64_t request_id, world_id;
+    int32_t status;          /* FSG_OK, FSG_ERR_BUDGET/NOMEM/TIMEOUT/CANCELLED/CHECKSUM */
+    int32_t ticks_done;
+    uint64_t tick;           /* world tick after the request                    */
+    uint64_t checksum;       /* grid checksum after the request (fsg_stats)     */
+    uint64_t cells;          /* cells of the world                              */
+    uint64_t t_submit_ns, t_admit_ns, t_done_ns;
+} fsg_serve_result;
+
+typedef struct fsg_serve_stats {
+    uint64_t submitted, completed, failed;
+    uint64_t queued, active, resident_worlds, resident_bytes;
+    uint64_t steps, cell_updates, evictions;
+} fsg_serve_stats;
+
+uint64_t fsg_now_ns(void);
+int fsg_serve_create(fsg_ctx *ctx, const fsg_serve_config *cfg, fsg_server **out);
+int fsg_serve_destroy(fsg_server *s);
+int fsg_serve_submit(fsg_server *s, const fsg_serve_request *req, uint64_t *request_id);
+int fsg_serve_cancel(fsg_server *s, uint64_t request_id);
+/* Completed results in completion order; *n = number written (<= cap). */
+int fsg_serve_poll(fsg_server *s, fsg_serve_result *out, int cap, int *n);
+/* Blocks until every submitted request has completed or timeout_ns passed. */
+int fsg_serve_drain(fsg_server *s, uint64_t timeout_ns);
+int fsg_serve_release(fsg_server *s, uint64_t world_id);
+int fsg_serve_stats_get(fsg_server *s, fsg_serve_stats *out);
+/* Test hook: overwrite one cell of an idle resident world, so that the next
+ * request on it fails the resident checksum check (verify_resident = 1). */
+int fsg_serve_corrupt(fsg_server *s, uint64_t world_id);
+
+/* ------------------------------------------------------------------------ */
 /* Host-side generator of the benchmark worlds (not on the device path).     */
 /* kind: 1 = basin terrain + sand discs (512^2 world), 2 = kind 1 + lava and  */
 /* oil discs, 3 = all-active striped world, 4 = kind 2 with disc counts       */

@@ -1,0 +1,87 @@
+"""Serving loop: resident worlds, continuous batching, admission by HBM
+budget, and failure paths forced from the host."""
+import numpy as np
+import pytest
+
+import falling_sand_engine_amd as fsg
+
+pytestmark = pytest.mark.gpu
+
+OK, BUDGET, TIMEOUT, CANCELLED, CHECKSUM = 0, -4, -7, -8, -6
+
+
+def run_alone(ctx, **req):
+    with fsg.Server(ctx) as s:
+        s.submit(**req)
+        assert s.drain(120) == 0
+        (r,) = s.poll()
+    return r
+
+
+def test_result_is_the_same_alone_and_in_the_mix(ctx):
+    a = dict(world_id=1, ticks=37, kind=4, width=512, height=384, seed=5, n_bodies=25)
+    alone = run_alone(ctx, **a)
+    assert alone["status"] == OK and alone["ticks_done"] == 37
+    with fsg.Server(ctx) as s:
+        s.submit(world_id=7, ticks=50, kind=4, width=640, height=320, seed=8, n_bodies=0)
+        s.submit(**a)
+        s.submit(world_id=9, ticks=3, kind=1, width=256, height=256, seed=2, n_bodies=10)
+        s.submit(world_id=7, ticks=20, kind=4, width=640, height=320, seed=8)   # steps the resident world 7
+        assert s.drain(120) == 0
+        res = {r["request_id"]: r for r in s.poll()}
+        st = s.stats()
+    assert all(r["status"] == OK for r in res.values())
+    mixed = [r for r in res.values() if r["world_id"] == 1][0]
+    assert mixed["checksum"] == alone["checksum"] and mixed["tick"] == 37
+    w7 = sorted([r for r in res.values() if r["world_id"] == 7], key=lambda r: r["request_id"])
+    assert [r["tick"] for r in w7] == [50, 70]
+    assert st["steps"] < 37 + 50 + 3 + 20   # requests shared steps
+
+
+def test_served_world_matches_direct_stepping(ctx):
+    r = run_alone(ctx, world_id=3, ticks=45, kind=4, width=384, height=256, seed=9)
+    with fsg.World(ctx, 384, 256, 9) as w:
+        w.upload(fsg.generate(4, 384, 256, 9))
+        w.tick(45)
+        assert w.stats().checksum == r["checksum"]
+
+
+def test_budget_admission_and_eviction():
+    c = fsg.Context(0, hbm_budget=12 * 1024 * 1024)   # 12 MiB of state
+    try:
+        with fsg.Server(c) as s:
+            for i in range(4):                          # 4 MiB each: the 4th forces an eviction
+                s.submit(world_id=100 + i, ticks=2, kind=1, width=512, height=512, seed=i)
+                assert s.drain(60) == 0
+            s.submit(world_id=999, ticks=1, kind=1, width=1024, height=1024, seed=1)  # 16 MiB: never fits
+            assert s.drain(60) == 0
+            res = s.poll()
+            st = s.stats()
+        assert [r["status"] for r in res] == [OK, OK, OK, OK, BUDGET]
+        assert st["evictions"] >= 1 and st["resident_bytes"] <= 12 * 1024 * 1024
+    finally:
+        c.close()
+
+
+def test_deadline_and_cancel(ctx):
+    with fsg.Server(ctx) as s:
+        s.submit(world_id=1, ticks=5, width=128, height=128, deadline_ns=1)   # already past
+        rid = s.submit(world_id=2, ticks=100000, width=256, height=256, seed=1)
+        import time
+        time.sleep(0.3)
+        assert s.cancel(rid) == 0
+        assert s.drain(60) == 0
+        res = {r["world_id"]: r for r in s.poll()}
+    assert res[1]["status"] == TIMEOUT
+    assert res[2]["status"] == CANCELLED and 0 < res[2]["ticks_done"] < 100000
+
+
+def test_corrupted_resident_state_is_detected(ctx):
+    with fsg.Server(ctx, verify_resident=True) as s:
+        s.submit(world_id=5, ticks=4, width=192, height=192, seed=3)
+        assert s.drain(60) == 0
+        s.corrupt(5)
+        s.submit(world_id=5, ticks=4, width=192, height=192, seed=3)
+        assert s.drain(60) == 0
+        res = s.poll()
+    assert [r["status"] for r in res] == [OK, CHECKSUM]
