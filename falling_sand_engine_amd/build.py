@@ -40,5 +40,20 @@ def build(verbose: bool = True) -> str:
     return LIB
 
 
+def build_cpp_tests(verbose: bool = True) -> str:
+    """The C++ facade test program (tests/cpp), linked against libfsg.so."""
+    src = os.path.join(ROOT, "tests", "cpp", "world_facade_test.cpp")
+    outdir = os.path.join(ROOT, "tests", "cpp", "build")
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "world_facade_test")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-L", LIBDIR, "-lfsg",
+           "-Wl,-rpath,$ORIGIN/../../../falling_sand_engine_amd/lib", "-o", out]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose="-q" not in sys.argv)
+    build_cpp_tests(verbose="-q" not in sys.argv)

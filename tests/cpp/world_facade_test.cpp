@@ -1,0 +1,35 @@
+// Drives the C++ facade (include/fsg_world.hpp) the way game::tick drives
+// ME::world: load a world, poke tiles with setTile, tick, read with getTile.
+// Prints one JSON line with the checksum and material counts.
+#include <cstdio>
+#include <vector>
+
+#include "fsg_world.hpp"
+
+int main() {
+    try {
+        fsg::context ctx(0);
+        fsg::world world(ctx);
+        const int W = 256, H = 192;
+        world.init(W, H, 7);
+        std::vector<fsg::MaterialInstance> cells((size_t)W * H);
+        fsg::check(fsg_generate(2, W, H, 7, cells.data()), "fsg_generate");
+        world.load(cells);
+        fsg::MaterialInstance sand{2, 320, 0xFF50C8E6u};
+        for (int x = 100; x < 110; x++) world.setTile(x, 5, sand);
+        world.setTile(-3, 5, sand);  // ignored, like world::setTile
+        world.tick(10);
+        const fsg::MaterialInstance wall = world.getTile(-1, 0);
+        const fsg::MaterialInstance t = world.getTile(50, 50);
+        const fsg_stats s = world.stats();
+        std::printf("{\"checksum\": %llu, \"wall\": %u, \"tile_50_50\": %llu, \"counts\": [",
+                    (unsigned long long)s.checksum, (unsigned)wall.mat,
+                    (unsigned long long)(t.mat | ((uint64_t)(uint16_t)t.temp << 16) | ((uint64_t)t.rgba << 32)));
+        for (int i = 0; i < 8; i++) std::printf("%llu%s", (unsigned long long)s.count[i], i < 7 ? ", " : "");
+        std::printf("]}\n");
+        return 0;
+    } catch (const fsg::error &e) {
+        std::fprintf(stderr, "fsg error: %s\n", e.what());
+        return 2;
+    }
+}
