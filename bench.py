@@ -26,6 +26,18 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE = 16
 
 
+def profiled_traffic(side):
+    """HBM bytes per tick launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE
+    (copy-kernel calibrated) in the committed profile summary, or None."""
+    path = os.path.join(ROOT, "profiles", "k_tick_world_16384_all_active.md")
+    if side != 16384 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -277,7 +289,9 @@ def main():
                                f"temperature on, {args.steps} ticks",
                    "cells": r["cells"], "cell_iter": 3, "chunk": "64x64", "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.side),
+                     "traffic_source": "profiles/k_tick_world_16384_all_active.md (rocprofv3 FETCH_SIZE x2.00 "
+                                       "+ WRITE_SIZE x1.00, bytes per launch)",
                      "kernel": "k_tick_world", "kernel_avg_ms": round(kernel_s * 1e3, 4)},
         "cpu_baseline": cpu,
         "secondary": secondary,
