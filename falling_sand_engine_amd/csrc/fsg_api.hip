@@ -102,6 +102,11 @@ MatTab build_mattab(const fsg_config &c) {
     return M;
 }
 
+void fill_pair_table(DevWorld &d) {
+    for (int it = 0; it < fsg::MAX_IT; it++)
+        for (uint32_t i = 0; i < 256; i++) d.pair[it][i] = (uint16_t)fsg::pair_code(d.mat, it, i >> 4, i & 15u);
+}
+
 bool cell_ok(const fsg_config &c, uint16_t mat) {
     return (mat & 0xFE00u) == 0 && (uint32_t)(mat & 0xFFu) < c.n_materials;
 }
@@ -285,6 +290,7 @@ extern "C" int fsg_world_create(fsg_ctx *c, int W, int H, uint64_t seed, const f
     w->d.skip = conf.chunk_skip ? 1 : 0;
     w->d.processed = w->d_stats + 17;
     w->d.mat = build_mattab(conf);
+    fill_pair_table(w->d);
     const uint64_t air = conf.mat_air;
     fsg::k_fill<<<grid_for(cells, 256), 256, 0, c->stream>>>(w->d.buf[0], cells, air);
     fsg::k_fill<<<grid_for(cells, 256), 256, 0, c->stream>>>(w->d.buf[1], cells, air);

@@ -47,6 +47,7 @@ struct DevWorld {
     uint32_t pad;
     unsigned long long *processed; // chunk updates run (skip mode)
     MatTab mat;
+    uint16_t pair[MAX_IT][256];  // pair_code(mat, it, t, u) at [it][t * 16 + u], built on the host
 };
 
 __host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {

@@ -14,6 +14,8 @@
 
 namespace fsg {
 
+static_assert(NT == 256, "one pair-table entry per thread and iteration");
+
 __constant__ int kOX[4] = {0, 1, 1, 0};
 __constant__ int kOY[4] = {0, 1, 0, 1};
 
@@ -59,7 +61,7 @@ __device__ __forceinline__ uint32_t band_bits(int x0, int x1, int y0, int y1) {
 // pair of cells of the LDS region (the chunk and its HALO ring) take part in
 // a rule under some random draw, can any cell change alone, or is any
 // temperature stencil of the chunk non-zero?
-__device__ bool tile_unstable(const MatTab &M, uint64_t (*L)[LW], const uint16_t *T0, bool temp_enabled) {
+__device__ bool tile_unstable(const MatTab &M, uint64_t (*L)[LW], const uint32_t *T0, bool temp_enabled) {
     bool u = false;
     constexpr uint32_t KV = K_SINK | K_GRISE | K_REACT, KD = K_SINK | K_GRISE;
     for (int i = threadIdx.x; i < LW * LH; i += NT) {
@@ -91,7 +93,7 @@ __device__ bool tile_unstable(const MatTab &M, uint64_t (*L)[LW], const uint16_t
 // last phase leaves the one ring the temperature stencil reads).  With
 // `track`, returns the band bits of chunk cells that changed.
 template <bool TRACK>
-__device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint64_t (*L)[LW], const uint16_t *T, int it,
+__device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint64_t (*L)[LW], const uint32_t *T, int it,
                                               uint64_t key, int ox, int oy, int need, int gx0, int gy0,
                                               bool has_wood) {
     // first/last block origin (chunk coordinates) with the phase's parity
@@ -159,7 +161,7 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
 }
 
 __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, int tile,
-                                            uint64_t (*L)[LW], uint16_t (*lut)[256], uint32_t *red) {
+                                            uint64_t (*L)[LW], uint32_t (*lut)[256], uint32_t *red) {
     const int tid = threadIdx.x;
     const MatTab &M = w.mat;
     const int tx = tile % w.tiles_x, ty = tile / w.tiles_x;
@@ -190,30 +192,64 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         prev_changed = forced || (self & F_ANY);
     }
 
+    // Every global load of the pair table and the tile is issued before the
+    // first LDS write, so a workgroup waits for one memory latency, not one
+    // per loop trip.
     const int ci = (int)M.cell_iter;
-    for (int i = tid; i < ci * 256; i += NT)
-        lut[i >> 8][i & 255] = (uint16_t)pair_code(M, i >> 8, (uint32_t)(i >> 4) & 15u, (uint32_t)i & 15u);
+    uint32_t pv[MAX_IT];
+#pragma unroll
+    for (int k = 0; k < MAX_IT; k++)
+        if (k < ci) pv[k] = w.pair[k][tid];
     bool wood = false;
     // A chunk whose halo lies inside an even-width world takes the fast
     // paths: 16-byte loads/stores with no bounds or wall checks.
     const bool interior = gx0 >= 0 && gy0 >= 0 && gx0 + LW <= w.W && gy0 + LH <= w.H && (w.W & 1) == 0;
     if (interior) {
+        constexpr int NV = LH * (LW / 2);        // 16-byte vectors in the tile
+        constexpr int KV = (NV + NT - 1) / NT;   // per thread
         const uint64_t *__restrict__ src = in + (size_t)gy0 * w.W + gx0;
-        for (int i = tid; i < LH * (LW / 2); i += NT) {
-            const int ly = i / (LW / 2), c2 = i - ly * (LW / 2);
-            const uint4 v = *reinterpret_cast<const uint4 *>(src + (size_t)__umul24(ly, w.W) + 2 * c2);
-            *reinterpret_cast<uint4 *>(&L[ly][2 * c2]) = v;
-            if (M.reactions) wood |= (v.x & 0xFFu) == M.wood || (v.z & 0xFFu) == M.wood;
+        uint4 v[KV];
+#pragma unroll
+        for (int k = 0; k < KV; k++) {
+            const int i = tid + k * NT;
+            if (k < KV - 1 || i < NV) {
+                const int ly = i / (LW / 2), c2 = i - ly * (LW / 2);
+                v[k] = *reinterpret_cast<const uint4 *>(src + (size_t)__umul24(ly, w.W) + 2 * c2);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KV; k++) {
+            const int i = tid + k * NT;
+            if (k < KV - 1 || i < NV) {
+                const int ly = i / (LW / 2), c2 = i - ly * (LW / 2);
+                *reinterpret_cast<uint4 *>(&L[ly][2 * c2]) = v[k];
+                wood |= (v[k].x & 0xFFu) == M.wood || (v[k].z & 0xFFu) == M.wood;
+            }
         }
     } else {
-        for (int i = tid; i < LW * LH; i += NT) {
+        constexpr int NC = LW * LH;
+        constexpr int KC = (NC + NT - 1) / NT;
+        uint64_t v[KC];
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            const int i = tid + k * NT;
             const int ly = i / LW, lx = i - ly * LW;
             const int gx = gx0 + lx, gy = gy0 + ly;
-            const uint64_t c = (gx >= 0 && gy >= 0 && gx < w.W && gy < w.H) ? in[(size_t)gy * w.W + gx] : WALL;
-            L[ly][lx] = c;
-            wood |= cbyte(c) == M.wood;
+            v[k] = ((k < KC - 1 || i < NC) && gx >= 0 && gy >= 0 && gx < w.W && gy < w.H)
+                       ? in[(size_t)gy * w.W + gx] : WALL;
+        }
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            const int i = tid + k * NT;
+            if (k < KC - 1 || i < NC) {
+                L[i / LW][i % LW] = v[k];
+                wood |= cbyte(v[k]) == M.wood;
+            }
         }
     }
+#pragma unroll
+    for (int k = 0; k < MAX_IT; k++)
+        if (k < ci) lut[k][tid] = pv[k];
     // burning wood is the one reaction the pair table cannot see
     const bool has_wood = __syncthreads_or(wood && M.reactions) != 0;
 
@@ -314,7 +350,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
 
 __global__ __launch_bounds__(NT) void k_tick_world(DevWorld w, uint32_t k) {
     __shared__ __align__(16) uint64_t L[LH][LW];
-    __shared__ uint16_t lut[MAX_IT][256];
+    __shared__ uint32_t lut[MAX_IT][256];
     __shared__ uint32_t red[NT / 64];
     tile_update(w, w.tick + k, blockIdx.x, L, lut, red);
 }
@@ -325,7 +361,7 @@ __global__ __launch_bounds__(NT) void k_tick_batch(const DevWorld *__restrict__ 
                                                    const uint32_t *__restrict__ tile_start,
                                                    int nworlds, uint32_t k) {
     __shared__ __align__(16) uint64_t L[LH][LW];
-    __shared__ uint16_t lut[MAX_IT][256];
+    __shared__ uint32_t lut[MAX_IT][256];
     __shared__ uint32_t red[NT / 64];
     const uint32_t gt = blockIdx.x;
     int lo = 0, hi = nworlds - 1;
