@@ -131,22 +131,41 @@ int launch_tick(fsg_world *w) {
 } // namespace
 
 extern "C" int fsg_rule_eval(const fsg_config *cfg, int it, uint32_t r, uint32_t s, uint64_t q[4], int form) {
-    if (!cfg || !q || validate_config(*cfg) || it < 0 || it >= (int)cfg->cell_iter) return FSG_ERR_INVALID;
+    if (!cfg || !q || it < 0 || it >= (int)cfg->cell_iter || validate_config(*cfg)) return FSG_ERR_INVALID;
     const MatTab M = build_mattab(*cfg);
     if (form == 0) return fsg::block_rule(M, it, r, s, q) ? 1 : 0;
     if (form != 1) return FSG_ERR_INVALID;
-    auto code = [&](uint64_t t, uint64_t u) { return fsg::pair_code(M, it, fsg::cid(t), fsg::cid(u)); };
-    const uint32_t VL = code(q[0], q[2]), VR = code(q[1], q[3]), DL = code(q[0], q[3]), DR = code(q[1], q[2]);
-    const uint32_t HT = code(q[0], q[1]), HTR = code(q[1], q[0]), HB = code(q[2], q[3]);
+    // The tick kernel's path: compact cells whose origins are the block's
+    // own cells, the pair-table decision or the staged rule for reacting
+    // blocks, then the colour of each cell taken from its origin (or the
+    // base colour of a reaction product).
+    uint32_t c[4];
+    for (int k = 0; k < 4; k++) c[k] = (uint32_t)q[k] | ((uint32_t)(k & 1) << 9) | ((uint32_t)(k >> 1) << 12);
+    auto code = [&](uint32_t t, uint32_t u) { return fsg::pair_code(M, it, fsg::cid(t), fsg::cid(u)); };
+    const uint32_t VL = code(c[0], c[2]), VR = code(c[1], c[3]), DL = code(c[0], c[3]), DR = code(c[1], c[2]);
+    const uint32_t HT = code(c[0], c[1]), HTR = code(c[1], c[0]), HB = code(c[2], c[3]);
     bool react = ((VL | VR) & fsg::K_REACT) != 0;
-    for (int i = 0; i < 4; i++)
-        react = react || (M.reactions && fsg::cbyte(q[i]) == M.wood && fsg::ctemp(q[i]) > M.burn);
-    if (react) return fsg::block_rule(M, it, r, s, q) ? 1 : 0;
-    const uint32_t movA = (q[0] >> 8) & 1u, movB = (q[1] >> 8) & 1u;
-    if (fsg::block_inert(VL, VR, DL, DR, HT, HB, movA, movB)) return 0;
-    const uint64_t o0 = q[0], o1 = q[1], o2 = q[2], o3 = q[3];
-    fsg::apply_decision(fsg::block_decide(VL, VR, DL, DR, HT, HTR, HB, movA, movB, r, s), q);
-    return (q[0] != o0 || q[1] != o1 || q[2] != o2 || q[3] != o3) ? 1 : 0;
+    for (int k = 0; k < 4; k++)
+        react = react || (M.reactions && fsg::cbyte(c[k]) == M.wood && fsg::ctemp(c[k]) > M.burn);
+    const uint32_t movA = (c[0] >> 8) & 1u, movB = (c[1] >> 8) & 1u;
+    if (react) {
+        fsg::block_rule(M, it, r, s, c);
+    } else {
+        if (fsg::block_inert(VL, VR, DL, DR, HT, HB, movA, movB)) return 0;
+        fsg::apply_decision(fsg::block_decide(VL, VR, DL, DR, HT, HTR, HB, movA, movB, r, s), c);
+    }
+    uint64_t o[4];
+    bool ch = false;
+    for (int k = 0; k < 4; k++) {
+        const uint32_t org = ((c[k] >> 9) & 1u) | (((c[k] >> 12) & 1u) << 1);
+        const uint32_t colour = (c[k] & fsg::C_XF) ? M.rgba[c[k] & 15u] : (uint32_t)(q[org] >> 32);
+        o[k] = (uint64_t)(c[k] & fsg::C_KEEP) | ((uint64_t)colour << 32);
+    }
+    for (int k = 0; k < 4; k++) {
+        ch = ch || o[k] != q[k];
+        q[k] = o[k];
+    }
+    return ch ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -72,9 +72,26 @@ __host__ __device__ __forceinline__ uint32_t cid(uint64_t c) { return (uint32_t)
 __host__ __device__ __forceinline__ uint32_t cbyte(uint64_t c) { return (uint32_t)c & 0xFFu; }
 __host__ __device__ __forceinline__ int ctemp(uint64_t c) { return (int)(int16_t)(uint16_t)(c >> 16); }
 
+// Compact cell, the form the tick kernel keeps in LDS: the stored cell's low
+// word, whose bits 9-15 are zero in every stored cell, with those bits reused:
+//   bits 0-7   material byte, bit 8 MOVED              (as stored)
+//   bits 9-11  origin x mod 8, bits 12-14 origin y mod 8 (LDS coordinates of
+//              the cell at the start of the tick; a cell moves at most one
+//              step per phase, so with |displacement| <= 3 the residues name
+//              the origin, and moves stay plain permutations)
+//   bit 15     XF: made by a reaction, so its colour is the material's base
+//              colour rather than the origin cell's
+//   bits 16-31 temperature                             (as stored)
+// The colour (high word) stays outside LDS until the chunk is written back.
+constexpr uint32_t C_ORG = 0x7E00u, C_XF = 0x8000u, C_KEEP = 0xFFFF01FFu;
+
 // Reaction product: material m, same temperature, base colour, flags clear.
 __host__ __device__ __forceinline__ uint64_t xform(const MatTab &M, uint64_t c, uint32_t m) {
     return (uint64_t)m | (c & 0xFFFF0000ull) | ((uint64_t)M.rgba[m & 15] << 32);
+}
+// The same on a compact cell: the colour is implied by XF.
+__host__ __device__ __forceinline__ uint32_t xform(const MatTab &, uint32_t c, uint32_t m) {
+    return m | (c & (0xFFFF0000u | C_ORG)) | C_XF;
 }
 
 // Block random words for the block whose top-left world cell is (x0, y0).
@@ -84,12 +101,13 @@ __host__ __device__ __forceinline__ void block_hash(uint64_t key, int x0, int y0
     s = lowbias32(r ^ (uint32_t)(key >> 32));
 }
 
-// One Margolus block, q = {TL, TR, BL, BR}.  Returns true if a cell changed.
-// Stages: R reactions, V vertical, D diagonal, F friction stop, H horizontal;
-// each stage restates one group of reference rules (world.cpp:1148-1889).
-__host__ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uint32_t r, uint32_t s,
-                                           uint64_t q[4]) {
-    const uint64_t o0 = q[0], o1 = q[1], o2 = q[2], o3 = q[3];
+// One Margolus block, q = {TL, TR, BL, BR} (stored cells, or compact cells
+// in the kernel).  Returns true if a cell changed.  Stages: R reactions, V
+// vertical, D diagonal, F friction stop, H horizontal; each stage restates
+// one group of reference rules (world.cpp:1148-1889).
+template <typename Cell>
+__host__ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uint32_t r, uint32_t s, Cell q[4]) {
+    const Cell o0 = q[0], o1 = q[1], o2 = q[2], o3 = q[3];
     const uint64_t pass = M.pass, sinkt = M.sink[it], flowt = M.flow[it];
     const uint32_t gasm = M.gas[it], airm = M.air, sandm = M.sand;
 
@@ -114,7 +132,7 @@ __host__ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uin
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     if (!done && bit(airm, cid(q[j]))) {
-                        const uint64_t t = q[i];
+                        const Cell t = q[i];
                         q[i] = q[j];
                         q[j] = t;
                         done = true;
@@ -131,8 +149,8 @@ __host__ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uin
 #pragma unroll
     for (int col = 0; col < 2; col++) {
         const int o = 1 - col;
-        uint64_t t = q[col];
-        const uint64_t u = q[col + 2];
+        Cell t = q[col];
+        const Cell u = q[col + 2];
         const uint32_t mt = cid(t), mu = cid(u);
         const int sr = nib(sinkt, mt);
         if (nib(pass, mu) < sr) {
@@ -175,12 +193,12 @@ __host__ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uin
             else cr = false;
         }
         if (cl) {
-            uint64_t a2 = q[0];
+            Cell a2 = q[0];
             const bool sand = bit(sandm, ma);
             if (sand && ((s >> 11) & 511u) == 0) a2 &= ~MOVED;
             if (sand && bit(sandm, cid(q[2])) && ((s >> 20) & 3u) == 0) q[2] |= MOVED;
             if (bit(airm, cid(q[1]))) {
-                const uint64_t side = q[1];
+                const Cell side = q[1];
                 q[1] = q[3];
                 q[0] = side;
             } else {
@@ -190,12 +208,12 @@ __host__ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uin
             return true;
         }
         if (cr) {
-            uint64_t b2 = q[1];
+            Cell b2 = q[1];
             const bool sand = bit(sandm, mb);
             if (sand && ((s >> 11) & 511u) == 0) b2 &= ~MOVED;
             if (sand && bit(sandm, cid(q[3])) && ((s >> 20) & 3u) == 0) q[3] |= MOVED;
             if (bit(airm, cid(q[0]))) {
-                const uint64_t side = q[0];
+                const Cell side = q[0];
                 q[0] = q[2];
                 q[1] = side;
             } else {
@@ -211,13 +229,13 @@ __host__ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uin
             else gr = false;
         }
         if (gl) {
-            const uint64_t t = q[2];
+            const Cell t = q[2];
             q[2] = q[1];
             q[1] = t;
             return true;
         }
         if (gr) {
-            const uint64_t t = q[3];
+            const Cell t = q[3];
             q[3] = q[0];
             q[0] = t;
             return true;
@@ -243,7 +261,7 @@ __host__ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uin
         const bool f = nib(pass, mr) < nib(flowt, ml) || nib(pass, ml) < nib(flowt, mr) ||
                        (bit(gasm, ml) && bit(airm, mr)) || (bit(gasm, mr) && bit(airm, ml));
         if (f && ((s >> (23 + row)) & 1u)) {
-            const uint64_t t = q[i];
+            const Cell t = q[i];
             q[i] = q[j];
             q[j] = t;
         }
@@ -369,8 +387,9 @@ __host__ __device__ __forceinline__ bool block_inert(uint32_t VL, uint32_t VR, u
 }
 
 // Host form of the decision's effect on q (the kernel scatters to LDS).
-__host__ __device__ __forceinline__ void apply_decision(uint32_t dec, uint64_t q[4]) {
-    uint64_t v[4];
+template <typename Cell>
+__host__ __device__ __forceinline__ void apply_decision(uint32_t dec, Cell q[4]) {
+    Cell v[4];
     for (int k = 0; k < 4; k++) {
         v[k] = q[k];
         if ((dec >> (8 + k)) & 1u) v[k] |= MOVED;

@@ -1,39 +1,32 @@
 // fsg_kernels.hip -- HIP kernels of the falling-sand tick for gfx950.
 //
 // k_tick_world / k_tick_batch: one workgroup updates one 64x64 chunk for one
-// whole tick.  It stages the chunk plus a 4-cell halo (72x72 cells, 41 KB) in
-// LDS with coalesced 8-byte loads, runs every Margolus phase of the tick and
-// the temperature step on the LDS copy (blocks are disjoint inside a phase,
-// so no atomics are needed; halo cells are recomputed redundantly and shrink
-// by one ring per phase), and writes the chunk interior once.  HBM traffic is
-// one read and one write of each cell per tick plus the halo re-read that the
-// L2/MALL mostly absorbs.  This replaces the reference's 16-thread chunk pool
-// (world.cpp:1057-1077) and its 3 sequential passes per chunk.
+// whole tick.  It stages the chunk plus a 4-cell halo (72x72 cells) in LDS as
+// 32-bit compact cells (the stored low word, with the cell's origin and a
+// reaction mark in its spare bits: fsg_device.hpp) and keeps the colours
+// (the high words) in registers.  It runs every Margolus phase of the tick
+// and the temperature step on the LDS copy (blocks are disjoint inside a
+// phase, so no atomics are needed; halo cells are recomputed redundantly and
+// shrink by one ring per phase), then lays the colours out in LDS, gathers
+// each output cell's colour from its origin and writes the chunk interior
+// once.  About 24 KB of LDS per workgroup keeps several workgroups resident
+// per CU, so one workgroup's loads and stores overlap the others' phases.
+// HBM traffic is one read and one write of each cell per tick plus the halo
+// re-read that the L2/MALL mostly absorbs.  This replaces the reference's
+// 16-thread chunk pool (world.cpp:1057-1077) and its 3 sequential passes per
+// chunk.
 #include "fsg_device.hpp"
 #include "fsg_kernels.hpp"
 
 namespace fsg {
 
 static_assert(NT == 256, "one pair-table entry per thread and iteration");
+static_assert(TW == 64 && TH == 64 && HALO == 4, "thread layout of the write-back");
 
 __constant__ int kOX[4] = {0, 1, 1, 0};
 __constant__ int kOY[4] = {0, 1, 0, 1};
 
-__device__ __forceinline__ uint64_t temp_step(const MatTab &M, uint64_t (*L)[LW], int ly, int lx) {
-    const uint64_t c = L[ly][lx];
-    const int T = ctemp(c);
-    const uint64_t nu = L[ly - 1][lx], nd = L[ly + 1][lx], nl = L[ly][lx - 1], nr = L[ly][lx + 1];
-    const int tu = cid(nu) == WALL ? T : ctemp(nu);
-    const int td = cid(nd) == WALL ? T : ctemp(nd);
-    const int tl = cid(nl) == WALL ? T : ctemp(nl);
-    const int tr = cid(nr) == WALL ? T : ctemp(nr);
-    int v = T + ((tu + td + tl + tr - 4 * T) >> 3);
-    if (M.any_add_temp) v += M.add_temp[cid(c)];
-    v = v > 32767 ? 32767 : (v < -32768 ? -32768 : v);
-    return (c & 0xFFFFFFFF0000FFFFull) | ((uint64_t)(uint16_t)v << 16);
-}
-
-__device__ __forceinline__ bool wood_hot(const MatTab &M, uint64_t c) {
+__device__ __forceinline__ bool wood_hot(const MatTab &M, uint32_t c) {
     return cbyte(c) == M.wood && ctemp(c) > M.burn;
 }
 
@@ -56,17 +49,38 @@ __device__ __forceinline__ uint32_t band_bits(int x0, int x1, int y0, int y1) {
            ((so && ea) ? F_SE : 0u);
 }
 
+// Temperature of a neighbour for the 5-point step: a WALL cell (outside the
+// world) counts as the cell itself.  WALLS = false where the region has none.
+template <bool WALLS>
+__device__ __forceinline__ int ntemp(int T, uint32_t n) {
+    return (WALLS && cid(n) == WALL) ? T : ctemp(n);
+}
+
+// New temperature of a cell with temperature T, material id m and the given
+// neighbour temperatures: integer diffusion with coefficient 1/8 and floor
+// rounding, plus the material's add_temp (addt, in LDS), clamped to int16.
+__device__ __forceinline__ int temp_of(const MatTab &M, const int32_t *addt, int T, uint32_t m, int tu, int td,
+                                       int tl, int tr) {
+    int v = T + ((tu + td + tl + tr - 4 * T) >> 3);
+    if (M.any_add_temp) {
+        v += addt[m];
+        v = min(max(v, -32768), 32767);
+    }
+    return v;
+}
+
 // Dirty-rect predicate over the staged tile, from the pair table of
 // iteration 0 (where every material with iterations > 0 is active): can any
 // pair of cells of the LDS region (the chunk and its HALO ring) take part in
 // a rule under some random draw, can any cell change alone, or is any
 // temperature stencil of the chunk non-zero?
-__device__ bool tile_unstable(const MatTab &M, uint64_t (*L)[LW], const uint32_t *T0, bool temp_enabled) {
+__device__ bool tile_unstable(const MatTab &M, const uint32_t (*L)[LW], const uint32_t *T0, const int32_t *addt,
+                              bool temp_enabled) {
     bool u = false;
     constexpr uint32_t KV = K_SINK | K_GRISE | K_REACT, KD = K_SINK | K_GRISE;
     for (int i = threadIdx.x; i < LW * LH; i += NT) {
         const int ly = i / LW, lx = i - ly * LW;
-        const uint64_t c = L[ly][lx];
+        const uint32_t c = L[ly][lx];
         const uint32_t t = cid(c) << 4;
         u |= single_unstable(M, c);
         if (lx + 1 < LW) u |= (T0[t | cid(L[ly][lx + 1])] & K_HSWAP) != 0;
@@ -79,9 +93,12 @@ __device__ bool tile_unstable(const MatTab &M, uint64_t (*L)[LW], const uint32_t
     if (temp_enabled) {
         for (int i = threadIdx.x; i < TW * TH; i += NT) {
             const int ly = HALO + i / TW, lx = HALO + (i & (TW - 1));
-            const uint64_t c = L[ly][lx];
+            const uint32_t c = L[ly][lx];
             if (cid(c) == WALL) continue;
-            u |= temp_step(M, L, ly, lx) != c;
+            const int T = ctemp(c);
+            const int v = temp_of(M, addt, T, cid(c), ntemp<true>(T, L[ly - 1][lx]), ntemp<true>(T, L[ly + 1][lx]),
+                                  ntemp<true>(T, L[ly][lx - 1]), ntemp<true>(T, L[ly][lx + 1]));
+            u |= v != T;
         }
     }
     return u;
@@ -93,7 +110,7 @@ __device__ bool tile_unstable(const MatTab &M, uint64_t (*L)[LW], const uint32_t
 // last phase leaves the one ring the temperature stencil reads).  With
 // `track`, returns the band bits of chunk cells that changed.
 template <bool TRACK>
-__device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint64_t (*L)[LW], const uint32_t *T, int it,
+__device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW], const uint32_t *T, int it,
                                               uint64_t key, int ox, int oy, int need, int gx0, int gy0,
                                               bool has_wood) {
     // first/last block origin (chunk coordinates) with the phase's parity
@@ -108,14 +125,13 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint64_t (*L)[LW]
         const int by = (int)(__umul24((uint32_t)b, magic) >> 16);
         const int bx = b - (int)__umul24((uint32_t)by, (uint32_t)nbx);
         const int lx = lx0 + 2 * bx, ly = ly0 + 2 * by;
-        uint64_t *p0 = &L[0][0] + __umul24((uint32_t)ly, (uint32_t)LW) + lx;
-        uint64_t q[4] = {p0[0], p0[1], p0[LW], p0[LW + 1]};
-        const uint32_t i0 = (uint32_t)q[0] & 15u, i1 = (uint32_t)q[1] & 15u;
-        const uint32_t i2 = (uint32_t)q[2] & 15u, i3 = (uint32_t)q[3] & 15u;
+        uint32_t *p0 = &L[0][0] + __umul24((uint32_t)ly, (uint32_t)LW) + lx;
+        uint32_t q[4] = {p0[0], p0[1], p0[LW], p0[LW + 1]};
+        const uint32_t i0 = q[0] & 15u, i1 = q[1] & 15u, i2 = q[2] & 15u, i3 = q[3] & 15u;
         const uint32_t VL = T[(i0 << 4) | i2], VR = T[(i1 << 4) | i3];
         const uint32_t DL = T[(i0 << 4) | i3], DR = T[(i1 << 4) | i2];
         const uint32_t HT = T[(i0 << 4) | i1], HB = T[(i2 << 4) | i3];
-        const uint32_t movA = ((uint32_t)q[0] >> 8) & 1u, movB = ((uint32_t)q[1] >> 8) & 1u;
+        const uint32_t movA = (q[0] >> 8) & 1u, movB = (q[1] >> 8) & 1u;
         bool hot = false;
         if (has_wood)
             hot = wood_hot(M, q[0]) || wood_hot(M, q[1]) || wood_hot(M, q[2]) || wood_hot(M, q[3]);
@@ -137,9 +153,9 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint64_t (*L)[LW]
             if (dec != 0) {
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    uint64_t v = q[k];
-                    if ((dec >> (8 + k)) & 1u) v |= MOVED;
-                    if ((dec >> (12 + k)) & 1u) v &= ~MOVED;
+                    uint32_t v = q[k];
+                    if ((dec >> (8 + k)) & 1u) v |= (uint32_t)MOVED;
+                    if ((dec >> (12 + k)) & 1u) v &= ~(uint32_t)MOVED;
                     const uint32_t d = (uint32_t)k ^ ((dec >> (2 * k)) & 3u);
                     p0[(d & 1u) + (d >> 1) * LW] = v;
                 }
@@ -160,8 +176,51 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
     return v;
 }
 
-__device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, int tile,
-                                            uint64_t (*L)[LW], uint32_t (*lut)[256], uint32_t *red) {
+// Final compact words of a thread's 2 x 8 output cells (columns lx, lx + 1,
+// rows ly0 .. ly0 + 7), with the temperature step applied when temp_now; the
+// stencil slides down the strip.  Returns whether a temperature changed.
+template <bool WALLS>
+__device__ __forceinline__ bool final_words(const MatTab &M, const uint32_t (*L)[LW], const int32_t *addt,
+                                            bool temp_now, int lx, int ly0, uint32_t (&fw)[2 * (TH / 8)]) {
+    bool tchg = false;
+    if (!temp_now) {
+#pragma unroll
+        for (int r = 0; r < TH / 8; r++) {
+            const uint2 c = *reinterpret_cast<const uint2 *>(&L[ly0 + r][lx]);
+            fw[2 * r] = c.x;
+            fw[2 * r + 1] = c.y;
+        }
+        return false;
+    }
+    uint2 up = *reinterpret_cast<const uint2 *>(&L[ly0 - 1][lx]);
+    uint2 cur = *reinterpret_cast<const uint2 *>(&L[ly0][lx]);
+#pragma unroll
+    for (int r = 0; r < TH / 8; r++) {
+        const uint2 dn = *reinterpret_cast<const uint2 *>(&L[ly0 + r + 1][lx]);
+        const uint32_t cl = L[ly0 + r][lx - 1], cr = L[ly0 + r][lx + 2];
+        const int t0 = ctemp(cur.x), t1 = ctemp(cur.y);
+        int v0 = t0, v1 = t1;
+        if (!WALLS || cid(cur.x) != WALL)
+            v0 = temp_of(M, addt, t0, cid(cur.x), ntemp<WALLS>(t0, up.x), ntemp<WALLS>(t0, dn.x),
+                         ntemp<WALLS>(t0, cl), ntemp<WALLS>(t0, cur.y));
+        if (!WALLS || cid(cur.y) != WALL)
+            v1 = temp_of(M, addt, t1, cid(cur.y), ntemp<WALLS>(t1, up.y), ntemp<WALLS>(t1, dn.y),
+                         ntemp<WALLS>(t1, cur.x), ntemp<WALLS>(t1, cr));
+        tchg |= (v0 != t0) | (v1 != t1);
+        fw[2 * r] = (cur.x & 0xFFFFu) | ((uint32_t)v0 << 16);
+        fw[2 * r + 1] = (cur.y & 0xFFFFu) | ((uint32_t)v1 << 16);
+        up = cur;
+        cur = dn;
+    }
+    return tchg;
+}
+
+constexpr int NV = LH * (LW / 2);       // 2-cell pairs in the LDS region
+constexpr int KV = (NV + NT - 1) / NT;  // pairs per thread
+
+__device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, int tile, uint32_t (*L)[LW],
+                                            uint32_t (*lut)[256], uint32_t *rgba, int32_t *addt,
+                                            uint32_t *red) {
     const int tid = threadIdx.x;
     const MatTab &M = w.mat;
     const int tx = tile % w.tiles_x, ty = tile / w.tiles_x;
@@ -192,71 +251,76 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         prev_changed = forced || (self & F_ANY);
     }
 
-    // Every global load of the pair table and the tile is issued before the
-    // first LDS write, so a workgroup waits for one memory latency, not one
-    // per loop trip.
+    // Every global load (pair table, colour and add_temp tables, the region
+    // as 16-byte cell pairs) is issued before the first LDS write, so a
+    // workgroup waits for one memory latency, not one per loop trip.
     const int ci = (int)M.cell_iter;
     uint32_t pv[MAX_IT];
 #pragma unroll
     for (int k = 0; k < MAX_IT; k++)
         if (k < ci) pv[k] = w.pair[k][tid];
-    bool wood = false;
+    uint32_t mrgba = 0;
+    int32_t madd = 0;
+    if (tid < 16) {
+        mrgba = M.rgba[tid];
+        madd = M.add_temp[tid];
+    }
     // A chunk whose halo lies inside an even-width world takes the fast
     // paths: 16-byte loads/stores with no bounds or wall checks.
     const bool interior = gx0 >= 0 && gy0 >= 0 && gx0 + LW <= w.W && gy0 + LH <= w.H && (w.W & 1) == 0;
+    uint4 v[KV];
     if (interior) {
-        constexpr int NV = LH * (LW / 2);        // 16-byte vectors in the tile
-        constexpr int KV = (NV + NT - 1) / NT;   // per thread
         const uint64_t *__restrict__ src = in + (size_t)gy0 * w.W + gx0;
-        uint4 v[KV];
 #pragma unroll
         for (int k = 0; k < KV; k++) {
-            const int i = tid + k * NT;
-            if (k < KV - 1 || i < NV) {
-                const int ly = i / (LW / 2), c2 = i - ly * (LW / 2);
+            const int j = tid + k * NT;
+            if (k < KV - 1 || j < NV) {
+                const int ly = j / (LW / 2), c2 = j - ly * (LW / 2);
                 v[k] = *reinterpret_cast<const uint4 *>(src + (size_t)__umul24(ly, w.W) + 2 * c2);
             }
         }
+    } else {
 #pragma unroll
         for (int k = 0; k < KV; k++) {
-            const int i = tid + k * NT;
-            if (k < KV - 1 || i < NV) {
-                const int ly = i / (LW / 2), c2 = i - ly * (LW / 2);
-                *reinterpret_cast<uint4 *>(&L[ly][2 * c2]) = v[k];
-                wood |= (v[k].x & 0xFFu) == M.wood || (v[k].z & 0xFFu) == M.wood;
+            const int j = tid + k * NT;
+            if (k < KV - 1 || j < NV) {
+                const int ly = j / (LW / 2), gx = gx0 + 2 * (j - ly * (LW / 2)), gy = gy0 + ly;
+                const bool rin = gy >= 0 && gy < w.H;
+                const int64_t row = (int64_t)gy * w.W;
+                const uint64_t a = (rin && gx >= 0 && gx < w.W) ? in[row + gx] : WALL;
+                const uint64_t b = (rin && gx + 1 >= 0 && gx + 1 < w.W) ? in[row + gx + 1] : WALL;
+                v[k] = make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
             }
         }
-    } else {
-        constexpr int NC = LW * LH;
-        constexpr int KC = (NC + NT - 1) / NT;
-        uint64_t v[KC];
+    }
+    // compact cells into LDS, colours kept in registers until the write-back
+    uint2 col[KV];
+    bool wood = false;
 #pragma unroll
-        for (int k = 0; k < KC; k++) {
-            const int i = tid + k * NT;
-            const int ly = i / LW, lx = i - ly * LW;
-            const int gx = gx0 + lx, gy = gy0 + ly;
-            v[k] = ((k < KC - 1 || i < NC) && gx >= 0 && gy >= 0 && gx < w.W && gy < w.H)
-                       ? in[(size_t)gy * w.W + gx] : WALL;
-        }
-#pragma unroll
-        for (int k = 0; k < KC; k++) {
-            const int i = tid + k * NT;
-            if (k < KC - 1 || i < NC) {
-                L[i / LW][i % LW] = v[k];
-                wood |= cbyte(v[k]) == M.wood;
-            }
+    for (int k = 0; k < KV; k++) {
+        const int j = tid + k * NT;
+        if (k < KV - 1 || j < NV) {
+            const int ly = j / (LW / 2), c2 = j - ly * (LW / 2);
+            const uint32_t org = (((uint32_t)(2 * c2) & 7u) << 9) | (((uint32_t)ly & 7u) << 12);
+            *reinterpret_cast<uint2 *>(&L[ly][2 * c2]) = make_uint2(v[k].x | org, v[k].z | org | (1u << 9));
+            col[k] = make_uint2(v[k].y, v[k].w);
+            wood |= (v[k].x & 0xFFu) == M.wood || (v[k].z & 0xFFu) == M.wood;
         }
     }
 #pragma unroll
     for (int k = 0; k < MAX_IT; k++)
         if (k < ci) lut[k][tid] = pv[k];
+    if (tid < 16) {
+        rgba[tid] = mrgba;
+        addt[tid] = madd;
+    }
     // burning wood is the one reaction the pair table cannot see
     const bool has_wood = __syncthreads_or(wood && M.reactions) != 0;
 
     if (skip && !prev_changed) {
         // the chunk did not change last tick: if nothing in its region can
         // change now it keeps its state, and both buffers already hold it
-        const bool pot = __syncthreads_or(tile_unstable(M, L, lut[0], M.temp_every != 0)) != 0;
+        const bool pot = __syncthreads_or(tile_unstable(M, L, lut[0], addt, M.temp_every != 0)) != 0;
         if (!pot) {
             if (tid == 0) {
                 w.flags[tick & 1][tile] = 0;
@@ -281,58 +345,51 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         }
     }
 
+    // Write-back.  Thread = 2-column strip x 8 rows of the chunk.
     const bool temp_now = M.temp_every != 0 && (tick % M.temp_every) == M.temp_phase;
+    const int j = tid & 31, rg = tid >> 5;
+    const int lx = HALO + 2 * j, ly0 = HALO + rg * (TH / 8);
+    uint32_t fw[2 * (TH / 8)];
+    const bool tchg = interior ? final_words<false>(M, L, addt, temp_now, lx, ly0, fw)
+                               : final_words<true>(M, L, addt, temp_now, lx, ly0, fw);
+    if (skip && tchg) bands |= band_bits(2 * j, 2 * j + 1, rg * (TH / 8), rg * (TH / 8) + TH / 8 - 1);
+    __syncthreads();
+    // the colour plane: each cell's colour at its origin
+#pragma unroll
+    for (int k = 0; k < KV; k++) {
+        const int jj = tid + k * NT;
+        if (k < KV - 1 || jj < NV) {
+            const int ly = jj / (LW / 2), c2 = jj - ly * (LW / 2);
+            *reinterpret_cast<uint2 *>(&L[ly][2 * c2]) = col[k];
+        }
+    }
+    __syncthreads();
+    auto colour = [&](uint32_t wd, int x, int y) -> uint32_t {
+        const int dx = (int)(((wd >> 9) - (uint32_t)x + 4u) & 7u) - 4;
+        const int dy = (int)(((wd >> 12) - (uint32_t)y + 4u) & 7u) - 4;
+        return (wd & C_XF) ? rgba[wd & 15u] : L[y + dy][x + dx];
+    };
     if (interior) {
-        // thread = 2-column strip x 8 rows; the stencil slides down the strip
-        const int j = tid & 31, rg = tid >> 5;
-        const int lx = HALO + 2 * j, ly0 = HALO + rg * (TH / 8);
         uint64_t *__restrict__ dst = out + (size_t)(gy0 + ly0) * w.W + gx0 + lx;
-        if (!temp_now) {
 #pragma unroll
-            for (int r = 0; r < TH / 8; r++)
-                *reinterpret_cast<uint4 *>(dst + (size_t)__umul24(r, w.W)) =
-                    *reinterpret_cast<const uint4 *>(&L[ly0 + r][lx]);
-        } else {
-            uint4 up = *reinterpret_cast<const uint4 *>(&L[ly0 - 1][lx]);
-            uint4 cur = *reinterpret_cast<const uint4 *>(&L[ly0][lx]);
-            int tu0 = (int)up.x >> 16, tu1 = (int)up.z >> 16;
-            bool tchg = false;
-#pragma unroll
-            for (int r = 0; r < TH / 8; r++) {
-                const uint4 dn = *reinterpret_cast<const uint4 *>(&L[ly0 + r + 1][lx]);
-                const int tl = (int)(uint32_t)L[ly0 + r][lx - 1] >> 16;
-                const int tr = (int)(uint32_t)L[ly0 + r][lx + 2] >> 16;
-                const int t0 = (int)cur.x >> 16, t1 = (int)cur.z >> 16;
-                const int td0 = (int)dn.x >> 16, td1 = (int)dn.z >> 16;
-                int v0 = t0 + ((tu0 + td0 + tl + t1 - 4 * t0) >> 3);
-                int v1 = t1 + ((tu1 + td1 + t0 + tr - 4 * t1) >> 3);
-                if (M.any_add_temp) {
-                    v0 += M.add_temp[cur.x & 15u];
-                    v1 += M.add_temp[cur.z & 15u];
-                }
-                v0 = min(max(v0, -32768), 32767);
-                v1 = min(max(v1, -32768), 32767);
-                tchg |= (v0 != t0) | (v1 != t1);
-                uint4 o = cur;
-                o.x = (cur.x & 0xFFFFu) | ((uint32_t)v0 << 16);
-                o.z = (cur.z & 0xFFFFu) | ((uint32_t)v1 << 16);
-                *reinterpret_cast<uint4 *>(dst + (size_t)__umul24(r, w.W)) = o;
-                tu0 = t0;
-                tu1 = t1;
-                cur = dn;
-            }
-            if (skip && tchg)
-                bands |= band_bits(2 * j, 2 * j + 1, rg * (TH / 8), rg * (TH / 8) + TH / 8 - 1);
+        for (int r = 0; r < TH / 8; r++) {
+            const uint32_t a = fw[2 * r], b = fw[2 * r + 1];
+            const uint32_t ca = colour(a, lx, ly0 + r), cb = colour(b, lx + 1, ly0 + r);
+            *reinterpret_cast<uint4 *>(dst + (size_t)__umul24(r, w.W)) =
+                make_uint4(a & C_KEEP, ca, b & C_KEEP, cb);
         }
     } else {
-        for (int i = tid; i < TW * TH; i += NT) {
-            const int ly = HALO + i / TW, lx = HALO + (i & (TW - 1));
-            const int gx = gx0 + lx, gy = gy0 + ly;
-            if (gx >= w.W || gy >= w.H) continue;
-            const uint64_t c0 = L[ly][lx];
-            const uint64_t c = temp_now ? temp_step(M, L, ly, lx) : c0;
-            out[(size_t)gy * w.W + gx] = c;
-            if (skip && c != c0) bands |= band_bits(lx - HALO, lx - HALO, ly - HALO, ly - HALO);
+#pragma unroll
+        for (int r = 0; r < TH / 8; r++) {
+            const int gy = gy0 + ly0 + r;
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int gx = gx0 + lx + c;
+                if (gx < w.W && gy < w.H) {
+                    const uint32_t a = fw[2 * r + c];
+                    out[(size_t)gy * w.W + gx] = (uint64_t)(a & C_KEEP) | ((uint64_t)colour(a, lx + c, ly0 + r) << 32);
+                }
+            }
         }
     }
     if (skip) {
@@ -348,20 +405,25 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     }
 }
 
-__global__ __launch_bounds__(NT) void k_tick_world(DevWorld w, uint32_t k) {
-    __shared__ __align__(16) uint64_t L[LH][LW];
+// Five workgroups per CU (95 VGPRs, 24 KB of LDS each).
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_tick_world(DevWorld w, uint32_t k) {
+    __shared__ __align__(16) uint32_t L[LH][LW];
     __shared__ uint32_t lut[MAX_IT][256];
+    __shared__ uint32_t rgba[16];
+    __shared__ int32_t addt[16];
     __shared__ uint32_t red[NT / 64];
-    tile_update(w, w.tick + k, blockIdx.x, L, lut, red);
+    tile_update(w, w.tick + k, blockIdx.x, L, lut, rgba, addt, red);
 }
 
 // Many worlds in one launch: tile_start[i] is the first global chunk index of
 // world i (prefix sums, tile_start[n] = total).
-__global__ __launch_bounds__(NT) void k_tick_batch(const DevWorld *__restrict__ worlds,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_tick_batch(const DevWorld *__restrict__ worlds,
                                                    const uint32_t *__restrict__ tile_start,
                                                    int nworlds, uint32_t k) {
-    __shared__ __align__(16) uint64_t L[LH][LW];
+    __shared__ __align__(16) uint32_t L[LH][LW];
     __shared__ uint32_t lut[MAX_IT][256];
+    __shared__ uint32_t rgba[16];
+    __shared__ int32_t addt[16];
     __shared__ uint32_t red[NT / 64];
     const uint32_t gt = blockIdx.x;
     int lo = 0, hi = nworlds - 1;
@@ -371,7 +433,7 @@ __global__ __launch_bounds__(NT) void k_tick_batch(const DevWorld *__restrict__ 
         else hi = mid - 1;
     }
     const DevWorld &w = worlds[lo];
-    tile_update(w, w.tick + k, (int)(gt - tile_start[lo]), L, lut, red);
+    tile_update(w, w.tick + k, (int)(gt - tile_start[lo]), L, lut, rgba, addt, red);
 }
 
 __global__ void k_advance(DevWorld *worlds, int nworlds, uint32_t ticks) {
