@@ -104,40 +104,73 @@ __device__ bool tile_unstable(const MatTab &M, const uint32_t (*L)[LW], const ui
     return u;
 }
 
+// Blocks a wave handles in one phase: at most ceil(36 * 36 / NT) rounds of 64.
+constexpr int QCAP = ((36 * 36 + NT - 1) / NT) * 64;
+
 // One Margolus phase over the blocks whose cells the rest of the tick still
 // reads: after this phase the tick needs `need` more rings around the chunk
 // (need = cell_iter - it, so the region shrinks by one ring per phase and the
-// last phase leaves the one ring the temperature stencil reads).  With
-// `track`, returns the band bits of chunk cells that changed.
+// last phase leaves the one ring the temperature stencil reads).  Two passes
+// per wave: the first tests every block against the pair table and queues
+// the ones some rule may change (ballot + mbcnt into the wave's LDS queue Q);
+// the second runs the random draw and the rule on the queued blocks only, so
+// no lane idles through the decision for an inert block.  Blocks of a phase
+// are disjoint and a wave's LDS accesses stay in order, so the passes need
+// no barrier.  With TRACK, returns the band bits of chunk cells that changed.
 template <bool TRACK>
 __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW], const uint32_t *T, int it,
                                               uint64_t key, int ox, int oy, int need, int gx0, int gy0,
-                                              bool has_wood) {
+                                              bool has_wood, uint16_t *queue) {
     // first/last block origin (chunk coordinates) with the phase's parity
     const int x0f = -need - ((need ^ ox) & 1), y0f = -need - ((need ^ oy) & 1);
     const int x0l = (TW - 1 + need) - (((TW - 1 + need) ^ ox) & 1);
     const int y0l = (TH - 1 + need) - (((TH - 1 + need) ^ oy) & 1);
     const int nbx = ((x0l - x0f) >> 1) + 1, nby = ((y0l - y0f) >> 1) + 1;
+    const int nb = nbx * nby;
     const uint32_t magic = (65536u + (uint32_t)nbx - 1u) / (uint32_t)nbx;  // b / nbx for b < 2^16/nbx
     const int lx0 = HALO + x0f, ly0 = HALO + y0f;
-    uint32_t bands = 0;
-    for (int b = threadIdx.x; b < nbx * nby; b += NT) {
+    auto block_at = [&](int b) -> uint32_t * {
         const int by = (int)(__umul24((uint32_t)b, magic) >> 16);
         const int bx = b - (int)__umul24((uint32_t)by, (uint32_t)nbx);
-        const int lx = lx0 + 2 * bx, ly = ly0 + 2 * by;
-        uint32_t *p0 = &L[0][0] + __umul24((uint32_t)ly, (uint32_t)LW) + lx;
+        return &L[0][0] + __umul24((uint32_t)(ly0 + 2 * by), (uint32_t)LW) + lx0 + 2 * bx;
+    };
+    const int lane = threadIdx.x & 63;
+    uint16_t *Q = queue + (threadIdx.x >> 6) * QCAP;
+    int nq = 0;  // wave-uniform
+    for (int b0 = threadIdx.x & ~63; b0 < nb; b0 += NT) {
+        const int b = b0 + lane;
+        bool act = false;
+        if (b < nb) {
+            const uint32_t *p0 = block_at(b);
+            const uint32_t q0 = p0[0], q1 = p0[1], q2 = p0[LW], q3 = p0[LW + 1];
+            const uint32_t i0 = q0 & 15u, i1 = q1 & 15u, i2 = q2 & 15u, i3 = q3 & 15u;
+            const uint32_t VL = T[(i0 << 4) | i2], VR = T[(i1 << 4) | i3];
+            const uint32_t DL = T[(i0 << 4) | i3], DR = T[(i1 << 4) | i2];
+            const uint32_t HT = T[(i0 << 4) | i1], HB = T[(i2 << 4) | i3];
+            act = !block_inert(VL, VR, DL, DR, HT, HB, (q0 >> 8) & 1u, (q1 >> 8) & 1u);
+            if (has_wood) act = act || wood_hot(M, q0) || wood_hot(M, q1) || wood_hot(M, q2) || wood_hot(M, q3);
+        }
+        const uint64_t m = __ballot(act);
+        if (act) {
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            Q[nq + (int)below] = (uint16_t)b;
+        }
+        nq += __popcll(m);
+    }
+    uint32_t bands = 0;
+    for (int i = lane; i < nq; i += 64) {
+        uint32_t *p0 = block_at(Q[i]);
+        const int off = (int)(p0 - &L[0][0]);
+        const int ly = (int)__umul24((uint32_t)off, 58255u) >> 22;  // off / LW for off < LW * LH
+        const int lx = off - ly * LW;
         uint32_t q[4] = {p0[0], p0[1], p0[LW], p0[LW + 1]};
         const uint32_t i0 = q[0] & 15u, i1 = q[1] & 15u, i2 = q[2] & 15u, i3 = q[3] & 15u;
         const uint32_t VL = T[(i0 << 4) | i2], VR = T[(i1 << 4) | i3];
-        const uint32_t DL = T[(i0 << 4) | i3], DR = T[(i1 << 4) | i2];
-        const uint32_t HT = T[(i0 << 4) | i1], HB = T[(i2 << 4) | i3];
-        const uint32_t movA = (q[0] >> 8) & 1u, movB = (q[1] >> 8) & 1u;
+        uint32_t r, s;
+        block_hash(key, gx0 + lx, gy0 + ly, r, s);
         bool hot = false;
         if (has_wood)
             hot = wood_hot(M, q[0]) || wood_hot(M, q[1]) || wood_hot(M, q[2]) || wood_hot(M, q[3]);
-        if (!hot && block_inert(VL, VR, DL, DR, HT, HB, movA, movB)) continue;
-        uint32_t r, s;
-        block_hash(key, gx0 + lx, gy0 + ly, r, s);
         bool wrote = false;
         if (hot || ((VL | VR) & K_REACT)) {
             if (block_rule(M, it, r, s, q)) {
@@ -148,8 +181,9 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
                 wrote = true;
             }
         } else {
-            const uint32_t HTR = T[(i1 << 4) | i0];
-            const uint32_t dec = block_decide(VL, VR, DL, DR, HT, HTR, HB, movA, movB, r, s);
+            const uint32_t DL = T[(i0 << 4) | i3], DR = T[(i1 << 4) | i2];
+            const uint32_t HT = T[(i0 << 4) | i1], HB = T[(i2 << 4) | i3], HTR = T[(i1 << 4) | i0];
+            const uint32_t dec = block_decide(VL, VR, DL, DR, HT, HTR, HB, (q[0] >> 8) & 1u, (q[1] >> 8) & 1u, r, s);
             if (dec != 0) {
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
@@ -220,7 +254,7 @@ constexpr int KV = (NV + NT - 1) / NT;  // pairs per thread
 
 __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, int tile, uint32_t (*L)[LW],
                                             uint32_t (*lut)[256], uint32_t *rgba, int32_t *addt,
-                                            uint32_t *red) {
+                                            uint16_t *queue, uint32_t *red) {
     const int tid = threadIdx.x;
     const MatTab &M = w.mat;
     const int tx = tile % w.tiles_x, ty = tile / w.tiles_x;
@@ -338,9 +372,9 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
             const uint64_t key = splitmix64(seedmix ^ g);
             const int ox = kOX[g & 3], oy = kOY[g & 3];
             if (skip)
-                bands |= run_phase<true>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood);
+                bands |= run_phase<true>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue);
             else
-                run_phase<false>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood);
+                run_phase<false>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue);
             __syncthreads();
         }
     }
@@ -411,8 +445,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
     __shared__ uint32_t lut[MAX_IT][256];
     __shared__ uint32_t rgba[16];
     __shared__ int32_t addt[16];
+    __shared__ uint16_t queue[(NT / 64) * QCAP];
     __shared__ uint32_t red[NT / 64];
-    tile_update(w, w.tick + k, blockIdx.x, L, lut, rgba, addt, red);
+    tile_update(w, w.tick + k, blockIdx.x, L, lut, rgba, addt, queue, red);
 }
 
 // Many worlds in one launch: tile_start[i] is the first global chunk index of
@@ -424,6 +459,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
     __shared__ uint32_t lut[MAX_IT][256];
     __shared__ uint32_t rgba[16];
     __shared__ int32_t addt[16];
+    __shared__ uint16_t queue[(NT / 64) * QCAP];
     __shared__ uint32_t red[NT / 64];
     const uint32_t gt = blockIdx.x;
     int lo = 0, hi = nworlds - 1;
@@ -433,7 +469,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
         else hi = mid - 1;
     }
     const DevWorld &w = worlds[lo];
-    tile_update(w, w.tick + k, (int)(gt - tile_start[lo]), L, lut, rgba, addt, red);
+    tile_update(w, w.tick + k, (int)(gt - tile_start[lo]), L, lut, rgba, addt, queue, red);
 }
 
 __global__ void k_advance(DevWorld *worlds, int nworlds, uint32_t ticks) {
