@@ -317,11 +317,22 @@ __host__ __device__ __forceinline__ uint32_t pair_code(const MatTab &M, int it, 
 //   bits 12-15 MOVED cleared on original cell k (bit 12 + k)
 // All intermediate values are 0/1 integers, so the stages combine with plain
 // bit operations instead of divergent branches.  Same result as block_rule
-// for blocks where no reaction fires.
+// for blocks where no reaction fires.  GAS_H = false compiles the form for
+// blocks with no K_GRISE bit in VL, VR, DL, DR and no K_HSWAP bit in HT, HB
+// (the gas and horizontal terms are then zero).
+template <bool GAS_H = true>
 __host__ __device__ __forceinline__ uint32_t block_decide(uint32_t VL, uint32_t VR, uint32_t DL, uint32_t DR,
                                                           uint32_t HT, uint32_t HTR, uint32_t HB,
                                                           uint32_t movA, uint32_t movB, uint32_t r,
                                                           uint32_t s) {
+    if (!GAS_H) {
+        VL &= ~K_GRISE;
+        VR &= ~K_GRISE;
+        DL &= ~K_GRISE;
+        DR &= ~K_GRISE;
+        HT = 0;
+        HB = 0;
+    }
     const uint32_t sL = VL & 1u, sR = VR & 1u, dL = DL & 1u, dR = DR & 1u, htr = HTR & 1u;
     const uint32_t gL = (VL >> 1) & 1u, gR = (VR >> 1) & 1u, gdl = (DL >> 1) & 1u, gdr = (DR >> 1) & 1u;
     const uint32_t sandA = (VL >> 4) & 1u, sandB = (VR >> 4) & 1u;

@@ -110,13 +110,18 @@ constexpr int QCAP = ((36 * 36 + NT - 1) / NT) * 64;
 // One Margolus phase over the blocks whose cells the rest of the tick still
 // reads: after this phase the tick needs `need` more rings around the chunk
 // (need = cell_iter - it, so the region shrinks by one ring per phase and the
-// last phase leaves the one ring the temperature stencil reads).  Two passes
-// per wave: the first tests every block against the pair table and queues
-// the ones some rule may change (ballot + mbcnt into the wave's LDS queue Q);
-// the second runs the random draw and the rule on the queued blocks only, so
-// no lane idles through the decision for an inert block.  Blocks of a phase
-// are disjoint and a wave's LDS accesses stay in order, so the passes need
-// no barrier.  With TRACK, returns the band bits of chunk cells that changed.
+// last phase leaves the one ring the temperature stencil reads).
+//
+// Two passes per wave.  The first tests every block against the pair table
+// and queues the ones some rule may change (ballot + mbcnt into the wave's
+// LDS queue Q): "plain" blocks (no reaction, no gas or horizontal relation:
+// sand over air, most of a world) from the front, the others from the back.
+// The second pass runs the random draw and the rule on the queued blocks
+// only, plain ones with the decision form that omits the gas and horizontal
+// terms, so no lane idles through the decision for an inert block.  Blocks
+// of a phase are disjoint and a wave's LDS accesses stay in order, so the
+// passes need no barrier.  With TRACK, returns the band bits of chunk cells
+// that changed.
 template <bool TRACK>
 __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW], const uint32_t *T, int it,
                                               uint64_t key, int ox, int oy, int need, int gx0, int gy0,
@@ -136,10 +141,10 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
     };
     const int lane = threadIdx.x & 63;
     uint16_t *Q = queue + (threadIdx.x >> 6) * QCAP;
-    int nq = 0;  // wave-uniform
+    int nplain = 0, nother = 0;  // wave-uniform
     for (int b0 = threadIdx.x & ~63; b0 < nb; b0 += NT) {
         const int b = b0 + lane;
-        bool act = false;
+        bool act = false, plain = false;
         if (b < nb) {
             const uint32_t *p0 = block_at(b);
             const uint32_t q0 = p0[0], q1 = p0[1], q2 = p0[LW], q3 = p0[LW + 1];
@@ -148,21 +153,66 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
             const uint32_t DL = T[(i0 << 4) | i3], DR = T[(i1 << 4) | i2];
             const uint32_t HT = T[(i0 << 4) | i1], HB = T[(i2 << 4) | i3];
             act = !block_inert(VL, VR, DL, DR, HT, HB, (q0 >> 8) & 1u, (q1 >> 8) & 1u);
-            if (has_wood) act = act || wood_hot(M, q0) || wood_hot(M, q1) || wood_hot(M, q2) || wood_hot(M, q3);
+            plain = ((((VL | VR) & (K_GRISE | K_REACT)) | ((DL | DR) & K_GRISE) | ((HT | HB) & K_HSWAP)) == 0);
+            if (has_wood && (wood_hot(M, q0) || wood_hot(M, q1) || wood_hot(M, q2) || wood_hot(M, q3))) {
+                act = true;
+                plain = false;
+            }
+            plain = plain && act;
         }
-        const uint64_t m = __ballot(act);
-        if (act) {
-            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            Q[nq + (int)below] = (uint16_t)b;
+        const uint64_t mp = __ballot(plain), mo = __ballot(act && !plain);
+        if (plain) {
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mp >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mp, 0u));
+            Q[nplain + (int)below] = (uint16_t)b;
+        } else if (act) {
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mo >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mo, 0u));
+            Q[QCAP - 1 - (nother + (int)below)] = (uint16_t)b;
         }
-        nq += __popcll(m);
+        nplain += __popcll(mp);
+        nother += __popcll(mo);
     }
     uint32_t bands = 0;
-    for (int i = lane; i < nq; i += 64) {
-        uint32_t *p0 = block_at(Q[i]);
+    // block position from its LDS pointer (for the hash and the bands)
+    auto pos = [&](const uint32_t *p0, int &lx, int &ly) {
         const int off = (int)(p0 - &L[0][0]);
-        const int ly = (int)__umul24((uint32_t)off, 58255u) >> 22;  // off / LW for off < LW * LH
-        const int lx = off - ly * LW;
+        ly = (int)(__umul24((uint32_t)off, 58255u) >> 22);  // off / LW for off < LW * LH
+        lx = off - ly * LW;
+    };
+    auto track = [&](bool wrote, int lx, int ly) {
+        if (TRACK && wrote) {
+            const int cx = lx - HALO, cy = ly - HALO;  // chunk coordinates of the block
+            if (cx + 1 >= 0 && cx <= TW - 1 && cy + 1 >= 0 && cy <= TH - 1)
+                bands |= band_bits(max(cx, 0), min(cx + 1, TW - 1), max(cy, 0), min(cy + 1, TH - 1));
+        }
+    };
+    auto scatter = [&](uint32_t *p0, const uint32_t (&q)[4], uint32_t dec) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint32_t v = q[k];
+            if ((dec >> (8 + k)) & 1u) v |= (uint32_t)MOVED;
+            if ((dec >> (12 + k)) & 1u) v &= ~(uint32_t)MOVED;
+            const uint32_t d = (uint32_t)k ^ ((dec >> (2 * k)) & 3u);
+            p0[(d & 1u) + (d >> 1) * LW] = v;
+        }
+    };
+    for (int i = lane; i < nplain; i += 64) {
+        uint32_t *p0 = block_at(Q[i]);
+        int lx, ly;
+        pos(p0, lx, ly);
+        const uint32_t q[4] = {p0[0], p0[1], p0[LW], p0[LW + 1]};
+        const uint32_t i0 = q[0] & 15u, i1 = q[1] & 15u, i2 = q[2] & 15u, i3 = q[3] & 15u;
+        uint32_t r, s;
+        block_hash(key, gx0 + lx, gy0 + ly, r, s);
+        const uint32_t dec = block_decide<false>(T[(i0 << 4) | i2], T[(i1 << 4) | i3], T[(i0 << 4) | i3],
+                                                 T[(i1 << 4) | i2], 0u, T[(i1 << 4) | i0], 0u,
+                                                 (q[0] >> 8) & 1u, (q[1] >> 8) & 1u, r, s);
+        if (dec != 0) scatter(p0, q, dec);
+        track(dec != 0, lx, ly);
+    }
+    for (int i = lane; i < nother; i += 64) {
+        uint32_t *p0 = block_at(Q[QCAP - 1 - i]);
+        int lx, ly;
+        pos(p0, lx, ly);
         uint32_t q[4] = {p0[0], p0[1], p0[LW], p0[LW + 1]};
         const uint32_t i0 = q[0] & 15u, i1 = q[1] & 15u, i2 = q[2] & 15u, i3 = q[3] & 15u;
         const uint32_t VL = T[(i0 << 4) | i2], VR = T[(i1 << 4) | i3];
@@ -181,26 +231,13 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
                 wrote = true;
             }
         } else {
-            const uint32_t DL = T[(i0 << 4) | i3], DR = T[(i1 << 4) | i2];
-            const uint32_t HT = T[(i0 << 4) | i1], HB = T[(i2 << 4) | i3], HTR = T[(i1 << 4) | i0];
-            const uint32_t dec = block_decide(VL, VR, DL, DR, HT, HTR, HB, (q[0] >> 8) & 1u, (q[1] >> 8) & 1u, r, s);
-            if (dec != 0) {
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    uint32_t v = q[k];
-                    if ((dec >> (8 + k)) & 1u) v |= (uint32_t)MOVED;
-                    if ((dec >> (12 + k)) & 1u) v &= ~(uint32_t)MOVED;
-                    const uint32_t d = (uint32_t)k ^ ((dec >> (2 * k)) & 3u);
-                    p0[(d & 1u) + (d >> 1) * LW] = v;
-                }
-                wrote = true;
-            }
+            const uint32_t dec = block_decide<true>(VL, VR, T[(i0 << 4) | i3], T[(i1 << 4) | i2], T[(i0 << 4) | i1],
+                                                    T[(i1 << 4) | i0], T[(i2 << 4) | i3], (q[0] >> 8) & 1u,
+                                                    (q[1] >> 8) & 1u, r, s);
+            if (dec != 0) scatter(p0, q, dec);
+            wrote = dec != 0;
         }
-        if (TRACK && wrote) {
-            const int cx = lx - HALO, cy = ly - HALO;  // chunk coordinates of the block
-            if (cx + 1 >= 0 && cx <= TW - 1 && cy + 1 >= 0 && cy <= TH - 1)
-                bands |= band_bits(max(cx, 0), min(cx + 1, TW - 1), max(cy, 0), min(cy + 1, TH - 1));
-        }
+        track(wrote, lx, ly);
     }
     return bands;
 }
