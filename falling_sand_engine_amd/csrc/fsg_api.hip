@@ -232,7 +232,8 @@ extern "C" int fsg_ctx_copy_timed(fsg_ctx *c, size_t bytes, int reps, double *ms
         return FSG_ERR_NOMEM;
     }
     int rc = FSG_OK;
-    const unsigned grid = 256 * 16;  // 16 workgroups per CU, grid-stride
+    if (n16 > (size_t)0x7FFFFFFF * 256) return FSG_ERR_INVALID;
+    const unsigned grid = grid_for(n16, 256);
     (void)hipMemsetAsync(a, 1, n16 * 16, c->stream);
     fsg::k_copy<<<grid, 256, 0, c->stream>>>((const uint4 *)a, (uint4 *)b, n16);  // warm-up
     if (hipEventRecord(c->e0, c->stream) != hipSuccess) rc = FSG_ERR_DEVICE;

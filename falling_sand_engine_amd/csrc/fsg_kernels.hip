@@ -565,12 +565,13 @@ __global__ void k_fill(uint64_t *cells, size_t n, uint64_t v) {
     if (i < n) cells[i] = v;
 }
 
-// Streaming copy (16 B per lane per iteration), the calibration kernel for
-// HBM byte counters.
+// Streaming copy, one 16-byte load and store per thread over a grid that
+// covers the buffer: the calibration kernel for the HBM byte counters and
+// the reference for attainable HBM bandwidth.
 __global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
                                               size_t n16) {
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (size_t)gridDim.x * 256)
-        dst[i] = src[i];
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n16) dst[i] = src[i];
 }
 
 __global__ void k_set_cells(uint64_t *cells, int W, int H, const int32_t *xy, const uint64_t *vals,
