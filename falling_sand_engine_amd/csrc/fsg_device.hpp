@@ -174,18 +174,20 @@ __host__ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uin
     }
     if (vm) return true;
 
-    // D: TL->BR or TR->BL for sinking cells (sand needs MOVED and shouldMove),
-    // else gas BL->TR or BR->TL into air
+    // D: TL->BR or TR->BL for sinking cells (sand needs MOVED, or the 1/16
+    // unstick draw r[16:19] / r[20:23] == 0, and shouldMove), else gas BL->TR
+    // or BR->TL into air
     {
         const uint32_t ma = cid(q[0]), mb = cid(q[1]);
         bool cl = nib(pass, cid(q[3])) < nib(sinkt, ma);
         bool cr = nib(pass, cid(q[2])) < nib(sinkt, mb);
+        const bool ua = ((r >> 16) & 15u) == 0, ub = ((r >> 20) & 15u) == 0;
         if (cl && bit(sandm, ma)) {
-            if (!(q[0] & MOVED)) cl = false;
+            if (!(q[0] & MOVED) && !ua) cl = false;
             else if ((s & 31u) == 0) { cl = false; q[0] &= ~MOVED; }
         }
         if (cr && bit(sandm, mb)) {
-            if (!(q[1] & MOVED)) cr = false;
+            if (!(q[1] & MOVED) && !ub) cr = false;
             else if (((s >> 5) & 31u) == 0) { cr = false; q[1] &= ~MOVED; }
         }
         if (cl && cr) {
@@ -195,6 +197,7 @@ __host__ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uin
         if (cl) {
             Cell a2 = q[0];
             const bool sand = bit(sandm, ma);
+            if (sand && ua) a2 |= MOVED;
             if (sand && ((s >> 11) & 511u) == 0) a2 &= ~MOVED;
             if (sand && bit(sandm, cid(q[2])) && ((s >> 20) & 3u) == 0) q[2] |= MOVED;
             if (bit(airm, cid(q[1]))) {
@@ -210,6 +213,7 @@ __host__ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uin
         if (cr) {
             Cell b2 = q[1];
             const bool sand = bit(sandm, mb);
+            if (sand && ub) b2 |= MOVED;
             if (sand && ((s >> 11) & 511u) == 0) b2 &= ~MOVED;
             if (sand && bit(sandm, cid(q[3])) && ((s >> 20) & 3u) == 0) q[3] |= MOVED;
             if (bit(airm, cid(q[0]))) {
@@ -342,6 +346,7 @@ __host__ __device__ __forceinline__ uint32_t block_decide(uint32_t VL, uint32_t 
     const uint32_t hT = (HT >> 2) & 1u, hB = (HB >> 2) & 1u;
     const uint32_t T1 = (r & 0xFu) == 0, T2 = (r & 0xF0u) == 0, T5 = (r & 0xC00u) == 0, T6 = (r & 0x3000u) == 0;
     const uint32_t T3 = (r >> 8) & 1u, T4 = (r >> 9) & 1u, T7 = (r >> 14) & 1u, T8 = (r >> 15) & 1u;
+    const uint32_t UA = (r & 0xF0000u) == 0, UB = (r & 0xF00000u) == 0;  // stopped sand unsticks
     const uint32_t S1 = (s & 0x1Fu) == 0, S2 = (s & 0x3E0u) == 0, S4 = (s & 0xFF800u) == 0;
     const uint32_t S5 = (s & 0x300000u) == 0;
     const uint32_t S3 = (s >> 10) & 1u, S6 = (s >> 22) & 1u, S7 = (s >> 23) & 1u, S8 = (s >> 24) & 1u;
@@ -364,14 +369,16 @@ __host__ __device__ __forceinline__ uint32_t block_decide(uint32_t VL, uint32_t 
     // D: sinking diagonals with sand friction, then gas diagonals
     const uint32_t nvm = vm ^ 1u;
     const uint32_t cl0 = nvm & dL, cr0 = nvm & dR;
-    const uint32_t failA = cl0 & sandA & movA & S1, failB = cr0 & sandB & movB & S2;
-    const uint32_t cl1 = cl0 & ((sandA ^ 1u) | (movA & (failA ^ 1u)));
-    const uint32_t cr1 = cr0 & ((sandB ^ 1u) | (movB & (failB ^ 1u)));
+    const uint32_t mvA = movA | UA, mvB = movB | UB;
+    const uint32_t failA = cl0 & sandA & mvA & S1, failB = cr0 & sandB & mvB & S2;
+    const uint32_t cl1 = cl0 & ((sandA ^ 1u) | (mvA & (failA ^ 1u)));
+    const uint32_t cr1 = cr0 & ((sandB ^ 1u) | (mvB & (failB ^ 1u)));
     const uint32_t both = cl1 & cr1;
     const uint32_t cl = cl1 & ((both & S3) ^ 1u);
     const uint32_t cr = cr1 & ((both & (S3 ^ 1u)) ^ 1u);
     uint32_t clr = failA | (failB << 1) | (cl & sandA & S4) | ((cr & sandB & S4) << 1);
     set |= ((cl & sandA & sandC & S5) << 2) | ((cr & sandB & sandD & S5) << 3);
+    set |= (cl & sandA & UA) | ((cr & sandB & UB) << 1);  // an unstuck slider moves (S4 may clear it)
     y |= cl * (0xC3u ^ (airB * 0x44u));  // swap TL/BR, or 3-cycle TL->BR->TR->TL when TR is air
     y |= cr * (0x3Cu ^ (airA * 0x11u));  // swap TR/BL, or 3-cycle TR->BL->TL->TR when TL is air
     const uint32_t none = nvm & (cl ^ 1u) & (cr ^ 1u);

@@ -142,7 +142,8 @@ static inline uint64_t transform(const props *P, uint64_t c, int m) {
 /* ----------------------------------------------------------------------- */
 /* q[0]=TL (x0,y0)  q[1]=TR (x0+1,y0)  q[2]=BL (x0,y0+1)  q[3]=BR (x0+1,y0+1)
  * Random bits: r[0:3]/r[4:7] sand vertical skip L/R, r[8]/r[9] MOVED on fall
- * L/R, r[10:11]/r[12:13] nudge L/R, r[14]/r[15] gas rise skip L/R;
+ * L/R, r[10:11]/r[12:13] nudge L/R, r[14]/r[15] gas rise skip L/R,
+ * r[16:19]/r[20:23] stopped sand unsticks TL/TR;
  * s[0:4]/s[5:9] slide refused L/R, s[10] slide side choice, s[11:19] MOVED
  * cleared after a slide, s[20:21] slide nudge, s[22] gas diagonal choice,
  * s[23]/s[24] horizontal swap top/bottom row. */
@@ -222,21 +223,29 @@ static int block_rule(const props *P, const fsg_config *cfg, int it, uint32_t r,
         const int ma = mid(q[0]), mb = mid(q[1]);
         int cl = pass_rank(P, mid(q[3])) < sink_rank(P, ma, it); /* TL -> BR */
         int cr = pass_rank(P, mid(q[2])) < sink_rank(P, mb, it); /* TR -> BL */
-        /* Sand friction (world.cpp:1612-1700): a sand cell slides only while
+        /* Sand friction (world.cpp:1612-1700): a sand cell slides while
          * MOVED is set, and refuses with p=1/32 (shouldMove, rand()%(2*slip)
-         * at 1656), which also stops it (moved = false, 1695-1700). */
+         * at 1656), which also stops it (moved = false, 1695-1700).  A
+         * stopped sand cell whose diagonal is open comes unstuck with p=1/16
+         * (r[16:19] == 0 for TL, r[20:23] == 0 for TR) and slides with MOVED
+         * set: the block-local form of the friction pillar check, which
+         * unsticks a stopped cell above a drop (world.cpp:1616-1644) and
+         * cannot be restated exactly because it reads 10 rows below the
+         * cell. */
+        const int ua = ((r >> 16) & 15u) == 0, ub = ((r >> 20) & 15u) == 0;
         if (cl && is_sand(P, ma)) {
-            if (!(q[0] & MOVED)) cl = 0;
+            if (!(q[0] & MOVED) && !ua) cl = 0;
             else if ((s & 31u) == 0) { cl = 0; q[0] &= ~MOVED; }
         }
         if (cr && is_sand(P, mb)) {
-            if (!(q[1] & MOVED)) cr = 0;
+            if (!(q[1] & MOVED) && !ub) cr = 0;
             else if (((s >> 5) & 31u) == 0) { cr = 0; q[1] &= ~MOVED; }
         }
         if (cl && cr) { if ((s >> 10) & 1u) cl = 0; else cr = 0; }
         if (cl) {
             uint64_t a2 = q[0];
             const int sand = is_sand(P, ma);
+            if (sand && ua) a2 |= MOVED; /* unstuck (world.cpp:1636) */
             /* moved cleared with p=1/512 after a slide (1/(20*slip), 1688). */
             if (sand && ((s >> 11) & 511u) == 0) a2 &= ~MOVED;
             /* the cell below the slider is nudged with p=1/4 (1658-1671). */
@@ -258,6 +267,7 @@ static int block_rule(const props *P, const fsg_config *cfg, int it, uint32_t r,
         if (cr) {
             uint64_t b2 = q[1];
             const int sand = is_sand(P, mb);
+            if (sand && ub) b2 |= MOVED;
             if (sand && ((s >> 11) & 511u) == 0) b2 &= ~MOVED;
             if (sand && is_sand(P, mid(q[3])) && ((s >> 20) & 3u) == 0) q[3] |= MOVED;
             if (is_air(P, mid(q[0]))) {

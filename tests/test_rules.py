@@ -86,13 +86,24 @@ def test_block_wood_burns_to_steam_and_air(orc):
 def test_block_sand_slide_needs_moved(orc):
     cfg = orc.default_config()
     base = [cell(SAND), cell(STONE), cell(STONE), cell(AIR)]
-    ch, q = orc.block_rule(cfg, 0, 0, 0xFFFFFFFF, base)
-    assert not ch  # stuck (MOVED clear): no slide
+    ch, q = orc.block_rule(cfg, 0, 0x00010000, 0xFFFFFFFF, base)
+    assert not ch  # stuck (MOVED clear) and r[16:19] != 0: no slide
+    ch, q = orc.block_rule(cfg, 0, 0x00F00000, 0xFFFFFFFF, base)
+    assert [v & 0xFF for v in q] == [AIR, STONE, STONE, SAND]  # r[16:19] == 0: unstuck, slides
+    assert q[3] & 0x100  # and moves on with MOVED set
+    ch, q = orc.block_rule(cfg, 0, 0, 0xFFF007FF, base)  # s[11:19] == 0 still clears MOVED after the slide
+    assert [v & 0xFF for v in q] == [AIR, STONE, STONE, SAND] and not (q[3] & 0x100)
     moving = [cell(SAND, moved=True), cell(STONE), cell(STONE), cell(AIR)]
-    ch, q = orc.block_rule(cfg, 0, 0, 0xFFFFFFFF, moving)
+    ch, q = orc.block_rule(cfg, 0, 0x00010000, 0xFFFFFFFF, moving)
     assert [v & 0xFF for v in q] == [AIR, STONE, STONE, SAND]
-    ch, q = orc.block_rule(cfg, 0, 0, 0xFFFFFFE0, moving)    # s[0:4] == 0: refuses, stops
+    ch, q = orc.block_rule(cfg, 0, 0x00010000, 0xFFFFFFE0, moving)    # s[0:4] == 0: refuses, stops
     assert [v & 0xFF for v in q] == [SAND, STONE, STONE, AIR] and not (q[0] & 0x100)
+    # the right column draws from r[20:23]
+    right = [cell(STONE), cell(SAND), cell(AIR), cell(STONE)]
+    ch, q = orc.block_rule(cfg, 0, 0x00100000, 0xFFFFFFFF, right)
+    assert not ch
+    ch, q = orc.block_rule(cfg, 0, 0x000F0000, 0xFFFFFFFF, right)
+    assert [v & 0xFF for v in q] == [STONE, AIR, SAND, STONE] and q[2] & 0x100
 
 
 def test_block_slide_three_cycle(orc):
