@@ -101,6 +101,44 @@ __host__ __device__ __forceinline__ void block_hash(uint64_t key, int x0, int y0
     s = lowbias32(r ^ (uint32_t)(key >> 32));
 }
 
+// Stage R of the block rule, reactions: water directly above lava becomes
+// steam and every lava cell of the block becomes stone; hot wood becomes
+// steam and trades places with the block's first air cell (world.cpp:1181-1204,
+// 1519-1537).  Returns true if a reaction fired (the block then does not also
+// move).  Requires M.reactions.
+template <typename Cell>
+__host__ __device__ __forceinline__ bool block_react(const MatTab &M, Cell q[4]) {
+    bool fired = false;
+    const bool wl = cbyte(q[0]) == M.water && cbyte(q[2]) == M.lava;
+    const bool wr = cbyte(q[1]) == M.water && cbyte(q[3]) == M.lava;
+    if (wl || wr) {
+        if (wl) q[0] = xform(M, q[0], M.steam);
+        if (wr) q[1] = xform(M, q[1], M.steam);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (cbyte(q[i]) == M.lava) q[i] = xform(M, q[i], M.stone);
+        fired = true;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (cbyte(q[i]) == M.wood && ctemp(q[i]) > M.burn) {
+            q[i] = xform(M, q[i], M.steam);
+            bool done = false;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (!done && bit(M.air, cid(q[j]))) {
+                    const Cell t = q[i];
+                    q[i] = q[j];
+                    q[j] = t;
+                    done = true;
+                }
+            }
+            fired = true;
+        }
+    }
+    return fired;
+}
+
 // One Margolus block, q = {TL, TR, BL, BR} (stored cells, or compact cells
 // in the kernel).  Returns true if a cell changed.  Stages: R reactions, V
 // vertical, D diagonal, F friction stop, H horizontal; each stage restates
@@ -111,38 +149,7 @@ __host__ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uin
     const uint64_t pass = M.pass, sinkt = M.sink[it], flowt = M.flow[it];
     const uint32_t gasm = M.gas[it], airm = M.air, sandm = M.sand;
 
-    // R: water above lava -> steam, lava of the block -> stone; hot wood -> steam
-    if (M.reactions) {
-        bool fired = false;
-        const bool wl = cbyte(q[0]) == M.water && cbyte(q[2]) == M.lava;
-        const bool wr = cbyte(q[1]) == M.water && cbyte(q[3]) == M.lava;
-        if (wl || wr) {
-            if (wl) q[0] = xform(M, q[0], M.steam);
-            if (wr) q[1] = xform(M, q[1], M.steam);
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                if (cbyte(q[i]) == M.lava) q[i] = xform(M, q[i], M.stone);
-            fired = true;
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            if (cbyte(q[i]) == M.wood && ctemp(q[i]) > M.burn) {
-                q[i] = xform(M, q[i], M.steam);
-                bool done = false;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    if (!done && bit(airm, cid(q[j]))) {
-                        const Cell t = q[i];
-                        q[i] = q[j];
-                        q[j] = t;
-                        done = true;
-                    }
-                }
-                fired = true;
-            }
-        }
-        if (fired) return q[0] != o0 || q[1] != o1 || q[2] != o2 || q[3] != o3;
-    }
+    if (M.reactions && block_react(M, q)) return q[0] != o0 || q[1] != o1 || q[2] != o2 || q[3] != o3;
 
     // V: per column, left then right
     bool vm = false;

@@ -223,7 +223,7 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
             hot = wood_hot(M, q[0]) || wood_hot(M, q[1]) || wood_hot(M, q[2]) || wood_hot(M, q[3]);
         bool wrote = false;
         if (hot || ((VL | VR) & K_REACT)) {
-            if (block_rule(M, it, r, s, q)) {
+            if (block_react(M, q)) {
                 p0[0] = q[0];
                 p0[1] = q[1];
                 p0[LW] = q[2];
@@ -365,7 +365,11 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         }
     }
     // compact cells into LDS, colours kept in registers until the write-back
-    uint2 col[KV];
+    // (not those of the last LDS row: a cell moves at most one row per
+    // phase, so no cell of the chunk comes from HALO rows away)
+    static_assert(MAX_IT < HALO, "write-back colours cover the rings cells can come from");
+    constexpr int KC = (LW / 2 * (LH - 1) + NT - 1) / NT;  // pair slots holding rows 0 .. LH-2
+    uint2 col[KC];
     bool wood = false;
 #pragma unroll
     for (int k = 0; k < KV; k++) {
@@ -374,7 +378,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
             const int ly = j / (LW / 2), c2 = j - ly * (LW / 2);
             const uint32_t org = (((uint32_t)(2 * c2) & 7u) << 9) | (((uint32_t)ly & 7u) << 12);
             *reinterpret_cast<uint2 *>(&L[ly][2 * c2]) = make_uint2(v[k].x | org, v[k].z | org | (1u << 9));
-            col[k] = make_uint2(v[k].y, v[k].w);
+            if (k < KC) col[k] = make_uint2(v[k].y, v[k].w);
             wood |= (v[k].x & 0xFFu) == M.wood || (v[k].z & 0xFFu) == M.wood;
         }
     }
@@ -427,9 +431,9 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     __syncthreads();
     // the colour plane: each cell's colour at its origin
 #pragma unroll
-    for (int k = 0; k < KV; k++) {
+    for (int k = 0; k < KC; k++) {
         const int jj = tid + k * NT;
-        if (k < KV - 1 || jj < NV) {
+        if (jj < NV) {
             const int ly = jj / (LW / 2), c2 = jj - ly * (LW / 2);
             *reinterpret_cast<uint2 *>(&L[ly][2 * c2]) = col[k];
         }
