@@ -62,6 +62,29 @@ def bench_c3(fsg, steps, warmup, side=16384, seed=3):
     return {"wall_s": wall, "dev_ms": ms_dev, "cells": side * side, "counts": list(st.count)[:8]}
 
 
+def bench_c1(fsg, ticks=1000, side=512, seed=1):
+    """The reference's own CPU-runnable case: 512^2 basin world with sand
+    discs, 1000 ticks on the GPU, and the same 1000 ticks of the sequential
+    tick restatement on the host's cores."""
+    from oracle import pyoracle
+    cells = fsg.generate(1, side, side, seed)
+    with fsg.Context(0) as ctx, fsg.World(ctx, side, side, seed, fsg.default_config()) as w:
+        w.upload(cells)
+        w.sync()
+        t0 = time.perf_counter()
+        ms = w.step_timed(ticks)
+        wall = time.perf_counter() - t0
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    pyoracle.refseq(cells.copy(), pyoracle.default_config(), seed, ticks, threads)
+    cpu_s = time.perf_counter() - t0
+    cu = side * side * ticks
+    return {"workload": f"basin world {side}x{side} with sand discs, {ticks} ticks",
+            "gpu_gcell_s": round(cu / wall / 1e9, 2), "gpu_dev_ms": round(ms, 2),
+            "cpu_refseq_gcell_s": round(cu / cpu_s / 1e9, 4), "cpu_threads": threads,
+            "cpu_refseq_s": round(cpu_s, 2)}
+
+
 def bench_c2(fsg, ticks=500, side=4096, seed=2):
     """4096^2 basin world with lava and oil, reactions on, temperature every
     tick, exact dirty-rect skipping on 64x64 chunks; also run without
@@ -106,9 +129,10 @@ def uneven_batch_sizes(n=256, seed=4):
     return sizes
 
 
-def bench_c4(fsg, ticks=256, n=256):
+def bench_c4(fsg, ticks=256, n=256, tune_ticks=32):
     """256 worlds of uneven sizes: one launch per tick over all of their
-    chunks (16-tick HIP graphs) vs one launch per world per tick."""
+    chunks (HIP graphs of ticks, launch shape picked by the autotuner on
+    copies of the same worlds) vs one launch per world per tick."""
     sizes = uneven_batch_sizes(n)
     with fsg.Context(0) as ctx:
         sets = []
@@ -119,6 +143,9 @@ def bench_c4(fsg, ticks=256, n=256):
                 w.upload(fsg.generate(4, W, H, 1000 + i))
                 ws.append(w)
             sets.append(ws)
+        t0 = time.perf_counter()
+        tune = ctx.autotune(sets[0], sample_ticks=tune_ticks)
+        tune_s = time.perf_counter() - t0
         fsg.batch_step(ctx, sets[0], 16)        # warm-up + graph capture
         for w in sets[1]:
             w.tick(16)
@@ -134,13 +161,18 @@ def bench_c4(fsg, ticks=256, n=256):
             for w in ws:
                 w.close()
     cells = sum(W * H for W, H in sizes)
+    cand = sorted(tune["candidates"], key=lambda c: c["ms"])
     return {"workload": f"{n} worlds, sides log-uniform [256,2048], aspect [0.5,2], {ticks} ticks",
             "cells": cells,
             "batched_gcell_s": round(cells * ticks / wall_b / 1e9, 2),
             "perworld_gcell_s": round(cells * ticks / wall_p / 1e9, 2),
             "ratio": round(wall_p / wall_b, 3),
             "batched_dev_ms": round(ms_b, 2), "perworld_dev_ms": round(ms_p, 2),
-            "bit_exact": exact}
+            "bit_exact": exact,
+            "tuning": tune["best"],
+            "autotune": {"sample_ticks": tune_ticks, "seconds": round(tune_s, 2),
+                         "ms_per_tick": {f"T{c['graph_ticks']}/skip{c['chunk_skip']}/occ{c['occupancy']}":
+                                         round(c["ms"] / tune_ticks, 3) for c in cand}}}
 
 
 def bench_c5(fsg, rates=(30.0, 120.0, 400.0), duration_s=6.0, budget=64 << 30, seed=5):
@@ -193,19 +225,33 @@ def bench_c5(fsg, rates=(30.0, 120.0, 400.0), duration_s=6.0, budget=64 << 30, s
 
 def cpu_baseline(threads=None, side_w=8192, side_h=4096, ticks=64):
     """refseq (the reference's sequential chunk-checkerboard tick, restated)
-    on a bounded sample of the all-active world."""
+    on a bounded sample of the all-active world, on the host's cores; also
+    its 1-thread rate and the deterministic oracle's (1 thread) on smaller
+    samples of the same world."""
     import falling_sand_engine_amd as fsg
     from oracle import pyoracle
-    threads = threads or min(16, os.cpu_count() or 1)
+    hw = os.cpu_count() or 1
+    threads = threads or min(16, hw)
     cells = fsg.generate(3, side_w, side_h, 3)
     cfg = pyoracle.default_config(reactions=0, temp_every=1)
     t0 = time.perf_counter()
     pyoracle.refseq(cells, cfg, 3, ticks, threads)
     dt = time.perf_counter() - t0
+    small = fsg.generate(3, 2048, 2048, 3)
+    t0 = time.perf_counter()
+    pyoracle.refseq(small.copy(), cfg, 3, 16, 1)
+    dt1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pyoracle.run(small.copy(), cfg, 3, 0, 16)
+    dto = time.perf_counter() - t0
     return {"value": side_w * side_h * ticks / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"refseq restatement of world::tick, all-active striped world {side_w}x{side_h}, "
-                      f"{ticks} ticks, {threads} threads, {dt:.2f} s"}
+                      f"{ticks} ticks, {threads} threads, {dt:.2f} s",
+            "host_threads": hw,
+            "refseq_1_thread": round(2048 * 2048 * 16 / dt1 / 1e9, 4),
+            "det_oracle_1_thread": round(2048 * 2048 * 16 / dto / 1e9, 4),
+            "small_sample": "2048x2048 striped world, 16 ticks"}
 
 
 def main():
@@ -253,6 +299,9 @@ def main():
 
     secondary = {}
     if not args.no_secondary:
+        log("[bench] C1 512^2 world")
+        secondary["c1_small_world"] = bench_c1(fsg)
+        log(f"[bench] {secondary['c1_small_world']}")
         log("[bench] C2 dirty-chunk world")
         secondary["c2_dirty_chunks"] = bench_c2(fsg)
         log(f"[bench] {secondary['c2_dirty_chunks']}")
