@@ -74,6 +74,27 @@ class Context:
     def inject_alloc_failures(self, n: int):
         check(lib().fsg_ctx_inject_alloc_failures(self.handle, n))
 
+    def tuning(self) -> dict:
+        t = _abi.Tuning()
+        check(lib().fsg_ctx_get_tuning(self.handle, C.byref(t)), "fsg_ctx_get_tuning")
+        return {"graph_ticks": t.graph_ticks, "chunk_skip": t.chunk_skip, "occupancy": t.occupancy}
+
+    def set_tuning(self, graph_ticks: int = 16, chunk_skip: int = -1, occupancy: int = 5):
+        t = _abi.Tuning(graph_ticks, chunk_skip, occupancy, 0)
+        check(lib().fsg_ctx_set_tuning(self.handle, C.byref(t)), "fsg_ctx_set_tuning")
+
+    def autotune(self, worlds, sample_ticks: int = 32) -> dict:
+        """Times every batch tuning on copies of `worlds` (which do not
+        advance) and keeps the fastest; returns the report."""
+        rep = _abi.TuneReport()
+        check(lib().fsg_autotune_batch(self.handle, _handles(worlds), len(worlds), sample_ticks, C.byref(rep)),
+              "fsg_autotune_batch")
+
+        def td(t):
+            return {"graph_ticks": t.graph_ticks, "chunk_skip": t.chunk_skip, "occupancy": t.occupancy}
+        return {"best": td(rep.best), "from_cache": bool(rep.from_cache), "size_class": rep.size_class,
+                "candidates": [dict(td(rep.candidate[i]), ms=rep.ms[i]) for i in range(rep.n_candidates)]}
+
     def copy_timed(self, nbytes: int, reps: int = 10) -> float:
         """Mean ms of a plain device copy of nbytes (HBM calibration)."""
         ms = C.c_double()

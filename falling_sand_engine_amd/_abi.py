@@ -86,6 +86,20 @@ class ServeStats(C.Structure):
                                           "evictions")]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("graph_ticks", C.c_int32), ("chunk_skip", C.c_int32), ("occupancy", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+TUNE_MAX_CANDIDATES = 32
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("best", Tuning), ("n_candidates", C.c_int32), ("from_cache", C.c_int32),
+                ("size_class", C.c_uint64), ("candidate", Tuning * TUNE_MAX_CANDIDATES),
+                ("ms", C.c_double * TUNE_MAX_CANDIDATES)]
+
+
 P = C.c_void_p
 u64p = C.POINTER(C.c_uint64)
 
@@ -116,6 +130,9 @@ SIGNATURES = {
     "fsg_batch_step": (C.c_int, [P, P, C.c_int, C.c_int]),
     "fsg_batch_step_timed": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "fsg_perworld_step_timed": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "fsg_ctx_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+    "fsg_ctx_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+    "fsg_autotune_batch": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(TuneReport)]),
     "fsg_generate": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, P]),
     "fsg_rule_eval": (C.c_int, [C.POINTER(Config), C.c_int, C.c_uint32, C.c_uint32, P, C.c_int]),
     "fsg_now_ns": (C.c_uint64, []),

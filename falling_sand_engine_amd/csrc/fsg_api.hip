@@ -8,7 +8,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <string>
 #include <new>
 #include <unordered_map>
 #include <vector>
@@ -23,7 +25,6 @@ using namespace fsg_impl;
 
 namespace fsg_impl {
 
-constexpr int kGraphTicks = 16;
 
 int dev_alloc(fsg_ctx *c, void **p, size_t bytes) {
     *p = nullptr;
@@ -121,10 +122,13 @@ unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) 
 
 int launch_tick(fsg_world *w) {
     const unsigned tiles = (unsigned)(w->d.tiles_x * w->d.tiles_y);
+    // chunk flags left stale by a tick without skipping: process every chunk once
+    if (w->d.skip && !w->flags_valid) w->d.force_tick = w->d.tick;
     fsg::k_tick_world<<<tiles, fsg::NT, 0, w->ctx->stream>>>(w->d, 0u);
     HIPCHK(hipGetLastError());
     w->d.tick++;
     w->offered += tiles;
+    w->flags_valid = w->d.skip != 0;
     return FSG_OK;
 }
 
@@ -524,7 +528,7 @@ fsg::StampWorld stamp_world_of(fsg_world *w, uint32_t pix_start) {
     const uint64_t t = w->d.tick;
     sw.cells = w->d.buf[t & 1];
     sw.claims = w->d.buf[(t + 1) & 1];
-    sw.flags = w->d.skip ? w->d.flags[(t + 1) & 1] : nullptr;  // flags the next tick reads
+    sw.flags = w->flags_valid ? w->d.flags[(t + 1) & 1] : nullptr;  // flags the next tick reads
     sw.pixels = w->d_pixels;
     sw.pix_body = w->d_pixbody;
     sw.bodies = w->d_bodies;
@@ -743,6 +747,9 @@ int batch_prepare(fsg_ctx *c, fsg_world *const *ws, int n, bool *graph_ok) {
     uint64_t total = 0;
     for (int i = 0; i < n; i++) {
         dw[i] = ws[i]->d;
+        // the tuning may override the world's skipping; stale flags force a full tick
+        if (c->tune.chunk_skip >= 0) dw[i].skip = (uint32_t)c->tune.chunk_skip;
+        if (dw[i].skip && !ws[i]->flags_valid) dw[i].force_tick = ws[i]->d.tick;
         start[i] = (uint32_t)total;
         total += (uint64_t)ws[i]->d.tiles_x * ws[i]->d.tiles_y;
     }
@@ -753,7 +760,8 @@ int batch_prepare(fsg_ctx *c, fsg_world *const *ws, int n, bool *graph_ok) {
     // host staging buffers must outlive the async copies
     HIPCHK(hipStreamSynchronize(c->stream));
     const std::vector<fsg_world *> key(ws, ws + n);
-    *graph_ok = c->graph && key == c->graph_key && start == c->graph_tiles;
+    *graph_ok = c->graph && key == c->graph_key && start == c->graph_tiles && c->graph_ticks == c->tune.graph_ticks &&
+                c->graph_occ == c->tune.occupancy;
     if (!*graph_ok) {
         c->graph_key = key;
         c->graph_tiles = start;
@@ -764,16 +772,22 @@ int batch_prepare(fsg_ctx *c, fsg_world *const *ws, int n, bool *graph_ok) {
 
 int batch_run(fsg_ctx *c, fsg_world *const *ws, int n, int ticks, bool graph_ok) {
     const unsigned tiles = c->total_tiles;
+    const int T = c->tune.graph_ticks;
+    auto tick_launch = [&](uint32_t k) {
+        if (c->tune.occupancy == 4)
+            fsg::k_tick_batch4<<<tiles, fsg::NT, 0, c->stream>>>(c->d_worlds, c->d_tile_start, n, k);
+        else
+            fsg::k_tick_batch<<<tiles, fsg::NT, 0, c->stream>>>(c->d_worlds, c->d_tile_start, n, k);
+    };
     int done = 0;
-    if (ticks >= kGraphTicks) {
+    if (T > 1 && ticks >= T) {
         if (!graph_ok) {
             if (c->graph) (void)hipGraphExecDestroy(c->graph);
             c->graph = nullptr;
             hipGraph_t g;
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < kGraphTicks; k++)
-                fsg::k_tick_batch<<<tiles, fsg::NT, 0, c->stream>>>(c->d_worlds, c->d_tile_start, n, (uint32_t)k);
-            fsg::k_advance<<<grid_for(n, 256), 256, 0, c->stream>>>(c->d_worlds, n, kGraphTicks);
+            for (int k = 0; k < T; k++) tick_launch((uint32_t)k);
+            fsg::k_advance<<<grid_for(n, 256), 256, 0, c->stream>>>(c->d_worlds, n, (uint32_t)T);
             HIPCHK(hipStreamEndCapture(c->stream, &g));
             const hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -781,19 +795,22 @@ int batch_run(fsg_ctx *c, fsg_world *const *ws, int n, int ticks, bool graph_ok)
                 c->graph = nullptr;
                 return FSG_ERR_DEVICE;
             }
+            c->graph_ticks = T;
+            c->graph_occ = c->tune.occupancy;
         }
-        for (; done + kGraphTicks <= ticks; done += kGraphTicks) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+        for (; done + T <= ticks; done += T) HIPCHK(hipGraphLaunch(c->graph, c->stream));
     }
     const int rem = ticks - done;
     if (rem > 0) {
-        for (int k = 0; k < rem; k++)
-            fsg::k_tick_batch<<<tiles, fsg::NT, 0, c->stream>>>(c->d_worlds, c->d_tile_start, n, (uint32_t)k);
+        for (int k = 0; k < rem; k++) tick_launch((uint32_t)k);
         fsg::k_advance<<<grid_for(n, 256), 256, 0, c->stream>>>(c->d_worlds, n, (uint32_t)rem);
         HIPCHK(hipGetLastError());
     }
+    const bool skip_all = c->tune.chunk_skip > 0, skip_none = c->tune.chunk_skip == 0;
     for (int i = 0; i < n; i++) {
         ws[i]->d.tick += (uint64_t)ticks;
         ws[i]->offered += (uint64_t)ticks * ws[i]->d.tiles_x * ws[i]->d.tiles_y;
+        ws[i]->flags_valid = skip_all || (!skip_none && ws[i]->d.skip);
     }
     return FSG_OK;
 }
@@ -840,5 +857,142 @@ extern "C" int fsg_perworld_step_timed(fsg_ctx *c, fsg_world *const *ws, int n, 
     float f = 0.f;
     HIPCHK(hipEventElapsedTime(&f, c->e0, c->e1));
     if (ms) *ms = f;
+    return FSG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batch autotuner
+// ---------------------------------------------------------------------------
+namespace {
+
+static_assert(sizeof(fsg_tuning) == 16 && sizeof(fsg_tune_report) == 800, "ABI layout of the tuning structs");
+
+bool tuning_ok(const fsg_tuning &t) {
+    const bool ticks_ok = t.graph_ticks == 1 || t.graph_ticks == 4 || t.graph_ticks == 8 || t.graph_ticks == 16 ||
+                          t.graph_ticks == 32;
+    return ticks_ok && t.chunk_skip >= -1 && t.chunk_skip <= 1 && (t.occupancy == 4 || t.occupancy == 5) &&
+           t.reserved == 0;
+}
+
+// World-size class of a batch: for each log2 area bucket (2^10 .. 2^25
+// cells), the bit length of the number of worlds in it, 4 bits per bucket.
+uint64_t size_class(fsg_world *const *ws, int n) {
+    int count[16] = {0};
+    for (int i = 0; i < n; i++) {
+        const uint64_t a = (uint64_t)ws[i]->d.W * ws[i]->d.H;
+        int lg = 0;
+        while ((2ull << lg) <= a) lg++;
+        count[std::min(std::max(lg - 10, 0), 15)]++;
+    }
+    uint64_t key = 0;
+    for (int b = 0; b < 16; b++) {
+        int bits = 0;
+        while ((1 << bits) <= count[b] && bits < 15) bits++;
+        key |= (uint64_t)bits << (4 * b);
+    }
+    return key;
+}
+
+std::mutex g_tune_mu;
+std::map<std::pair<std::string, uint64_t>, fsg_tuning> g_tune_cache;
+
+// Copies the state of src (both buffers, flags, tick) into dst, a world of
+// the same size and configuration.
+int copy_state(fsg_world *dst, const fsg_world *src) {
+    fsg_ctx *c = dst->ctx;
+    const size_t cells = (size_t)src->d.W * src->d.H, nt = (size_t)src->d.tiles_x * src->d.tiles_y;
+    for (int b = 0; b < 2; b++) {
+        HIPCHK(hipMemcpyAsync(dst->d.buf[b], src->d.buf[b], cells * 8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(dst->d.flags[b], src->d.flags[b], nt * 4, hipMemcpyDeviceToDevice, c->stream));
+    }
+    dst->d.tick = src->d.tick;
+    dst->d.force_tick = src->d.force_tick;
+    dst->flags_valid = src->flags_valid;
+    return FSG_OK;
+}
+
+}  // namespace
+
+extern "C" int fsg_ctx_get_tuning(fsg_ctx *c, fsg_tuning *out) {
+    if (!c || !out) return FSG_ERR_INVALID;
+    *out = c->tune;
+    return FSG_OK;
+}
+
+extern "C" int fsg_ctx_set_tuning(fsg_ctx *c, const fsg_tuning *t) {
+    if (!c || !t || !tuning_ok(*t)) return FSG_ERR_INVALID;
+    c->tune = *t;
+    return FSG_OK;
+}
+
+extern "C" int fsg_autotune_batch(fsg_ctx *c, fsg_world *const *ws, int n, int sample_ticks,
+                                  fsg_tune_report *rep) {
+    if (!c || !ws || n <= 0 || sample_ticks <= 0 || sample_ticks > 4096) return FSG_ERR_INVALID;
+    for (int i = 0; i < n; i++)
+        if (!ws[i] || ws[i]->ctx != c) return FSG_ERR_INVALID;
+    HIPCHK(hipSetDevice(c->device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, c->device));
+    const auto key = std::make_pair(std::string(prop.gcnArchName), size_class(ws, n));
+    fsg_tune_report r{};
+    r.size_class = key.second;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto hit = g_tune_cache.find(key);
+        if (hit != g_tune_cache.end()) {
+            c->tune = hit->second;
+            r.best = hit->second;
+            r.from_cache = 1;
+            if (rep) *rep = r;
+            return FSG_OK;
+        }
+    }
+    // copies of the worlds, restored before every timed run
+    std::vector<fsg_world *> cl(n, nullptr);
+    int rc = FSG_OK;
+    for (int i = 0; i < n && !rc; i++) {
+        rc = fsg_world_create(c, ws[i]->d.W, ws[i]->d.H, ws[i]->d.seed, &ws[i]->cfg, &cl[i]);
+        if (!rc) rc = copy_state(cl[i], ws[i]);
+    }
+    const fsg_tuning before = c->tune;
+    std::vector<fsg_tuning> cand;
+    for (int occ : {5, 4})
+        for (int skip : {1, 0})
+            for (int T : {16, 32, 8, 4, 1}) cand.push_back(fsg_tuning{T, skip, occ, 0});
+    double best_ms = 1e30;
+    for (size_t k = 0; k < cand.size() && !rc; k++) {
+        c->tune = cand[k];
+        float f = 0.f;
+        for (int pass = 0; pass < 2 && !rc; pass++) {  // pass 0 builds the graph, pass 1 is timed
+            for (int i = 0; i < n && !rc; i++) rc = copy_state(cl[i], ws[i]);
+            bool graph_ok = false;
+            if (!rc) rc = batch_prepare(c, cl.data(), n, &graph_ok);
+            if (!rc && hipEventRecord(c->e0, c->stream) != hipSuccess) rc = FSG_ERR_DEVICE;
+            if (!rc) rc = batch_run(c, cl.data(), n, sample_ticks, graph_ok);
+            if (!rc && (hipEventRecord(c->e1, c->stream) != hipSuccess || hipEventSynchronize(c->e1) != hipSuccess ||
+                        hipEventElapsedTime(&f, c->e0, c->e1) != hipSuccess))
+                rc = FSG_ERR_DEVICE;
+        }
+        if (rc) break;
+        r.candidate[k] = cand[k];
+        r.ms[k] = f;
+        r.n_candidates = (int)k + 1;
+        if (f < best_ms) {
+            best_ms = f;
+            r.best = cand[k];
+        }
+    }
+    for (fsg_world *w : cl)
+        if (w) fsg_world_destroy(w);
+    if (rc) {
+        c->tune = before;
+        return rc;
+    }
+    c->tune = r.best;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = r.best;
+    }
+    if (rep) *rep = r;
     return FSG_OK;
 }

@@ -33,12 +33,17 @@ struct fsg_ctx {
     std::vector<uint32_t> graph_tiles;
     hipGraphExec_t graph = nullptr;
     uint32_t total_tiles = 0;
+    // batch tuning (fsg_ctx_set_tuning / fsg_autotune_batch) and the tuning
+    // the cached graph was captured with
+    fsg_tuning tune{16, -1, 5, 0};
+    int graph_ticks = 0, graph_occ = 0;
 };
 
 struct fsg_world {
     fsg_ctx *ctx = nullptr;
     fsg_config cfg{};
     DevWorld d{};
+    bool flags_valid = false;  // the chunk flags describe the last tick (it ran with skipping on)
     size_t bytes = 0;
     uint64_t offered = 0;
     unsigned long long *d_stats = nullptr;  // [17] census scratch + [1] processed counter

@@ -125,7 +125,7 @@ constexpr int QCAP = ((36 * 36 + NT - 1) / NT) * 64;
 template <bool TRACK>
 __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW], const uint32_t *T, int it,
                                               uint64_t key, int ox, int oy, int need, int gx0, int gy0,
-                                              bool has_wood, uint16_t *queue) {
+                                              bool has_wood, uint16_t *queue, const uint32_t *perm) {
     // first/last block origin (chunk coordinates) with the phase's parity
     const int x0f = -need - ((need ^ ox) & 1), y0f = -need - ((need ^ oy) & 1);
     const int x0l = (TW - 1 + need) - (((TW - 1 + need) ^ ox) & 1);
@@ -185,14 +185,16 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
                 bands |= band_bits(max(cx, 0), min(cx + 1, TW - 1), max(cy, 0), min(cy + 1, TH - 1));
         }
     };
+    // perm[y] holds, in byte k, the LDS offset (from the block's top-left
+    // cell) of the destination of cell k under scatter mask y
     auto scatter = [&](uint32_t *p0, const uint32_t (&q)[4], uint32_t dec) {
+        const uint32_t po = perm[dec & 0xFFu];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             uint32_t v = q[k];
             if ((dec >> (8 + k)) & 1u) v |= (uint32_t)MOVED;
             if ((dec >> (12 + k)) & 1u) v &= ~(uint32_t)MOVED;
-            const uint32_t d = (uint32_t)k ^ ((dec >> (2 * k)) & 3u);
-            p0[(d & 1u) + (d >> 1) * LW] = v;
+            p0[(po >> (8 * k)) & 0xFFu] = v;
         }
     };
     for (int i = lane; i < nplain; i += 64) {
@@ -385,6 +387,16 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
 #pragma unroll
     for (int k = 0; k < MAX_IT; k++)
         if (k < ci) lut[k][tid] = pv[k];
+    {
+        // scatter offsets for mask y = tid: cell k goes to k ^ ((y >> 2k) & 3)
+        uint32_t po = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t d = (uint32_t)k ^ (((uint32_t)tid >> (2 * k)) & 3u);
+            po |= ((d & 1u) + (d >> 1) * (uint32_t)LW) << (8 * k);
+        }
+        lut[MAX_IT][tid] = po;
+    }
     if (tid < 16) {
         rgba[tid] = mrgba;
         addt[tid] = madd;
@@ -413,9 +425,9 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
             const uint64_t key = splitmix64(seedmix ^ g);
             const int ox = kOX[g & 3], oy = kOY[g & 3];
             if (skip)
-                bands |= run_phase<true>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue);
+                bands |= run_phase<true>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue, lut[MAX_IT]);
             else
-                run_phase<false>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue);
+                run_phase<false>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue, lut[MAX_IT]);
             __syncthreads();
         }
     }
@@ -439,10 +451,15 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         }
     }
     __syncthreads();
+    // the base colours sit right after the region in the same LDS array, so
+    // a cell's colour is one load whose index a select picks
+    const uint32_t *Lf = &L[0][0];
+    const bool xf_on = M.reactions != 0;  // only reactions make XF cells
     auto colour = [&](uint32_t wd, int x, int y) -> uint32_t {
         const int dx = (int)(((wd >> 9) - (uint32_t)x + 4u) & 7u) - 4;
         const int dy = (int)(((wd >> 12) - (uint32_t)y + 4u) & 7u) - 4;
-        return (wd & C_XF) ? rgba[wd & 15u] : L[y + dy][x + dx];
+        const uint32_t idx = (uint32_t)((y + dy) * LW + (x + dx));
+        return Lf[(xf_on && (wd & C_XF)) ? (uint32_t)(LH * LW) + (wd & 15u) : idx];
     };
     if (interior) {
         uint64_t *__restrict__ dst = out + (size_t)(gy0 + ly0) * w.W + gx0 + lx;
@@ -482,9 +499,10 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
 
 // Five workgroups per CU (95 VGPRs, 24 KB of LDS each).
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_tick_world(DevWorld w, uint32_t k) {
-    __shared__ __align__(16) uint32_t L[LH][LW];
-    __shared__ uint32_t lut[MAX_IT][256];
-    __shared__ uint32_t rgba[16];
+    __shared__ __align__(16) uint32_t Lmem[LH * LW + 16];  // the region, then the 16 base colours
+    __shared__ uint32_t lut[MAX_IT + 1][256];               // pair tables + scatter offsets
+    uint32_t(*L)[LW] = reinterpret_cast<uint32_t(*)[LW]>(Lmem);
+    uint32_t *rgba = Lmem + LH * LW;
     __shared__ int32_t addt[16];
     __shared__ uint16_t queue[(NT / 64) * QCAP];
     __shared__ uint32_t red[NT / 64];
@@ -493,12 +511,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
 
 // Many worlds in one launch: tile_start[i] is the first global chunk index of
 // world i (prefix sums, tile_start[n] = total).
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_tick_batch(const DevWorld *__restrict__ worlds,
-                                                   const uint32_t *__restrict__ tile_start,
-                                                   int nworlds, uint32_t k) {
-    __shared__ __align__(16) uint32_t L[LH][LW];
-    __shared__ uint32_t lut[MAX_IT][256];
-    __shared__ uint32_t rgba[16];
+__device__ __forceinline__ void tick_batch_chunk(const DevWorld *__restrict__ worlds,
+                                                 const uint32_t *__restrict__ tile_start, int nworlds, uint32_t k) {
+    __shared__ __align__(16) uint32_t Lmem[LH * LW + 16];  // the region, then the 16 base colours
+    __shared__ uint32_t lut[MAX_IT + 1][256];               // pair tables + scatter offsets
+    uint32_t(*L)[LW] = reinterpret_cast<uint32_t(*)[LW]>(Lmem);
+    uint32_t *rgba = Lmem + LH * LW;
     __shared__ int32_t addt[16];
     __shared__ uint16_t queue[(NT / 64) * QCAP];
     __shared__ uint32_t red[NT / 64];
@@ -511,6 +529,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
     }
     const DevWorld &w = worlds[lo];
     tile_update(w, w.tick + k, (int)(gt - tile_start[lo]), L, lut, rgba, addt, queue, red);
+}
+
+// Two register budgets of the batch tick, one of the autotuner's choices:
+// five workgroups per CU (95 VGPRs) or four (up to 128).
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_tick_batch(
+    const DevWorld *__restrict__ worlds, const uint32_t *__restrict__ tile_start, int nworlds, uint32_t k) {
+    tick_batch_chunk(worlds, tile_start, nworlds, k);
+}
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_tick_batch4(
+    const DevWorld *__restrict__ worlds, const uint32_t *__restrict__ tile_start, int nworlds, uint32_t k) {
+    tick_batch_chunk(worlds, tile_start, nworlds, k);
 }
 
 __global__ void k_advance(DevWorld *worlds, int nworlds, uint32_t ticks) {

@@ -31,6 +31,8 @@ struct StampArgs {
 __global__ void k_tick_world(DevWorld w, uint32_t k);
 __global__ void k_tick_batch(const DevWorld *__restrict__ worlds, const uint32_t *__restrict__ tile_start,
                              int nworlds, uint32_t k);
+__global__ void k_tick_batch4(const DevWorld *__restrict__ worlds, const uint32_t *__restrict__ tile_start,
+                              int nworlds, uint32_t k);
 __global__ void k_advance(DevWorld *worlds, int nworlds, uint32_t ticks);
 __global__ void k_temperature(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int W, int H,
                               MatTab M);

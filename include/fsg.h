@@ -236,6 +236,39 @@ int fsg_perworld_step_timed(fsg_ctx *ctx, fsg_world *const *worlds, int n, int t
                             double *elapsed_ms);
 
 /* ------------------------------------------------------------------------ */
+/* Autotuner for batched launches (no reference counterpart: the reference  */
+/* ticks one world on a CPU thread pool).  Every choice gives bit-identical  */
+/* results; only the speed differs.                                          */
+/* ------------------------------------------------------------------------ */
+typedef struct fsg_tuning {
+    int32_t graph_ticks; /* ticks per captured HIP graph block: 1 (no graph), 4, 8, 16 or 32 */
+    int32_t chunk_skip;  /* dirty-chunk skipping in batches: -1 each world's own setting, 0 off, 1 on */
+    int32_t occupancy;   /* tick-kernel register budget: 4 or 5 workgroups per CU */
+    int32_t reserved;    /* 0 */
+} fsg_tuning;
+
+#define FSG_TUNE_MAX_CANDIDATES 32
+typedef struct fsg_tune_report {
+    fsg_tuning best;
+    int32_t n_candidates;  /* candidates timed (0 when the choice came from the cache) */
+    int32_t from_cache;    /* 1: the cache held a choice for this GPU and world-size class */
+    uint64_t size_class;   /* the cache key's world-size signature */
+    fsg_tuning candidate[FSG_TUNE_MAX_CANDIDATES];
+    double ms[FSG_TUNE_MAX_CANDIDATES]; /* device time of sample_ticks ticks per candidate */
+} fsg_tune_report;
+
+/* Current batch tuning of the context (default {16, -1, 5, 0}). */
+int fsg_ctx_get_tuning(fsg_ctx *ctx, fsg_tuning *out);
+int fsg_ctx_set_tuning(fsg_ctx *ctx, const fsg_tuning *t);
+/* Times every candidate tuning on copies of the given worlds (the worlds
+ * themselves do not advance), sample_ticks ticks each, and makes the fastest
+ * the context's tuning.  Choices are cached per GPU architecture and
+ * world-size class (a histogram of log2 world areas), so a later call on a
+ * similar batch returns at once. */
+int fsg_autotune_batch(fsg_ctx *ctx, fsg_world *const *worlds, int n, int sample_ticks,
+                       fsg_tune_report *report);
+
+/* ------------------------------------------------------------------------ */
 /* Serving: a request loop that keeps worlds resident in HBM and steps them  */
 /* in continuous batches.  The reference runs one world at a fixed 30 TPS    */
 /* from game::run (game.cpp:1009-1018); there is no request API to mirror,   */

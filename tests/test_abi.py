@@ -43,6 +43,8 @@ def test_struct_sizes():
     assert C.sizeof(_abi.Body) == 32
     assert C.sizeof(_abi.Stats) == 168
     assert C.sizeof(_abi.CellEdit) == 16
+    assert C.sizeof(_abi.Tuning) == 16
+    assert C.sizeof(_abi.TuneReport) == 800
 
 
 def test_default_config_matches_oracle_table(orc):
@@ -65,6 +67,10 @@ def test_invalid_arguments_are_rejected_without_device():
     out = np.empty((4, 4), dtype=np.uint64)
     assert lib.fsg_generate(9, 4, 4, 0, out.ctypes.data) == _abi.FSG_ERR_INVALID
     assert lib.fsg_generate(1, 0, 4, 0, out.ctypes.data) == _abi.FSG_ERR_INVALID
+    t = _abi.Tuning(16, -1, 5, 0)
+    assert lib.fsg_ctx_get_tuning(None, C.byref(t)) == _abi.FSG_ERR_INVALID
+    assert lib.fsg_ctx_set_tuning(None, C.byref(t)) == _abi.FSG_ERR_INVALID
+    assert lib.fsg_autotune_batch(None, None, 0, 8, None) == _abi.FSG_ERR_INVALID
 
 
 def test_ctx_create_without_gpu_reports_device_error():

@@ -87,3 +87,49 @@ def test_invalid_cells_rejected(ctx):
         cfg = fsg.default_config(cell_iter=4)   # the fused tick supports cell_iter <= 3
         with pytest.raises(fsg.FsgError):
             fsg.World(ctx, 64, 64, 0, cfg)
+
+
+@pytest.mark.parametrize("tuning", [(1, -1, 5), (4, 1, 5), (8, 0, 4), (32, 1, 4), (16, 0, 5)])
+def test_every_tuning_is_bit_exact(ctx, tuning):
+    """Graph block size, skipping override and kernel variant change speed only."""
+    T, skip, occ = tuning
+    worlds = uneven_worlds(ctx, 6, 700, skip=1, lo=64, hi=320)
+    alone = uneven_worlds(ctx, 6, 700, skip=1, lo=64, hi=320)
+    ctx.set_tuning(graph_ticks=T, chunk_skip=skip, occupancy=occ)
+    try:
+        fsg.batch_step(ctx, [w for w, _, _ in worlds], 41)
+        fsg.batch_step(ctx, [w for w, _, _ in worlds[:3]], 7)   # another world set, then back
+        fsg.batch_step(ctx, [w for w, _, _ in worlds], 5)
+    finally:
+        ctx.set_tuning()
+    for i, (w, _, _) in enumerate(alone):
+        w.tick(53 if i < 3 else 46)
+    for (wb, _, _), (wa, _, _) in zip(worlds, alone):
+        assert np.array_equal(wb.download(), wa.download())
+    # a world stepped alone afterwards (its own skipping) still matches
+    worlds[0][0].tick(9)
+    alone[0][0].tick(9)
+    assert np.array_equal(worlds[0][0].download(), alone[0][0].download())
+    for w, _, _ in worlds + alone:
+        w.close()
+
+
+def test_autotune_leaves_worlds_untouched_and_caches(ctx):
+    worlds = uneven_worlds(ctx, 5, 900, skip=1, lo=64, hi=256)
+    before = [w.download() for w, _, _ in worlds]
+    base = ctx.usage()
+    rep = ctx.autotune([w for w, _, _ in worlds], sample_ticks=8)
+    assert not rep["from_cache"]
+    assert len(rep["candidates"]) == 20
+    assert all(c["ms"] > 0 for c in rep["candidates"])
+    best = min(rep["candidates"], key=lambda c: c["ms"])
+    assert rep["best"] == {k: best[k] for k in ("graph_ticks", "chunk_skip", "occupancy")}
+    assert ctx.tuning() == rep["best"]
+    assert ctx.usage() == base                      # the copies are gone
+    for (w, _, _), b in zip(worlds, before):
+        assert w.tick_count == 0 and np.array_equal(w.download(), b)
+    again = ctx.autotune([w for w, _, _ in worlds], sample_ticks=8)
+    assert again["from_cache"] and again["best"] == rep["best"]
+    ctx.set_tuning()
+    for w, _, _ in worlds:
+        w.close()
