@@ -451,15 +451,13 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         }
     }
     __syncthreads();
-    // the base colours sit right after the region in the same LDS array, so
-    // a cell's colour is one load whose index a select picks
+    // a reaction product takes its material's base colour; any other cell
+    // the colour of its origin, which is within 4 cells in x and y
     const uint32_t *Lf = &L[0][0];
-    const bool xf_on = M.reactions != 0;  // only reactions make XF cells
     auto colour = [&](uint32_t wd, int x, int y) -> uint32_t {
         const int dx = (int)(((wd >> 9) - (uint32_t)x + 4u) & 7u) - 4;
         const int dy = (int)(((wd >> 12) - (uint32_t)y + 4u) & 7u) - 4;
-        const uint32_t idx = (uint32_t)((y + dy) * LW + (x + dx));
-        return Lf[(xf_on && (wd & C_XF)) ? (uint32_t)(LH * LW) + (wd & 15u) : idx];
+        return (wd & C_XF) ? rgba[wd & 15u] : Lf[(y + dy) * LW + (x + dx)];
     };
     if (interior) {
         uint64_t *__restrict__ dst = out + (size_t)(gy0 + ly0) * w.W + gx0 + lx;
