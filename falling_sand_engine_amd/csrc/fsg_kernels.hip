@@ -288,8 +288,18 @@ __device__ __forceinline__ bool final_words(const MatTab &M, const uint32_t (*L)
     return tchg;
 }
 
-constexpr int NV = LH * (LW / 2);       // 2-cell pairs in the LDS region
-constexpr int KV = (NV + NT - 1) / NT;  // pairs per thread
+// Region map of the load and of the colour plane.  The region is 72 rows of
+// 36 two-cell pairs.  Thread t < 252 owns pair column t % 36 of the rows
+// 1 + t / 36 + 7k (k < 10), which are rows 1 .. 70: every row a cell of the
+// chunk can come from in one tick (MAX_IT rows at most), so these are also
+// the colours the thread holds.  Threads t < 72 load the two edge rows 0
+// and 71 besides.  The rows of one thread are 7 apart, so its addresses
+// differ by constants: the address arithmetic is done once, not per pair.
+constexpr int PR = LW / 2;         // two-cell pairs per region row
+constexpr int RS = NT / PR;        // rows one sweep of the workgroup covers (7)
+constexpr int KC = (LH - 2) / RS;  // sweeps over rows 1 .. LH-2 (10)
+static_assert(RS * KC == LH - 2 && HALO - MAX_IT >= 1, "sweeps cover rows 1 .. LH-2 exactly");
+static_assert((uint64_t)FSG_MAX_SIDE * 8u * LH < (1ull << 32), "32-bit byte offsets within a region");
 
 __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, int tile, uint32_t (*L)[LW],
                                             uint32_t (*lut)[256], uint32_t *rgba, int32_t *addt,
@@ -341,48 +351,61 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     // A chunk whose halo lies inside an even-width world takes the fast
     // paths: 16-byte loads/stores with no bounds or wall checks.
     const bool interior = gx0 >= 0 && gy0 >= 0 && gx0 + LW <= w.W && gy0 + LH <= w.H && (w.W & 1) == 0;
-    uint4 v[KV];
+    const bool mine = tid < RS * PR, edge = tid < 2 * PR;
+    const int rr = tid / PR, c2 = tid - rr * PR;                      // sweep rows 1 + rr + RS * k
+    const int ey = tid < PR ? 0 : LH - 1, ec2 = tid < PR ? tid : tid - PR;  // edge row (edge threads)
+    uint4 v[KC], ve;
     if (interior) {
-        const uint64_t *__restrict__ src = in + (size_t)gy0 * w.W + gx0;
+        // a uniform row base per sweep plus one 32-bit offset per thread
+        const char *__restrict__ src = reinterpret_cast<const char *>(in + (size_t)gy0 * w.W + gx0);
+        const uint32_t rowb = (uint32_t)w.W * 8u;
+        const uint32_t off = (uint32_t)(1 + rr) * rowb + 16u * (uint32_t)c2;
+        if (mine) {
 #pragma unroll
-        for (int k = 0; k < KV; k++) {
-            const int j = tid + k * NT;
-            if (k < KV - 1 || j < NV) {
-                const int ly = j / (LW / 2), c2 = j - ly * (LW / 2);
-                v[k] = *reinterpret_cast<const uint4 *>(src + (size_t)__umul24(ly, w.W) + 2 * c2);
-            }
+            for (int k = 0; k < KC; k++)
+                v[k] = *reinterpret_cast<const uint4 *>(src + (off + (uint32_t)(RS * k) * rowb));
         }
+        if (edge) ve = *reinterpret_cast<const uint4 *>(src + (uint32_t)ey * rowb + 16u * (uint32_t)ec2);
     } else {
+        auto load = [&](int ly, int p) -> uint4 {
+            const int gx = gx0 + 2 * p, gy = gy0 + ly;
+            const bool rin = gy >= 0 && gy < w.H;
+            const int64_t row = (int64_t)gy * w.W;
+            const uint64_t a = (rin && gx >= 0 && gx < w.W) ? in[row + gx] : WALL;
+            const uint64_t b = (rin && gx + 1 >= 0 && gx + 1 < w.W) ? in[row + gx + 1] : WALL;
+            return make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
+        };
+        if (mine) {
 #pragma unroll
-        for (int k = 0; k < KV; k++) {
-            const int j = tid + k * NT;
-            if (k < KV - 1 || j < NV) {
-                const int ly = j / (LW / 2), gx = gx0 + 2 * (j - ly * (LW / 2)), gy = gy0 + ly;
-                const bool rin = gy >= 0 && gy < w.H;
-                const int64_t row = (int64_t)gy * w.W;
-                const uint64_t a = (rin && gx >= 0 && gx < w.W) ? in[row + gx] : WALL;
-                const uint64_t b = (rin && gx + 1 >= 0 && gx + 1 < w.W) ? in[row + gx + 1] : WALL;
-                v[k] = make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
-            }
+            for (int k = 0; k < KC; k++) v[k] = load(1 + rr + RS * k, c2);
+        }
+        if (edge) ve = load(ey, ec2);
+    }
+    // compact cells into LDS (origin residues in bits 9-14), colours kept in
+    // registers until the write-back
+    uint32_t cola[KC], colb[KC];  // colours of the pair's two cells
+    bool wood = false;
+    if (mine) {
+        uint32_t *Lt = &L[1 + rr][2 * c2];
+        const uint32_t orgx = ((uint32_t)(2 * c2) & 7u) << 9, oy = (uint32_t)(1 + rr) << 12;
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            const uint32_t org = orgx | ((oy + ((uint32_t)(RS * k) << 12)) & 0x7000u);  // (row & 7) << 12
+            *reinterpret_cast<uint2 *>(Lt + RS * k * LW) = make_uint2(v[k].x | org, v[k].z | org | (1u << 9));
+            cola[k] = v[k].y;
+            colb[k] = v[k].w;
         }
     }
-    // compact cells into LDS, colours kept in registers until the write-back
-    // (not those of the last LDS row: a cell moves at most one row per
-    // phase, so no cell of the chunk comes from HALO rows away)
-    static_assert(MAX_IT < HALO, "write-back colours cover the rings cells can come from");
-    constexpr int KC = (LW / 2 * (LH - 1) + NT - 1) / NT;  // pair slots holding rows 0 .. LH-2
-    uint2 col[KC];
-    bool wood = false;
+    if (edge) {
+        const uint32_t org = (((uint32_t)(2 * ec2) & 7u) << 9) | (((uint32_t)ey & 7u) << 12);
+        *reinterpret_cast<uint2 *>(&L[ey][2 * ec2]) = make_uint2(ve.x | org, ve.z | org | (1u << 9));
+    }
+    if (M.reactions) {
+        if (mine) {
 #pragma unroll
-    for (int k = 0; k < KV; k++) {
-        const int j = tid + k * NT;
-        if (k < KV - 1 || j < NV) {
-            const int ly = j / (LW / 2), c2 = j - ly * (LW / 2);
-            const uint32_t org = (((uint32_t)(2 * c2) & 7u) << 9) | (((uint32_t)ly & 7u) << 12);
-            *reinterpret_cast<uint2 *>(&L[ly][2 * c2]) = make_uint2(v[k].x | org, v[k].z | org | (1u << 9));
-            if (k < KC) col[k] = make_uint2(v[k].y, v[k].w);
-            wood |= (v[k].x & 0xFFu) == M.wood || (v[k].z & 0xFFu) == M.wood;
+            for (int k = 0; k < KC; k++) wood |= (v[k].x & 0xFFu) == M.wood || (v[k].z & 0xFFu) == M.wood;
         }
+        if (edge) wood |= (ve.x & 0xFFu) == M.wood || (ve.z & 0xFFu) == M.wood;
     }
 #pragma unroll
     for (int k = 0; k < MAX_IT; k++)
@@ -441,13 +464,13 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
                                : final_words<true>(M, L, addt, temp_now, lx, ly0, fw);
     if (skip && tchg) bands |= band_bits(2 * j, 2 * j + 1, rg * (TH / 8), rg * (TH / 8) + TH / 8 - 1);
     __syncthreads();
-    // the colour plane: each cell's colour at its origin
+    // the colour plane: each cell's colour at its origin (rows 1 .. LH-2)
+    if (mine) {
+        uint32_t *Lt = &L[1 + rr][2 * c2];
 #pragma unroll
-    for (int k = 0; k < KC; k++) {
-        const int jj = tid + k * NT;
-        if (jj < NV) {
-            const int ly = jj / (LW / 2), c2 = jj - ly * (LW / 2);
-            *reinterpret_cast<uint2 *>(&L[ly][2 * c2]) = col[k];
+        for (int k = 0; k < KC; k++) {
+            Lt[RS * k * LW] = cola[k];
+            Lt[RS * k * LW + 1] = colb[k];
         }
     }
     __syncthreads();
