@@ -474,22 +474,43 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         }
     }
     __syncthreads();
-    // a reaction product takes its material's base colour; any other cell
-    // the colour of its origin, which is within 4 cells in x and y
+    // A reaction product takes its material's base colour; any other cell
+    // the colour of its origin, which is
+    // within 3 cells in x and y: the signed 3-bit differences of the origin
+    // residues and the cell's own coordinates.
+    // (The offset dy * LW + dx is pinned as two v_bfe_i32 and one
+    // v_mad_i32_i24: left to itself the compiler rewrites the extracts as
+    // shift pairs and folds y into a 32-bit multiply, ~10 instead of 6 VALU
+    // per cell.  Register-only instructions.)
     const uint32_t *Lf = &L[0][0];
-    auto colour = [&](uint32_t wd, int x, int y) -> uint32_t {
-        const int dx = (int)(((wd >> 9) - (uint32_t)x + 4u) & 7u) - 4;
-        const int dy = (int)(((wd >> 12) - (uint32_t)y + 4u) & 7u) - 4;
-        return (wd & C_XF) ? rgba[wd & 15u] : Lf[(y + dy) * LW + (x + dx)];
+    auto origin = [&](uint32_t wd, int x, int y) -> int {
+        int dx, dy, d;
+        asm("v_bfe_i32 %0, %1, 9, 3" : "=v"(dx) : "v"(wd - ((uint32_t)x << 9)));    // bits 9-11
+        asm("v_bfe_i32 %0, %1, 12, 3" : "=v"(dy) : "v"(wd - ((uint32_t)y << 12)));  // bits 12-14
+        asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(dy), "s"(LW), "v"(dx));
+        return d + (y * LW + x);
     };
+    auto colour = [&](uint32_t wd, int x, int y) -> uint32_t {
+        return *((wd & C_XF) ? rgba + (wd & 15u) : Lf + origin(wd, x, y));
+    };
+    const bool xf_on = M.reactions != 0;  // only reactions make XF cells
     if (interior) {
-        uint64_t *__restrict__ dst = out + (size_t)(gy0 + ly0) * w.W + gx0 + lx;
+        // a uniform base plus a 32-bit offset per row
+        char *__restrict__ dstb = reinterpret_cast<char *>(out + (size_t)(gy0 + HALO) * w.W + (gx0 + HALO));
+        const uint32_t rowb = (uint32_t)w.W * 8u;
+        const uint32_t voff = (uint32_t)(ly0 - HALO) * rowb + 16u * (uint32_t)j;
 #pragma unroll
         for (int r = 0; r < TH / 8; r++) {
             const uint32_t a = fw[2 * r], b = fw[2 * r + 1];
-            const uint32_t ca = colour(a, lx, ly0 + r), cb = colour(b, lx + 1, ly0 + r);
-            *reinterpret_cast<uint4 *>(dst + (size_t)__umul24(r, w.W)) =
-                make_uint4(a & C_KEEP, ca, b & C_KEEP, cb);
+            uint32_t ca, cb;
+            if (xf_on) {
+                ca = colour(a, lx, ly0 + r);
+                cb = colour(b, lx + 1, ly0 + r);
+            } else {
+                ca = Lf[origin(a, lx, ly0 + r)];
+                cb = Lf[origin(b, lx + 1, ly0 + r)];
+            }
+            *reinterpret_cast<uint4 *>(dstb + (voff + (uint32_t)r * rowb)) = make_uint4(a & C_KEEP, ca, b & C_KEEP, cb);
         }
     } else {
 #pragma unroll
