@@ -251,8 +251,9 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
 
 // Final compact words of a thread's 2 x 8 output cells (columns lx, lx + 1,
 // rows ly0 .. ly0 + 7), with the temperature step applied when temp_now; the
-// stencil slides down the strip.  Returns whether a temperature changed.
-template <bool WALLS>
+// stencil slides down the strip.  With TCHG, returns whether a temperature
+// changed (only chunk skipping needs to know).
+template <bool WALLS, bool TCHG = true>
 __device__ __forceinline__ bool final_words(const MatTab &M, const uint32_t (*L)[LW], const int32_t *addt,
                                             bool temp_now, int lx, int ly0, uint32_t (&fw)[2 * (TH / 8)]) {
     bool tchg = false;
@@ -279,7 +280,7 @@ __device__ __forceinline__ bool final_words(const MatTab &M, const uint32_t (*L)
         if (!WALLS || cid(cur.y) != WALL)
             v1 = temp_of(M, addt, t1, cid(cur.y), ntemp<WALLS>(t1, up.y), ntemp<WALLS>(t1, dn.y),
                          ntemp<WALLS>(t1, cur.x), ntemp<WALLS>(t1, cr));
-        tchg |= (v0 != t0) | (v1 != t1);
+        if (TCHG) tchg |= (v0 != t0) | (v1 != t1);
         fw[2 * r] = (cur.x & 0xFFFFu) | ((uint32_t)v0 << 16);
         fw[2 * r + 1] = (cur.y & 0xFFFFu) | ((uint32_t)v1 << 16);
         up = cur;
@@ -460,8 +461,10 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     const int j = tid & 31, rg = tid >> 5;
     const int lx = HALO + 2 * j, ly0 = HALO + rg * (TH / 8);
     uint32_t fw[2 * (TH / 8)];
-    const bool tchg = interior ? final_words<false>(M, L, addt, temp_now, lx, ly0, fw)
-                               : final_words<true>(M, L, addt, temp_now, lx, ly0, fw);
+    const bool tchg = skip ? (interior ? final_words<false>(M, L, addt, temp_now, lx, ly0, fw)
+                                       : final_words<true>(M, L, addt, temp_now, lx, ly0, fw))
+                           : (interior ? final_words<false, false>(M, L, addt, temp_now, lx, ly0, fw)
+                                       : final_words<true, false>(M, L, addt, temp_now, lx, ly0, fw));
     if (skip && tchg) bands |= band_bits(2 * j, 2 * j + 1, rg * (TH / 8), rg * (TH / 8) + TH / 8 - 1);
     __syncthreads();
     // the colour plane: each cell's colour at its origin (rows 1 .. LH-2)
