@@ -15,14 +15,20 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._abi import (AIR, GAS, MOVED, SAND, SOLID, SOUP, Body, Cell, CellEdit, Config,  # noqa: F401
-                   FsgError, Stats, StampResult, check, lib)
+from ._abi import (AIR, GAS, LAYER_CELLS, LAYER_EMISSION, LAYER_FIRE, MOVED, SAND, SOLID, SOUP,  # noqa: F401
+                   Body, Cell, CellEdit, Config, FsgError, RenderParams, RenderStats, Stats, StampResult,
+                   check, lib)
 
 __all__ = ["Context", "World", "default_config", "generate", "make_cell", "cell_mat",
            "cell_temp", "cell_rgba", "batch_step", "FsgError", "Body", "Config"]
 
 MAT_AIR, MAT_STONE, MAT_SAND, MAT_WATER, MAT_OIL, MAT_STEAM, MAT_LAVA, MAT_WOOD = range(8)
 MATERIAL_NAMES = ["air", "stone", "sand", "water", "oil", "steam", "lava", "wood"]
+
+# Material::emitColor of the benchmark materials, 0xAARRGGBB: water and lava
+# glow (game_datastruct.cpp:91-92), the others emit nothing.
+EMIT_DEFAULT = [0, 0, 0, 0x3000AFB5, 0, 0, 0xFFFF6900, 0] + [0] * 8
+NO_FIRE = 255
 
 
 def default_config(**overrides) -> Config:
@@ -225,6 +231,41 @@ class World:
         out = np.empty(npixels, dtype=np.uint64)
         check(lib().fsg_world_unstamp_bodies(self.handle, out.ctypes.data, npixels), "fsg_world_unstamp_bodies")
         return out
+
+    # --- texture pass (game.cpp:1994-2067) -----------------------------------
+    def render(self, layers=LAYER_CELLS, rect=None, fire_mat=NO_FIRE, emit=None, stats=False,
+               count_moved=True):
+        """Updates the RGBA8 layers over rect = (x0, y0, w, h) (whole world
+        by default).  With stats, waits and returns the movingTiles census."""
+        self._flush()
+        p = RenderParams()
+        p.layers = layers
+        p.x0, p.y0, p.w, p.h = rect if rect is not None else (0, 0, 0, 0)
+        p.fire_mat = fire_mat
+        p.count_moved = 1 if count_moved else 0
+        for m, e in enumerate(EMIT_DEFAULT if emit is None else emit):
+            p.emit[m] = e
+        st = RenderStats()
+        check(lib().fsg_world_render(self.handle, C.byref(p), C.byref(st) if stats else None),
+              "fsg_world_render")
+        if stats:
+            return {"moved": [int(v) for v in st.moved], "fire_cells": int(st.fire_cells),
+                    "had_dirty": bool(st.had_dirty)}
+        return None
+
+    def layer(self, layer, x0=0, y0=0, w=None, h=None) -> np.ndarray:
+        """Host copy of a layer rectangle as (h, w) uint32 texels (bytes R,G,B,A)."""
+        w = self.width if w is None else w
+        h = self.height if h is None else h
+        out = np.empty((h, w), dtype=np.uint32)
+        check(lib().fsg_world_layer_download(self.handle, layer, x0, y0, w, h, out.ctypes.data),
+              "fsg_world_layer_download")
+        return out
+
+    def layer_ptr(self, layer) -> int:
+        ptr = C.c_void_p()
+        check(lib().fsg_world_layer(self.handle, layer, C.byref(ptr)), "fsg_world_layer")
+        return ptr.value or 0
 
     def close(self):
         if self.handle:

@@ -100,6 +100,20 @@ class TuneReport(C.Structure):
                 ("ms", C.c_double * TUNE_MAX_CANDIDATES)]
 
 
+LAYER_CELLS, LAYER_EMISSION, LAYER_FIRE = 1, 2, 4
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("layers", C.c_uint32), ("x0", C.c_int32), ("y0", C.c_int32), ("w", C.c_int32),
+                ("h", C.c_int32), ("fire_mat", C.c_uint32), ("count_moved", C.c_uint32),
+                ("emit", C.c_uint32 * 16)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("moved", C.c_uint64 * 16), ("fire_cells", C.c_uint64), ("had_dirty", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
 P = C.c_void_p
 u64p = C.POINTER(C.c_uint64)
 
@@ -127,6 +141,9 @@ SIGNATURES = {
     "fsg_world_reset_activity": (C.c_int, [P]),
     "fsg_world_stamp_bodies": (C.c_int, [P, P, C.c_int, P, C.c_size_t, C.POINTER(StampResult)]),
     "fsg_world_unstamp_bodies": (C.c_int, [P, P, C.c_size_t]),
+    "fsg_world_render": (C.c_int, [P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
+    "fsg_world_layer": (C.c_int, [P, C.c_uint32, C.POINTER(P)]),
+    "fsg_world_layer_download": (C.c_int, [P, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int, P]),
     "fsg_batch_step": (C.c_int, [P, P, C.c_int, C.c_int]),
     "fsg_batch_step_timed": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "fsg_perworld_step_timed": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(C.c_double)]),

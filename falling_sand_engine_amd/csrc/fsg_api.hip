@@ -129,6 +129,7 @@ int launch_tick(fsg_world *w) {
     w->d.tick++;
     w->offered += tiles;
     w->flags_valid = w->d.skip != 0;
+    w->prev_valid = true;
     return FSG_OK;
 }
 
@@ -353,6 +354,7 @@ extern "C" int fsg_world_destroy(fsg_world *w) {
     dev_free(w->d_state);
     dev_free(w->d_disp);
     dev_free(w->d_dispxy);
+    for (uint32_t *l : w->d_layer) dev_free(l);
     dev_free(w->d_sw);
     delete w;
     return FSG_OK;
@@ -372,6 +374,7 @@ extern "C" int fsg_world_set_tick(fsg_world *w, uint64_t tick) {
     if ((tick & 1) != (w->d.tick & 1)) std::swap(w->d.buf[0], w->d.buf[1]);
     w->d.tick = tick;
     w->d.force_tick = tick;
+    w->prev_valid = false;
     return FSG_OK;
 }
 
@@ -384,6 +387,7 @@ extern "C" int fsg_world_upload(fsg_world *w, const fsg_cell *cells, size_t n) {
     HIPCHK(hipStreamSynchronize(w->ctx->stream));
     w->d.force_tick = w->d.tick;
     w->stamped = false;
+    w->prev_valid = false;
     return FSG_OK;
 }
 
@@ -809,6 +813,7 @@ int batch_run(fsg_ctx *c, fsg_world *const *ws, int n, int ticks, bool graph_ok)
     const bool skip_all = c->tune.chunk_skip > 0, skip_none = c->tune.chunk_skip == 0;
     for (int i = 0; i < n; i++) {
         ws[i]->d.tick += (uint64_t)ticks;
+        if (ticks > 0) ws[i]->prev_valid = true;
         ws[i]->offered += (uint64_t)ticks * ws[i]->d.tiles_x * ws[i]->d.tiles_y;
         ws[i]->flags_valid = skip_all || (!skip_none && ws[i]->d.skip);
     }

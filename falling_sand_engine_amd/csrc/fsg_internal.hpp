@@ -44,6 +44,8 @@ struct fsg_world {
     fsg_config cfg{};
     DevWorld d{};
     bool flags_valid = false;  // the chunk flags describe the last tick (it ran with skipping on)
+    bool prev_valid = false;   // buf[(tick + 1) & 1] holds the state before the last tick
+    uint32_t *d_layer[3] = {nullptr, nullptr, nullptr};  // texture layers (fsg_world_render)
     size_t bytes = 0;
     uint64_t offered = 0;
     unsigned long long *d_stats = nullptr;  // [17] census scratch + [1] processed counter

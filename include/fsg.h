@@ -224,6 +224,45 @@ int fsg_world_stamp_bodies(fsg_world *w, const fsg_body *bodies, int nbodies,
 int fsg_world_unstamp_bodies(fsg_world *w, fsg_cell *pixels_out, size_t npixels);
 
 /* ------------------------------------------------------------------------ */
+/* Texture pass (game.cpp:1994-2067): the grid as RGBA8 layers in device     */
+/* memory, byte order R,G,B,A per texel, row-major with pitch width texels.  */
+/* The reference rewrites the texels of dirty cells after every tick; every  */
+/* texel it writes is a function of the cell alone (fire layer: of the cell  */
+/* and the old texel), so one pass over a rectangle gives the same texture   */
+/* without per-cell dirty flags.  Layers are allocated (zeroed: transparent) */
+/* on first use and counted against the context's HBM budget.                */
+/* ------------------------------------------------------------------------ */
+#define FSG_LAYER_CELLS 1u    /* pixelsCells_ar: cell colour; AIR cells transparent 0 */
+#define FSG_LAYER_EMISSION 2u /* pixelsEmission_ar: Material::emitColor; AIR 0        */
+#define FSG_LAYER_FIRE 4u     /* pixelsFire_ar: fire cells' colour; AIR 0; others kept */
+
+typedef struct fsg_render_params {
+    uint32_t layers;         /* FSG_LAYER_* mask                                      */
+    int32_t x0, y0, w, h;    /* rectangle, clipped to the world; w or h <= 0: whole world */
+    uint32_t fire_mat;       /* material id drawn on the fire layer (FIRE); 255: none  */
+    uint32_t count_moved;    /* 1: fill fsg_render_stats (reads the state before the last tick) */
+    uint32_t emit[16];       /* Material::emitColor per material id, 0xAARRGGBB        */
+} fsg_render_params;
+
+typedef struct fsg_render_stats {
+    uint64_t moved[16];      /* movingTiles (game.cpp:1999): cells in the rectangle whose
+                              * material or colour changed in the last tick, by new material */
+    uint64_t fire_cells;     /* fire cells drawn (hadFire)                             */
+    uint32_t had_dirty;      /* any moved cell (hadDirty)                              */
+    uint32_t reserved;
+} fsg_render_stats;
+
+/* Updates the requested layers over the rectangle.  Enqueued on the ctx
+ * stream; with stats != NULL it waits and fills *stats (all zero if the
+ * world has not ticked since its last upload). */
+int fsg_world_render(fsg_world *w, const fsg_render_params *p, fsg_render_stats *stats);
+/* Device address of a layer (NULL until first rendered); pitch = width texels. */
+int fsg_world_layer(fsg_world *w, uint32_t layer, uint32_t **device_ptr);
+/* Rectangle device->host copy of a layer (R_UpdateImageBytes' source). */
+int fsg_world_layer_download(fsg_world *w, uint32_t layer, int x0, int y0, int rw, int rh,
+                             uint32_t *out);
+
+/* ------------------------------------------------------------------------ */
 /* Batching: many worlds, one launch per tick over a multi-world chunk list, */
 /* ticks captured as HIP graphs.  Results are bit-identical to stepping each */
 /* world alone.                                                              */

@@ -648,6 +648,78 @@ uint64_t orc_checksum(const uint64_t *cells, size_t n) {
 }
 
 /* ----------------------------------------------------------------------- */
+/* Texture pass (game.cpp:1994-2067)                                        */
+/* ----------------------------------------------------------------------- */
+/* The build's cell colour is rgba with bytes R,G,B,A in memory; the
+ * reference keeps MaterialInstance::color = 0x00RRGGBB and Material::alpha
+ * apart.  Each texel is assembled byte by byte as game.cpp does it, from
+ * those reference-side values. */
+static uint32_t texel_of_bytes(const uint8_t b[4]) {
+    return (uint32_t)b[0] | (uint32_t)b[1] << 8 | (uint32_t)b[2] << 16 | (uint32_t)b[3] << 24;
+}
+
+/* dpixels_ar for a non-AIR cell (game.cpp:2017-2021). */
+static uint32_t cell_texel(uint64_t cell) {
+    const uint32_t rgba = (uint32_t)(cell >> 32);
+    const uint32_t color = (rgba & 0xFFu) << 16 | ((rgba >> 8) & 0xFFu) << 8 | ((rgba >> 16) & 0xFFu);
+    const uint8_t alpha = (uint8_t)(rgba >> 24);
+    uint8_t b[4];
+    b[2] = (uint8_t)((color >> 0) & 0xff);
+    b[1] = (uint8_t)((color >> 8) & 0xff);
+    b[0] = (uint8_t)((color >> 16) & 0xff);
+    b[3] = alpha;
+    return texel_of_bytes(b);
+}
+
+/* dpixelsEmission_ar for a non-AIR cell (game.cpp:2023-2027). */
+static uint32_t emit_texel(uint32_t emit) {
+    uint8_t b[4];
+    b[2] = (uint8_t)((emit >> 0) & 0xff);
+    b[1] = (uint8_t)((emit >> 8) & 0xff);
+    b[0] = (uint8_t)((emit >> 16) & 0xff);
+    b[3] = (uint8_t)((emit >> 24) & 0xff);
+    return texel_of_bytes(b);
+}
+
+int orc_render(const uint64_t *cells, const uint64_t *prev, int W, int H, const fsg_config *cfg,
+               const fsg_render_params *p, uint32_t *lc, uint32_t *le, uint32_t *lf,
+               fsg_render_stats *st) {
+    if (!cells || !cfg || !p || W <= 0 || H <= 0) return -1;
+    if (st) memset(st, 0, sizeof(*st));
+    int64_t x0 = 0, y0 = 0, x1 = W, y1 = H;
+    if (p->w > 0 && p->h > 0) {
+        x0 = p->x0 < 0 ? 0 : p->x0;
+        y0 = p->y0 < 0 ? 0 : p->y0;
+        x1 = (int64_t)p->x0 + p->w < W ? (int64_t)p->x0 + p->w : W;
+        y1 = (int64_t)p->y0 + p->h < H ? (int64_t)p->y0 + p->h : H;
+    }
+    for (int64_t y = y0; y < y1; y++)
+        for (int64_t x = x0; x < x1; x++) {
+            const size_t i = (size_t)y * (size_t)W + (size_t)x;
+            const uint64_t c = cells[i];
+            const uint32_t mat = (uint32_t)(c & 0xFF);
+            const int air = mat < cfg->n_materials && cfg->materials[mat].type == FSG_AIR;
+            if (prev && st && ((c ^ prev[i]) & 0xFFFFFFFF000000FFull)) {
+                st->moved[mat & 15]++; /* movingTiles[mat->id]++ (game.cpp:1999) */
+                st->had_dirty = 1;
+            }
+            if (air) { /* game.cpp:2000-2016 */
+                if (lc && (p->layers & FSG_LAYER_CELLS)) lc[i] = 0;
+                if (le && (p->layers & FSG_LAYER_EMISSION)) le[i] = 0;
+                if (lf && (p->layers & FSG_LAYER_FIRE)) lf[i] = 0;
+                continue;
+            }
+            if (lc && (p->layers & FSG_LAYER_CELLS)) lc[i] = cell_texel(c);
+            if (le && (p->layers & FSG_LAYER_EMISSION)) le[i] = emit_texel(p->emit[mat & 15]);
+            if (mat == p->fire_mat) { /* game.cpp:2029-2035 */
+                if (lf && (p->layers & FSG_LAYER_FIRE)) lf[i] = cell_texel(c);
+                if (st) st->fire_cells++;
+            }
+        }
+    return 0;
+}
+
+/* ----------------------------------------------------------------------- */
 /* The benchmark material table                                             */
 /* ----------------------------------------------------------------------- */
 /* Eight materials: the reference's GENERIC_AIR, STONE, GENERIC_SAND, WATER,

@@ -60,6 +60,13 @@ int orc_unstamp(uint64_t *cells, int W, int H, const fsg_config *cfg, const fsg_
                 int nbodies, uint64_t *pixels, size_t npixels);
 int orc_body_pixel_xy(const fsg_body *b, int tx, int ty, int *wx, int *wy);
 
+/* Texture pass (game.cpp:1994-2067) over p's rectangle: the cells, emission
+ * and fire layers (any may be NULL), and with prev (the state before the
+ * last tick) and st the movingTiles census.  Layers are updated in place. */
+int orc_render(const uint64_t *cells, const uint64_t *prev, int W, int H, const fsg_config *cfg,
+               const fsg_render_params *p, uint32_t *lc, uint32_t *le, uint32_t *lf,
+               fsg_render_stats *st);
+
 void orc_counts(const uint64_t *cells, size_t n, uint64_t counts[16]);
 uint64_t orc_checksum(const uint64_t *cells, size_t n);
 

@@ -45,6 +45,8 @@ def test_struct_sizes():
     assert C.sizeof(_abi.CellEdit) == 16
     assert C.sizeof(_abi.Tuning) == 16
     assert C.sizeof(_abi.TuneReport) == 800
+    assert C.sizeof(_abi.RenderParams) == 92
+    assert C.sizeof(_abi.RenderStats) == 144
 
 
 def test_default_config_matches_oracle_table(orc):
@@ -71,6 +73,12 @@ def test_invalid_arguments_are_rejected_without_device():
     assert lib.fsg_ctx_get_tuning(None, C.byref(t)) == _abi.FSG_ERR_INVALID
     assert lib.fsg_ctx_set_tuning(None, C.byref(t)) == _abi.FSG_ERR_INVALID
     assert lib.fsg_autotune_batch(None, None, 0, 8, None) == _abi.FSG_ERR_INVALID
+    rp = _abi.RenderParams()
+    assert lib.fsg_world_render(None, C.byref(rp), None) == _abi.FSG_ERR_INVALID
+    ptr = C.c_void_p()
+    assert lib.fsg_world_layer(None, _abi.LAYER_CELLS, C.byref(ptr)) == _abi.FSG_ERR_INVALID
+    tex = np.empty((4, 4), dtype=np.uint32)
+    assert lib.fsg_world_layer_download(None, _abi.LAYER_CELLS, 0, 0, 4, 4, tex.ctypes.data) == _abi.FSG_ERR_INVALID
 
 
 def test_ctx_create_without_gpu_reports_device_error():
