@@ -223,6 +223,44 @@ def bench_c5(fsg, rates=(30.0, 120.0, 400.0), duration_s=6.0, budget=64 << 30, s
             "rates": out}
 
 
+def bench_render(fsg, side=16384, seed=3, reps=20):
+    """Texture pass (game.cpp:1994-2067) over the whole all-active world after
+    10 ticks, device time from HIP events on the ctx stream: the cells layer
+    alone (8 B read + 4 B written per cell), the three layers (8 B read, 8 B
+    written, 4 B per fire-layer texel that changes: AIR and fire cells), and
+    the three layers with the movingTiles census (8 B more read; each call
+    also waits for the 17 counters)."""
+    cells = fsg.generate(3, side, side, seed)
+    cfg = fsg.default_config(reactions=0, temp_every=1)
+    n = side * side
+    allm = fsg.LAYER_CELLS | fsg.LAYER_EMISSION | fsg.LAYER_FIRE
+    fire = fsg.MAT_WATER  # stands in for the reference's FIRE material
+    out = {"workload": f"all-active striped world {side}x{side} after 10 ticks, whole-world render, "
+                       f"fire layer drawn for water", "reps": reps}
+    with fsg.Context(0) as ctx, fsg.World(ctx, side, side, seed, cfg) as w:
+        w.upload(cells)
+        del cells
+        w.tick(10)
+        st = w.stats()
+        fire_texels = int(st.count[fsg.MAT_AIR] + st.count[fire])
+        out["fire_layer_texels"] = fire_texels
+        cases = [("cells_layer", fsg.LAYER_CELLS, False, 12 * n),
+                 ("three_layers", allm, False, 16 * n + 4 * fire_texels),
+                 ("three_layers_census", allm, True, 24 * n + 4 * fire_texels)]
+        for name, layers, census, nbytes in cases:
+            w.render(layers, fire_mat=fire, stats=census)  # allocation + warm-up
+            ctx.timer_start()
+            for _ in range(reps):
+                w.render(layers, fire_mat=fire, stats=census)
+            ms = ctx.timer_stop() / reps
+            gbs = nbytes / (ms / 1e3) / 1e9
+            out[name] = {"ms": round(ms, 4), "gcell_s": round(n / ms / 1e6, 1),
+                         "algorithmic_bytes_per_cell": round(nbytes / n, 3),
+                         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}}
+    return out
+
+
 def cpu_baseline(threads=None, side_w=8192, side_h=4096, ticks=64):
     """refseq (the reference's sequential chunk-checkerboard tick, restated)
     on a bounded sample of the all-active world, on the host's cores; also
@@ -244,6 +282,18 @@ def cpu_baseline(threads=None, side_w=8192, side_h=4096, ticks=64):
     t0 = time.perf_counter()
     pyoracle.run(small.copy(), cfg, 3, 0, 16)
     dto = time.perf_counter() - t0
+    # texture pass, restated per cell as game.cpp:1994-2067 does it (1 thread)
+    rp = fsg.RenderParams()
+    rp.layers = fsg.LAYER_CELLS | fsg.LAYER_EMISSION | fsg.LAYER_FIRE
+    rp.fire_mat = fsg.MAT_WATER
+    rp.count_moved = 1
+    for m, e in enumerate(fsg.EMIT_DEFAULT):
+        rp.emit[m] = e
+    prev = fsg.generate(3, 4096, 4096, 3)
+    cur = pyoracle.run(prev.copy(), cfg, 3, 0, 1)
+    t0 = time.perf_counter()
+    pyoracle.render(cur, cfg, rp, prev=prev)
+    dtr = time.perf_counter() - t0
     return {"value": side_w * side_h * ticks / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"refseq restatement of world::tick, all-active striped world {side_w}x{side_h}, "
@@ -251,7 +301,9 @@ def cpu_baseline(threads=None, side_w=8192, side_h=4096, ticks=64):
             "host_threads": hw,
             "refseq_1_thread": round(2048 * 2048 * 16 / dt1 / 1e9, 4),
             "det_oracle_1_thread": round(2048 * 2048 * 16 / dto / 1e9, 4),
-            "small_sample": "2048x2048 striped world, 16 ticks"}
+            "small_sample": "2048x2048 striped world, 16 ticks",
+            "render_oracle_1_thread_gcell_s": round(4096 * 4096 / dtr / 1e9, 4),
+            "render_sample": "texture pass, three layers + census, 4096x4096 striped world, 1 thread"}
 
 
 def main():
@@ -262,9 +314,9 @@ def main():
     ap.add_argument("--side", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
-    ap.add_argument("--mode", default="all", choices=["all", "c3", "copy", "c5"],
+    ap.add_argument("--mode", default="all", choices=["all", "c3", "copy", "c5", "render"],
                     help="all: the JSON line; c3: only the timed tick loop (for profiling); "
-                         "copy: the HBM copy calibration kernel")
+                         "copy: the HBM copy calibration kernel; render: only the texture pass")
     ap.add_argument("--copy-bytes", type=int, default=2 << 30)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -283,6 +335,9 @@ def main():
         return
     if args.mode == "c5":
         print(json.dumps(bench_c5(fsg)))
+        return
+    if args.mode == "render":
+        print(json.dumps(bench_render(fsg, side=args.side, reps=max(1, args.steps))))
         return
     if args.mode == "c3":
         r = bench_c3(fsg, args.steps, args.warmup, args.side)
@@ -311,6 +366,9 @@ def main():
         log("[bench] C5 serving")
         secondary["c5_serving"] = bench_c5(fsg)
         log(f"[bench] {secondary['c5_serving']}")
+        log("[bench] texture pass")
+        secondary["render"] = bench_render(fsg)
+        log(f"[bench] {secondary['render']}")
         with fsg.Context(0) as ctx:
             cms = ctx.copy_timed(2 << 30, 10)
         secondary["copy_kernel_GBs"] = round(2 * (2 << 30) / (cms / 1e3) / 1e9, 1)

@@ -77,6 +77,15 @@ class Context:
         check(lib().fsg_ctx_usage(self.handle, C.byref(v)), "fsg_ctx_usage")
         return v.value
 
+    def timer_start(self):
+        """HIP event on the ctx stream; timer_stop() returns the device ms since."""
+        check(lib().fsg_ctx_timer_start(self.handle), "fsg_ctx_timer_start")
+
+    def timer_stop(self) -> float:
+        ms = C.c_double()
+        check(lib().fsg_ctx_timer_stop(self.handle, C.byref(ms)), "fsg_ctx_timer_stop")
+        return ms.value
+
     def inject_alloc_failures(self, n: int):
         check(lib().fsg_ctx_inject_alloc_failures(self.handle, n))
 

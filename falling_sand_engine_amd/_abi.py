@@ -126,6 +126,8 @@ SIGNATURES = {
     "fsg_ctx_usage": (C.c_int, [P, u64p]),
     "fsg_ctx_inject_alloc_failures": (C.c_int, [P, C.c_int]),
     "fsg_ctx_copy_timed": (C.c_int, [P, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "fsg_ctx_timer_start": (C.c_int, [P]),
+    "fsg_ctx_timer_stop": (C.c_int, [P, C.POINTER(C.c_double)]),
     "fsg_world_create": (C.c_int, [P, C.c_int, C.c_int, C.c_uint64, C.POINTER(Config), C.POINTER(P)]),
     "fsg_world_destroy": (C.c_int, [P]),
     "fsg_world_info": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), u64p]),

@@ -191,7 +191,8 @@ extern "C" int fsg_ctx_create(const fsg_ctx_config *cfg, fsg_ctx **out) {
     c->device = dev;
     c->budget = cfg ? cfg->hbm_budget : 0;
     if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
+        hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess ||
+        hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) {
         delete c;
         return FSG_ERR_DEVICE;
     }
@@ -208,6 +209,8 @@ extern "C" int fsg_ctx_destroy(fsg_ctx *c) {
     dev_free(c->d_tile_start);
     (void)hipEventDestroy(c->e0);
     (void)hipEventDestroy(c->e1);
+    (void)hipEventDestroy(c->t0);
+    (void)hipEventDestroy(c->t1);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return FSG_OK;
@@ -223,6 +226,24 @@ extern "C" int fsg_ctx_usage(fsg_ctx *c, uint64_t *bytes) {
 extern "C" int fsg_ctx_inject_alloc_failures(fsg_ctx *c, int n) {
     if (!c || n < 0) return FSG_ERR_INVALID;
     c->inject = n;
+    return FSG_OK;
+}
+
+extern "C" int fsg_ctx_timer_start(fsg_ctx *c) {
+    if (!c) return FSG_ERR_INVALID;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipEventRecord(c->t0, c->stream));
+    return FSG_OK;
+}
+
+extern "C" int fsg_ctx_timer_stop(fsg_ctx *c, double *ms) {
+    if (!c || !ms) return FSG_ERR_INVALID;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipEventRecord(c->t1, c->stream));
+    HIPCHK(hipEventSynchronize(c->t1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, c->t0, c->t1));
+    *ms = f;
     return FSG_OK;
 }
 
@@ -355,6 +376,7 @@ extern "C" int fsg_world_destroy(fsg_world *w) {
     dev_free(w->d_disp);
     dev_free(w->d_dispxy);
     for (uint32_t *l : w->d_layer) dev_free(l);
+    dev_free(w->d_tally);
     dev_free(w->d_sw);
     delete w;
     return FSG_OK;

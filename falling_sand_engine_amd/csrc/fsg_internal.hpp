@@ -22,6 +22,7 @@ struct fsg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;  // fsg_ctx_timer_start/stop
     uint64_t budget = 0, used = 0;
     int inject = 0;
     std::mutex mu;
@@ -46,6 +47,7 @@ struct fsg_world {
     bool flags_valid = false;  // the chunk flags describe the last tick (it ran with skipping on)
     bool prev_valid = false;   // buf[(tick + 1) & 1] holds the state before the last tick
     uint32_t *d_layer[3] = {nullptr, nullptr, nullptr};  // texture layers (fsg_world_render)
+    unsigned long long *d_tally = nullptr;                 // render census replicas
     size_t bytes = 0;
     uint64_t offered = 0;
     unsigned long long *d_stats = nullptr;  // [17] census scratch + [1] processed counter

@@ -25,6 +25,10 @@ static_assert(sizeof(fsg_render_params) == 92 && sizeof(fsg_render_stats) == 144
 namespace {
 
 constexpr int RT = 256;  // threads per workgroup; one quad of 4 cells each
+// The census lands in NREP replicas of the 17 counters, one 256-byte line
+// each, picked by workgroup: a whole-world pass has ~10^5 workgroups, and
+// their atomics on one line would serialise in the L2.
+constexpr int NREP = 64, REP_STRIDE = 32;  // u64 per replica slot
 
 struct RenderTab {
     uint32_t emit[16];  // emission texel (R,G,B,A bytes) per material id
@@ -34,7 +38,8 @@ struct RenderTab {
     uint32_t vec;       // width % 4 == 0: 16-byte accesses are aligned
 };
 
-// tally: [0..15] changed cells by material, [16] fire cells drawn.
+// tally (replica r at r * REP_STRIDE): [0..15] changed cells by material,
+// [16] fire cells drawn.
 __global__ __launch_bounds__(RT) void k_render(const uint64_t *__restrict__ cur, const uint64_t *__restrict__ prev,
                                                int W, int x0, int x1, int y0, RenderTab t,
                                                uint32_t *__restrict__ lc, uint32_t *__restrict__ le,
@@ -49,6 +54,8 @@ __global__ __launch_bounds__(RT) void k_render(const uint64_t *__restrict__ cur,
     const int xs = 4 * ((x0 >> 2) + (int)blockIdx.x * RT + tid);  // first cell of the quad
     const bool want_c = t.layers & FSG_LAYER_CELLS, want_e = t.layers & FSG_LAYER_EMISSION;
     const bool want_f = t.layers & FSG_LAYER_FIRE;
+    uint32_t key[4] = {16u, 16u, 16u, 16u};  // census: material of a changed cell, 16 = none
+    uint32_t fires = 0;                      // fire cells drawn by this thread
     if (xs < x1) {
         const bool full = t.vec && xs >= x0 && xs + 4 <= x1;
         uint64_t c[4], p[4];
@@ -81,7 +88,7 @@ __global__ __launch_bounds__(RT) void k_render(const uint64_t *__restrict__ cur,
                     if (prev) p[k] = prev[row + xs + k];
                 }
         }
-        uint32_t tc[4], te[4];
+        uint32_t tc[4], te[4], fmask = 0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const uint32_t m = (uint32_t)c[k] & 15u;
@@ -89,9 +96,9 @@ __global__ __launch_bounds__(RT) void k_render(const uint64_t *__restrict__ cur,
             tc[k] = air ? 0u : (uint32_t)(c[k] >> 32);
             te[k] = air ? 0u : emit[m];
             if (in[k]) {
-                if (want_f && (air || fire)) lf[row + xs + k] = tc[k];
-                if (tally && fire && !air) atomicAdd(&hist[16], 1u);
-                if (prev && ((c[k] ^ p[k]) & 0xFFFFFFFF000000FFull)) atomicAdd(&hist[m], 1u);
+                fmask |= (uint32_t)(air || fire) << k;  // fire-layer texels that change
+                fires += fire && !air;
+                if (prev && ((c[k] ^ p[k]) & 0xFFFFFFFF000000FFull)) key[k] = m;
             }
         }
         if (full) {
@@ -105,10 +112,36 @@ __global__ __launch_bounds__(RT) void k_render(const uint64_t *__restrict__ cur,
                     if (want_e) le[row + xs + k] = te[k];
                 }
         }
+        if (want_f) {
+            if (full && fmask == 15u) {
+                *reinterpret_cast<uint4 *>(lf + row + xs) = make_uint4(tc[0], tc[1], tc[2], tc[3]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if ((fmask >> k) & 1u) lf[row + xs + k] = tc[k];
+            }
+        }
     }
     if (tally) {
+        // Census per wave: one ballot per distinct changed material (a wave
+        // sees a handful), one LDS add each, instead of an atomic per cell.
+        const int lane = tid & 63;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint64_t todo = __ballot(key[k] != 16u);
+            while (todo) {
+                const uint32_t mat = (uint32_t)__builtin_amdgcn_readlane((int)key[k], __builtin_ctzll(todo));
+                const uint64_t same = __ballot(key[k] == mat);
+                if (lane == 0) atomicAdd(&hist[mat], (uint32_t)__popcll(same));
+                todo &= ~same;
+            }
+        }
+        uint32_t wf = fires;
+        for (int off = 32; off > 0; off >>= 1) wf += __shfl_xor(wf, off, 64);
+        if (lane == 0 && wf) atomicAdd(&hist[16], wf);
         __syncthreads();
-        if (tid < 17 && hist[tid]) atomicAdd(&tally[tid], (unsigned long long)hist[tid]);
+        unsigned long long *rep = tally + (size_t)((blockIdx.x + blockIdx.y * gridDim.x) % NREP) * REP_STRIDE;
+        if (tid < 17 && hist[tid]) atomicAdd(&rep[tid], (unsigned long long)hist[tid]);
     }
 }
 
@@ -172,17 +205,25 @@ extern "C" int fsg_world_render(fsg_world *w, const fsg_render_params *p, fsg_re
     t.vec = (W % 4) == 0;
     const uint64_t *cur = w->d.buf[w->d.tick & 1];
     const uint64_t *prev = (stats && p->count_moved && w->prev_valid) ? w->d.buf[(w->d.tick + 1) & 1] : nullptr;
-    unsigned long long *tally = stats ? w->d_stats : nullptr;  // census scratch, [0..16]
-    if (tally) HIPCHK(hipMemsetAsync(tally, 0, 17 * sizeof(unsigned long long), c->stream));
+    constexpr size_t tally_bytes = (size_t)NREP * REP_STRIDE * sizeof(unsigned long long);
+    if (stats && !w->d_tally) {
+        void *q = nullptr;
+        if (fsg_impl::dev_alloc(c, &q, tally_bytes)) return FSG_ERR_NOMEM;
+        w->d_tally = static_cast<unsigned long long *>(q);
+    }
+    unsigned long long *tally = stats ? w->d_tally : nullptr;
+    if (tally) HIPCHK(hipMemsetAsync(tally, 0, tally_bytes, c->stream));
     const int quads = (int)(((x1 + 3) >> 2) - (x0 >> 2));
     const dim3 grid((unsigned)((quads + RT - 1) / RT), (unsigned)(y1 - y0));
     k_render<<<grid, RT, 0, c->stream>>>(cur, prev, W, (int)x0, (int)x1, (int)y0, t, w->d_layer[0], w->d_layer[1],
                                          w->d_layer[2], tally);
     HIPCHK(hipGetLastError());
     if (stats) {
-        unsigned long long h[17];
-        HIPCHK(hipMemcpyAsync(h, tally, sizeof h, hipMemcpyDeviceToHost, c->stream));
+        unsigned long long rep[NREP * REP_STRIDE], h[17] = {};
+        HIPCHK(hipMemcpyAsync(rep, tally, tally_bytes, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        for (int r = 0; r < NREP; r++)
+            for (int k = 0; k < 17; k++) h[k] += rep[r * REP_STRIDE + k];
         for (int m = 0; m < 16; m++) {
             stats->moved[m] = h[m];
             stats->had_dirty |= h[m] != 0;

@@ -139,6 +139,11 @@ int fsg_ctx_inject_alloc_failures(fsg_ctx *ctx, int n);
  * with HIP events (ms = mean over `reps`).  It has no counterpart in the
  * reference: it calibrates HBM counters and the achievable copy bandwidth. */
 int fsg_ctx_copy_timed(fsg_ctx *ctx, size_t bytes, int reps, double *ms);
+/* HIP events recorded on the ctx stream around whatever the caller enqueues
+ * in between (device time of any call sequence on this context; no
+ * reference counterpart).  stop waits for the second event. */
+int fsg_ctx_timer_start(fsg_ctx *ctx);
+int fsg_ctx_timer_stop(fsg_ctx *ctx, double *elapsed_ms);
 
 /* ------------------------------------------------------------------------ */
 /* World: replaces ME::world's grid and tick (world.hpp:63-193).             */

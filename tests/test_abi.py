@@ -73,6 +73,8 @@ def test_invalid_arguments_are_rejected_without_device():
     assert lib.fsg_ctx_get_tuning(None, C.byref(t)) == _abi.FSG_ERR_INVALID
     assert lib.fsg_ctx_set_tuning(None, C.byref(t)) == _abi.FSG_ERR_INVALID
     assert lib.fsg_autotune_batch(None, None, 0, 8, None) == _abi.FSG_ERR_INVALID
+    assert lib.fsg_ctx_timer_start(None) == _abi.FSG_ERR_INVALID
+    assert lib.fsg_ctx_timer_stop(None, None) == _abi.FSG_ERR_INVALID
     rp = _abi.RenderParams()
     assert lib.fsg_world_render(None, C.byref(rp), None) == _abi.FSG_ERR_INVALID
     ptr = C.c_void_p()
