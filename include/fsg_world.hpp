@@ -9,6 +9,7 @@
 //   world.setTile(x, y, cell)         world::setTile (world.cpp:1004-1008)
 //   world.stampBodies(...)            the stamp loop of game.cpp:1760-1815
 //   world.unstampBodies(...)          the unstamp loop of game.cpp:1896-1983
+//   world.updateTextures(...)         the texture pass of game.cpp:1994-2067
 //
 // Error behaviour follows the reference: getTile out of bounds returns the
 // solid wall cell, setTile out of bounds is ignored.  Device failures, which
@@ -126,6 +127,23 @@ class world {
     // Pulls the pixels back into `pixels` (RigidBody::tiles).
     void unstampBodies(std::vector<MaterialInstance> &pixels) {
         check(fsg_world_unstamp_bodies(w_, pixels.data(), pixels.size()), "fsg_world_unstamp_bodies");
+    }
+
+    // The texture pass after a tick: updates the requested layers over the
+    // rectangle in p and returns the movingTiles census (hadDirty, hadFire).
+    fsg_render_stats updateTextures(const fsg_render_params &p) {
+        flush();
+        fsg_render_stats st{};
+        check(fsg_world_render(w_, &p, &st), "fsg_world_render");
+        return st;
+    }
+
+    // Host copy of a layer rectangle (R,G,B,A bytes per texel), the source of
+    // R_UpdateImageBytes.
+    std::vector<uint32_t> layer(uint32_t which, int x0, int y0, int w, int h) {
+        std::vector<uint32_t> out((size_t)w * h);
+        check(fsg_world_layer_download(w_, which, x0, y0, w, h, out.data()), "fsg_world_layer_download");
+        return out;
     }
 
     fsg_stats stats() {

@@ -22,9 +22,16 @@ int main() {
         const fsg::MaterialInstance wall = world.getTile(-1, 0);
         const fsg::MaterialInstance t = world.getTile(50, 50);
         const fsg_stats s = world.stats();
-        std::printf("{\"checksum\": %llu, \"wall\": %u, \"tile_50_50\": %llu, \"counts\": [",
+        // the texture pass: cells layer of the whole world, then one texel read back
+        fsg_render_params rp{};
+        rp.layers = FSG_LAYER_CELLS;
+        rp.fire_mat = 255;
+        world.updateTextures(rp);
+        const std::vector<uint32_t> texel = world.layer(FSG_LAYER_CELLS, 50, 50, 1, 1);
+        std::printf("{\"checksum\": %llu, \"wall\": %u, \"tile_50_50\": %llu, \"texel_50_50\": %u, \"counts\": [",
                     (unsigned long long)s.checksum, (unsigned)wall.mat,
-                    (unsigned long long)(t.mat | ((uint64_t)(uint16_t)t.temp << 16) | ((uint64_t)t.rgba << 32)));
+                    (unsigned long long)(t.mat | ((uint64_t)(uint16_t)t.temp << 16) | ((uint64_t)t.rgba << 32)),
+                    (unsigned)texel[0]);
         for (int i = 0; i < 8; i++) std::printf("%llu%s", (unsigned long long)s.count[i], i < 7 ? ", " : "");
         std::printf("]}\n");
         return 0;
