@@ -241,6 +241,32 @@ class World:
         check(lib().fsg_world_unstamp_bodies(self.handle, out.ctypes.data, npixels), "fsg_world_unstamp_bodies")
         return out
 
+    # --- island check: world::physicsCheck (world.cpp:3330-3429) ---------------
+    def physicsCheck(self, seeds):  # noqa: N802
+        """Runs the island check for seed cells [(x, y), ...] in order.  Returns
+        one dict per seed: status (ISLAND_*), count, box (min_x, min_y, w, h) and,
+        for a new body, its (h, w) uint64 pixel image (air outside the island)."""
+        self._flush()
+        n = len(seeds)
+        arr = (_abi.Island * max(n, 1))()
+        for i, (x, y) in enumerate(seeds):
+            arr[i].x, arr[i].y = int(x), int(y)
+        need = C.c_size_t()
+        rc = lib().fsg_world_physics_check(self.handle, arr, n, None, 0, C.byref(need))
+        pix = np.empty(0, dtype=np.uint64)
+        if rc == _abi.FSG_ERR_INVALID and need.value:
+            pix = np.empty(need.value, dtype=np.uint64)
+            rc = lib().fsg_world_physics_check(self.handle, arr, n, pix.ctypes.data, pix.size, C.byref(need))
+        check(rc, "fsg_world_physics_check")
+        out = []
+        for i in range(n):
+            r = arr[i]
+            d = {"status": r.status, "count": r.count, "box": (r.min_x, r.min_y, r.w, r.h)}
+            if r.status == _abi.ISLAND_BODY:
+                d["pixels"] = pix[r.pixel_offset:r.pixel_offset + r.w * r.h].reshape(r.h, r.w).copy()
+            out.append(d)
+        return out
+
     # --- texture pass (game.cpp:1994-2067) -----------------------------------
     def render(self, layers=LAYER_CELLS, rect=None, fire_mat=NO_FIRE, emit=None, stats=False,
                count_moved=True):

@@ -101,6 +101,14 @@ class TuneReport(C.Structure):
 
 
 LAYER_CELLS, LAYER_EMISSION, LAYER_FIRE = 1, 2, 4
+ISLAND_NONE, ISLAND_ATTACHED, ISLAND_REMOVED, ISLAND_BODY = 0, 1, 2, 3
+ISLAND_MAX_CELLS, ISLAND_MIN_BODY = 1000, 11
+
+
+class Island(C.Structure):
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("status", C.c_int32), ("count", C.c_uint32),
+                ("min_x", C.c_int32), ("min_y", C.c_int32), ("w", C.c_int32), ("h", C.c_int32),
+                ("pixel_offset", C.c_uint64)]
 
 
 class RenderParams(C.Structure):
@@ -143,6 +151,7 @@ SIGNATURES = {
     "fsg_world_reset_activity": (C.c_int, [P]),
     "fsg_world_stamp_bodies": (C.c_int, [P, P, C.c_int, P, C.c_size_t, C.POINTER(StampResult)]),
     "fsg_world_unstamp_bodies": (C.c_int, [P, P, C.c_size_t]),
+    "fsg_world_physics_check": (C.c_int, [P, P, C.c_int, P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fsg_world_render": (C.c_int, [P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "fsg_world_layer": (C.c_int, [P, C.c_uint32, C.POINTER(P)]),
     "fsg_world_layer_download": (C.c_int, [P, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int, P]),

@@ -229,6 +229,43 @@ int fsg_world_stamp_bodies(fsg_world *w, const fsg_body *bodies, int nbodies,
 int fsg_world_unstamp_bodies(fsg_world *w, fsg_cell *pixels_out, size_t npixels);
 
 /* ------------------------------------------------------------------------ */
+/* Island check: world::physicsCheck (world.cpp:3330-3429) for a list of     */
+/* seed cells.  The 4-connected SOLID component of each seed is searched up  */
+/* to FSG_ISLAND_MAX_CELLS cells; a smaller one is cleared to air (colour 0, */
+/* temperature 0: Tiles_NOTHING) and, above FSG_ISLAND_MIN_BODY cells, its   */
+/* cells come back as the pixel image of a new rigid body (the reference's   */
+/* makeRigidBody texture).  Seeds act in array order, as consecutive         */
+/* physicsCheck calls would: a seed whose island an earlier seed removed     */
+/* finds air.  The reference calls it once per tick at a random cell of the  */
+/* tick zone (world.cpp:1929-1934) and from tools (game.cpp:700, 902-906).   */
+/* ------------------------------------------------------------------------ */
+#define FSG_ISLAND_MAX_CELLS 1000 /* larger components stay part of the world */
+#define FSG_ISLAND_MIN_BODY 11    /* smaller islands are only deleted         */
+
+typedef enum fsg_island_status {
+    FSG_ISLAND_NONE = 0,     /* seed out of the world or not SOLID             */
+    FSG_ISLAND_ATTACHED = 1, /* component larger than FSG_ISLAND_MAX_CELLS     */
+    FSG_ISLAND_REMOVED = 2,  /* 1..10 cells, cleared                           */
+    FSG_ISLAND_BODY = 3      /* 11..1000 cells, cleared, pixels returned       */
+} fsg_island_status;
+
+typedef struct fsg_island {
+    int32_t x, y;          /* in: seed cell                                       */
+    int32_t status;        /* out: fsg_island_status                              */
+    uint32_t count;        /* out: cells of the island (REMOVED, BODY)            */
+    int32_t min_x, min_y;  /* out: bounding box origin (REMOVED, BODY)            */
+    int32_t w, h;          /* out: bounding box size                              */
+    uint64_t pixel_offset; /* out: BODY: first cell of its w*h row-major image in
+                            * pixels_out (air where the box is not the island)    */
+} fsg_island;
+
+/* Runs the check for n seeds.  *pixels_needed gets the image cells the
+ * BODY islands need; if pixels_cap is smaller, nothing is changed and
+ * FSG_ERR_INVALID is returned (call again with a larger buffer). */
+int fsg_world_physics_check(fsg_world *w, fsg_island *seeds, int n, fsg_cell *pixels_out,
+                            size_t pixels_cap, size_t *pixels_needed);
+
+/* ------------------------------------------------------------------------ */
 /* Texture pass (game.cpp:1994-2067): the grid as RGBA8 layers in device     */
 /* memory, byte order R,G,B,A per texel, row-major with pitch width texels.  */
 /* The reference rewrites the texels of dirty cells after every tick; every  */

@@ -648,6 +648,116 @@ uint64_t orc_checksum(const uint64_t *cells, size_t n) {
 }
 
 /* ----------------------------------------------------------------------- */
+/* Island check: world::physicsCheck (world.cpp:3330-3429)                  */
+/* ----------------------------------------------------------------------- */
+static int solid_cell(const fsg_config *cfg, uint64_t c) {
+    const uint32_t m = (uint32_t)(c & 0xFF);
+    return m < cfg->n_materials && cfg->materials[m].type == FSG_SOLID;
+}
+
+typedef struct { int x, y, dir; } flood_frame;
+
+/* physicsCheck_flood (world.cpp:3407-3425): the recursive 4-way DFS, with an
+ * explicit stack in the same visit order (x+1, y+1, x-1, y-1); a call
+ * returns at once when more than 1000 cells were counted or out of bounds. */
+static void flood(const uint64_t *cells, int W, int H, const fsg_config *cfg, int x0, int y0,
+                  uint8_t *visited, int *count, int *minX, int *maxX, int *minY, int *maxY,
+                  flood_frame *st) {
+    static const int ddx[4] = {1, 0, -1, 0}, ddy[4] = {0, 1, 0, -1};
+    int top = 0;
+    st[0].x = x0, st[0].y = y0, st[0].dir = -1;
+    while (top >= 0) {
+        flood_frame *f = &st[top];
+        if (f->dir < 0) { /* entering physicsCheck_flood(f->x, f->y) */
+            const int x = f->x, y = f->y;
+            if (*count > 1000 || x < 0 || x >= W || y < 0 || y >= H ||
+                visited[(size_t)x + (size_t)y * W] || !solid_cell(cfg, cells[(size_t)x + (size_t)y * W])) {
+                top--;
+                continue;
+            }
+            if (x < *minX) *minX = x;
+            if (x > *maxX) *maxX = x;
+            if (y < *minY) *minY = y;
+            if (y > *maxY) *maxY = y;
+            visited[(size_t)x + (size_t)y * W] = 1;
+            (*count)++;
+            f->dir = 0;
+        } else if (f->dir < 4) {
+            const int d = f->dir++;
+            top++;
+            st[top].x = f->x + ddx[d], st[top].y = f->y + ddy[d], st[top].dir = -1;
+        } else {
+            top--;
+        }
+    }
+}
+
+static void physics_check_seq(uint64_t *cells, int W, int H, const fsg_config *cfg, fsg_island *seeds, int n,
+                              uint64_t *pixels, size_t *needed, uint8_t *visited, flood_frame *st) {
+    const uint64_t nothing = (uint64_t)cfg->mat_air; /* Tiles_NOTHING: colour 0, temperature 0 */
+    *needed = 0;
+    for (int i = 0; i < n; i++) {
+        fsg_island *s = &seeds[i];
+        const int x = s->x, y = s->y;
+        s->status = FSG_ISLAND_NONE;
+        s->count = 0;
+        s->min_x = s->min_y = s->w = s->h = 0;
+        s->pixel_offset = 0;
+        if (x < 0 || y < 0 || x >= W || y >= H || !solid_cell(cfg, cells[(size_t)x + (size_t)y * W])) continue;
+        memset(visited, 0, (size_t)W * H);
+        int count = 0, minX = W, maxX = 0, minY = H, maxY = 0;
+        flood(cells, W, H, cfg, x, y, visited, &count, &minX, &maxX, &minY, &maxY, st);
+        if (count > 1000) {
+            s->status = FSG_ISLAND_ATTACHED;
+            continue;
+        }
+        if (count <= 0) continue;
+        const int bw = maxX - minX + 1, bh = maxY - minY + 1;
+        s->count = (uint32_t)count;
+        s->min_x = minX, s->min_y = minY, s->w = bw, s->h = bh;
+        s->status = count > 10 ? FSG_ISLAND_BODY : FSG_ISLAND_REMOVED;
+        if (s->status == FSG_ISLAND_BODY) {
+            s->pixel_offset = *needed;
+            if (pixels)
+                for (size_t k = 0; k < (size_t)bw * bh; k++) pixels[*needed + k] = nothing;
+        }
+        for (int yy = minY; yy <= maxY; yy++)
+            for (int xx = minX; xx <= maxX; xx++) {
+                const size_t j = (size_t)xx + (size_t)yy * W;
+                if (!visited[j]) continue;
+                if (s->status == FSG_ISLAND_BODY && pixels)
+                    pixels[*needed + (size_t)(yy - minY) * bw + (xx - minX)] = cells[j];
+                cells[j] = nothing;
+            }
+        if (s->status == FSG_ISLAND_BODY) *needed += (size_t)bw * bh;
+    }
+}
+
+int orc_physics_check(uint64_t *cells, int W, int H, const fsg_config *cfg, fsg_island *seeds, int n,
+                      uint64_t *pixels_out, size_t pixels_cap, size_t *pixels_needed) {
+    if (!cells || !cfg || W <= 0 || H <= 0 || n < 0 || (n && !seeds)) return -1;
+    uint8_t *visited = (uint8_t *)malloc((size_t)W * H);
+    uint64_t *copy = (uint64_t *)malloc((size_t)W * H * 8);
+    fsg_island *sc = (fsg_island *)malloc(sizeof(fsg_island) * (size_t)(n ? n : 1));
+    flood_frame *st = (flood_frame *)malloc(sizeof(flood_frame) * 1100);
+    if (!visited || !copy || !sc || !st) {
+        free(visited), free(copy), free(sc), free(st);
+        return -2;
+    }
+    /* dry run on a copy: the image cells needed */
+    size_t needed = 0;
+    memcpy(copy, cells, (size_t)W * H * 8);
+    memcpy(sc, seeds, sizeof(fsg_island) * (size_t)n);
+    physics_check_seq(copy, W, H, cfg, sc, n, NULL, &needed, visited, st);
+    if (pixels_needed) *pixels_needed = needed;
+    int rc = 0;
+    if (needed && (!pixels_out || pixels_cap < needed)) rc = -1;
+    else physics_check_seq(cells, W, H, cfg, seeds, n, pixels_out, &needed, visited, st);
+    free(visited), free(copy), free(sc), free(st);
+    return rc;
+}
+
+/* ----------------------------------------------------------------------- */
 /* Texture pass (game.cpp:1994-2067)                                        */
 /* ----------------------------------------------------------------------- */
 /* The build's cell colour is rgba with bytes R,G,B,A in memory; the

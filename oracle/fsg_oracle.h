@@ -60,6 +60,12 @@ int orc_unstamp(uint64_t *cells, int W, int H, const fsg_config *cfg, const fsg_
                 int nbodies, uint64_t *pixels, size_t npixels);
 int orc_body_pixel_xy(const fsg_body *b, int tx, int ty, int *wx, int *wy);
 
+/* world::physicsCheck (world.cpp:3330-3429) for seeds in order (each acts on
+ * the grid the previous ones left).  Returns -1 without changing anything if
+ * the body images need more than pixels_cap cells (*pixels_needed). */
+int orc_physics_check(uint64_t *cells, int W, int H, const fsg_config *cfg, fsg_island *seeds, int n,
+                      uint64_t *pixels_out, size_t pixels_cap, size_t *pixels_needed);
+
 /* Texture pass (game.cpp:1994-2067) over p's rectangle: the cells, emission
  * and fire layers (any may be NULL), and with prev (the state before the
  * last tick) and st the movingTiles census.  Layers are updated in place. */

@@ -9,7 +9,7 @@ import subprocess
 
 import numpy as np
 
-from falling_sand_engine_amd._abi import Body, CellEdit, Config, RenderParams, RenderStats
+from falling_sand_engine_amd._abi import Body, CellEdit, Config, Island, RenderParams, RenderStats
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
@@ -45,6 +45,7 @@ def lib():
                                 C.POINTER(u32), P, u32, P, P]),
             "orc_unstamp": (i32, [P, i32, i32, cfgp, P, i32, P, C.c_size_t]),
             "orc_body_pixel_xy": (i32, [P, i32, i32, C.POINTER(i32), C.POINTER(i32)]),
+            "orc_physics_check": (i32, [P, i32, i32, cfgp, P, i32, P, C.c_size_t, C.POINTER(C.c_size_t)]),
             "orc_render": (i32, [P, P, i32, i32, cfgp, C.POINTER(RenderParams), P, P, P,
                                  C.POINTER(RenderStats)]),
             "orc_counts": (None, [P, C.c_size_t, P]),
@@ -192,3 +193,27 @@ def unstamp(cells, cfg, bodies, pixels):
     px = np.array(pixels, dtype=np.uint64, copy=True)
     lib().orc_unstamp(p, W, H, C.byref(cfg), barr, nb, px.ctypes.data, px.size)
     return px
+
+
+def physics_check(cells, cfg, seeds):
+    """world::physicsCheck for seeds [(x, y), ...] in order, in place on cells.
+    Returns per-seed dicts like World.physicsCheck (pixels for bodies)."""
+    ptr, W, H = _grid(cells)
+    n = len(seeds)
+    arr = (Island * max(n, 1))()
+    for i, (x, y) in enumerate(seeds):
+        arr[i].x, arr[i].y = int(x), int(y)
+    need = C.c_size_t()
+    rc = lib().orc_physics_check(ptr, W, H, C.byref(cfg), arr, n, None, 0, C.byref(need))
+    pix = np.empty(max(need.value, 1), dtype=np.uint64)
+    if rc != 0:
+        rc = lib().orc_physics_check(ptr, W, H, C.byref(cfg), arr, n, pix.ctypes.data, pix.size, C.byref(need))
+    assert rc == 0, rc
+    out = []
+    for i in range(n):
+        r = arr[i]
+        d = {"status": r.status, "count": r.count, "box": (r.min_x, r.min_y, r.w, r.h)}
+        if r.status == 3:
+            d["pixels"] = pix[r.pixel_offset:r.pixel_offset + r.w * r.h].reshape(r.h, r.w).copy()
+        out.append(d)
+    return out
