@@ -261,6 +261,41 @@ def bench_render(fsg, side=16384, seed=3, reps=20):
     return out
 
 
+def fragments(side, seed, p=0.5):
+    """A world of stone fragments near the percolation threshold (islands of
+    every size up to one spanning component) among air, sand and water."""
+    import numpy as np
+    import falling_sand_engine_amd as fsg
+    rng = np.random.default_rng(seed)
+    mats = np.where(rng.random((side, side)) < p, fsg.MAT_STONE,
+                    rng.choice([fsg.MAT_AIR, fsg.MAT_SAND, fsg.MAT_WATER], size=(side, side)))
+    return mats.astype(np.uint64) | (np.uint64(320) << np.uint64(16)) | (np.uint64(0xFF808080) << np.uint64(32))
+
+
+def bench_islands(fsg, side=4096, nseeds=4096, seed=9):
+    """world::physicsCheck for a batch of random seed cells on a world of stone
+    fragments: device time of the whole call (search launch, host dedupe,
+    image/clear launch, copies).  BFS levels and barriers bound it, not HBM."""
+    import numpy as np
+    cells = fragments(side, seed)
+    rng = np.random.default_rng(seed)
+    seeds = [(int(x), int(y)) for x, y in zip(rng.integers(0, side, nseeds), rng.integers(0, side, nseeds))]
+    with fsg.Context(0) as ctx, fsg.World(ctx, side, side, seed, fsg.default_config()) as w:
+        w.upload(cells)
+        w.physicsCheck(seeds[:64])  # warm-up (removes those islands)
+        arr = (fsg._abi.Island * nseeds)()
+        for i, (x, y) in enumerate(seeds):
+            arr[i].x, arr[i].y = x, y
+        need = fsg.C.c_size_t()
+        ctx.timer_start()  # the C call alone: search, host dedupe, images and clearing
+        fsg.check(fsg.lib().fsg_world_physics_check(w.handle, arr, nseeds, fsg.C.byref(need)))
+        ms = ctx.timer_stop()
+    st = [arr[i].status for i in range(nseeds)]
+    return {"workload": f"{side}x{side} world of stone fragments (p=0.5), {nseeds} random seed cells",
+            "ms": round(ms, 3), "seeds_per_s": round(nseeds / (ms / 1e3)),
+            "bodies": st.count(3), "removed_small": st.count(2), "attached": st.count(1), "none": st.count(0)}
+
+
 def cpu_baseline(threads=None, side_w=8192, side_h=4096, ticks=64):
     """refseq (the reference's sequential chunk-checkerboard tick, restated)
     on a bounded sample of the all-active world, on the host's cores; also
@@ -294,6 +329,15 @@ def cpu_baseline(threads=None, side_w=8192, side_h=4096, ticks=64):
     t0 = time.perf_counter()
     pyoracle.render(cur, cfg, rp, prev=prev)
     dtr = time.perf_counter() - t0
+    # island check as the reference runs it: one seed at a time, a W*H visited
+    # array cleared per call, recursive fill (restated iteratively)
+    import numpy as np
+    frag = fragments(4096, 9)
+    rng = np.random.default_rng(9)
+    iseeds = [(int(x), int(y)) for x, y in zip(rng.integers(0, 4096, 64), rng.integers(0, 4096, 64))]
+    t0 = time.perf_counter()
+    pyoracle.physics_check(frag, pyoracle.default_config(), iseeds)
+    dti = time.perf_counter() - t0
     return {"value": side_w * side_h * ticks / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"refseq restatement of world::tick, all-active striped world {side_w}x{side_h}, "
@@ -303,7 +347,10 @@ def cpu_baseline(threads=None, side_w=8192, side_h=4096, ticks=64):
             "det_oracle_1_thread": round(2048 * 2048 * 16 / dto / 1e9, 4),
             "small_sample": "2048x2048 striped world, 16 ticks",
             "render_oracle_1_thread_gcell_s": round(4096 * 4096 / dtr / 1e9, 4),
-            "render_sample": "texture pass, three layers + census, 4096x4096 striped world, 1 thread"}
+            "render_sample": "texture pass, three layers + census, 4096x4096 striped world, 1 thread",
+            "islands_oracle_1_thread_seeds_per_s": round(64 / dti, 1),
+            "islands_sample": "physicsCheck restated, 64 seeds one at a time, 4096x4096 fragment world; "
+                              "the oracle makes two passes (a dry run sizes the body images)"}
 
 
 def main():
@@ -314,7 +361,7 @@ def main():
     ap.add_argument("--side", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
-    ap.add_argument("--mode", default="all", choices=["all", "c3", "copy", "c5", "render"],
+    ap.add_argument("--mode", default="all", choices=["all", "c3", "copy", "c5", "render", "islands"],
                     help="all: the JSON line; c3: only the timed tick loop (for profiling); "
                          "copy: the HBM copy calibration kernel; render: only the texture pass")
     ap.add_argument("--copy-bytes", type=int, default=2 << 30)
@@ -335,6 +382,9 @@ def main():
         return
     if args.mode == "c5":
         print(json.dumps(bench_c5(fsg)))
+        return
+    if args.mode == "islands":
+        print(json.dumps(bench_islands(fsg)))
         return
     if args.mode == "render":
         print(json.dumps(bench_render(fsg, side=args.side, reps=max(1, args.steps))))
@@ -369,6 +419,9 @@ def main():
         log("[bench] texture pass")
         secondary["render"] = bench_render(fsg)
         log(f"[bench] {secondary['render']}")
+        log("[bench] island check")
+        secondary["islands"] = bench_islands(fsg)
+        log(f"[bench] {secondary['islands']}")
         with fsg.Context(0) as ctx:
             cms = ctx.copy_timed(2 << 30, 10)
         secondary["copy_kernel_GBs"] = round(2 * (2 << 30) / (cms / 1e3) / 1e9, 1)

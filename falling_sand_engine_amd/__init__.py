@@ -252,12 +252,10 @@ class World:
         for i, (x, y) in enumerate(seeds):
             arr[i].x, arr[i].y = int(x), int(y)
         need = C.c_size_t()
-        rc = lib().fsg_world_physics_check(self.handle, arr, n, None, 0, C.byref(need))
-        pix = np.empty(0, dtype=np.uint64)
-        if rc == _abi.FSG_ERR_INVALID and need.value:
-            pix = np.empty(need.value, dtype=np.uint64)
-            rc = lib().fsg_world_physics_check(self.handle, arr, n, pix.ctypes.data, pix.size, C.byref(need))
-        check(rc, "fsg_world_physics_check")
+        check(lib().fsg_world_physics_check(self.handle, arr, n, C.byref(need)), "fsg_world_physics_check")
+        pix = np.empty(need.value, dtype=np.uint64)
+        if need.value:
+            check(lib().fsg_world_island_pixels(self.handle, pix.ctypes.data, pix.size), "fsg_world_island_pixels")
         out = []
         for i in range(n):
             r = arr[i]

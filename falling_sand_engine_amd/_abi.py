@@ -151,7 +151,8 @@ SIGNATURES = {
     "fsg_world_reset_activity": (C.c_int, [P]),
     "fsg_world_stamp_bodies": (C.c_int, [P, P, C.c_int, P, C.c_size_t, C.POINTER(StampResult)]),
     "fsg_world_unstamp_bodies": (C.c_int, [P, P, C.c_size_t]),
-    "fsg_world_physics_check": (C.c_int, [P, P, C.c_int, P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "fsg_world_physics_check": (C.c_int, [P, P, C.c_int, C.POINTER(C.c_size_t)]),
+    "fsg_world_island_pixels": (C.c_int, [P, P, C.c_size_t]),
     "fsg_world_render": (C.c_int, [P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "fsg_world_layer": (C.c_int, [P, C.c_uint32, C.POINTER(P)]),
     "fsg_world_layer_download": (C.c_int, [P, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int, P]),

@@ -377,6 +377,8 @@ extern "C" int fsg_world_destroy(fsg_world *w) {
     dev_free(w->d_dispxy);
     for (uint32_t *l : w->d_layer) dev_free(l);
     dev_free(w->d_tally);
+    dev_free(w->d_isl);
+    dev_free(w->d_isl_pix);
     dev_free(w->d_sw);
     delete w;
     return FSG_OK;

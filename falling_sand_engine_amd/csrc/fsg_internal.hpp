@@ -48,6 +48,10 @@ struct fsg_world {
     bool prev_valid = false;   // buf[(tick + 1) & 1] holds the state before the last tick
     uint32_t *d_layer[3] = {nullptr, nullptr, nullptr};  // texture layers (fsg_world_render)
     unsigned long long *d_tally = nullptr;                 // render census replicas
+    void *d_isl = nullptr;                                 // island check scratch
+    size_t isl_cap = 0;
+    uint64_t *d_isl_pix = nullptr;                         // body images of the last island check
+    size_t isl_pix_cap = 0, isl_pix_used = 0;
     size_t bytes = 0;
     uint64_t offered = 0;
     unsigned long long *d_stats = nullptr;  // [17] census scratch + [1] processed counter

@@ -174,10 +174,21 @@ __global__ void k_fill_u64(uint64_t *p, size_t n, uint64_t v) {
 
 } // namespace
 
-extern "C" int fsg_world_physics_check(fsg_world *w, fsg_island *seeds, int n, fsg_cell *pixels_out,
-                                       size_t pixels_cap, size_t *pixels_needed) {
+// Grows *buf to at least `bytes` (contents are not kept).
+static int grow(fsg_ctx *c, void **buf, size_t *cap, size_t bytes) {
+    if (*cap >= bytes) return FSG_OK;
+    fsg_impl::dev_free(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    if (fsg_impl::dev_alloc(c, buf, bytes)) return FSG_ERR_NOMEM;
+    *cap = bytes;
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_physics_check(fsg_world *w, fsg_island *seeds, int n, size_t *pixels_needed) {
     if (!w || n < 0 || (n > 0 && !seeds) || n > (1 << 20)) return FSG_ERR_INVALID;
     if (pixels_needed) *pixels_needed = 0;
+    w->isl_pix_used = 0;
     if (n == 0) return FSG_OK;
     fsg_ctx *c = w->ctx;
     HIPCHK(hipSetDevice(c->device));
@@ -190,92 +201,90 @@ extern "C" int fsg_world_physics_check(fsg_world *w, fsg_island *seeds, int n, f
         xy[2 * i] = seeds[i].x;
         xy[2 * i + 1] = seeds[i].y;
     }
-    // scratch: seed cells, island cell lists, per-seed results, kept islands
-    const size_t b_xy = xy.size() * 4, b_lists = (size_t)n * CAP * 4, b_out = (size_t)n * sizeof(SeedOut);
-    const size_t b_win = (size_t)n * sizeof(Winner);
-    char *scratch = nullptr;
-    if (fsg_impl::dev_alloc(c, reinterpret_cast<void **>(&scratch), b_xy + b_lists + b_out + b_win)) return FSG_ERR_NOMEM;
+    // scratch (kept by the world): seed cells, island cell lists, per-seed
+    // results, kept islands
+    const size_t b_xy = (xy.size() * 4 + 255) & ~(size_t)255, b_lists = (size_t)n * CAP * 4;
+    const size_t b_out = (size_t)n * sizeof(SeedOut), b_win = (size_t)n * sizeof(Winner);
+    if (int rc = grow(c, &w->d_isl, &w->isl_cap, b_xy + b_lists + b_out + b_win)) return rc;
+    char *scratch = static_cast<char *>(w->d_isl);
     int32_t *d_xy = reinterpret_cast<int32_t *>(scratch);
     uint32_t *d_lists = reinterpret_cast<uint32_t *>(scratch + b_xy);
     SeedOut *d_out = reinterpret_cast<SeedOut *>(scratch + b_xy + b_lists);
     Winner *d_win = reinterpret_cast<Winner *>(scratch + b_xy + b_lists + b_out);
     std::vector<SeedOut> res((size_t)n);
-    int rc = FSG_OK;
     uint64_t *cur = w->d.buf[w->d.tick & 1];
-    if (hipMemcpyAsync(d_xy, xy.data(), b_xy, hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = FSG_ERR_DEVICE;
-    if (!rc) {
-        k_island_search<<<n, IT, 0, c->stream>>>(cur, W, H, solid, d_xy, d_lists, d_out);
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(res.data(), d_out, b_out, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-            hipStreamSynchronize(c->stream) != hipSuccess)
-            rc = FSG_ERR_DEVICE;
-    }
+    HIPCHK(hipMemcpyAsync(d_xy, xy.data(), xy.size() * 4, hipMemcpyHostToDevice, c->stream));
+    k_island_search<<<n, IT, 0, c->stream>>>(cur, W, H, solid, d_xy, d_lists, d_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(res.data(), d_out, b_out, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     // keep the first seed of each island, in seed order
     std::vector<Winner> win;
     std::unordered_set<uint32_t> gone;
     size_t needed = 0;
-    for (int i = 0; i < n && !rc; i++) {
+    for (int i = 0; i < n; i++) {
         SeedOut &r = res[(size_t)i];
-        if (r.status == FSG_ISLAND_REMOVED || r.status == FSG_ISLAND_BODY) {
-            if (!gone.insert(r.minidx).second) {
-                r.status = FSG_ISLAND_NONE;  // removed by an earlier seed: air now
-                r.count = 0;
-                continue;
-            }
-            Winner v{};
-            v.seed = (uint32_t)i;
-            v.count = r.count;
-            v.minx = r.minx;
-            v.miny = r.miny;
-            v.w = r.maxx - r.minx + 1;
-            v.body = r.status == FSG_ISLAND_BODY;
-            v.off = needed;
-            if (v.body) needed += (size_t)v.w * (size_t)(r.maxy - r.miny + 1);
-            win.push_back(v);
+        if (r.status != FSG_ISLAND_REMOVED && r.status != FSG_ISLAND_BODY) continue;
+        if (!gone.insert(r.minidx).second) {
+            r.status = FSG_ISLAND_NONE;  // removed by an earlier seed: air now
+            continue;
         }
+        Winner v{};
+        v.seed = (uint32_t)i;
+        v.count = r.count;
+        v.minx = r.minx;
+        v.miny = r.miny;
+        v.w = r.maxx - r.minx + 1;
+        v.body = r.status == FSG_ISLAND_BODY;
+        v.off = needed;
+        if (v.body) needed += (size_t)v.w * (size_t)(r.maxy - r.miny + 1);
+        win.push_back(v);
     }
-    if (pixels_needed) *pixels_needed = needed;
-    if (!rc && needed && (!pixels_out || pixels_cap < needed)) rc = FSG_ERR_INVALID;
-    uint64_t *d_pix = nullptr;
-    if (!rc && !win.empty()) {
-        if (needed && fsg_impl::dev_alloc(c, reinterpret_cast<void **>(&d_pix), needed * 8)) rc = FSG_ERR_NOMEM;
+    if (!win.empty()) {
+        if (needed) {
+            void *pb = w->d_isl_pix;
+            if (int rc = grow(c, &pb, &w->isl_pix_cap, needed * 8)) {
+                w->d_isl_pix = nullptr;
+                return rc;
+            }
+            w->d_isl_pix = static_cast<uint64_t *>(pb);
+        }
         const uint64_t air = (uint64_t)w->cfg.mat_air;  // Tiles_NOTHING: colour 0, temperature 0
-        if (!rc && hipMemcpyAsync(d_win, win.data(), win.size() * sizeof(Winner), hipMemcpyHostToDevice,
-                                  c->stream) != hipSuccess)
-            rc = FSG_ERR_DEVICE;
-        if (!rc && needed) {
-            k_fill_u64<<<fsg_impl::grid_for(needed, 256), 256, 0, c->stream>>>(d_pix, needed, air);
-            if (hipGetLastError() != hipSuccess) rc = FSG_ERR_DEVICE;
+        HIPCHK(hipMemcpyAsync(d_win, win.data(), win.size() * sizeof(Winner), hipMemcpyHostToDevice, c->stream));
+        if (needed) {
+            k_fill_u64<<<fsg_impl::grid_for(needed, 256), 256, 0, c->stream>>>(w->d_isl_pix, needed, air);
+            HIPCHK(hipGetLastError());
         }
-        if (!rc) {
-            k_island_apply<<<(unsigned)win.size(), IT, 0, c->stream>>>(cur, W, d_lists, d_win, air, d_pix);
-            if (hipGetLastError() != hipSuccess) rc = FSG_ERR_DEVICE;
-        }
-        if (!rc && needed &&
-            hipMemcpyAsync(pixels_out, d_pix, needed * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-            rc = FSG_ERR_DEVICE;
-        if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = FSG_ERR_DEVICE;
+        k_island_apply<<<(unsigned)win.size(), IT, 0, c->stream>>>(cur, W, d_lists, d_win, air, w->d_isl_pix);
+        HIPCHK(hipGetLastError());
         // the grid changed outside a tick: every chunk runs at the next tick
         w->d.force_tick = w->d.tick;
     }
-    fsg_impl::dev_free(d_pix);
-    fsg_impl::dev_free(scratch);
-    if (rc) return rc;
-    for (const Winner &v : win) seeds[v.seed].pixel_offset = v.off;
+    w->isl_pix_used = needed;
+    if (pixels_needed) *pixels_needed = needed;
     for (int i = 0; i < n; i++) {
         const SeedOut &r = res[(size_t)i];
         fsg_island &s = seeds[i];
+        const bool kept = r.status == FSG_ISLAND_REMOVED || r.status == FSG_ISLAND_BODY;
         s.status = r.status;
-        s.count = (r.status == FSG_ISLAND_REMOVED || r.status == FSG_ISLAND_BODY) ? r.count : 0u;
-        if (r.status == FSG_ISLAND_REMOVED || r.status == FSG_ISLAND_BODY) {
-            s.min_x = r.minx;
-            s.min_y = r.miny;
-            s.w = r.maxx - r.minx + 1;
-            s.h = r.maxy - r.miny + 1;
-        } else {
-            s.min_x = s.min_y = s.w = s.h = 0;
-        }
-        if (r.status != FSG_ISLAND_BODY) s.pixel_offset = 0;
+        s.count = kept ? r.count : 0u;
+        s.min_x = kept ? r.minx : 0;
+        s.min_y = kept ? r.miny : 0;
+        s.w = kept ? r.maxx - r.minx + 1 : 0;
+        s.h = kept ? r.maxy - r.miny + 1 : 0;
+        s.pixel_offset = 0;
     }
+    for (const Winner &v : win)
+        if (v.body) seeds[v.seed].pixel_offset = v.off;
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_island_pixels(fsg_world *w, fsg_cell *out, size_t cap) {
+    if (!w || (w->isl_pix_used && (!out || cap < w->isl_pix_used))) return FSG_ERR_INVALID;
+    if (!w->isl_pix_used) return FSG_OK;
+    fsg_ctx *c = w->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(out, w->d_isl_pix, w->isl_pix_used * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return FSG_OK;
 }

@@ -259,11 +259,13 @@ typedef struct fsg_island {
                             * pixels_out (air where the box is not the island)    */
 } fsg_island;
 
-/* Runs the check for n seeds.  *pixels_needed gets the image cells the
- * BODY islands need; if pixels_cap is smaller, nothing is changed and
- * FSG_ERR_INVALID is returned (call again with a larger buffer). */
-int fsg_world_physics_check(fsg_world *w, fsg_island *seeds, int n, fsg_cell *pixels_out,
-                            size_t pixels_cap, size_t *pixels_needed);
+/* Runs the check for n seeds (at most 2^20) and clears the islands it
+ * removes.  The BODY images stay in device memory owned by the world until
+ * the next check; *pixels_needed (may be NULL) gets their total cell count. */
+int fsg_world_physics_check(fsg_world *w, fsg_island *seeds, int n, size_t *pixels_needed);
+/* Copies the body images of the last check (pixel_offset indexes them);
+ * FSG_ERR_INVALID if cap is smaller than their cell count. */
+int fsg_world_island_pixels(fsg_world *w, fsg_cell *out, size_t cap);
 
 /* ------------------------------------------------------------------------ */
 /* Texture pass (game.cpp:1994-2067): the grid as RGBA8 layers in device     */

@@ -75,6 +75,9 @@ def test_invalid_arguments_are_rejected_without_device():
     assert lib.fsg_autotune_batch(None, None, 0, 8, None) == _abi.FSG_ERR_INVALID
     assert lib.fsg_ctx_timer_start(None) == _abi.FSG_ERR_INVALID
     assert lib.fsg_ctx_timer_stop(None, None) == _abi.FSG_ERR_INVALID
+    assert lib.fsg_world_physics_check(None, None, 0, None) == _abi.FSG_ERR_INVALID
+    assert lib.fsg_world_island_pixels(None, None, 0) == _abi.FSG_ERR_INVALID
+    assert C.sizeof(_abi.Island) == 40
     rp = _abi.RenderParams()
     assert lib.fsg_world_render(None, C.byref(rp), None) == _abi.FSG_ERR_INVALID
     ptr = C.c_void_p()

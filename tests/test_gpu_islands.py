@@ -39,7 +39,7 @@ def test_islands_match_oracle(ctx, orc, W, H, p, nseeds):
     assert any(d["status"] == _abi.ISLAND_BODY for d in ref)
 
 
-def test_small_pixel_buffer_changes_nothing(ctx, orc):
+def test_pixel_copy_and_argument_checks(ctx, orc):
     cfg = orc.default_config()
     W, H = 96, 64
     c = random_fragments(W, H, 5, p=0.5)
@@ -48,17 +48,26 @@ def test_small_pixel_buffer_changes_nothing(ctx, orc):
     for i, (x, y) in enumerate(seeds):
         arr[i].x, arr[i].y = x, y
     need = C.c_size_t()
+    lib = fsg.lib()
     with fsg.World(ctx, W, H, 1, cfg) as w:
         w.upload(c)
-        lib = fsg.lib()
-        rc = lib.fsg_world_physics_check(w.handle, arr, len(seeds), None, 0, C.byref(need))
-        assert rc == _abi.FSG_ERR_INVALID and need.value > 0
+        assert lib.fsg_world_physics_check(w.handle, arr, len(seeds), C.byref(need)) == _abi.FSG_OK
+        assert need.value > 0
         small = np.empty(need.value - 1, dtype=np.uint64)
-        rc = lib.fsg_world_physics_check(w.handle, arr, len(seeds), small.ctypes.data, small.size, C.byref(need))
-        assert rc == _abi.FSG_ERR_INVALID
-        assert np.array_equal(w.download(), c)  # untouched
-        assert lib.fsg_world_physics_check(w.handle, None, 3, None, 0, None) == _abi.FSG_ERR_INVALID
-        assert lib.fsg_world_physics_check(w.handle, arr, 0, None, 0, None) == _abi.FSG_OK
+        assert lib.fsg_world_island_pixels(w.handle, small.ctypes.data, small.size) == _abi.FSG_ERR_INVALID
+        pix = np.empty(need.value, dtype=np.uint64)
+        assert lib.fsg_world_island_pixels(w.handle, pix.ctypes.data, pix.size) == _abi.FSG_OK
+        ref = orc.physics_check(c, cfg, seeds)
+        for i, r in enumerate(ref):
+            assert (arr[i].status, arr[i].count) == (r["status"], r["count"])
+            if r["status"] == _abi.ISLAND_BODY:
+                o, n = arr[i].pixel_offset, arr[i].w * arr[i].h
+                assert np.array_equal(pix[o:o + n].reshape(arr[i].h, arr[i].w), r["pixels"])
+        assert np.array_equal(w.download(), c)
+        assert lib.fsg_world_physics_check(w.handle, None, 3, None) == _abi.FSG_ERR_INVALID
+        assert lib.fsg_world_physics_check(w.handle, arr, 0, None) == _abi.FSG_OK
+        # a check with no bodies leaves no images to copy
+        assert lib.fsg_world_island_pixels(w.handle, None, 0) == _abi.FSG_OK
 
 
 @pytest.mark.parametrize("skip", [0, 1])
