@@ -181,6 +181,11 @@ class World:
         assert a.size == self.width * self.height
         check(lib().fsg_world_upload(self.handle, a.ctypes.data, a.size), "fsg_world_upload")
 
+    def generate(self, kind: int, seed: int):
+        """Fills the grid with generate(kind, width, height, seed), on the device."""
+        self._edits = []
+        check(lib().fsg_world_generate(self.handle, kind, seed), "fsg_world_generate")
+
     def download(self, x0=0, y0=0, w=None, h=None) -> np.ndarray:
         self._flush()
         w = self.width if w is None else w

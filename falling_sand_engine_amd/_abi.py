@@ -142,6 +142,7 @@ SIGNATURES = {
     "fsg_world_set_tick": (C.c_int, [P, C.c_uint64]),
     "fsg_world_upload": (C.c_int, [P, P, C.c_size_t]),
     "fsg_world_download": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "fsg_world_generate": (C.c_int, [P, C.c_int, C.c_uint64]),
     "fsg_world_set_cells": (C.c_int, [P, P, C.c_size_t]),
     "fsg_world_step": (C.c_int, [P, C.c_int]),
     "fsg_world_step_timed": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),

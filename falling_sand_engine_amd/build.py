@@ -14,8 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libfsg.so")
 
-SOURCES = ["fsg_kernels.hip", "fsg_api.hip", "fsg_serve.hip", "fsg_render.hip", "fsg_island.hip", "fsg_host.cpp"]
-HEADERS = ["fsg_device.hpp", "fsg_kernels.hpp", "fsg_internal.hpp"]
+SOURCES = ["fsg_kernels.hip", "fsg_api.hip", "fsg_serve.hip", "fsg_render.hip", "fsg_island.hip", "fsg_generate.hip", "fsg_host.cpp"]
+HEADERS = ["fsg_device.hpp", "fsg_kernels.hpp", "fsg_internal.hpp", "fsg_generate.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FSG_ARCH", "gfx950")

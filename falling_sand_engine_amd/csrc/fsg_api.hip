@@ -375,10 +375,12 @@ extern "C" int fsg_world_destroy(fsg_world *w) {
     dev_free(w->d_state);
     dev_free(w->d_disp);
     dev_free(w->d_dispxy);
+    dev_free(w->d_tgt);
     for (uint32_t *l : w->d_layer) dev_free(l);
     dev_free(w->d_tally);
     dev_free(w->d_isl);
     dev_free(w->d_isl_pix);
+    dev_free(w->d_gen);
     dev_free(w->d_sw);
     delete w;
     return FSG_OK;
@@ -563,6 +565,7 @@ fsg::StampWorld stamp_world_of(fsg_world *w, uint32_t pix_start) {
     sw.state = w->d_state;
     sw.disp = w->d_disp;
     sw.disp_xy = w->d_dispxy;
+    sw.tgt = w->d_tgt;
     sw.W = w->d.W;
     sw.H = w->d.H;
     sw.tiles_x = w->d.tiles_x;
@@ -573,15 +576,15 @@ fsg::StampWorld stamp_world_of(fsg_world *w, uint32_t pix_start) {
 }
 
 // Claim rounds over a batch of worlds whose table is already on the device.
-int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t total, const MatTab &M,
-                     bool unstamp) {
-    if (total == 0) return FSG_OK;
+int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t total, uint32_t max_npix,
+                 const MatTab &M, bool unstamp) {
+    if (total == 0 || max_npix == 0) return FSG_OK;
     fsg::StampArgs a;
     a.worlds = d_table;
     a.nworlds = n;
     a.total = total;
     a.mat = M;
-    const unsigned g = grid_for(total, 256);
+    const dim3 g(grid_for(max_npix, 256), (unsigned)n);  // y: world, x: its pixels
     fsg::k_claims_init<<<g, 256, 0, c->stream>>>(a);
     for (int d = 0; d < 5; d++) {
         if (unstamp) {
@@ -607,7 +610,7 @@ int stamp_single(fsg_world *w, bool unstamp) {
     }
     w->sw_host = stamp_world_of(w, 0);
     HIPCHK(hipMemcpyAsync(w->d_sw, &w->sw_host, sizeof(fsg::StampWorld), hipMemcpyHostToDevice, c->stream));
-    return stamp_rounds(c, w->d_sw, 1, (uint32_t)w->npix, w->d.mat, unstamp);
+    return stamp_rounds(c, w->d_sw, 1, (uint32_t)w->npix, (uint32_t)w->npix, w->d.mat, unstamp);
 }
 
 int ensure_stamp_buffers(fsg_world *w, size_t npix, int nb) {
@@ -619,10 +622,11 @@ int ensure_stamp_buffers(fsg_world *w, size_t npix, int nb) {
         dev_free(w->d_state);
         dev_free(w->d_disp);
         dev_free(w->d_dispxy);
+        dev_free(w->d_tgt);
         w->d_pixels = nullptr;
         w->d_pixbody = nullptr;
         w->d_state = nullptr;
-        w->d_disp = w->d_dispxy = nullptr;
+        w->d_disp = w->d_dispxy = w->d_tgt = nullptr;
         w->pix_cap = 0;
         if (dev_alloc(c, &p, npix * 8)) return FSG_ERR_NOMEM;
         w->d_pixels = (uint64_t *)p;
@@ -634,6 +638,8 @@ int ensure_stamp_buffers(fsg_world *w, size_t npix, int nb) {
         w->d_disp = (uint64_t *)p;
         if (dev_alloc(c, &p, npix * 8)) return FSG_ERR_NOMEM;
         w->d_dispxy = (uint64_t *)p;
+        if (dev_alloc(c, &p, npix * 8)) return FSG_ERR_NOMEM;
+        w->d_tgt = (uint64_t *)p;
         w->pix_cap = npix;
     }
     if (nb > w->body_cap) {

@@ -9,6 +9,8 @@
 #include <cstring>
 #include <vector>
 
+#include "fsg_generate.hpp"
+
 #define GOLDEN 0x9E3779B97F4A7C15ull
 
 extern "C" const char *fsg_status_string(int st) {
@@ -159,6 +161,28 @@ void disc(fsg_cell *g, int W, int H, int cx, int cy, int rad, int m, uint64_t se
     }
 }
 
+void basin_discs(int W, int H, uint64_t seed, int n_sand, int n_lava, int n_oil, std::vector<fsg_gen_disc> &out) {
+    out.clear();
+    Stream rs{fsg_splitmix64(seed ^ 0x5A4Dull)};
+    for (int i = 0; i < n_sand; i++) {
+        const int cx = (int)rs.below((uint32_t)W), cy = (int)rs.below((uint32_t)H);
+        const int r = 4 + (int)rs.below(17);
+        out.push_back({cx, cy, r, M_SAND});
+    }
+    Stream rl{fsg_splitmix64(seed ^ 0x1A7Aull)};
+    for (int i = 0; i < n_lava; i++) {
+        const int cx = (int)rl.below((uint32_t)W), cy = (int)rl.below((uint32_t)H);
+        const int r = 8 + (int)rl.below(25);
+        out.push_back({cx, cy, r, M_LAVA});
+    }
+    Stream ro{fsg_splitmix64(seed ^ 0x0111ull)};
+    for (int i = 0; i < n_oil; i++) {
+        const int cx = (int)ro.below((uint32_t)W), cy = (int)ro.below((uint32_t)H);
+        const int r = 4 + (int)ro.below(17);
+        out.push_back({cx, cy, r, M_OIL});
+    }
+}
+
 void gen_basin(fsg_cell *g, int W, int H, uint64_t seed, int n_sand, int n_lava,
                int n_oil) {
     // terrain: stone below H - noise*0.4*H
@@ -173,23 +197,24 @@ void gen_basin(fsg_cell *g, int W, int H, uint64_t seed, int n_sand, int n_lava,
     for (int y = level; y < H; y++)
         for (int x = 0; x < W; x++)
             if (g[(size_t)y * W + x].mat == M_AIR) g[(size_t)y * W + x] = make(M_WATER, x, y, seed);
-    Stream rs{fsg_splitmix64(seed ^ 0x5A4Dull)};
-    for (int i = 0; i < n_sand; i++) {
-        const int cx = (int)rs.below((uint32_t)W), cy = (int)rs.below((uint32_t)H);
-        const int r = 4 + (int)rs.below(17);
-        disc(g, W, H, cx, cy, r, M_SAND, seed);
+    std::vector<fsg_gen_disc> discs;
+    basin_discs(W, H, seed, n_sand, n_lava, n_oil, discs);
+    for (const fsg_gen_disc &d : discs) disc(g, W, H, d.cx, d.cy, d.r, d.m, seed);
+}
+
+// Disc counts of a basin kind (1, 2, 4); false for other kinds.
+bool basin_counts(int kind, int W, int H, int &n_sand, int &n_lava, int &n_oil) {
+    switch (kind) {
+    case 1: n_sand = 64, n_lava = 0, n_oil = 0; return true;
+    case 2: n_sand = 64, n_lava = 256, n_oil = 128; return true;
+    case 4: {
+        // kind 2 mix scaled to the world's area relative to 4096^2
+        const double a = (double)W * (double)H / (4096.0 * 4096.0);
+        auto sc = [&](int n) { int v = (int)(n * a + 0.5); return v < 1 ? 1 : v; };
+        n_sand = sc(64), n_lava = sc(256), n_oil = sc(128);
+        return true;
     }
-    Stream rl{fsg_splitmix64(seed ^ 0x1A7Aull)};
-    for (int i = 0; i < n_lava; i++) {
-        const int cx = (int)rl.below((uint32_t)W), cy = (int)rl.below((uint32_t)H);
-        const int r = 8 + (int)rl.below(25);
-        disc(g, W, H, cx, cy, r, M_LAVA, seed);
-    }
-    Stream ro{fsg_splitmix64(seed ^ 0x0111ull)};
-    for (int i = 0; i < n_oil; i++) {
-        const int cx = (int)ro.below((uint32_t)W), cy = (int)ro.below((uint32_t)H);
-        const int r = 4 + (int)ro.below(17);
-        disc(g, W, H, cx, cy, r, M_OIL, seed);
+    default: return false;
     }
 }
 
@@ -221,19 +246,27 @@ void gen_striped(fsg_cell *g, int W, int H, uint64_t seed) {
 
 extern "C" int fsg_generate(int kind, int W, int H, uint64_t seed, fsg_cell *out) {
     if (!out || W <= 0 || H <= 0 || W > FSG_MAX_SIDE || H > FSG_MAX_SIDE) return FSG_ERR_INVALID;
-    fsg_cell *g = out;
-    switch (kind) {
-    case 1: gen_basin(g, W, H, seed, 64, 0, 0); break;
-    case 2: gen_basin(g, W, H, seed, 64, 256, 128); break;
-    case 3: gen_striped(g, W, H, seed); break;
-    case 4: {
-        // kind 2 mix scaled to the world's area relative to 4096^2
-        const double a = (double)W * (double)H / (4096.0 * 4096.0);
-        auto sc = [&](int n) { int v = (int)(n * a + 0.5); return v < 1 ? 1 : v; };
-        gen_basin(g, W, H, seed, sc(64), sc(256), sc(128));
-        break;
+    if (kind == 3) {
+        gen_striped(out, W, H, seed);
+        return FSG_OK;
     }
-    default: return FSG_ERR_INVALID;
-    }
+    int ns, nl, no;
+    if (!basin_counts(kind, W, H, ns, nl, no)) return FSG_ERR_INVALID;
+    gen_basin(out, W, H, seed, ns, nl, no);
+    return FSG_OK;
+}
+
+// The generator's plan for the device path (fsg_world_generate): the disc
+// list of a basin kind in drawing order and the constants of make().
+int fsg_gen_plan(int kind, int W, int H, uint64_t seed, std::vector<fsg_gen_disc> &discs, uint32_t rgba[8],
+                 int16_t &ambient, int16_t &lava_temp) {
+    for (int i = 0; i < 8; i++) rgba[i] = kMats[i].rgba;
+    ambient = kAmbient;
+    lava_temp = kLavaTemp;
+    discs.clear();
+    if (kind == 3) return FSG_OK;
+    int ns, nl, no;
+    if (!basin_counts(kind, W, H, ns, nl, no)) return FSG_ERR_INVALID;
+    basin_discs(W, H, seed, ns, nl, no, discs);
     return FSG_OK;
 }

@@ -8,6 +8,7 @@
 
 #include "fsg.h"
 #include "fsg_device.hpp"
+#include "fsg_generate.hpp"
 #include "fsg_kernels.hpp"
 
 using fsg::DevWorld;
@@ -52,6 +53,9 @@ struct fsg_world {
     size_t isl_cap = 0;
     uint64_t *d_isl_pix = nullptr;                         // body images of the last island check
     size_t isl_pix_cap = 0, isl_pix_used = 0;
+    void *d_gen = nullptr;                                 // device generator scratch
+    size_t gen_cap = 0;
+    std::vector<fsg_gen_disc> gen_discs;                   // host copy of the last disc upload
     size_t bytes = 0;
     uint64_t offered = 0;
     unsigned long long *d_stats = nullptr;  // [17] census scratch + [1] processed counter
@@ -61,6 +65,7 @@ struct fsg_world {
     fsg_body *d_bodies = nullptr;
     uint8_t *d_state = nullptr;
     uint64_t *d_disp = nullptr, *d_dispxy = nullptr;
+    uint64_t *d_tgt = nullptr;  // per pixel: own cell under the pose (stamp kernels)
     size_t pix_cap = 0;
     int body_cap = 0;
     size_t npix = 0;
@@ -85,8 +90,9 @@ int batch_run(fsg_ctx *c, fsg_world *const *ws, int n, int ticks, bool graph_ok)
 // The world's entry in a stamp batch (pixels numbered from pix_start).
 fsg::StampWorld stamp_world_of(fsg_world *w, uint32_t pix_start);
 // Stamp (or unstamp) claim rounds over a device table of n worlds.
-int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t total, const MatTab &M,
-                 bool unstamp);
+// max_npix: the largest pixel count of one world (the launch's x extent).
+int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t total, uint32_t max_npix,
+                 const MatTab &M, bool unstamp);
 // Uploads pixels/bodies of a world for stamping (validates them).
 int set_bodies(fsg_world *w, const fsg_body *bodies, int nb, const fsg_cell *pixels, size_t npix);
 }  // namespace fsg_impl

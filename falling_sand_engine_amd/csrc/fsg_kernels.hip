@@ -670,45 +670,48 @@ __global__ void k_set_cells(uint64_t *cells, int W, int H, const int32_t *xy, co
 __constant__ int kDX[5] = {0, 1, -1, 0, 0};
 __constant__ int kDY[5] = {0, 0, 0, 1, -1};
 
-// Pixel g of the batch -> world, local pixel, target cell of round d
-// (d = -1: the pixel's own cell).  False if the pixel is not live (no body,
-// body disabled, air pixel) or the target is out of its world.
+// The pixel of this thread (blockIdx.y = world, x over its pixels) and, for
+// round d (d = -1: the pixel's own cell), its target cell.  The own cell of a
+// live pixel (not air, body present and enabled) is computed once per stamp
+// or unstamp by k_claims_init into tgt; a round adds its offset.
 struct PixelRef {
     const StampWorld *w;
     uint32_t p;
 };
 
-__device__ __forceinline__ bool pixel_ref(const StampArgs &a, uint32_t g, PixelRef &ref) {
-    if (g >= a.total) return false;
-    int lo = 0, hi = a.nworlds - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (a.worlds[mid].pix_start <= g) lo = mid;
-        else hi = mid - 1;
-    }
-    ref.w = &a.worlds[lo];
-    ref.p = g - ref.w->pix_start;
-    if (ref.p >= ref.w->npix) return false;
-    return !bit(a.mat.air, cid(ref.w->pixels[ref.p]));
+__device__ __forceinline__ bool pixel_ref(const StampArgs &a, PixelRef &ref) {
+    ref.w = &a.worlds[blockIdx.y];
+    ref.p = blockIdx.x * blockDim.x + threadIdx.x;
+    return ref.p < ref.w->npix;
 }
 
+constexpr uint32_t NOT_LIVE = 0x80000000u;
+
 __device__ __forceinline__ bool pixel_target(const PixelRef &ref, int d, int &wx, int &wy) {
-    const StampWorld &W = *ref.w;
-    const uint32_t b = W.pix_body[ref.p];
-    if (b == 0xFFFFFFFFu) return false;
-    const fsg_body B = W.bodies[b];
-    if (!B.enabled) return false;
-    const uint32_t k = ref.p - B.pixel_offset;
-    const int tx = (int)(k % (uint32_t)B.mat_w), ty = (int)(k / (uint32_t)B.mat_w);
-    const int64_t fx = (int64_t)tx * B.cos_q16 - (int64_t)(ty + 1) * B.sin_q16 + B.pos_x_q16;
-    const int64_t fy = (int64_t)tx * B.sin_q16 + (int64_t)(ty + 1) * B.cos_q16 + B.pos_y_q16;
-    wx = (int)(fx >> 16);
-    wy = (int)(fy >> 16);
+    const uint64_t t = ref.w->tgt[ref.p];
+    if ((uint32_t)t == NOT_LIVE) return false;
+    wx = (int)(uint32_t)t;
+    wy = (int)(uint32_t)(t >> 32);
     if (d >= 0) {
         wx += kDX[d];
         wy += kDY[d];
     }
-    return wx >= 0 && wy >= 0 && wx < W.W && wy < W.H;
+    return wx >= 0 && wy >= 0 && wx < ref.w->W && wy < ref.w->H;
+}
+
+// Own cell of pixel p under its body's pose (Q16.16 rotation of the pixel's
+// lower-left corner, as the reference rotates (tx, ty + 1)), or NOT_LIVE.
+__device__ __forceinline__ uint64_t own_cell(const StampArgs &a, const StampWorld &W, uint32_t p) {
+    if (bit(a.mat.air, cid(W.pixels[p]))) return NOT_LIVE;
+    const uint32_t b = W.pix_body[p];
+    if (b == 0xFFFFFFFFu) return NOT_LIVE;
+    const fsg_body B = W.bodies[b];
+    if (!B.enabled) return NOT_LIVE;
+    const uint32_t k = p - B.pixel_offset;
+    const int tx = (int)(k % (uint32_t)B.mat_w), ty = (int)(k / (uint32_t)B.mat_w);
+    const int64_t fx = (int64_t)tx * B.cos_q16 - (int64_t)(ty + 1) * B.sin_q16 + B.pos_x_q16;
+    const int64_t fy = (int64_t)tx * B.sin_q16 + (int64_t)(ty + 1) * B.cos_q16 + B.pos_y_q16;
+    return (uint64_t)(uint32_t)(int)(fx >> 16) | ((uint64_t)(uint32_t)(int)(fy >> 16) << 32);
 }
 
 // Chunks whose cells a stamp round touches must run on the next tick (their
@@ -719,31 +722,21 @@ __device__ __forceinline__ void mark_tile(const StampWorld &W, int x, int y) {
 
 __global__ void k_claims_init(StampArgs a) {
     PixelRef ref;
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.total) return;
-    {
-        int lo = 0, hi = a.nworlds - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (a.worlds[mid].pix_start <= g) lo = mid;
-            else hi = mid - 1;
-        }
-        const uint32_t p = g - a.worlds[lo].pix_start;
-        if (p < a.worlds[lo].npix) a.worlds[lo].state[p] = 0;
-    }
-    if (!pixel_ref(a, g, ref)) return;
+    if (!pixel_ref(a, ref)) return;
+    const StampWorld &W = *ref.w;
+    W.state[ref.p] = 0;
+    W.tgt[ref.p] = own_cell(a, W, ref.p);
     for (int d = 0; d < 5; d++) {
         int wx, wy;
         if (!pixel_target(ref, d, wx, wy)) continue;
-        ref.w->claims[(size_t)wy * ref.w->W + wx] = ~0ull;
-        mark_tile(*ref.w, wx, wy);
+        W.claims[(size_t)wy * W.W + wx] = ~0ull;
+        mark_tile(W, wx, wy);
     }
 }
 
 __global__ void k_stamp_propose(StampArgs a, int d) {
     PixelRef ref;
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!pixel_ref(a, g, ref) || ref.w->state[ref.p]) return;
+    if (!pixel_ref(a, ref) || ref.w->state[ref.p]) return;
     int wx, wy;
     if (!pixel_target(ref, d, wx, wy)) return;
     const size_t ci = (size_t)wy * ref.w->W + wx;
@@ -754,8 +747,7 @@ __global__ void k_stamp_propose(StampArgs a, int d) {
 
 __global__ void k_stamp_resolve(StampArgs a, int d) {
     PixelRef ref;
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!pixel_ref(a, g, ref) || ref.w->state[ref.p]) return;
+    if (!pixel_ref(a, ref) || ref.w->state[ref.p]) return;
     int wx, wy;
     if (!pixel_target(ref, d, wx, wy)) return;
     const StampWorld &W = *ref.w;
@@ -774,8 +766,7 @@ __global__ void k_stamp_resolve(StampArgs a, int d) {
 
 __global__ void k_unstamp_propose(StampArgs a, int d) {
     PixelRef ref;
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!pixel_ref(a, g, ref) || ref.w->state[ref.p]) return;
+    if (!pixel_ref(a, ref) || ref.w->state[ref.p]) return;
     int wx, wy;
     if (!pixel_target(ref, d, wx, wy)) return;
     const size_t ci = (size_t)wy * ref.w->W + wx;
@@ -785,8 +776,7 @@ __global__ void k_unstamp_propose(StampArgs a, int d) {
 
 __global__ void k_unstamp_resolve(StampArgs a, int d) {
     PixelRef ref;
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!pixel_ref(a, g, ref) || ref.w->state[ref.p]) return;
+    if (!pixel_ref(a, ref) || ref.w->state[ref.p]) return;
     int wx, wy;
     if (!pixel_target(ref, d, wx, wy)) return;
     const StampWorld &W = *ref.w;
@@ -801,8 +791,7 @@ __global__ void k_unstamp_resolve(StampArgs a, int d) {
 // A pixel that found nothing is lost when the cell under it is air.
 __global__ void k_unstamp_finish(StampArgs a) {
     PixelRef ref;
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!pixel_ref(a, g, ref) || ref.w->state[ref.p]) return;
+    if (!pixel_ref(a, ref) || ref.w->state[ref.p]) return;
     int wx, wy;
     if (!pixel_target(ref, -1, wx, wy)) return;
     if (bit(a.mat.air, cid(ref.w->cells[(size_t)wy * ref.w->W + wx]))) ref.w->pixels[ref.p] = (uint64_t)a.mat.air_id;

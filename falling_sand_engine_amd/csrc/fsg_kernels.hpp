@@ -17,10 +17,12 @@ struct StampWorld {
     uint8_t *state;            // per pixel: 0 open, 1 placed/found, 2/3 placed over sand/soup
     uint64_t *disp;            // displaced cell per pixel
     uint64_t *disp_xy;         // its position (x | y << 32)
+    uint64_t *tgt;             // per pixel: its own cell under the pose, x | y << 32 (x = INT32_MIN: not live)
     int32_t W, H, tiles_x;
     uint32_t npix, pix_start, pad;
 };
 
+// Stamp kernels run on a 2D grid: blockIdx.y = world, x over its pixels.
 struct StampArgs {
     const StampWorld *worlds;  // device array
     int nworlds;

@@ -388,10 +388,11 @@ void settle_pending(fsg_server *S, std::vector<fsg_serve_result> &out) {
 int stamp_batch(fsg_server *S, const std::vector<Resident *> &bw, bool unstamp) {
     fsg_ctx *c = S->ctx;
     S->table.clear();
-    uint32_t total = 0;
+    uint32_t total = 0, max_npix = 0;
     for (Resident *r : bw) {
         S->table.push_back(stamp_world_of(r->w, total));
         total += (uint32_t)r->w->npix;
+        max_npix = std::max(max_npix, (uint32_t)r->w->npix);
     }
     if ((int)S->table.size() > S->table_cap) {
         dev_free(S->d_table);
@@ -403,7 +404,7 @@ int stamp_batch(fsg_server *S, const std::vector<Resident *> &bw, bool unstamp) 
     }
     HIPCHK(hipMemcpyAsync(S->d_table, S->table.data(), S->table.size() * sizeof(fsg::StampWorld),
                           hipMemcpyHostToDevice, c->stream));
-    const int rc = stamp_rounds(c, S->d_table, (int)S->table.size(), total, bw[0]->w->d.mat, unstamp);
+    const int rc = stamp_rounds(c, S->d_table, (int)S->table.size(), total, max_npix, bw[0]->w->d.mat, unstamp);
     HIPCHK(hipStreamSynchronize(c->stream));  // the host table is reused next call
     return rc;
 }

@@ -160,6 +160,11 @@ int fsg_world_set_tick(fsg_world *w, uint64_t tick); /* resume at tick */
 /* Whole-grid host->device copy (the chunk load path writes real_tiles,
  * world.cpp:2653-2665).  n must equal width*height. */
 int fsg_world_upload(fsg_world *w, const fsg_cell *cells, size_t n);
+/* The benchmark world generator (fsg_generate) run on the device straight
+ * into the world's grid: the same cells as fsg_generate + fsg_world_upload
+ * without the host work (no reference counterpart: the reference fills its
+ * grid from chunk files or its own generator, world.cpp:43-172, 2653-2665). */
+int fsg_world_generate(fsg_world *w, int kind, uint64_t seed);
 /* Rectangle device->host copy (getTile over a region, world.cpp:999-1002). */
 int fsg_world_download(fsg_world *w, fsg_cell *out, int x0, int y0, int rw, int rh);
 
