@@ -4,8 +4,8 @@
 //   1. admits queued requests in arrival order while the context's HBM state
 //      budget allows (evicting idle resident worlds, least recently used
 //      first); a request whose world is busy waits for the earlier one;
-//   2. finishes the host-side generation of newly created worlds and
-//      uploads them;
+//   2. writes the cells of newly created worlds with the device generator
+//      (fsg_world_generate) and uploads their bodies;
 //   3. advances all admitted requests together: one batched tick launch per
 //      tick (16-tick HIP graphs when no world has bodies), with the rigid
 //      bodies of every world stamped before and unstamped after each tick in
@@ -23,7 +23,6 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
-#include <future>
 #include <list>
 #include <memory>
 #include <mutex>
@@ -61,6 +60,7 @@ struct Resident {
     bool checksum_known = false;
     std::vector<HostBody> hb;
     std::vector<fsg_body> poses;
+    std::vector<fsg_cell> px;  // body pixels as uploaded (kept while the copy may be in flight)
 };
 
 struct Req {
@@ -72,10 +72,6 @@ struct Req {
     Resident *res = nullptr;
 };
 
-struct Pending {  // a created world whose cells are being generated on the host
-    Req req;
-    std::future<std::vector<fsg_cell>> cells;
-};
 
 constexpr double kDt = 1.0 / 30.3;  // b2world->Step interval (world.cpp:2276 at maxTps 30)
 constexpr double kGravity = 20.0;   // b2World gravity (0, 20), world.cpp:147-148
@@ -192,8 +188,6 @@ struct fsg_server {
     fsg_serve_stats st{};
     // worker-owned
     std::unordered_map<uint64_t, std::unique_ptr<Resident>> res;
-    std::list<Pending> pending;
-    std::atomic<size_t> n_pending{0};  // pending.size() for fsg_serve_stats_get
     std::vector<Req> active;
     fsg::StampWorld *d_table = nullptr;
     int table_cap = 0;
@@ -274,7 +268,7 @@ void admit(fsg_server *S, std::vector<fsg_serve_result> &out) {
         }
         const uint64_t wid = q.r.world_id;
         if (std::find(blocked.begin(), blocked.end(), wid) != blocked.end() ||
-            S->active.size() + S->pending.size() >= cap) {
+            S->active.size() >= cap) {
             blocked.push_back(wid);
             ++it;
             continue;
@@ -315,7 +309,7 @@ void admit(fsg_server *S, std::vector<fsg_serve_result> &out) {
         }
         const uint64_t need = world_bytes(rq.width, rq.height);
         if (!make_room(S, need)) {
-            if (S->active.empty() && S->pending.empty()) {  // can never fit
+            if (S->active.empty()) {  // can never fit
                 finish(S, q, FSG_ERR_BUDGET, out);
                 it = S->queue.erase(it);
                 continue;
@@ -327,7 +321,7 @@ void admit(fsg_server *S, std::vector<fsg_serve_result> &out) {
         fsg_world *w = nullptr;
         const int rc = fsg_world_create(S->ctx, rq.width, rq.height, rq.seed, &cfg, &w);
         if (rc != FSG_OK) {
-            if (rc == FSG_ERR_BUDGET && !(S->active.empty() && S->pending.empty())) break;
+            if (rc == FSG_ERR_BUDGET && !S->active.empty()) break;
             finish(S, q, rc, out);
             it = S->queue.erase(it);
             continue;
@@ -336,52 +330,27 @@ void admit(fsg_server *S, std::vector<fsg_serve_result> &out) {
         r->id = wid;
         r->w = w;
         r->busy = true;
-        q.res = r.get();
+        Resident *rp = r.get();
+        q.res = rp;
         q.t_admit = now;
         S->res[wid] = std::move(r);
-        Pending p;
-        p.req = q;
-        const int kind = rq.kind, W = rq.width, H = rq.height;
-        const uint64_t seed = rq.seed;
-        p.cells = std::async(std::launch::async, [kind, W, H, seed]() {
-            std::vector<fsg_cell> cells((size_t)W * H);
-            if (fsg_generate(kind, W, H, seed, cells.data()) != FSG_OK) cells.clear();
-            return cells;
-        });
-        S->pending.push_back(std::move(p));
-        S->n_pending++;
-        it = S->queue.erase(it);
-    }
-}
-
-// Uploads generated worlds whose host generation has finished.
-void settle_pending(fsg_server *S, std::vector<fsg_serve_result> &out) {
-    for (auto it = S->pending.begin(); it != S->pending.end();) {
-        if (it->cells.wait_for(std::chrono::seconds(0)) != std::future_status::ready) {
-            ++it;
-            continue;
+        // the world's cells come from the device generator; bodies from the
+        // host kinematic model
+        int grc = fsg_world_generate(w, rq.kind, rq.seed);
+        if (grc == FSG_OK && rq.n_bodies > 0) {
+            make_bodies(rq.seed, std::min(rq.n_bodies, 4096), rq.width, rq.height, rp->hb, rp->px);
+            poses_of(rp->hb, rp->poses);
+            grc = set_bodies(w, rp->poses.data(), (int)rp->poses.size(), rp->px.data(), rp->px.size());
         }
-        Req q = it->req;
-        std::vector<fsg_cell> cells = it->cells.get();
-        it = S->pending.erase(it);
-        S->n_pending--;
-        Resident *r = q.res;
-        int rc = cells.empty() ? FSG_ERR_INVALID : fsg_world_upload(r->w, cells.data(), cells.size());
-        if (rc == FSG_OK && q.r.n_bodies > 0) {
-            std::vector<fsg_cell> px;
-            make_bodies(q.r.seed, std::min(q.r.n_bodies, 4096), q.r.width, q.r.height, r->hb, px);
-            poses_of(r->hb, r->poses);
-            rc = set_bodies(r->w, r->poses.data(), (int)r->poses.size(), px.data(), px.size());
-            if (rc == FSG_OK) rc = fsg_world_sync(r->w);  // px is a local
-        }
-        std::lock_guard<std::mutex> lk(S->mu);  // active is read by fsg_serve_cancel
-        if (rc != FSG_OK) {
-            finish(S, q, rc, out);
-            r->busy = false;
-            drop_world(S, r->id);
+        if (grc != FSG_OK) {
+            finish(S, q, grc, out);
+            rp->busy = false;
+            drop_world(S, wid);
+            it = S->queue.erase(it);
             continue;
         }
         S->active.push_back(q);
+        it = S->queue.erase(it);
     }
 }
 
@@ -456,12 +425,11 @@ void worker(fsg_server *S) {
             }
             S->corruptions.clear();
             admit(S, out);
-            if (S->active.empty() && S->n_pending.load() == 0 && out.empty()) {
+            if (S->active.empty() && out.empty()) {
                 S->cv.wait_for(lk, std::chrono::milliseconds(2));
                 continue;
             }
         }
-        settle_pending(S, out);
         if (!S->active.empty()) {
             // cancelled or late active requests leave before the step
             const uint64_t now = fsg_now_ns();
@@ -523,8 +491,6 @@ void worker(fsg_server *S) {
     for (Req &q : S->queue) finish(S, q, FSG_ERR_CANCELLED, out);
     S->active.clear();
     S->queue.clear();
-    for (auto &p : S->pending) p.cells.wait();
-    S->pending.clear();
     for (auto &r : out) S->done.push_back(r);
 }
 
@@ -634,7 +600,7 @@ extern "C" int fsg_serve_stats_get(fsg_server *S, fsg_serve_stats *out) {
     std::lock_guard<std::mutex> lk(S->mu);
     *out = S->st;
     out->queued = S->queue.size();
-    out->active = S->active.size() + S->n_pending.load();
+    out->active = S->active.size();
     out->resident_worlds = S->res.size();
     uint64_t used = 0;
     fsg_ctx_usage(S->ctx, &used);
