@@ -175,7 +175,7 @@ def bench_c4(fsg, ticks=256, n=256, tune_ticks=32):
                                          round(c["ms"] / tune_ticks, 3) for c in cand}}}
 
 
-def bench_c5(fsg, rates=(30.0, 120.0, 400.0), duration_s=6.0, budget=64 << 30, seed=5):
+def bench_c5(fsg, rates=(30.0, 120.0, 400.0, 1000.0), duration_s=6.0, budget=64 << 30, seed=5):
     """Serving: Poisson arrivals at each offered rate for duration_s.  Each
     request creates a world (side uniform in {512..4096 step 64}, config-2
     mix scaled to its area, 0-200 rigid bodies) or steps a resident one,
@@ -365,6 +365,7 @@ def main():
                     help="all: the JSON line; c3: only the timed tick loop (for profiling); "
                          "copy: the HBM copy calibration kernel; render: only the texture pass")
     ap.add_argument("--copy-bytes", type=int, default=2 << 30)
+    ap.add_argument("--rate", type=float, default=0.0, help="c5 mode: one offered rate (req/s) instead of the set")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 "
@@ -381,7 +382,7 @@ def main():
         print(json.dumps({"mode": "copy", "bytes": args.copy_bytes, "ms": ms, "GBs_read_plus_write": gbs}))
         return
     if args.mode == "c5":
-        print(json.dumps(bench_c5(fsg)))
+        print(json.dumps(bench_c5(fsg, rates=(args.rate,)) if args.rate > 0 else bench_c5(fsg)))
         return
     if args.mode == "islands":
         print(json.dumps(bench_islands(fsg)))

@@ -26,6 +26,32 @@ using namespace fsg_impl;
 namespace fsg_impl {
 
 
+// Device memory with a per-context cache of freed blocks.  hipFree waits for
+// the whole device, which would stall a serving loop that creates and evicts
+// worlds while other worlds tick; a freed block instead goes back to its
+// context and is handed out again for the same rounded size.  A context's
+// work runs on its one stream, so a reused block is only touched after the
+// work that used it before.  The cache is trimmed past cache_cap and emptied
+// by fsg_ctx_destroy.
+namespace {
+std::mutex g_alloc_mu;
+std::unordered_map<void *, std::pair<fsg_ctx *, size_t>> g_live;  // block -> (ctx, rounded bytes)
+
+void trim_cache_locked(fsg_ctx *c, size_t keep) {
+    while (c->cached > keep && !c->cache.empty()) {
+        auto it = std::prev(c->cache.end());  // largest first
+        (void)hipFree(it->second);
+        c->cached -= it->first;
+        c->cache.erase(it);
+    }
+}
+} // namespace
+
+void trim_cache(fsg_ctx *c, size_t keep) {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    trim_cache_locked(c, keep);
+}
+
 int dev_alloc(fsg_ctx *c, void **p, size_t bytes) {
     *p = nullptr;
     if (c->inject > 0) {
@@ -33,16 +59,46 @@ int dev_alloc(fsg_ctx *c, void **p, size_t bytes) {
         return FSG_ERR_NOMEM;
     }
     if (bytes == 0) bytes = 8;
-    if (hipMalloc(p, bytes) != hipSuccess) {
-        *p = nullptr;
-        (void)hipGetLastError();
-        return FSG_ERR_NOMEM;
+    bytes = (bytes + 255) & ~(size_t)255;
+    {
+        std::lock_guard<std::mutex> lk(g_alloc_mu);
+        auto it = c->cache.find(bytes);
+        if (it != c->cache.end()) {
+            *p = it->second;
+            c->cache.erase(it);
+            c->cached -= bytes;
+            g_live[*p] = {c, bytes};
+            return FSG_OK;
+        }
     }
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        trim_cache(c, 0);  // give the cached blocks back and try once more
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            *p = nullptr;
+            (void)hipGetLastError();
+            return FSG_ERR_NOMEM;
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    g_live[*p] = {c, bytes};
     return FSG_OK;
 }
 
 void dev_free(void *p) {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    auto it = g_live.find(p);
+    if (it == g_live.end()) {
+        (void)hipFree(p);
+        return;
+    }
+    fsg_ctx *c = it->second.first;
+    const size_t bytes = it->second.second;
+    g_live.erase(it);
+    c->cache.emplace(bytes, p);
+    c->cached += bytes;
+    if (c->cached > c->cache_cap) trim_cache_locked(c, c->cache_cap);
 }
 
 int validate_config(const fsg_config &c) {
@@ -207,6 +263,7 @@ extern "C" int fsg_ctx_destroy(fsg_ctx *c) {
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     dev_free(c->d_worlds);
     dev_free(c->d_tile_start);
+    trim_cache(c, 0);
     (void)hipEventDestroy(c->e0);
     (void)hipEventDestroy(c->e1);
     (void)hipEventDestroy(c->t0);
@@ -454,8 +511,8 @@ extern "C" int fsg_world_set_cells(fsg_world *w, const fsg_cell_edit *edits, siz
     fsg_ctx *c = w->ctx;
     HIPCHK(hipSetDevice(c->device));
     void *dxy = nullptr, *dv = nullptr;
-    if (hipMalloc(&dxy, xy.size() * 4) != hipSuccess) return FSG_ERR_NOMEM;
-    if (hipMalloc(&dv, vals.size() * 8) != hipSuccess) {
+    if (dev_alloc(c, &dxy, xy.size() * 4)) return FSG_ERR_NOMEM;
+    if (dev_alloc(c, &dv, vals.size() * 8)) {
         dev_free(dxy);
         return FSG_ERR_NOMEM;
     }

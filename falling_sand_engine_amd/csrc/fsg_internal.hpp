@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -39,6 +40,9 @@ struct fsg_ctx {
     // the cached graph was captured with
     fsg_tuning tune{16, -1, 5, 0};
     int graph_ticks = 0, graph_occ = 0;
+    // freed device blocks by rounded size (dev_alloc / dev_free)
+    std::multimap<size_t, void *> cache;
+    size_t cached = 0, cache_cap = (size_t)16 << 30;
 };
 
 struct fsg_world {
@@ -81,6 +85,7 @@ struct fsg_world {
 namespace fsg_impl {
 int dev_alloc(fsg_ctx *c, void **p, size_t bytes);
 void dev_free(void *p);
+void trim_cache(fsg_ctx *c, size_t keep);  // hipFree cached blocks down to `keep` bytes
 unsigned grid_for(size_t n, unsigned block);
 // Uploads the world table of a batch; *graph_ok tells whether the cached
 // graph matches this world set.

@@ -189,9 +189,14 @@ struct fsg_server {
     // worker-owned
     std::unordered_map<uint64_t, std::unique_ptr<Resident>> res;
     std::vector<Req> active;
-    fsg::StampWorld *d_table = nullptr;
+    // per-step staging in pinned host memory, one upload each: the stamp
+    // tables of the step (stamp at [0, n), unstamp at [n, 2n)) and every
+    // world's body poses; reused after the step's final synchronisation
+    fsg::StampWorld *d_table = nullptr, *h_table = nullptr;
     int table_cap = 0;
-    std::vector<fsg::StampWorld> table;
+    fsg_body *d_pose = nullptr, *h_pose = nullptr;
+    size_t pose_cap = 0;
+    std::vector<size_t> pose_off;
     std::thread th;
 };
 
@@ -354,28 +359,56 @@ void admit(fsg_server *S, std::vector<fsg_serve_result> &out) {
     }
 }
 
+// Claim rounds of the worlds with bodies: stamp (half 0) or unstamp (half 1)
+// table, bodies read from the step's pose staging.
 int stamp_batch(fsg_server *S, const std::vector<Resident *> &bw, bool unstamp) {
     fsg_ctx *c = S->ctx;
-    S->table.clear();
+    const int n = (int)bw.size();
+    fsg::StampWorld *h = S->h_table + (unstamp ? n : 0);
     uint32_t total = 0, max_npix = 0;
-    for (Resident *r : bw) {
-        S->table.push_back(stamp_world_of(r->w, total));
-        total += (uint32_t)r->w->npix;
-        max_npix = std::max(max_npix, (uint32_t)r->w->npix);
+    for (int i = 0; i < n; i++) {
+        h[i] = stamp_world_of(bw[i]->w, total);
+        h[i].bodies = S->d_pose + S->pose_off[(size_t)i];
+        total += (uint32_t)bw[i]->w->npix;
+        max_npix = std::max(max_npix, (uint32_t)bw[i]->w->npix);
     }
-    if ((int)S->table.size() > S->table_cap) {
+    fsg::StampWorld *d = S->d_table + (unstamp ? n : 0);
+    HIPCHK(hipMemcpyAsync(d, h, (size_t)n * sizeof(fsg::StampWorld), hipMemcpyHostToDevice, c->stream));
+    return stamp_rounds(c, d, n, total, max_npix, bw[0]->w->d.mat, unstamp);
+}
+
+// Grows the pinned host / device staging of a step.
+int ensure_staging(fsg_server *S, int nworlds, size_t nbodies) {
+    fsg_ctx *c = S->ctx;
+    if (nworlds > S->table_cap) {
+        const int cap = std::max(nworlds, 2 * S->table_cap);
         dev_free(S->d_table);
+        if (S->h_table) (void)hipHostFree(S->h_table);
         S->d_table = nullptr;
+        S->h_table = nullptr;
+        S->table_cap = 0;
         void *p;
-        if (dev_alloc(c, &p, S->table.size() * sizeof(fsg::StampWorld))) return FSG_ERR_NOMEM;
+        if (dev_alloc(c, &p, 2 * (size_t)cap * sizeof(fsg::StampWorld))) return FSG_ERR_NOMEM;
         S->d_table = (fsg::StampWorld *)p;
-        S->table_cap = (int)S->table.size();
+        if (hipHostMalloc(&p, 2 * (size_t)cap * sizeof(fsg::StampWorld)) != hipSuccess) return FSG_ERR_NOMEM;
+        S->h_table = (fsg::StampWorld *)p;
+        S->table_cap = cap;
     }
-    HIPCHK(hipMemcpyAsync(S->d_table, S->table.data(), S->table.size() * sizeof(fsg::StampWorld),
-                          hipMemcpyHostToDevice, c->stream));
-    const int rc = stamp_rounds(c, S->d_table, (int)S->table.size(), total, max_npix, bw[0]->w->d.mat, unstamp);
-    HIPCHK(hipStreamSynchronize(c->stream));  // the host table is reused next call
-    return rc;
+    if (nbodies > S->pose_cap) {
+        const size_t cap = std::max(nbodies, 2 * S->pose_cap);
+        dev_free(S->d_pose);
+        if (S->h_pose) (void)hipHostFree(S->h_pose);
+        S->d_pose = nullptr;
+        S->h_pose = nullptr;
+        S->pose_cap = 0;
+        void *p;
+        if (dev_alloc(c, &p, cap * sizeof(fsg_body))) return FSG_ERR_NOMEM;
+        S->d_pose = (fsg_body *)p;
+        if (hipHostMalloc(&p, cap * sizeof(fsg_body)) != hipSuccess) return FSG_ERR_NOMEM;
+        S->h_pose = (fsg_body *)p;
+        S->pose_cap = cap;
+    }
+    return FSG_OK;
 }
 
 // Advances every active request by T ticks.
@@ -388,11 +421,20 @@ int run_step(fsg_server *S, int T) {
     }
     fsg_ctx *c = S->ctx;
     if (!bw.empty()) {
+        size_t nb = 0;
         for (Resident *r : bw) {
             poses_of(r->hb, r->poses);
-            HIPCHK(hipMemcpyAsync(r->w->d_bodies, r->poses.data(), r->poses.size() * sizeof(fsg_body),
-                                  hipMemcpyHostToDevice, c->stream));
+            nb += r->poses.size();
         }
+        if (int rc = ensure_staging(S, (int)bw.size(), nb)) return rc;
+        S->pose_off.clear();
+        size_t off = 0;
+        for (Resident *r : bw) {
+            std::memcpy(S->h_pose + off, r->poses.data(), r->poses.size() * sizeof(fsg_body));
+            S->pose_off.push_back(off);
+            off += r->poses.size();
+        }
+        HIPCHK(hipMemcpyAsync(S->d_pose, S->h_pose, nb * sizeof(fsg_body), hipMemcpyHostToDevice, c->stream));
         if (int rc = stamp_batch(S, bw, false)) return rc;
     }
     bool graph_ok = false;
@@ -520,6 +562,9 @@ extern "C" int fsg_serve_destroy(fsg_server *S) {
     for (auto &kv : S->res) fsg_world_destroy(kv.second->w);
     S->res.clear();
     dev_free(S->d_table);
+    dev_free(S->d_pose);
+    if (S->h_table) (void)hipHostFree(S->h_table);
+    if (S->h_pose) (void)hipHostFree(S->h_pose);
     delete S;
     return FSG_OK;
 }
