@@ -644,6 +644,7 @@ int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t tot
     const dim3 g(grid_for(max_npix, 256), (unsigned)n);  // y: world, x: its pixels
     fsg::k_claims_init<<<g, 256, 0, c->stream>>>(a);
     for (int d = 0; d < 5; d++) {
+        if (d > 0) fsg::k_claims_round<<<g, 256, 0, c->stream>>>(a, d);
         if (unstamp) {
             fsg::k_unstamp_propose<<<g, 256, 0, c->stream>>>(a, d);
             fsg::k_unstamp_resolve<<<g, 256, 0, c->stream>>>(a, d);

@@ -720,18 +720,31 @@ __device__ __forceinline__ void mark_tile(const StampWorld &W, int x, int y) {
     if (W.flags) atomicOr(&W.flags[(y / TH) * W.tiles_x + (x / TW)], F_WAKE_ALL);
 }
 
+// Round 0: every live pixel's own cell and its claim slot.  Each later round
+// d first clears the slots of its open pixels' round-d targets
+// (k_claims_round): a slot a winner of an earlier round holds may be reset,
+// because the propose kernels also require the cell to be free (stamp) or to
+// hold the pixel's material (unstamp), so the earliest round and then the
+// lowest pixel still win.
 __global__ void k_claims_init(StampArgs a) {
     PixelRef ref;
     if (!pixel_ref(a, ref)) return;
     const StampWorld &W = *ref.w;
     W.state[ref.p] = 0;
     W.tgt[ref.p] = own_cell(a, W, ref.p);
-    for (int d = 0; d < 5; d++) {
-        int wx, wy;
-        if (!pixel_target(ref, d, wx, wy)) continue;
-        W.claims[(size_t)wy * W.W + wx] = ~0ull;
-        mark_tile(W, wx, wy);
-    }
+    int wx, wy;
+    if (!pixel_target(ref, 0, wx, wy)) return;
+    W.claims[(size_t)wy * W.W + wx] = ~0ull;
+    mark_tile(W, wx, wy);
+}
+
+__global__ void k_claims_round(StampArgs a, int d) {
+    PixelRef ref;
+    if (!pixel_ref(a, ref) || ref.w->state[ref.p]) return;
+    int wx, wy;
+    if (!pixel_target(ref, d, wx, wy)) return;
+    ref.w->claims[(size_t)wy * ref.w->W + wx] = ~0ull;
+    mark_tile(*ref.w, wx, wy);
 }
 
 __global__ void k_stamp_propose(StampArgs a, int d) {

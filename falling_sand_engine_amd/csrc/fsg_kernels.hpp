@@ -43,6 +43,7 @@ __global__ void k_fill(uint64_t *cells, size_t n, uint64_t v);
 __global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16);
 __global__ void k_set_cells(uint64_t *cells, int W, int H, const int32_t *xy, const uint64_t *vals, size_t n);
 __global__ void k_claims_init(StampArgs a);
+__global__ void k_claims_round(StampArgs a, int d);
 __global__ void k_stamp_propose(StampArgs a, int d);
 __global__ void k_stamp_resolve(StampArgs a, int d);
 __global__ void k_unstamp_propose(StampArgs a, int d);
