@@ -263,6 +263,8 @@ extern "C" int fsg_ctx_destroy(fsg_ctx *c) {
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     dev_free(c->d_worlds);
     dev_free(c->d_tile_start);
+    dev_free(c->d_open);
+    dev_free(c->d_nopen);
     trim_cache(c, 0);
     (void)hipEventDestroy(c->e0);
     (void)hipEventDestroy(c->e1);
@@ -633,27 +635,50 @@ fsg::StampWorld stamp_world_of(fsg_world *w, uint32_t pix_start) {
 }
 
 // Claim rounds over a batch of worlds whose table is already on the device.
+// Round 0 runs over every pixel (y: world, x: its pixels); the pixels it
+// leaves open are then compacted into one list that rounds 1-4 (and the
+// unstamp's last step) walk, so those rounds cost what is left, not npix.
 int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t total, uint32_t max_npix,
                  const MatTab &M, bool unstamp) {
     if (total == 0 || max_npix == 0) return FSG_OK;
+    if (!c->d_nopen) {
+        void *p;
+        if (dev_alloc(c, &p, sizeof(uint32_t))) return FSG_ERR_NOMEM;
+        c->d_nopen = (uint32_t *)p;
+    }
+    if (c->open_cap < total) {
+        dev_free(c->d_open);
+        c->d_open = nullptr;
+        c->open_cap = 0;
+        void *p;
+        if (dev_alloc(c, &p, (size_t)total * sizeof(uint64_t))) return FSG_ERR_NOMEM;
+        c->d_open = (uint64_t *)p;
+        c->open_cap = total;
+    }
     fsg::StampArgs a;
     a.worlds = d_table;
     a.nworlds = n;
     a.total = total;
     a.mat = M;
+    a.open = c->d_open;
+    a.nopen = c->d_nopen;
+    const int propose = unstamp ? fsg::OP_UNPROPOSE : fsg::OP_PROPOSE;
+    const int resolve = unstamp ? fsg::OP_UNRESOLVE : fsg::OP_RESOLVE;
     const dim3 g(grid_for(max_npix, 256), (unsigned)n);  // y: world, x: its pixels
     fsg::k_claims_init<<<g, 256, 0, c->stream>>>(a);
-    for (int d = 0; d < 5; d++) {
-        if (d > 0) fsg::k_claims_round<<<g, 256, 0, c->stream>>>(a, d);
-        if (unstamp) {
-            fsg::k_unstamp_propose<<<g, 256, 0, c->stream>>>(a, d);
-            fsg::k_unstamp_resolve<<<g, 256, 0, c->stream>>>(a, d);
-        } else {
-            fsg::k_stamp_propose<<<g, 256, 0, c->stream>>>(a, d);
-            fsg::k_stamp_resolve<<<g, 256, 0, c->stream>>>(a, d);
-        }
+    fsg::k_stamp_step<<<g, 256, 0, c->stream>>>(a, propose, 0);
+    fsg::k_stamp_step<<<g, 256, 0, c->stream>>>(a, resolve, 0);
+    fsg::k_zero_u32<<<1, 1, 0, c->stream>>>(a.nopen);
+    const dim3 gc(grid_for(max_npix, 256 * fsg::STAMP_CPT), (unsigned)n);
+    fsg::k_stamp_compact<<<gc, 256, 0, c->stream>>>(a);
+    // the list holds at most `total` entries; the loop kernels stride over it
+    const unsigned gl = std::min(grid_for(total, 256), 1024u);
+    for (int d = 1; d < 5; d++) {
+        fsg::k_stamp_step_list<<<gl, 256, 0, c->stream>>>(a, fsg::OP_CLAIMS, d);
+        fsg::k_stamp_step_list<<<gl, 256, 0, c->stream>>>(a, propose, d);
+        fsg::k_stamp_step_list<<<gl, 256, 0, c->stream>>>(a, resolve, d);
     }
-    if (unstamp) fsg::k_unstamp_finish<<<g, 256, 0, c->stream>>>(a);
+    if (unstamp) fsg::k_stamp_step_list<<<gl, 256, 0, c->stream>>>(a, fsg::OP_UNFINISH, 0);
     HIPCHK(hipGetLastError());
     return FSG_OK;
 }

@@ -43,6 +43,10 @@ struct fsg_ctx {
     // freed device blocks by rounded size (dev_alloc / dev_free)
     std::multimap<size_t, void *> cache;
     size_t cached = 0, cache_cap = (size_t)16 << 30;
+    // stamp claim rounds: pixels still open after round 0 and their count
+    uint64_t *d_open = nullptr;
+    uint32_t *d_nopen = nullptr;
+    size_t open_cap = 0;
 };
 
 struct fsg_world {

@@ -721,8 +721,8 @@ __device__ __forceinline__ void mark_tile(const StampWorld &W, int x, int y) {
 }
 
 // Round 0: every live pixel's own cell and its claim slot.  Each later round
-// d first clears the slots of its open pixels' round-d targets
-// (k_claims_round): a slot a winner of an earlier round holds may be reset,
+// d first clears the slots of its open pixels' round-d targets (OP_CLAIMS):
+// a slot a winner of an earlier round holds may be reset,
 // because the propose kernels also require the cell to be free (stamp) or to
 // hold the pixel's material (unstamp), so the earliest round and then the
 // lowest pixel still win.
@@ -738,76 +738,117 @@ __global__ void k_claims_init(StampArgs a) {
     mark_tile(W, wx, wy);
 }
 
-__global__ void k_claims_round(StampArgs a, int d) {
-    PixelRef ref;
-    if (!pixel_ref(a, ref) || ref.w->state[ref.p]) return;
-    int wx, wy;
-    if (!pixel_target(ref, d, wx, wy)) return;
-    ref.w->claims[(size_t)wy * ref.w->W + wx] = ~0ull;
-    mark_tile(*ref.w, wx, wy);
-}
+// The per-pixel steps of a claim round (d: round).  Each runs in two launch
+// forms: over all pixels (grid: y = world, x = pixel), and for rounds after
+// the first over the compacted list of pixels still open after round 0
+// (most pixels are placed in round 0, so later rounds touch only the few
+// left instead of reading every pixel's state).
 
-__global__ void k_stamp_propose(StampArgs a, int d) {
-    PixelRef ref;
-    if (!pixel_ref(a, ref) || ref.w->state[ref.p]) return;
-    int wx, wy;
-    if (!pixel_target(ref, d, wx, wy)) return;
-    const size_t ci = (size_t)wy * ref.w->W + wx;
-    const uint32_t m = cid(ref.w->cells[ci]);
-    if (!(bit(a.mat.air, m) || bit(a.mat.sand, m) || bit(a.mat.soup, m))) return;
-    atomicMin(reinterpret_cast<unsigned long long *>(&ref.w->claims[ci]), ((unsigned long long)d << 32) | ref.p);
-}
-
-__global__ void k_stamp_resolve(StampArgs a, int d) {
-    PixelRef ref;
-    if (!pixel_ref(a, ref) || ref.w->state[ref.p]) return;
-    int wx, wy;
-    if (!pixel_target(ref, d, wx, wy)) return;
+__device__ __forceinline__ void stamp_step(const StampArgs &a, const PixelRef &ref, int op, int d) {
     const StampWorld &W = *ref.w;
+    if (W.state[ref.p]) return;
+    int wx, wy;
+    if (!pixel_target(ref, op == OP_UNFINISH ? -1 : d, wx, wy)) return;
     const size_t ci = (size_t)wy * W.W + wx;
-    if (W.claims[ci] != (((uint64_t)d << 32) | ref.p)) return;
-    const uint64_t old = W.cells[ci];
-    if (!bit(a.mat.air, cid(old))) {
-        W.disp[ref.p] = old;
-        W.disp_xy[ref.p] = (uint64_t)(uint32_t)wx | ((uint64_t)(uint32_t)wy << 32);
-        W.state[ref.p] = bit(a.mat.sand, cid(old)) ? 2 : 3;  // placed over sand / soup
-    } else {
-        W.state[ref.p] = 1;  // placed
+    const uint64_t mine = ((uint64_t)d << 32) | ref.p;
+    switch (op) {
+    case OP_CLAIMS:
+        W.claims[ci] = ~0ull;
+        mark_tile(W, wx, wy);
+        break;
+    case OP_PROPOSE: {
+        const uint32_t m = cid(W.cells[ci]);
+        if (bit(a.mat.air, m) || bit(a.mat.sand, m) || bit(a.mat.soup, m))
+            atomicMin(reinterpret_cast<unsigned long long *>(&W.claims[ci]), (unsigned long long)mine);
+        break;
     }
-    W.cells[ci] = W.pixels[ref.p];
+    case OP_RESOLVE: {
+        if (W.claims[ci] != mine) break;
+        const uint64_t old = W.cells[ci];
+        if (!bit(a.mat.air, cid(old))) {
+            W.disp[ref.p] = old;
+            W.disp_xy[ref.p] = (uint64_t)(uint32_t)wx | ((uint64_t)(uint32_t)wy << 32);
+            W.state[ref.p] = bit(a.mat.sand, cid(old)) ? 2 : 3;  // placed over sand / soup
+        } else {
+            W.state[ref.p] = 1;  // placed
+        }
+        W.cells[ci] = W.pixels[ref.p];
+        break;
+    }
+    case OP_UNPROPOSE:
+        if (cbyte(W.cells[ci]) == cbyte(W.pixels[ref.p]))
+            atomicMin(reinterpret_cast<unsigned long long *>(&W.claims[ci]), (unsigned long long)mine);
+        break;
+    case OP_UNRESOLVE: {
+        if (W.claims[ci] != mine) break;
+        const uint64_t c = W.cells[ci];
+        W.pixels[ref.p] = c;
+        W.cells[ci] = (uint64_t)a.mat.air_id | (c & 0xFFFF0000ull);
+        W.state[ref.p] = 1;
+        break;
+    }
+    case OP_UNFINISH:  // a pixel that found nothing is lost when the cell under it is air
+        if (bit(a.mat.air, cid(W.cells[ci]))) W.pixels[ref.p] = (uint64_t)a.mat.air_id;
+        break;
+    }
 }
 
-__global__ void k_unstamp_propose(StampArgs a, int d) {
+__global__ void k_stamp_step(StampArgs a, int op, int d) {
     PixelRef ref;
-    if (!pixel_ref(a, ref) || ref.w->state[ref.p]) return;
-    int wx, wy;
-    if (!pixel_target(ref, d, wx, wy)) return;
-    const size_t ci = (size_t)wy * ref.w->W + wx;
-    if (cbyte(ref.w->cells[ci]) != cbyte(ref.w->pixels[ref.p])) return;
-    atomicMin(reinterpret_cast<unsigned long long *>(&ref.w->claims[ci]), ((unsigned long long)d << 32) | ref.p);
+    if (!pixel_ref(a, ref)) return;
+    stamp_step(a, ref, op, d);
 }
 
-__global__ void k_unstamp_resolve(StampArgs a, int d) {
-    PixelRef ref;
-    if (!pixel_ref(a, ref) || ref.w->state[ref.p]) return;
-    int wx, wy;
-    if (!pixel_target(ref, d, wx, wy)) return;
-    const StampWorld &W = *ref.w;
-    const size_t ci = (size_t)wy * W.W + wx;
-    if (W.claims[ci] != (((uint64_t)d << 32) | ref.p)) return;
-    const uint64_t c = W.cells[ci];
-    W.pixels[ref.p] = c;
-    W.cells[ci] = (uint64_t)a.mat.air_id | (c & 0xFFFF0000ull);
-    W.state[ref.p] = 1;
+__global__ void k_stamp_step_list(StampArgs a, int op, int d) {
+    const uint32_t n = *a.nopen;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint64_t e = a.open[i];
+        PixelRef ref;
+        ref.w = &a.worlds[(uint32_t)(e >> 32)];
+        ref.p = (uint32_t)e;
+        stamp_step(a, ref, op, d);
+    }
 }
 
-// A pixel that found nothing is lost when the cell under it is air.
-__global__ void k_unstamp_finish(StampArgs a) {
-    PixelRef ref;
-    if (!pixel_ref(a, ref) || ref.w->state[ref.p]) return;
-    int wx, wy;
-    if (!pixel_target(ref, -1, wx, wy)) return;
-    if (bit(a.mat.air, cid(ref.w->cells[(size_t)wy * ref.w->W + wx]))) ref.w->pixels[ref.p] = (uint64_t)a.mat.air_id;
+// The live pixels still open after round 0, appended to a.open.  Nearly
+// every wave of a rotated body holds a pixel that lost its round-0 claim, so
+// the list is reserved once per workgroup of STAMP_CPT * 256 pixels (one
+// global atomic per workgroup, offsets by a workgroup prefix sum) rather than
+// once per wave, whose same-address atomics serialise.
+__global__ __launch_bounds__(256) void k_stamp_compact(StampArgs a) {
+    __shared__ uint32_t wtot[4], base_s;
+    const StampWorld &W = a.worlds[blockIdx.y];
+    const uint32_t p0 = blockIdx.x * (256u * STAMP_CPT) + threadIdx.x;
+    uint32_t bits = 0;
+#pragma unroll
+    for (int k = 0; k < STAMP_CPT; k++) {
+        const uint32_t p = p0 + 256u * k;
+        if (p < W.npix && !W.state[p] && (uint32_t)W.tgt[p] != NOT_LIVE) bits |= 1u << k;
+    }
+    const uint32_t mine = __popc(bits);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t incl = mine;  // inclusive prefix over the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) wtot[wv] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+        base_s = t ? atomicAdd(a.nopen, t) : 0u;
+    }
+    __syncthreads();
+    if (!bits) return;
+    uint32_t k = base_s + incl - mine;
+    for (int w = 0; w < wv; w++) k += wtot[w];
+    const uint64_t hi = (uint64_t)blockIdx.y << 32;
+#pragma unroll
+    for (int b = 0; b < STAMP_CPT; b++)
+        if (bits >> b & 1u) a.open[k++] = hi | (p0 + 256u * b);
 }
+
+__global__ void k_zero_u32(uint32_t *p) { *p = 0; }
 
 } // namespace fsg

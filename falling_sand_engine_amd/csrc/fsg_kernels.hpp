@@ -28,6 +28,8 @@ struct StampArgs {
     int nworlds;
     uint32_t total;            // pixels over all worlds
     MatTab mat;                // shared material table
+    uint64_t *open;            // pixels open after round 0: world << 32 | pixel
+    uint32_t *nopen;           // their count
 };
 
 __global__ void k_tick_world(DevWorld w, uint32_t k);
@@ -43,11 +45,12 @@ __global__ void k_fill(uint64_t *cells, size_t n, uint64_t v);
 __global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16);
 __global__ void k_set_cells(uint64_t *cells, int W, int H, const int32_t *xy, const uint64_t *vals, size_t n);
 __global__ void k_claims_init(StampArgs a);
-__global__ void k_claims_round(StampArgs a, int d);
-__global__ void k_stamp_propose(StampArgs a, int d);
-__global__ void k_stamp_resolve(StampArgs a, int d);
-__global__ void k_unstamp_propose(StampArgs a, int d);
-__global__ void k_unstamp_resolve(StampArgs a, int d);
-__global__ void k_unstamp_finish(StampArgs a);
+// Per-pixel steps of a stamp claim round (k_stamp_step, k_stamp_step_list).
+enum StampOp { OP_CLAIMS = 0, OP_PROPOSE, OP_RESOLVE, OP_UNPROPOSE, OP_UNRESOLVE, OP_UNFINISH };
+__global__ void k_stamp_step(StampArgs a, int op, int d);
+__global__ void k_stamp_step_list(StampArgs a, int op, int d);
+constexpr int STAMP_CPT = 8;  // pixels per thread of k_stamp_compact (grid x: npix / (256 * STAMP_CPT))
+__global__ void k_stamp_compact(StampArgs a);
+__global__ void k_zero_u32(uint32_t *p);
 
 } // namespace fsg
