@@ -291,6 +291,58 @@ class World:
                     "had_dirty": bool(st.had_dirty)}
         return None
 
+    def pack(self, x0, y0, ncx, ncy, phase=None, layer2=None, background=None, download=True):
+        """Chunk .pack images (Chunk::ChunkWrite, chunk.cpp:219-330) of the
+        ncx x ncy chunks whose top-left cell is (x0, y0).  layer2 (uint64) and
+        background (uint32) are region-shaped arrays or None; phase is one int8
+        per chunk or None.  Returns a list of bytes objects (download=True) or
+        the image sizes (the images stay on the device for unpack(None))."""
+        self._flush()
+        n = ncx * ncy
+        sizes = np.zeros(n, dtype=np.uintp)
+        ph = None if phase is None else np.ascontiguousarray(phase, dtype=np.int8)
+        l2 = None if layer2 is None else np.ascontiguousarray(layer2, dtype=np.uint64)
+        bg = None if background is None else np.ascontiguousarray(background, dtype=np.uint32)
+        shape = (ncy * _abi.CHUNK, ncx * _abi.CHUNK)
+        assert ph is None or ph.size == n
+        assert l2 is None or l2.shape == shape
+        assert bg is None or bg.shape == shape
+        check(lib().fsg_world_pack(self.handle, x0, y0, ncx, ncy, None if ph is None else ph.ctypes.data,
+                                   None if l2 is None else l2.ctypes.data, None if bg is None else bg.ctypes.data,
+                                   sizes.ctypes.data), "fsg_world_pack")
+        if not download:
+            return [int(v) for v in sizes]
+        buf = np.empty(max(int(sizes.sum()), 1), dtype=np.uint8)
+        check(lib().fsg_world_pack_download(self.handle, buf.ctypes.data, buf.size), "fsg_world_pack_download")
+        out, o = [], 0
+        for v in sizes:
+            out.append(buf[o:o + int(v)].tobytes())
+            o += int(v)
+        return out
+
+    def unpack(self, x0, y0, ncx, ncy, images=None, want_layer2=False, want_background=False):
+        """Chunk::ChunkRead (chunk.cpp:74-217) of ncx x ncy images into the grid
+        at (x0, y0); images None reads the device-resident images of the last
+        pack().  Returns (status per chunk, phases, layer2 or None,
+        background or None)."""
+        self._flush()
+        n = ncx * ncy
+        status = np.zeros(n, dtype=np.int32)
+        phase = np.zeros(n, dtype=np.int8)
+        shape = (ncy * _abi.CHUNK, ncx * _abi.CHUNK)
+        l2 = np.zeros(shape, dtype=np.uint64) if want_layer2 else None
+        bg = np.zeros(shape, dtype=np.uint32) if want_background else None
+        src = sizes = None
+        if images is not None:
+            assert len(images) == n
+            sizes = np.array([len(b) for b in images], dtype=np.uintp)
+            src = np.frombuffer(b"".join(images) or b"\0", dtype=np.uint8)
+        check(lib().fsg_world_unpack(self.handle, x0, y0, ncx, ncy, None if src is None else src.ctypes.data,
+                                     None if sizes is None else sizes.ctypes.data, phase.ctypes.data,
+                                     None if l2 is None else l2.ctypes.data, None if bg is None else bg.ctypes.data,
+                                     status.ctypes.data), "fsg_world_unpack")
+        return status, phase, l2, bg
+
     def layer(self, layer, x0=0, y0=0, w=None, h=None) -> np.ndarray:
         """Host copy of a layer rectangle as (h, w) uint32 texels (bytes R,G,B,A)."""
         w = self.width if w is None else w

@@ -143,6 +143,9 @@ SIGNATURES = {
     "fsg_world_upload": (C.c_int, [P, P, C.c_size_t]),
     "fsg_world_download": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "fsg_world_generate": (C.c_int, [P, C.c_int, C.c_uint64]),
+    "fsg_world_pack": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, P, P]),
+    "fsg_world_pack_download": (C.c_int, [P, P, C.c_size_t]),
+    "fsg_world_unpack": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, P, P, P, P]),
     "fsg_world_set_cells": (C.c_int, [P, P, C.c_size_t]),
     "fsg_world_step": (C.c_int, [P, C.c_int]),
     "fsg_world_step_timed": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
@@ -211,3 +214,6 @@ def check(status, what=""):
     if status != FSG_OK:
         raise FsgError(status, what)
     return status
+
+
+CHUNK = 128  # FSG_CHUNK: chunk side of the .pack images (const.h:19-20)

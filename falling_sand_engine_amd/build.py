@@ -14,8 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libfsg.so")
 
-SOURCES = ["fsg_kernels.hip", "fsg_api.hip", "fsg_serve.hip", "fsg_render.hip", "fsg_island.hip", "fsg_generate.hip", "fsg_host.cpp"]
-HEADERS = ["fsg_device.hpp", "fsg_kernels.hpp", "fsg_internal.hpp", "fsg_generate.hpp"]
+SOURCES = ["fsg_kernels.hip", "fsg_api.hip", "fsg_serve.hip", "fsg_render.hip", "fsg_island.hip", "fsg_generate.hip", "fsg_pack.hip", "fsg_host.cpp"]
+HEADERS = ["fsg_device.hpp", "fsg_kernels.hpp", "fsg_internal.hpp", "fsg_generate.hpp", "fsg_lz4.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FSG_ARCH", "gfx950")
@@ -48,6 +48,14 @@ def build_cpp_tests(verbose: bool = True) -> str:
     out = os.path.join(outdir, "world_facade_test")
     cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-L", LIBDIR, "-lfsg",
            "-Wl,-rpath,$ORIGIN/../../../falling_sand_engine_amd/lib", "-o", out]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    # the LZ4 block decoder of the device codec, built for the host
+    src = os.path.join(ROOT, "tests", "cpp", "lz4_host.cpp")
+    lz4 = os.path.join(outdir, "liblz4host.so")
+    cmd = [HIPCC, "--offload-host-only", "-x", "hip", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", CSRC, src,
+           "-o", lz4]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -441,6 +441,9 @@ extern "C" int fsg_world_destroy(fsg_world *w) {
     dev_free(w->d_isl_pix);
     dev_free(w->d_gen);
     dev_free(w->d_sw);
+    for (void *p : {w->d_pack_slots, w->d_pack_img, w->d_pack_small, w->d_pack_aux, w->d_pack_in, w->d_pack_raw,
+                    w->d_pack_dec})
+        dev_free(p);
     delete w;
     return FSG_OK;
 }

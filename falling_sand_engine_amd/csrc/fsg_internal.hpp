@@ -64,6 +64,15 @@ struct fsg_world {
     void *d_gen = nullptr;                                 // device generator scratch
     size_t gen_cap = 0;
     std::vector<fsg_gen_disc> gen_discs;                   // host copy of the last disc upload
+    // chunk .pack codec (fsg_pack.hip): encoder slots, assembled images, sizes/offsets,
+    // layer2/background upload, decoder input, record staging, descriptors/statuses
+    void *d_pack_slots = nullptr, *d_pack_img = nullptr, *d_pack_small = nullptr, *d_pack_aux = nullptr;
+    void *d_pack_in = nullptr, *d_pack_raw = nullptr, *d_pack_dec = nullptr;
+    size_t pack_slots_cap = 0, pack_img_cap = 0, pack_small_cap = 0, pack_aux_cap = 0;
+    size_t pack_in_cap = 0, pack_raw_cap = 0, pack_dec_cap = 0;
+    size_t pack_n = 0, pack_total = 0;     // chunks and bytes of the last fsg_world_pack
+    std::vector<int32_t> pack_sizes;       // its block sizes (A, B per chunk)
+    std::vector<int8_t> pack_phase;        // its generation phases
     size_t bytes = 0;
     uint64_t offered = 0;
     unsigned long long *d_stats = nullptr;  // [17] census scratch + [1] processed counter

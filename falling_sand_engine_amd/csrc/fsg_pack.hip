@@ -1,0 +1,608 @@
+// fsg_pack.hip -- chunk .pack images made and read on the device.
+//
+// A .pack image (chunk.cpp:219-330 ChunkWrite, read back by chunk.cpp:74-217
+// ChunkRead) holds one 128x128 chunk:
+//   i8 generation phase, i32 src_size (393216), i32 compressed_size,
+//   i32 src_size2 (65536), i32 compressed_size2,
+//   LZ4 block of the 12-byte MaterialInstanceData records {u32 material
+//   index, u32 colour, i16 temperature, 2 pad bytes} of tiles[] then layer2[],
+//   LZ4 block of the u32 background.
+// The cell's material field (with its moved bit), colour and temperature map
+// one to one onto a record, so an image round-trips a chunk exactly.
+//
+// Encoding: one workgroup per chunk and block.  The reference asks LZ4 for a
+// hash-chain search (LZ4_compress_fast, acceleration 10); any parse of the
+// block format decodes the same, so the encoder uses one that parallelises:
+// every byte is compared with the byte one record back and one record row
+// (128 records) back, runs of equal bytes of 4 or more become matches with
+// that offset, and the rest are literals.  A thread parses one record row;
+// matches do not cross rows, literal runs do (a prefix scan over the rows
+// gives each row the literals carried into it).  Runs of identical cells,
+// identical rows, and the fields that repeat between neighbouring records
+// (material, temperature, alpha, padding) all become matches.  The last 12
+// bytes of a block are literals, as the format requires.
+//
+// Decoding: one wave per block (fsg_lz4.hpp), into a record staging area,
+// then one workgroup per chunk checks every record and writes the cells.
+#include "fsg_internal.hpp"
+#include "fsg_lz4.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace {
+
+constexpr int CW = FSG_CHUNK;                   // chunk side (const.h:19-20)
+constexpr int RA = 12, RB = 4;                  // record sizes of the two blocks
+constexpr int NA = CW * CW * 2 * RA;            // tiles + layer2 records: 393216
+constexpr int NB = CW * CW * RB;                // background: 65536
+constexpr int bound(int n) { return n + n / 255 + 16; }  // LZ4_compressBound (lz4.h)
+constexpr int CAPA = bound(NA), CAPB = bound(NB);
+constexpr size_t SLOT = ((size_t)CAPA + CAPB + 255) & ~(size_t)255;  // encoder output per chunk
+constexpr size_t RAW = (size_t)NA + NB;          // decoder output per chunk
+constexpr int HDR = 17;
+constexpr int KX = 0, KL = 1, KU = 2;            // byte kinds: literal, as one record back, as one row back
+constexpr int PT = 256;                          // encoder threads (one per record row of block A)
+
+struct PackSrc {
+    const uint64_t *cells;  // world grid
+    int W;
+    const uint64_t *l2;     // region layer2 cells, row width rw (null: empty)
+    const uint32_t *bg;     // region background (null: 0)
+    int rw, x0, y0, ncx;
+};
+
+// Record i of block A (tiles rows 0..127, then layer2) or B of chunk (cx, cy).
+__device__ __forceinline__ uint64_t rec_a(const PackSrc &s, int cx, int cy, int i) {
+    const int row = (i >> 7) & (CW - 1), col = i & (CW - 1);
+    if (i < CW * CW) return s.cells[(size_t)(s.y0 + cy * CW + row) * s.W + (s.x0 + cx * CW + col)];
+    return s.l2 ? s.l2[(size_t)(cy * CW + row) * s.rw + cx * CW + col] : 0ull;
+}
+__device__ __forceinline__ uint64_t rec_b(const PackSrc &s, int cx, int cy, int i) {
+    const int row = i >> 7, col = i & (CW - 1);
+    return s.bg ? s.bg[(size_t)(cy * CW + row) * s.rw + cx * CW + col] : 0u;
+}
+
+// Byte b of the record of cell value v: material index (u32), colour (u32),
+// temperature (i16), padding; for block B the u32 itself.
+template <int R>
+__device__ __forceinline__ uint32_t rbyte(uint64_t v, int b) {
+    if (R == RB) return (uint32_t)(v >> (8 * b)) & 0xFFu;
+    switch (b) {
+    case 0: return (uint32_t)v & 0xFFu;
+    case 1: return (uint32_t)(v >> 8) & 0xFFu;
+    case 4: case 5: case 6: case 7: return (uint32_t)(v >> (32 + 8 * (b - 4))) & 0xFFu;
+    case 8: return (uint32_t)(v >> 16) & 0xFFu;
+    case 9: return (uint32_t)(v >> 24) & 0xFFu;
+    default: return 0u;
+    }
+}
+
+// Bit b set where byte b of two records is equal (x = their xor).
+template <int R>
+__device__ __forceinline__ uint32_t eqmask(uint64_t x) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int b = 0; b < R; b++) m |= (rbyte<R>(x, b) == 0u ? 1u : 0u) << b;
+    return m;
+}
+
+__device__ __forceinline__ int extlen(int v) { return v >= 15 ? (v - 15) / 255 + 1 : 0; }
+
+// Parses record row t (bytes [t*128*R, (t+1)*128*R) of an n-byte block) into
+// runs and hands each match to the sink; returns the literal bytes pending
+// at the row's end.
+template <int R, class Rec, class Sink>
+__device__ int walk_row(int t, int n, const Rec &rec, Sink &sink) {
+    constexpr uint32_t FULL = (1u << R) - 1;
+    const int i0 = t * CW;
+    const int forced = n - fsg_lz4::MFLIMIT;  // bytes from here on are literals
+    uint64_t left = i0 > 0 ? rec(i0 - 1) : 0ull;
+    int kind = -1, len = 0, pend = 0;
+    auto close = [&](int end) {
+        if (kind != KX && len >= fsg_lz4::MINMATCH) {
+            sink.match(pend, end - len, len, kind == KL ? R : CW * R);
+            pend = 0;
+        } else {
+            pend += len;
+        }
+    };
+    for (int j = 0; j < CW; j++) {
+        const int i = i0 + j;
+        const uint64_t v = rec(i);
+        const uint32_t mL = i > 0 ? eqmask<R>(v ^ left) : 0u;
+        const uint32_t mU = i >= CW ? eqmask<R>(v ^ rec(i - CW)) : 0u;
+        left = v;
+        const int p0 = i * R;
+        uint32_t live = FULL;
+        if (p0 + R > forced) live = p0 >= forced ? 0u : ((1u << (forced - p0)) - 1u);
+        const uint32_t isL = mL & live, isU = mU & ~mL & live;
+        const uint32_t same = kind == KL ? isL : kind == KU ? isU : kind == KX ? (~(isL | isU) & FULL) : 0u;
+        if (same == FULL) {
+            len += R;
+            continue;
+        }
+        for (int b = 0; b < R; b++) {
+            const int k = (isL >> b & 1u) ? KL : (isU >> b & 1u) ? KU : KX;
+            if (k == kind) {
+                len++;
+                continue;
+            }
+            if (kind >= 0) close(p0 + b);
+            kind = k;
+            len = 1;
+        }
+    }
+    close((i0 + CW) * R);
+    return pend;
+}
+// Pass 1: the sizes of the sequences whose match lies in the row.
+struct Sizer {
+    bool hm = false;
+    int lead = 0, rest = 0;
+    __device__ void match(int pend, int, int ml, int) {
+        if (!hm) {
+            hm = true;
+            lead = pend;
+            rest += 3 + extlen(ml - 4);
+        } else {
+            rest += 1 + extlen(pend) + pend + 2 + extlen(ml - 4);
+        }
+    }
+};
+
+// Output of one block: every store is checked against the block's capacity
+// (the size pass keeps the writes inside it; the check keeps a slot's
+// neighbours safe whatever happens).
+struct Out {
+    uint8_t *p;
+    int cap;
+    __device__ void put(int at, uint32_t v) const {
+        if (at >= 0 && at < cap) p[at] = (uint8_t)v;
+    }
+};
+
+__device__ __forceinline__ int put_len(const Out &o, int oc, int v) {
+    if (v >= 15) {
+        v -= 15;
+        while (v >= 255) {
+            o.put(oc++, 255);
+            v -= 255;
+        }
+        o.put(oc++, (uint32_t)v);
+    }
+    return oc;
+}
+
+// Copies raw block bytes [p, p + cnt) to output position at.
+template <int R, class Rec>
+__device__ void put_raw(const Out &o, int at, int p, int cnt, const Rec &rec) {
+    if (cnt <= 0) return;
+    int i = p / R, b = p - i * R;
+    uint64_t v = rec(i);
+    for (int k = 0; k < cnt; k++) {
+        o.put(at + k, rbyte<R>(v, b));
+        if (++b == R && k + 1 < cnt) {
+            b = 0;
+            v = rec(++i);
+        }
+    }
+}
+
+// Pass 2: writes the row's sequences.
+template <int R, class Rec>
+struct Writer {
+    Out o;
+    int oc, carry;
+    bool first = true;
+    const Rec *rec;
+    __device__ void match(int pend, int mstart, int ml, int off) {
+        const int ll = first ? carry + pend : pend;
+        o.put(oc++, (uint32_t)((min(ll, 15) << 4) | min(ml - 4, 15)));
+        oc = put_len(o, oc, ll);
+        put_raw<R>(o, oc + (ll - pend), mstart - pend, pend, *rec);  // carried bytes come from earlier rows
+        oc += ll;
+        o.put(oc++, (uint32_t)off & 0xFFu);
+        o.put(oc++, (uint32_t)off >> 8);
+        oc = put_len(o, oc, ml - 4);
+        first = false;
+    }
+};
+
+template <int R, class Rec>
+__device__ void encode_block(int n, const Rec &rec, uint8_t *out, int cap, int32_t *size_out) {
+    constexpr int S = CW * R;
+    const int rows = n / S, L = rows - 1, t = threadIdx.x;
+    __shared__ int hm[PT], lead[PT], trail[PT], size[PT], carry[PT], start[PT], nxt[PT];
+    __shared__ int total;
+    Sizer sz;
+    int pend = 0;
+    if (t < rows) {
+        pend = walk_row<R>(t, n, rec, sz);
+        hm[t] = sz.hm;
+        lead[t] = sz.hm ? sz.lead : S;
+        trail[t] = pend;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int c = 0;
+        for (int u = 0; u < rows; u++) {
+            carry[u] = c;
+            c = trail[u] + (hm[u] ? 0 : c);
+        }
+        int nx = L;
+        for (int u = L; u >= 0; u--) {
+            nxt[u] = nx;
+            if (hm[u]) nx = u;
+        }
+    }
+    __syncthreads();
+    if (t < rows) {
+        int s = 0;
+        if (hm[t]) s = sz.rest + extlen(carry[t] + lead[t]) + carry[t] + lead[t];
+        if (t == L) {
+            const int llf = hm[t] ? pend : carry[t] + S;
+            s += 1 + extlen(llf) + llf;
+        }
+        size[t] = s;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int o = 0;
+        for (int u = 0; u < rows; u++) {
+            start[u] = o;
+            o += size[u];
+        }
+        total = o;
+    }
+    __syncthreads();
+    if (total > cap) {  // cannot happen for these parses (sizes stay under the LZ4 bound)
+        if (t == 0) *size_out = -1;
+        return;
+    }
+    if (t < rows) {
+        const Out o{out, cap};
+        Writer<R, Rec> w{o, start[t], carry[t], true, &rec};
+        walk_row<R>(t, n, rec, w);
+        const int row0 = t * S, tstart = row0 + S - pend;  // first trailing literal byte
+        if (t == L) {
+            // the last sequence: literals only
+            const int llf = hm[t] ? pend : carry[t] + S;
+            int oc = w.oc;
+            o.put(oc++, (uint32_t)(min(llf, 15) << 4));
+            oc = put_len(o, oc, llf);
+            put_raw<R>(o, oc + (llf - pend), tstart, pend, rec);
+        } else {
+            // trailing literals belong to the first sequence of the next row with a match
+            // (or to the last sequence)
+            const int u = nxt[t];
+            const int ll0 = carry[u] + lead[u];
+            const int lit_out = start[u] + 1 + extlen(ll0);
+            const int lit_raw = u * S - carry[u];
+            put_raw<R>(o, lit_out + (tstart - lit_raw), tstart, pend, rec);
+        }
+    }
+    if (t == 0) *size_out = total;
+}
+
+struct RecA {
+    PackSrc s;
+    int cx, cy;
+    __device__ uint64_t operator()(int i) const { return rec_a(s, cx, cy, i); }
+};
+struct RecB {
+    PackSrc s;
+    int cx, cy;
+    __device__ uint64_t operator()(int i) const { return rec_b(s, cx, cy, i); }
+};
+
+// grid: x = chunk, y = block (0: tiles + layer2, 1: background)
+__global__ __launch_bounds__(PT) void k_pack_encode(PackSrc s, uint8_t *slots, int32_t *sizes) {
+    const int c = blockIdx.x, cx = c % s.ncx, cy = c / s.ncx;
+    uint8_t *slot = slots + (size_t)c * SLOT;
+    if (blockIdx.y == 0) encode_block<RA>(NA, RecA{s, cx, cy}, slot, CAPA, &sizes[2 * c]);
+    else encode_block<RB>(NB, RecB{s, cx, cy}, slot + CAPA, CAPB, &sizes[2 * c + 1]);
+}
+
+// Image offsets (exclusive scan of 17 + A + B over the chunks); off[n] = total,
+// or -1 when a block failed.  One workgroup.
+__global__ __launch_bounds__(1024) void k_pack_offsets(const int32_t *sizes, int n, int64_t *off) {
+    __shared__ int64_t part[1024];
+    __shared__ int bad;
+    const int t = threadIdx.x, per = (n + 1023) / 1024, a = min(n, t * per), b = min(n, a + per);
+    if (t == 0) bad = 0;
+    __syncthreads();
+    int64_t s = 0;
+    for (int c = a; c < b; c++) {
+        if (sizes[2 * c] < 0 || sizes[2 * c + 1] < 0) bad = 1;
+        s += HDR + sizes[2 * c] + sizes[2 * c + 1];
+    }
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        int64_t o = 0;
+        for (int k = 0; k < 1024; k++) {
+            const int64_t v = part[k];
+            part[k] = o;
+            o += v;
+        }
+        off[n] = bad ? -1 : o;
+    }
+    __syncthreads();
+    int64_t o = part[t];
+    for (int c = a; c < b; c++) {
+        off[c] = o;
+        o += HDR + sizes[2 * c] + sizes[2 * c + 1];
+    }
+}
+
+__device__ __forceinline__ void put_i32(uint8_t *p, int32_t v) {
+    for (int k = 0; k < 4; k++) p[k] = (uint8_t)((uint32_t)v >> (8 * k));
+}
+
+// Writes each chunk's image: header, then block A, then block B.
+__global__ __launch_bounds__(256) void k_pack_assemble(const uint8_t *slots, const int32_t *sizes,
+                                                       const int64_t *off, const int8_t *phase, uint8_t *img) {
+    const int c = blockIdx.x, t = threadIdx.x;
+    if (off[gridDim.x] < 0) return;
+    const int a = sizes[2 * c], b = sizes[2 * c + 1];
+    uint8_t *o = img + off[c];
+    const uint8_t *sa = slots + (size_t)c * SLOT, *sb = sa + CAPA;
+    if (t == 0) {
+        o[0] = (uint8_t)phase[c];
+        put_i32(o + 1, NA);
+        put_i32(o + 5, a);
+        put_i32(o + 9, NB);
+        put_i32(o + 13, b);
+    }
+    for (int k = t; k < a; k += 256) o[HDR + k] = sa[k];
+    for (int k = t; k < b; k += 256) o[HDR + a + k] = sb[k];
+}
+
+// Decoder input: per block, its bytes in the image buffer; ok = 0 skips it.
+struct DecDesc {
+    int64_t off;
+    int32_t n, ok;
+};
+
+// One wave per block (x = chunk * 2 + block); status 0 when the block decoded
+// to its full size.
+__global__ __launch_bounds__(64) void k_pack_decode(const uint8_t *in, const DecDesc *desc, uint8_t *raw,
+                                                    int32_t *status) {
+    const int b = blockIdx.x, c = b >> 1, which = b & 1;
+    const DecDesc d = desc[b];
+    const int cap = which ? NB : NA;
+    int r = -1;
+    if (d.ok) r = fsg_lz4::decode_block(in + d.off, d.n, raw + (size_t)c * RAW + (which ? NA : 0), cap,
+                                        fsg_lz4::WaveLane{(int)threadIdx.x});
+    if (threadIdx.x == 0) status[b] = r == cap ? 0 : -1;
+}
+
+__device__ __forceinline__ uint32_t ld32(const uint8_t *p) {
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+
+struct UnpackDst {
+    uint64_t *cells;  // world grid
+    int W, x0, y0, ncx, rw;
+    uint64_t *l2;     // region layer2 out (null: not kept)
+    uint32_t *bg;     // region background out (null: not kept)
+    uint32_t n_materials;
+};
+
+// One workgroup per chunk: records -> cells, after checking every material
+// index (a bad one fails the chunk and leaves its cells as they were).
+__global__ __launch_bounds__(256) void k_pack_cells(const uint8_t *raw, const int32_t *status, UnpackDst d,
+                                                    int32_t *chunk_status) {
+    __shared__ int bad;
+    const int c = blockIdx.x, t = threadIdx.x, cx = c % d.ncx, cy = c / d.ncx;
+    if (status[2 * c] != 0 || status[2 * c + 1] != 0) {
+        if (t == 0) chunk_status[c] = FSG_ERR_INVALID;
+        return;
+    }
+    if (t == 0) bad = 0;
+    __syncthreads();
+    const uint8_t *r = raw + (size_t)c * RAW;
+    int mybad = 0;
+    for (int i = t; i < 2 * CW * CW; i += 256) {
+        const uint32_t idx = ld32(r + (size_t)i * RA);
+        if (i < CW * CW) mybad |= (idx & 0xFFFFFE00u) != 0 || (idx & 0xFFu) >= d.n_materials;
+        else mybad |= idx > 0xFFFFu;
+    }
+    if (mybad) bad = 1;
+    __syncthreads();
+    if (bad) {
+        if (t == 0) chunk_status[c] = FSG_ERR_INVALID;
+        return;
+    }
+    for (int i = t; i < 2 * CW * CW; i += 256) {
+        const uint8_t *q = r + (size_t)i * RA;
+        const uint64_t v = (uint64_t)(ld32(q) & 0xFFFFu) | ((uint64_t)(ld32(q + 8) & 0xFFFFu) << 16) |
+                           ((uint64_t)ld32(q + 4) << 32);
+        const int row = (i >> 7) & (CW - 1), col = i & (CW - 1);
+        if (i < CW * CW) d.cells[(size_t)(d.y0 + cy * CW + row) * d.W + (d.x0 + cx * CW + col)] = v;
+        else if (d.l2) d.l2[(size_t)(cy * CW + row) * d.rw + cx * CW + col] = v;
+    }
+    if (d.bg)
+        for (int i = t; i < CW * CW; i += 256)
+            d.bg[(size_t)(cy * CW + (i >> 7)) * d.rw + cx * CW + (i & (CW - 1))] = ld32(r + NA + (size_t)i * RB);
+    if (t == 0) chunk_status[c] = FSG_OK;
+}
+
+using fsg_impl::dev_alloc;
+using fsg_impl::dev_free;
+
+// Grows a world-owned device buffer to `bytes` (contents not kept).
+int grow(fsg_ctx *c, void **p, size_t *cap, size_t bytes) {
+    if (*cap >= bytes) return FSG_OK;
+    dev_free(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (dev_alloc(c, p, bytes)) return FSG_ERR_NOMEM;
+    *cap = bytes;
+    return FSG_OK;
+}
+
+int region_ok(const fsg_world *w, int x0, int y0, int ncx, int ncy) {
+    if (x0 < 0 || y0 < 0 || ncx <= 0 || ncy <= 0 || ncx > w->d.W / CW || ncy > w->d.H / CW) return 0;
+    return (int64_t)x0 + (int64_t)ncx * CW <= w->d.W && (int64_t)y0 + (int64_t)ncy * CW <= w->d.H;
+}
+
+int32_t rd_i32(const uint8_t *p) {
+    int32_t v;
+    std::memcpy(&v, p, 4);
+    return v;
+}
+
+}  // namespace
+
+extern "C" int fsg_world_pack(fsg_world *w, int32_t x0, int32_t y0, int32_t ncx, int32_t ncy, const int8_t *phase,
+                              const fsg_cell *layer2, const uint32_t *background, size_t *image_bytes) {
+    if (!w || !image_bytes || !region_ok(w, x0, y0, ncx, ncy)) return FSG_ERR_INVALID;
+    fsg_ctx *c = w->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n = (size_t)ncx * ncy, rcells = n * CW * CW;
+    // per-chunk device state: encoder slots, images (worst case), sizes, offsets, phases;
+    // the region's layer2 / background when given
+    const size_t b_small = ((n * 8 + (n + 1) * 8 + n + 255) & ~(size_t)255);
+    if (grow(c, &w->d_pack_slots, &w->pack_slots_cap, n * SLOT) ||
+        grow(c, &w->d_pack_img, &w->pack_img_cap, n * (HDR + CAPA + CAPB)) ||
+        grow(c, &w->d_pack_small, &w->pack_small_cap, b_small))
+        return FSG_ERR_NOMEM;
+    const size_t b_l2 = layer2 ? rcells * 8 : 0, b_bg = background ? rcells * 4 : 0;
+    if (b_l2 + b_bg && grow(c, &w->d_pack_aux, &w->pack_aux_cap, b_l2 + b_bg)) return FSG_ERR_NOMEM;
+    int32_t *d_sizes = static_cast<int32_t *>(w->d_pack_small);
+    int64_t *d_off = reinterpret_cast<int64_t *>(d_sizes + 2 * n);
+    int8_t *d_phase = reinterpret_cast<int8_t *>(d_off + n + 1);
+    w->pack_phase.assign(n, 0);
+    if (phase) std::copy(phase, phase + n, w->pack_phase.begin());
+    HIPCHK(hipMemcpyAsync(d_phase, w->pack_phase.data(), n, hipMemcpyHostToDevice, c->stream));
+    PackSrc s{};
+    s.cells = w->d.buf[w->d.tick & 1];
+    s.W = w->d.W;
+    s.rw = ncx * CW;
+    s.x0 = x0;
+    s.y0 = y0;
+    s.ncx = ncx;
+    char *aux = static_cast<char *>(w->d_pack_aux);
+    if (layer2) {
+        HIPCHK(hipMemcpyAsync(aux, layer2, b_l2, hipMemcpyHostToDevice, c->stream));
+        s.l2 = reinterpret_cast<const uint64_t *>(aux);
+    }
+    if (background) {
+        HIPCHK(hipMemcpyAsync(aux + b_l2, background, b_bg, hipMemcpyHostToDevice, c->stream));
+        s.bg = reinterpret_cast<const uint32_t *>(aux + b_l2);
+    }
+    uint8_t *slots = static_cast<uint8_t *>(w->d_pack_slots), *img = static_cast<uint8_t *>(w->d_pack_img);
+    k_pack_encode<<<dim3((unsigned)n, 2), PT, 0, c->stream>>>(s, slots, d_sizes);
+    k_pack_offsets<<<1, 1024, 0, c->stream>>>(d_sizes, (int)n, d_off);
+    k_pack_assemble<<<(unsigned)n, 256, 0, c->stream>>>(slots, d_sizes, d_off, d_phase, img);
+    HIPCHK(hipGetLastError());
+    w->pack_sizes.resize(2 * n);
+    int64_t total = 0;
+    HIPCHK(hipMemcpyAsync(w->pack_sizes.data(), d_sizes, 2 * n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&total, d_off + n, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (total < 0) {
+        w->pack_n = 0;
+        return FSG_ERR_DEVICE;
+    }
+    for (size_t k = 0; k < n; k++) image_bytes[k] = HDR + (size_t)w->pack_sizes[2 * k] + w->pack_sizes[2 * k + 1];
+    w->pack_n = n;
+    w->pack_total = (size_t)total;
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_pack_download(fsg_world *w, void *out, size_t cap) {
+    if (!w || !out) return FSG_ERR_INVALID;
+    if (w->pack_n == 0) return FSG_ERR_STATE;
+    if (cap < w->pack_total) return FSG_ERR_INVALID;
+    fsg_ctx *c = w->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(out, w->d_pack_img, w->pack_total, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_unpack(fsg_world *w, int32_t x0, int32_t y0, int32_t ncx, int32_t ncy, const void *images,
+                                const size_t *image_bytes, int8_t *phase, fsg_cell *layer2, uint32_t *background,
+                                int32_t *status) {
+    if (!w || !status || !region_ok(w, x0, y0, ncx, ncy)) return FSG_ERR_INVALID;
+    const size_t n = (size_t)ncx * ncy, rcells = n * CW * CW;
+    if (images && !image_bytes) return FSG_ERR_INVALID;
+    if (!images && w->pack_n != n) return FSG_ERR_STATE;  // the last pack's images, same chunk count
+    fsg_ctx *c = w->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    // descriptors of the 2n blocks from the headers (chunk.cpp:83-145)
+    std::vector<DecDesc> desc(2 * n);
+    std::vector<int8_t> ph(n, 0);
+    const uint8_t *src = nullptr;
+    size_t in_total = 0;
+    if (images) {
+        const uint8_t *p = static_cast<const uint8_t *>(images);
+        for (size_t k = 0; k < n; k++) {
+            const size_t len = image_bytes[k];
+            DecDesc a{(int64_t)in_total + HDR, 0, 0}, b{0, 0, 0};
+            if (len >= (size_t)HDR) {
+                ph[k] = (int8_t)p[in_total];
+                const int32_t sa = rd_i32(p + in_total + 1), ca = rd_i32(p + in_total + 5);
+                const int32_t sb = rd_i32(p + in_total + 9), cb = rd_i32(p + in_total + 13);
+                if (sa == NA && sb == NB && ca >= 0 && cb >= 0 && (size_t)HDR + (size_t)ca + (size_t)cb <= len) {
+                    a.n = ca;
+                    a.ok = 1;
+                    b = DecDesc{a.off + ca, cb, 1};
+                }
+            }
+            desc[2 * k] = a;
+            desc[2 * k + 1] = b;
+            in_total += len;
+        }
+        if (grow(c, &w->d_pack_in, &w->pack_in_cap, std::max<size_t>(in_total, 1))) return FSG_ERR_NOMEM;
+        HIPCHK(hipMemcpyAsync(w->d_pack_in, images, in_total, hipMemcpyHostToDevice, c->stream));
+        src = static_cast<const uint8_t *>(w->d_pack_in);
+    } else {
+        int64_t o = 0;
+        for (size_t k = 0; k < n; k++) {
+            const int32_t ca = w->pack_sizes[2 * k], cb = w->pack_sizes[2 * k + 1];
+            desc[2 * k] = DecDesc{o + HDR, ca, 1};
+            desc[2 * k + 1] = DecDesc{o + HDR + ca, cb, 1};
+            ph[k] = w->pack_phase[k];
+            o += HDR + ca + cb;
+        }
+        src = static_cast<const uint8_t *>(w->d_pack_img);
+    }
+    const size_t b_desc = (2 * n * sizeof(DecDesc) + 255) & ~(size_t)255, b_st = (3 * n * 4 + 255) & ~(size_t)255;
+    const size_t b_l2 = layer2 ? rcells * 8 : 0, b_bg = background ? rcells * 4 : 0;
+    if (grow(c, &w->d_pack_raw, &w->pack_raw_cap, n * RAW) ||
+        grow(c, &w->d_pack_dec, &w->pack_dec_cap, b_desc + b_st + b_l2 + b_bg))
+        return FSG_ERR_NOMEM;
+    char *dec = static_cast<char *>(w->d_pack_dec);
+    DecDesc *d_desc = reinterpret_cast<DecDesc *>(dec);
+    int32_t *d_bst = reinterpret_cast<int32_t *>(dec + b_desc), *d_cst = d_bst + 2 * n;
+    UnpackDst d{};
+    d.cells = w->d.buf[w->d.tick & 1];
+    d.W = w->d.W;
+    d.x0 = x0;
+    d.y0 = y0;
+    d.ncx = ncx;
+    d.rw = ncx * CW;
+    d.l2 = layer2 ? reinterpret_cast<uint64_t *>(dec + b_desc + b_st) : nullptr;
+    d.bg = background ? reinterpret_cast<uint32_t *>(dec + b_desc + b_st + b_l2) : nullptr;
+    d.n_materials = w->cfg.n_materials;
+    HIPCHK(hipMemcpyAsync(d_desc, desc.data(), 2 * n * sizeof(DecDesc), hipMemcpyHostToDevice, c->stream));
+    if (b_l2 + b_bg) HIPCHK(hipMemsetAsync(dec + b_desc + b_st, 0, b_l2 + b_bg, c->stream));  // failed chunks: 0
+    uint8_t *raw = static_cast<uint8_t *>(w->d_pack_raw);
+    k_pack_decode<<<(unsigned)(2 * n), 64, 0, c->stream>>>(src, d_desc, raw, d_bst);
+    k_pack_cells<<<(unsigned)n, 256, 0, c->stream>>>(raw, d_bst, d, d_cst);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(status, d_cst, n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (layer2) HIPCHK(hipMemcpyAsync(layer2, d.l2, b_l2, hipMemcpyDeviceToHost, c->stream));
+    if (background) HIPCHK(hipMemcpyAsync(background, d.bg, b_bg, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (phase) std::copy(ph.begin(), ph.end(), phase);
+    // new cells, as after fsg_world_upload
+    w->d.force_tick = w->d.tick;
+    w->stamped = false;
+    w->prev_valid = false;
+    return FSG_OK;
+}

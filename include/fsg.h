@@ -168,6 +168,44 @@ int fsg_world_generate(fsg_world *w, int kind, uint64_t seed);
 /* Rectangle device->host copy (getTile over a region, world.cpp:999-1002). */
 int fsg_world_download(fsg_world *w, fsg_cell *out, int x0, int y0, int rw, int rh);
 
+/* ------------------------------------------------------------------------ */
+/* Chunk .pack images (Chunk::ChunkWrite chunk.cpp:219-330, Chunk::ChunkRead */
+/* chunk.cpp:74-217), made and read on the device for a region of           */
+/* ncx x ncy chunks of FSG_CHUNK^2 cells whose top-left cell is (x0, y0).    */
+/* An image is the reference's file: i8 generation phase, i32 393216,        */
+/* i32 compressed size, i32 65536, i32 compressed size, the LZ4 block of the */
+/* 12-byte {u32 material index, u32 colour, i16 temperature, pad} records of */
+/* tiles[] then layer2[], the LZ4 block of the u32 background.  Images are   */
+/* ordered chunk row by chunk row.  The blocks decode with lz4's             */
+/* LZ4_decompress_safe; their bytes are this library's own parse, not those  */
+/* of LZ4_compress_fast.                                                     */
+/* ------------------------------------------------------------------------ */
+#define FSG_CHUNK 128 /* CHUNK_W = CHUNK_H (const.h:19-20) */
+
+/* ChunkWrite for every chunk of the region: encodes the world's cells as
+ * tiles[], layer2 (region-shaped host array, ncx*128 wide; NULL = empty
+ * records) and background (likewise; NULL = 0), with generation phases
+ * phase[ncx*ncy] (NULL = 0).  image_bytes[ncx*ncy] receives each image's
+ * size; the images stay on the device until fsg_world_pack_download. */
+int fsg_world_pack(fsg_world *w, int32_t x0, int32_t y0, int32_t ncx, int32_t ncy, const int8_t *phase,
+                   const fsg_cell *layer2, const uint32_t *background, size_t *image_bytes);
+/* Copies the images of the last fsg_world_pack, back to back, to out
+ * (cap >= the sum of their sizes). */
+int fsg_world_pack_download(fsg_world *w, void *out, size_t cap);
+/* ChunkRead for every chunk of the region from images laid back to back in
+ * host memory (image_bytes[ncx*ncy] their sizes), or, with images = NULL,
+ * from the device-resident images of the last fsg_world_pack (same chunk
+ * count; image_bytes ignored).  status[k] = FSG_OK, or FSG_ERR_INVALID for a
+ * chunk whose header sizes, LZ4 blocks (a block that LZ4_decompress_safe
+ * rejects or that decodes short) or material indices are bad; such a chunk
+ * leaves its cells unchanged (ChunkRead logs the error and keeps what it
+ * decoded, chunk.cpp:155-160).  phase, layer2 and background (region-shaped,
+ * may be NULL) receive those parts of the images (zero layer2 and background
+ * for a failed chunk; its phase is the header's when the header was read). */
+int fsg_world_unpack(fsg_world *w, int32_t x0, int32_t y0, int32_t ncx, int32_t ncy, const void *images,
+                     const size_t *image_bytes, int8_t *phase, fsg_cell *layer2, uint32_t *background,
+                     int32_t *status);
+
 /* setTile (world.cpp:1004-1008) batched: edits are applied in array order
  * before the next tick; out-of-bounds edits are ignored like setTile's. */
 typedef struct fsg_cell_edit {

@@ -871,3 +871,183 @@ void orc_default_config(fsg_config *cfg) {
     set_mat(&cfg->materials[6], "lava", FSG_SOUP, 4, 1, 0, 0xC00069FFu);
     set_mat(&cfg->materials[7], "wood", FSG_SOLID, 14, 0, 0, 0xFF2D5280u);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Chunk .pack images (chunk.cpp:74-330).                                    */
+/* ------------------------------------------------------------------------ */
+
+static void put_le(uint8_t *p, uint32_t v, int nbytes) {
+    for (int k = 0; k < nbytes; k++) p[k] = (uint8_t)(v >> (8 * k));
+}
+
+static uint32_t get_le(const uint8_t *p, int nbytes) {
+    uint32_t v = 0;
+    for (int k = 0; k < nbytes; k++) v |= (uint32_t)p[k] << (8 * k);
+    return v;
+}
+
+/* chunk.cpp:238-242: buf[i] = {mat->id, color, temperature} for tiles, then
+ * layer2 at CHUNK_W * CHUNK_H + i; the record's last two bytes are padding. */
+void orc_chunk_records(const uint64_t *tiles, size_t tstride, const uint64_t *layer2, size_t lstride,
+                       const uint32_t *bg, size_t bstride, uint8_t *raw_a, uint8_t *raw_b) {
+    for (int layer = 0; layer < 2; layer++)
+        for (int y = 0; y < 128; y++)
+            for (int x = 0; x < 128; x++) {
+                uint64_t c = 0;
+                if (layer == 0) c = tiles[(size_t)y * tstride + x];
+                else if (layer2) c = layer2[(size_t)y * lstride + x];
+                uint8_t *r = raw_a + ((size_t)layer * 16384 + (size_t)y * 128 + x) * 12;
+                put_le(r, (uint32_t)(c & 0xFFFFu), 4);            /* index */
+                put_le(r + 4, (uint32_t)(c >> 32), 4);            /* color */
+                put_le(r + 8, (uint32_t)((c >> 16) & 0xFFFFu), 2); /* temperature */
+                r[10] = r[11] = 0;
+            }
+    for (int y = 0; y < 128; y++)
+        for (int x = 0; x < 128; x++) put_le(raw_b + ((size_t)y * 128 + x) * 4, bg ? bg[(size_t)y * bstride + x] : 0u, 4);
+}
+
+/* Reads an LZ4 length continuation: bytes are added while they are 255.
+ * Fails when the input position passes lim after a byte (lz4.c:1916-1940). */
+static int lz4_more(const uint8_t *src, int *ip, int lim, int *len, int cap) {
+    int s;
+    do {
+        s = src[(*ip)++];
+        *len += s;
+        if (*ip > lim || *len > cap) return -1;
+    } while (s == 255);
+    return 0;
+}
+
+int orc_lz4_decode(const uint8_t *src, int n, uint8_t *dst, int cap) {
+    if (n <= 0 || cap < 0) return -1;
+    if (cap == 0) return (n == 1 && src[0] == 0) ? 0 : -1;
+    int ip = 0, op = 0;
+    while (ip < n) {
+        const int token = src[ip++];
+        int lit = token >> 4;
+        if (lit == 15) {
+            if (ip >= n - 15) return -1; /* initial check of the literal-length read */
+            if (lz4_more(src, &ip, n - 15, &lit, cap)) return -1;
+        }
+        /* a literal run near either end must be the block's last sequence */
+        if (op + lit > cap - 12 || ip + lit > n - 8) {
+            if (ip + lit != n || op + lit > cap) return -1;
+            memcpy(dst + op, src + ip, (size_t)lit);
+            return op + lit;
+        }
+        memcpy(dst + op, src + ip, (size_t)lit);
+        ip += lit;
+        op += lit;
+        const int offset = (int)get_le(src + ip, 2);
+        ip += 2;
+        int mlen = token & 15;
+        if (mlen == 15 && lz4_more(src, &ip, n - 4, &mlen, cap)) return -1;
+        mlen += 4;
+        if (offset > op) return -1;         /* before the output start */
+        if (op + mlen > cap - 5) return -1; /* the last 5 bytes are literals */
+        /* offset 0: lz4.c zeroes the first word before copying the match onto
+         * itself (lz4.c:2328-2334), so the match is zeros */
+        for (int k = 0; k < mlen; k++, op++) dst[op] = offset ? dst[op - offset] : 0;
+    }
+    return -1;
+}
+
+static int lz4_put_len(uint8_t *dst, int op, int cap, int v) {
+    if (v < 15) return op;
+    for (v -= 15; v >= 255; v -= 255) {
+        if (op >= cap) return -1;
+        dst[op++] = 255;
+    }
+    if (op >= cap) return -1;
+    dst[op++] = (uint8_t)v;
+    return op;
+}
+
+/* One sequence: literals raw[ls, le), then (if mlen > 0) a match of mlen
+ * bytes at offset off. */
+static int lz4_put_seq(const uint8_t *raw, int ls, int le, int off, int mlen, uint8_t *dst, int op, int cap) {
+    const int lit = le - ls;
+    if (op >= cap) return -1;
+    dst[op++] = (uint8_t)(((lit < 15 ? lit : 15) << 4) | (mlen ? (mlen - 4 < 15 ? mlen - 4 : 15) : 0));
+    if ((op = lz4_put_len(dst, op, cap, lit)) < 0 || op + lit > cap) return -1;
+    memcpy(dst + op, raw + ls, (size_t)lit);
+    op += lit;
+    if (!mlen) return op;
+    if (op + 2 > cap) return -1;
+    put_le(dst + op, (uint32_t)off, 2);
+    return lz4_put_len(dst, op + 2, cap, mlen - 4);
+}
+
+int orc_lz4_encode_rows(const uint8_t *raw, int n, int rec, uint8_t *dst, int cap) {
+    const int row = 128 * rec;
+    if (rec <= 0 || n <= 12 || n % row) return -1;
+    int op = 0, lit0 = 0; /* literals pending since lit0 */
+    for (int r0 = 0; r0 < n; r0 += row) {
+        int kind = -1, start = r0;
+        for (int p = r0; p <= r0 + row; p++) {
+            int k = -2; /* the row's end closes the last run */
+            if (p < r0 + row) {
+                const int eql = p >= rec && raw[p] == raw[p - rec];
+                const int equ = p >= row && raw[p] == raw[p - row];
+                k = p >= n - 12 ? 0 : eql ? 1 : equ ? 2 : 0;
+            }
+            if (k == kind) continue;
+            if (kind > 0 && p - start >= 4) { /* a match run */
+                op = lz4_put_seq(raw, lit0, start, kind == 1 ? rec : row, p - start, dst, op, cap);
+                if (op < 0) return -1;
+                lit0 = p;
+            }
+            kind = k;
+            start = p;
+        }
+    }
+    return lz4_put_seq(raw, lit0, n, 0, 0, dst, op, cap);
+}
+
+size_t orc_pack_chunk(const uint8_t *raw_a, const uint8_t *raw_b, int8_t phase, uint8_t *img) {
+    const int ca = orc_lz4_encode_rows(raw_a, ORC_PACK_NA, 12, img + ORC_PACK_HDR, ORC_LZ4_BOUND(ORC_PACK_NA));
+    const int cb = orc_lz4_encode_rows(raw_b, ORC_PACK_NB, 4, img + ORC_PACK_HDR + ca, ORC_LZ4_BOUND(ORC_PACK_NB));
+    if (ca < 0 || cb < 0) return 0;
+    img[0] = (uint8_t)phase; /* chunk.cpp:312-318 */
+    put_le(img + 1, ORC_PACK_NA, 4);
+    put_le(img + 5, (uint32_t)ca, 4);
+    put_le(img + 9, ORC_PACK_NB, 4);
+    put_le(img + 13, (uint32_t)cb, 4);
+    return (size_t)ORC_PACK_HDR + (size_t)ca + (size_t)cb;
+}
+
+int orc_unpack_chunk(const uint8_t *img, size_t n, uint32_t n_materials, uint64_t *tiles, size_t tstride,
+                     uint64_t *layer2, size_t lstride, uint32_t *bg, size_t bstride, int8_t *phase) {
+    if (n < ORC_PACK_HDR) return FSG_ERR_INVALID;
+    if (phase) *phase = (int8_t)img[0];
+    const int32_t sa = (int32_t)get_le(img + 1, 4), ca = (int32_t)get_le(img + 5, 4);
+    const int32_t sb = (int32_t)get_le(img + 9, 4), cb = (int32_t)get_le(img + 13, 4);
+    /* chunk.cpp:118-134: the sizes must be the chunk's */
+    if (sa != ORC_PACK_NA || sb != ORC_PACK_NB || ca < 0 || cb < 0 ||
+        (size_t)ORC_PACK_HDR + (size_t)ca + (size_t)cb > n)
+        return FSG_ERR_INVALID;
+    uint8_t *ra = (uint8_t *)malloc(ORC_PACK_NA), *rb = (uint8_t *)malloc(ORC_PACK_NB);
+    int rc = FSG_ERR_INVALID;
+    if (!ra || !rb) goto done;
+    if (orc_lz4_decode(img + ORC_PACK_HDR, ca, ra, ORC_PACK_NA) != ORC_PACK_NA) goto done;
+    if (orc_lz4_decode(img + ORC_PACK_HDR + ca, cb, rb, ORC_PACK_NB) != ORC_PACK_NB) goto done;
+    for (int i = 0; i < 2 * 16384; i++) { /* chunk.cpp:167-171: index -> materials_array */
+        const uint32_t idx = get_le(ra + (size_t)i * 12, 4);
+        if (i < 16384 ? ((idx & 0xFFFFFE00u) || (idx & 0xFFu) >= n_materials) : idx > 0xFFFFu) goto done;
+    }
+    for (int i = 0; i < 2 * 16384; i++) {
+        const uint8_t *r = ra + (size_t)i * 12;
+        const uint64_t c = (uint64_t)(get_le(r, 4) & 0xFFFFu) | ((uint64_t)get_le(r + 8, 2) << 16) |
+                           ((uint64_t)get_le(r + 4, 4) << 32);
+        const int y = (i >> 7) & 127, x = i & 127;
+        if (i < 16384) tiles[(size_t)y * tstride + x] = c;
+        else if (layer2) layer2[(size_t)y * lstride + x] = c;
+    }
+    if (bg)
+        for (int i = 0; i < 16384; i++) bg[(size_t)(i >> 7) * bstride + (i & 127)] = get_le(rb + (size_t)i * 4, 4);
+    rc = FSG_OK;
+done:
+    free(ra);
+    free(rb);
+    return rc;
+}

@@ -73,6 +73,38 @@ int orc_render(const uint64_t *cells, const uint64_t *prev, int W, int H, const 
                const fsg_render_params *p, uint32_t *lc, uint32_t *le, uint32_t *lf,
                fsg_render_stats *st);
 
+/* Chunk .pack images (Chunk::ChunkWrite chunk.cpp:219-330, Chunk::ChunkRead
+ * chunk.cpp:74-217).  A chunk is 128x128 cells (const.h:19-20). */
+#define ORC_PACK_NA 393216 /* 2 * 128 * 128 MaterialInstanceData records of 12 bytes */
+#define ORC_PACK_NB 65536  /* 128 * 128 u32 background */
+#define ORC_PACK_HDR 17
+#define ORC_LZ4_BOUND(n) ((n) + (n) / 255 + 16) /* LZ4_compressBound (lz4.h) */
+
+/* The records of one chunk (chunk.cpp:238-242): raw_a = {u32 index, u32
+ * colour, i16 temperature, 2 zero bytes} of tiles then layer2, raw_b = the
+ * background.  Inputs are chunk windows with row strides in elements;
+ * layer2 / bg may be NULL (zero records). */
+void orc_chunk_records(const uint64_t *tiles, size_t tstride, const uint64_t *layer2, size_t lstride,
+                       const uint32_t *bg, size_t bstride, uint8_t *raw_a, uint8_t *raw_b);
+/* LZ4 block decoding with the acceptance rules of lz4.c's
+ * LZ4_decompress_safe for a full block (lz4.c:1949-2360): decoded size, or -1. */
+int orc_lz4_decode(const uint8_t *src, int n, uint8_t *dst, int cap);
+/* The parse of the library's encoder for an n-byte block of rec-byte
+ * records, 128 records a row: each byte is a match candidate against the
+ * byte one record back, else one row back; runs of 4+ such bytes inside a
+ * row become matches, the rest literals; the last 12 bytes are literals.
+ * Returns the compressed size, or -1 if cap is too small. */
+int orc_lz4_encode_rows(const uint8_t *raw, int n, int rec, uint8_t *dst, int cap);
+/* One .pack image of the records (header + block A + block B); returns its
+ * size.  img holds at least 17 + ORC_LZ4_BOUND(NA) + ORC_LZ4_BOUND(NB). */
+size_t orc_pack_chunk(const uint8_t *raw_a, const uint8_t *raw_b, int8_t phase, uint8_t *img);
+/* ChunkRead of one image into chunk windows (layer2 / bg may be NULL):
+ * FSG_OK, or FSG_ERR_INVALID for bad header sizes, a block that does not
+ * decode to its full size or a bad material index (nothing written then;
+ * *phase is still set once the header has been read). */
+int orc_unpack_chunk(const uint8_t *img, size_t n, uint32_t n_materials, uint64_t *tiles, size_t tstride,
+                     uint64_t *layer2, size_t lstride, uint32_t *bg, size_t bstride, int8_t *phase);
+
 void orc_counts(const uint64_t *cells, size_t n, uint64_t counts[16]);
 uint64_t orc_checksum(const uint64_t *cells, size_t n);
 

@@ -51,6 +51,12 @@ def lib():
             "orc_counts": (None, [P, C.c_size_t, P]),
             "orc_checksum": (u64, [P, C.c_size_t]),
             "refseq_run": (i32, [P, i32, i32, cfgp, u64, i32, i32]),
+            "orc_chunk_records": (None, [P, C.c_size_t, P, C.c_size_t, P, C.c_size_t, P, P]),
+            "orc_lz4_decode": (i32, [P, i32, P, i32]),
+            "orc_lz4_encode_rows": (i32, [P, i32, i32, P, i32]),
+            "orc_pack_chunk": (C.c_size_t, [P, P, C.c_int8, P]),
+            "orc_unpack_chunk": (i32, [P, C.c_size_t, u32, P, C.c_size_t, P, C.c_size_t, P, C.c_size_t,
+                                       C.POINTER(C.c_int8)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -217,3 +223,69 @@ def physics_check(cells, cfg, seeds):
             d["pixels"] = pix[r.pixel_offset:r.pixel_offset + r.w * r.h].reshape(r.h, r.w).copy()
         out.append(d)
     return out
+
+
+# --- chunk .pack images (chunk.cpp:74-330) ---------------------------------
+PACK_NA, PACK_NB, PACK_HDR = 393216, 65536, 17
+
+
+def lz4_bound(n: int) -> int:
+    return n + n // 255 + 16
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def chunk_records(tiles, layer2=None, bg=None):
+    """(raw_a, raw_b) records of one chunk; tiles/layer2 (uint64) and bg
+    (uint32) are 128x128 windows of any row stride."""
+    ra = np.zeros(PACK_NA, np.uint8)
+    rb = np.zeros(PACK_NB, np.uint8)
+
+    def win(a, dt):
+        if a is None:
+            return None, 0
+        assert a.dtype == dt and a.shape[0] >= 128 and a.shape[1] >= 128 and a.strides[1] == a.itemsize
+        return a, a.strides[0] // a.itemsize
+
+    t, ts = win(tiles, np.uint64)
+    l2, ls = win(layer2, np.uint64)
+    b, bs = win(bg, np.uint32)
+    lib().orc_chunk_records(_ptr(t), ts, _ptr(l2), ls, _ptr(b), bs, ra.ctypes.data, rb.ctypes.data)
+    return ra, rb
+
+
+def lz4_decode(src: bytes, cap: int):
+    """Decoded bytes, or None when the block is rejected or not cap long."""
+    s = np.frombuffer(bytes(src), np.uint8)
+    out = np.zeros(max(cap, 1), np.uint8)
+    r = lib().orc_lz4_decode(s.ctypes.data if len(s) else None, len(s), out.ctypes.data, cap)
+    return None if r != cap else out[:cap].tobytes()
+
+
+def lz4_encode_rows(raw, rec: int) -> bytes:
+    raw = np.ascontiguousarray(raw, np.uint8)
+    out = np.zeros(lz4_bound(len(raw)), np.uint8)
+    r = lib().orc_lz4_encode_rows(raw.ctypes.data, len(raw), rec, out.ctypes.data, len(out))
+    assert r >= 0, r
+    return out[:r].tobytes()
+
+
+def pack_chunk(raw_a, raw_b, phase: int = 0) -> bytes:
+    img = np.zeros(PACK_HDR + lz4_bound(PACK_NA) + lz4_bound(PACK_NB), np.uint8)
+    n = lib().orc_pack_chunk(raw_a.ctypes.data, raw_b.ctypes.data, phase, img.ctypes.data)
+    assert n > 0
+    return img[:n].tobytes()
+
+
+def unpack_chunk(img: bytes, n_materials: int):
+    """(status, tiles, layer2, bg, phase) of one image (arrays 128x128)."""
+    s = np.frombuffer(bytes(img), np.uint8)
+    tiles = np.zeros((128, 128), np.uint64)
+    l2 = np.zeros((128, 128), np.uint64)
+    bg = np.zeros((128, 128), np.uint32)
+    ph = C.c_int8(0)
+    rc = lib().orc_unpack_chunk(s.ctypes.data if len(s) else None, len(s), n_materials, tiles.ctypes.data, 128,
+                                l2.ctypes.data, 128, bg.ctypes.data, 128, C.byref(ph))
+    return rc, tiles, l2, bg, ph.value

@@ -1,0 +1,178 @@
+"""Device chunk .pack codec (fsg_world_pack / fsg_world_unpack) against the
+oracle: images byte-for-byte equal to the oracle's ChunkWrite restatement,
+golden images made by the reference's own lz4 read back exactly, corrupt
+images rejected chunk by chunk as the oracle rejects them, and round trips
+at full size."""
+import struct
+
+import numpy as np
+import pytest
+
+import falling_sand_engine_amd as fsg
+import packcases
+from falling_sand_engine_amd import _abi
+from test_pack import NMAT, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_images(orc, cells, x0, y0, ncx, ncy, phase=None, l2=None, bg=None):
+    out = []
+    for cy in range(ncy):
+        for cx in range(ncx):
+            t = cells[y0 + cy * 128:y0 + (cy + 1) * 128, x0 + cx * 128:x0 + (cx + 1) * 128]
+            lw = None if l2 is None else l2[cy * 128:(cy + 1) * 128, cx * 128:(cx + 1) * 128]
+            bw = None if bg is None else bg[cy * 128:(cy + 1) * 128, cx * 128:(cx + 1) * 128]
+            ra, rb = orc.chunk_records(np.ascontiguousarray(t), None if lw is None else np.ascontiguousarray(lw),
+                                       None if bw is None else np.ascontiguousarray(bw))
+            out.append(orc.pack_chunk(ra, rb, 0 if phase is None else int(phase[cy * ncx + cx])))
+    return out
+
+
+@pytest.mark.parametrize("kind,W,H,seed,x0,y0,ncx,ncy,aux", [
+    (2, 512, 512, 3, 0, 0, 4, 4, False),
+    (2, 640, 448, 8, 64, 128, 3, 2, True),   # region off the origin, layer2 and background
+    (3, 512, 256, 5, 0, 0, 4, 2, True),      # the all-active striped world
+    (4, 1024, 512, 21, 128, 0, 6, 4, False),
+])
+def test_pack_matches_oracle(ctx, orc, kind, W, H, seed, x0, y0, ncx, ncy, aux):
+    print(f"pack kind={kind} {W}x{H} region=({x0},{y0}) {ncx}x{ncy} aux={aux}", flush=True)
+    cells = fsg.generate(kind, W, H, seed)
+    n = ncx * ncy
+    phase = np.arange(n, dtype=np.int8) - 3
+    l2 = packcases.layer2_pattern(ncy * 128, ncx * 128, seed) if aux else None
+    bg = packcases.background_pattern(ncy * 128, ncx * 128, seed) if aux else None
+    with fsg.World(ctx, W, H, seed, orc.default_config()) as w:
+        w.upload(cells)
+        got = w.pack(x0, y0, ncx, ncy, phase=phase, layer2=l2, background=bg)
+        assert np.array_equal(w.download(), cells)  # packing leaves the grid alone
+    ref = oracle_images(orc, cells, x0, y0, ncx, ncy, phase, l2, bg)
+    assert len(got) == n
+    for k in range(n):
+        assert got[k] == ref[k], k
+
+
+def test_unpack_golden_reference_images(ctx, orc):
+    """Images compressed by the reference's LZ4_compress_fast read back
+    exactly: cells, layer2, background and phase."""
+    gold = load_golden()
+    names = sorted(gold)
+    print("unpack golden", names, flush=True)
+    imgs = [gold[k][0] for k in names]
+    with fsg.World(ctx, 128 * len(names), 128, 1, orc.default_config()) as w:
+        st, ph, l2, bg = w.unpack(0, 0, len(names), 1, imgs, want_layer2=True, want_background=True)
+        grid = w.download()
+    assert (st == 0).all(), st
+    for k, name in enumerate(names):
+        _img, c, tiles, tl2, tbg = gold[name]
+        sl = np.s_[:, k * 128:(k + 1) * 128]
+        assert ph[k] == c["phase"]
+        assert np.array_equal(grid[sl], tiles), name
+        assert np.array_equal(l2[sl], np.zeros_like(tiles) if tl2 is None else tl2), name
+        assert np.array_equal(bg[sl], np.zeros((128, 128), np.uint32) if tbg is None else tbg), name
+
+
+def test_round_trip_on_device_and_through_host(ctx, orc):
+    cells = fsg.generate(4, 768, 512, 77)
+    other = fsg.generate(2, 768, 512, 78)
+    l2 = packcases.layer2_pattern(512, 768, 1)
+    bg = packcases.background_pattern(512, 768, 2)
+    print("round trip 768x512", flush=True)
+    with fsg.World(ctx, 768, 512, 1, orc.default_config()) as w:
+        w.upload(cells)
+        imgs = w.pack(0, 0, 6, 4, layer2=l2, background=bg)
+        w.upload(other)
+        st, _ph, gl2, gbg = w.unpack(0, 0, 6, 4, None, want_layer2=True, want_background=True)  # device images
+        assert (st == 0).all()
+        assert np.array_equal(w.download(), cells)
+        assert np.array_equal(gl2, l2) and np.array_equal(gbg, bg)
+        # host images into a shifted region of another world
+    with fsg.World(ctx, 1024, 640, 1, orc.default_config()) as w2:
+        w2.upload(np.zeros((640, 1024), np.uint64))
+        st, _ph, _l, _b = w2.unpack(128, 64, 6, 4, imgs)
+        assert (st == 0).all()
+        g = w2.download()
+        assert np.array_equal(g[64:576, 128:896], cells)
+        assert not g[:64].any() and not g[576:].any() and not g[:, :128].any() and not g[:, 896:].any()
+
+
+def test_corrupt_images_fail_per_chunk(ctx, orc):
+    """Bad header sizes, truncation, byte flips and bad material indices fail
+    their own chunk (cells untouched) exactly when the oracle's ChunkRead does;
+    the other chunks are written."""
+    cells = fsg.generate(2, 1024, 512, 5)
+    with fsg.World(ctx, 1024, 512, 1, orc.default_config()) as w:
+        w.upload(cells)
+        imgs = w.pack(0, 0, 8, 4)
+    rng = np.random.default_rng(4)
+    bad = []
+    for k, img in enumerate(imgs):
+        b = bytearray(img)
+        m = k % 8
+        if m == 1:
+            b[1:5] = struct.pack("<i", 393216 + 12)
+        elif m == 2:
+            b = b[:len(b) - 1 - int(rng.integers(0, 40))]
+        elif m == 3:
+            for _ in range(3):
+                b[17 + int(rng.integers(0, len(b) - 17))] ^= 1 << int(rng.integers(0, 8))
+        elif m == 4:
+            b = b[:int(rng.integers(0, 17))]
+        elif m == 5:  # a record with material index 200 (decodes, then fails the index check)
+            t = cells[(k // 8) * 128:(k // 8 + 1) * 128, (k % 8) * 128:(k % 8 + 1) * 128].copy()
+            t[7, 9] = np.uint64(200) | (t[7, 9] & np.uint64(0xFFFFFFFFFFFF0000))
+            ra, rb = orc.chunk_records(np.ascontiguousarray(t))
+            b = bytearray(orc.pack_chunk(ra, rb, 0))
+        elif m == 6:
+            b[13:17] = struct.pack("<i", 10)  # background block size wrong
+        bad.append(bytes(b))
+    refs = [orc.unpack_chunk(b, NMAT) for b in bad]
+    expect = [r[0] for r in refs]
+    assert any(e != 0 for e in expect) and any(e == 0 for e in expect)
+    fill = np.full((512, 1024), np.uint64(1) | (np.uint64(320) << np.uint64(16)), np.uint64)
+    print("corrupt images", [int(e) for e in expect], flush=True)
+    with fsg.World(ctx, 1024, 512, 1, orc.default_config()) as w:
+        w.upload(fill)
+        st, _ph, _l, _b = w.unpack(0, 0, 8, 4, bad)
+        g = w.download()
+    assert list(st) == expect
+    for k, e in enumerate(expect):
+        sl = np.s_[(k // 8) * 128:(k // 8 + 1) * 128, (k % 8) * 128:(k % 8 + 1) * 128]
+        assert np.array_equal(g[sl], refs[k][1] if e == 0 else fill[sl]), k  # a flipped literal still reads
+
+
+def test_round_trip_16k_world(ctx, orc):
+    """The 16384^2 world: every chunk packed and read back on the device gives
+    the grid again (census and checksum equal, and a sampled rectangle)."""
+    cfg = orc.default_config()
+    print("round trip 16384^2", flush=True)
+    with fsg.World(ctx, 16384, 16384, 1, cfg) as w:
+        w.generate(2, 41)
+        before = w.stats()
+        sample = w.download(4096, 8192, 512, 256)
+        sizes = w.pack(0, 0, 128, 128, download=False)
+        assert len(sizes) == 128 * 128
+        w.generate(3, 42)
+        assert w.stats().checksum != before.checksum
+        st, _ph, _l, _b = w.unpack(0, 0, 128, 128, None)
+        assert (st == 0).all()
+        after = w.stats()
+        assert after.checksum == before.checksum
+        assert list(after.count) == list(before.count)
+        assert np.array_equal(w.download(4096, 8192, 512, 256), sample)
+
+
+def test_pack_arguments(ctx, orc):
+    with fsg.World(ctx, 256, 256, 1, orc.default_config()) as w:
+        w.upload(fsg.generate(2, 256, 256, 1))
+        for args in [(-1, 0, 1, 1), (0, 0, 3, 1), (128, 0, 2, 1), (0, 0, 0, 1), (0, 200, 1, 1)]:
+            with pytest.raises(fsg.FsgError):
+                w.pack(*args)
+        with pytest.raises(fsg.FsgError):  # nothing packed yet on this world
+            w.unpack(0, 0, 1, 1, None)
+        sizes = w.pack(0, 0, 2, 2, download=False)
+        with pytest.raises(fsg.FsgError):  # a different chunk count than the last pack
+            w.unpack(0, 0, 1, 1, None)
+        assert min(sizes) > 17
+        rc = fsg.lib().fsg_world_pack_download(w.handle, None, 0)
+        assert rc == _abi.FSG_ERR_INVALID

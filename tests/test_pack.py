@@ -1,0 +1,270 @@
+"""Chunk .pack codec on the CPU: the oracle's restatement of ChunkWrite /
+ChunkRead (chunk.cpp:74-330) and of LZ4 block decoding, pinned against
+  * the MaterialInstanceData layout (chunk.hpp:26-30) by the C layout rules,
+  * golden images compressed by the reference's own lz4 (tests/golden),
+  * that lz4 itself (oracle/_ref, built from source/libs/lz4/lz4.c) where it
+    is built: both directions of the codec and a corruption fuzz,
+and the library's LZ4 decoder built for the host against the oracle."""
+import ctypes as C
+import hashlib
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import falling_sand_engine_amd as fsg
+from falling_sand_engine_amd import _abi
+import lz4ref
+import packcases
+from oracle import pyoracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+NMAT = 8
+
+needs_ref = pytest.mark.skipif(lz4ref.lib() is None, reason="reference lz4 not built (make -C oracle ref)")
+
+
+def cell(mat, temp, rgba):
+    return np.uint64(mat | ((temp & 0xFFFF) << 16) | (rgba << 32))
+
+
+def load_golden():
+    meta = json.load(open(os.path.join(GOLD, "pack_vectors.json")))
+    out = {}
+    for name, c in meta["cases"].items():
+        img = open(os.path.join(GOLD, f"pack_{name}.pack"), "rb").read()
+        assert hashlib.sha256(img).hexdigest() == c["sha256"]
+        if c["kind"] == 0:
+            cells = np.zeros((c["H"], c["W"]), np.uint64)
+        else:
+            cells = fsg.generate(c["kind"], c["W"], c["H"], c["seed"])
+        tiles = packcases.chunk_of(cells, c["cx"], c["cy"])
+        l2 = None if c["l2"] is None else packcases.layer2_pattern(128, 128, c["l2"])
+        bg = None if c["bg"] is None else packcases.background_pattern(128, 128, c["bg"])
+        out[name] = (img, c, tiles, l2, bg)
+    return out
+
+
+def world_chunks(kind, W, H, seed):
+    cells = fsg.generate(kind, W, H, seed)
+    return [packcases.chunk_of(cells, cx, cy) for cy in range(H // 128) for cx in range(W // 128)]
+
+
+def test_record_layout_is_material_instance_data():
+    # typedef struct { u32 index; u32 color; i16 temperature; } (chunk.hpp:26-30,
+    # mat_instance_id = u32, mat_temperature = i16: game_datastruct.hpp:106-109)
+    class MID(C.Structure):
+        _fields_ = [("index", C.c_uint32), ("color", C.c_uint32), ("temperature", C.c_int16)]
+
+    assert C.sizeof(MID) == 12 and MID.color.offset == 4 and MID.temperature.offset == 8
+    tiles = np.zeros((128, 128), np.uint64)
+    tiles[0, 0] = cell(0x107, -5, 0x11223344)  # moved sand-like id with its moved bit
+    tiles[127, 127] = cell(3, 320, 0xFFAA0000)
+    l2 = np.zeros((128, 128), np.uint64)
+    l2[5, 9] = cell(1, 7, 0x01020304)
+    bg = np.arange(128 * 128, dtype=np.uint32).reshape(128, 128) * np.uint32(2654435761)
+    ra, rb = O.chunk_records(tiles, l2, bg)
+    recs = np.frombuffer(ra.tobytes(), dtype=np.dtype(MID))
+    assert recs.shape == (2 * 128 * 128,)
+    assert (recs[0]["index"], recs[0]["color"], recs[0]["temperature"]) == (0x107, 0x11223344, -5)
+    last = recs[128 * 128 - 1]
+    assert (last["index"], last["color"], last["temperature"]) == (3, 0xFFAA0000, 320)
+    r = recs[128 * 128 + 5 * 128 + 9]
+    assert (r["index"], r["color"], r["temperature"]) == (1, 0x01020304, 7)
+    assert all(ra[12 * i + 10] == 0 and ra[12 * i + 11] == 0 for i in range(0, 2 * 128 * 128, 977))
+    assert np.array_equal(np.frombuffer(rb.tobytes(), np.uint32), bg.ravel())
+
+
+def test_golden_images_read_back():
+    """Images made by the reference's own compressor read back to the chunk."""
+    for name, (img, c, tiles, l2, bg) in load_golden().items():
+        rc, t, l, b, ph = O.unpack_chunk(img, NMAT)
+        assert rc == 0, name
+        assert ph == c["phase"]
+        assert np.array_equal(t, tiles), name
+        assert np.array_equal(l, np.zeros_like(l) if l2 is None else l2), name
+        assert np.array_equal(b, np.zeros_like(b) if bg is None else bg), name
+        ra, rb = O.chunk_records(tiles, l2, bg)
+        assert hashlib.sha256(ra.tobytes()).hexdigest() == c["raw_a_sha256"]
+        assert hashlib.sha256(rb.tobytes()).hexdigest() == c["raw_b_sha256"]
+
+
+def test_pack_image_header_and_round_trip():
+    tiles = world_chunks(2, 256, 256, 3)[1]
+    l2 = packcases.layer2_pattern(128, 128, 1)
+    bg = packcases.background_pattern(128, 128, 2)
+    ra, rb = O.chunk_records(tiles, l2, bg)
+    img = O.pack_chunk(ra, rb, -7)
+    phase, sa, ca, sb, cb = struct.unpack("<biiii", img[:17])
+    assert (phase, sa, sb) == (-7, 393216, 65536)
+    assert 17 + ca + cb == len(img)
+    assert O.lz4_decode(img[17:17 + ca], sa) == ra.tobytes()
+    assert O.lz4_decode(img[17 + ca:], sb) == rb.tobytes()
+    rc, t, l, b, ph = O.unpack_chunk(img, NMAT)
+    assert rc == 0 and ph == -7
+    assert np.array_equal(t, tiles) and np.array_equal(l, l2) and np.array_equal(b, bg)
+
+
+def crafted_blocks():
+    """Block contents that exercise the encoder's parse."""
+    rng = np.random.default_rng(5)
+    out = []
+    for rec, n in ((12, O.PACK_NA), (4, O.PACK_NB)):
+        row = 128 * rec
+        out.append((rec, np.zeros(n, np.uint8)))                         # one long match per row
+        out.append((rec, rng.integers(0, 256, n).astype(np.uint8)))      # incompressible
+        a = rng.integers(0, 256, n).astype(np.uint8)                     # literal runs carried over rows
+        a[5 * row:6 * row] = 0
+        out.append((rec, a))
+        b = np.zeros(n, np.uint8)                                        # runs of exactly 3 and 4 equal bytes
+        for p in range(rec, n - 16, 37):
+            b[p:p + 3 + (p & 1)] = b[p - rec:p - rec + 3 + (p & 1)]
+            b[p + 8] = (p * 7) & 255
+        out.append((rec, b))
+        c = np.tile(rng.integers(0, 256, row).astype(np.uint8), n // row)  # every row as the one above
+        c[::row + 1] ^= 0x5A
+        out.append((rec, c))
+    return out
+
+
+def test_encoder_parse_round_trips():
+    for rec, raw in crafted_blocks():
+        z = O.lz4_encode_rows(raw, rec)
+        assert len(z) <= O.lz4_bound(len(raw))
+        assert O.lz4_decode(z, len(raw)) == raw.tobytes()
+    # a chunk of one repeated cell needs only the row matches and the tail
+    z = O.lz4_encode_rows(np.zeros(O.PACK_NA, np.uint8), 12)
+    assert len(z) < 3000
+
+
+@needs_ref
+def test_reference_lz4_reads_encoder_blocks():
+    """The reference's LZ4_decompress_safe (as chunk.cpp:151, 191 call it)
+    decodes every block of the library's parse to the records."""
+    cases = list(crafted_blocks())
+    for tiles in world_chunks(2, 512, 256, 9) + world_chunks(3, 256, 256, 1):
+        ra, rb = O.chunk_records(tiles, packcases.layer2_pattern(128, 128, 3), packcases.background_pattern(128, 128, 4))
+        cases += [(12, ra), (4, rb)]
+    for rec, raw in cases:
+        z = O.lz4_encode_rows(raw, rec)
+        r, out = lz4ref.decompress_safe(z, len(raw))
+        assert r == len(raw) and out == raw.tobytes()
+
+
+@needs_ref
+def test_oracle_decoder_reads_reference_compressor():
+    for tiles in world_chunks(2, 512, 256, 13):
+        ra, rb = O.chunk_records(tiles, None, packcases.background_pattern(128, 128, 6))
+        for accel in (1, 10):
+            assert O.lz4_decode(lz4ref.compress_fast(ra, accel), len(ra)) == ra.tobytes()
+            assert O.lz4_decode(lz4ref.compress_fast(rb, accel), len(rb)) == rb.tobytes()
+
+
+def corrupt_cases(blocks, n, seed):
+    """(block, capacity) pairs: byte and bit flips, truncations, wrong capacities."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        src, cap = blocks[k % len(blocks)]
+        b = bytearray(src)
+        kind = k % 5
+        if kind == 0:
+            for _ in range(int(rng.integers(1, 4))):
+                b[int(rng.integers(0, len(b)))] = int(rng.integers(0, 256))
+        elif kind == 1:
+            b = b[:int(rng.integers(0, len(b)))]
+        elif kind == 2:
+            b[int(rng.integers(0, len(b)))] ^= 1 << int(rng.integers(0, 8))
+        elif kind == 3:
+            cap = int(rng.integers(max(0, cap - 40), cap + 40))
+        else:
+            b = bytearray(rng.integers(0, 256, int(rng.integers(1, 300))).astype(np.uint8).tobytes())
+        out.append((bytes(b), cap))
+    return out
+
+
+def sample_blocks(use_ref):
+    blocks = []
+    for tiles in world_chunks(2, 256, 256, 5):
+        ra, rb = O.chunk_records(tiles, None, packcases.background_pattern(128, 128, 1))
+        blocks += [(O.lz4_encode_rows(ra, 12), O.PACK_NA), (O.lz4_encode_rows(rb, 4), O.PACK_NB)]
+        if use_ref:
+            blocks += [(lz4ref.compress_fast(ra, 10), O.PACK_NA), (lz4ref.compress_fast(rb, 1), O.PACK_NB)]
+    rng = np.random.default_rng(8)
+    for _ in range(16):
+        n = int(rng.integers(13, 3000))
+        raw = rng.integers(0, 4, n).astype(np.uint8)
+        blocks.append((O.lz4_encode_rows(np.resize(raw, 128 * 4 * (n // 512 + 1)), 4), 128 * 4 * (n // 512 + 1)))
+    for name, (img, *_rest) in load_golden().items():
+        _, _, ca, _, cb = struct.unpack("<biiii", img[:17])
+        blocks += [(img[17:17 + ca], O.PACK_NA), (img[17 + ca:], O.PACK_NB)]
+    return blocks
+
+
+@needs_ref
+def test_oracle_decoder_agrees_with_reference_on_corrupt_blocks():
+    """Accept (decoded to the full capacity) or reject as LZ4_decompress_safe
+    does, with the same bytes when accepted -- offset 0 included."""
+    for src, cap in corrupt_cases(sample_blocks(True), 3000, 11):
+        r, out = lz4ref.decompress_safe(src, cap)
+        mine = O.lz4_decode(src, cap)
+        assert (r == cap) == (mine is not None), (len(src), cap, r)
+        if mine is not None:
+            assert mine == out
+
+
+def host_decoder():
+    path = os.path.join(HERE, "cpp", "build", "liblz4host.so")
+    if not os.path.exists(path):
+        from falling_sand_engine_amd import build as fb
+        fb.build_cpp_tests(verbose=False)
+    L = C.CDLL(path)
+    L.fsg_test_lz4_decode.restype = C.c_int
+    L.fsg_test_lz4_decode.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    return L
+
+
+def test_library_decoder_matches_oracle():
+    """The decode kernel's code (fsg_lz4.hpp, 64 lanes taken in turn) gives the
+    oracle's result on good and corrupt blocks, writing nothing past the
+    capacity (guard bytes after it stay intact)."""
+    L = host_decoder()
+    blocks = sample_blocks(lz4ref.lib() is not None)
+    for src, cap in blocks + corrupt_cases(blocks, 2500, 12):
+        s = np.frombuffer(src, np.uint8) if src else np.zeros(1, np.uint8)
+        out = np.full(cap + 64, 0xA5, np.uint8)
+        r = L.fsg_test_lz4_decode(s.ctypes.data, len(src), out.ctypes.data, cap)
+        mine = out[:cap].tobytes() if r == cap else None
+        assert mine == O.lz4_decode(src, cap), (len(src), cap, r)
+        assert (out[cap:] == 0xA5).all()
+
+
+def test_unpack_rejects_bad_images():
+    tiles = world_chunks(2, 256, 128, 4)[0]
+    ra, rb = O.chunk_records(tiles)
+    img = bytearray(O.pack_chunk(ra, rb, 1))
+    _, _, ca, _, cb = struct.unpack("<biiii", bytes(img[:17]))
+    bad = []
+    b = bytearray(img)
+    b[1:5] = struct.pack("<i", 393215)  # src_size not the chunk's (chunk.cpp:118-119)
+    bad.append(bytes(b))
+    b = bytearray(img)
+    b[9:13] = struct.pack("<i", 1)      # src_size2 (chunk.cpp:132-133)
+    bad.append(bytes(b))
+    bad.append(bytes(img[:-1]))         # shorter than its sizes
+    bad.append(bytes(img[:10]))         # no header
+    b = bytearray(img)
+    b[5:9] = struct.pack("<i", ca - 1)  # block A one byte short
+    bad.append(bytes(b))
+    # a bad material index: an image of records with index 9 (8 materials)
+    t2 = tiles.copy()
+    t2[3, 3] = cell(9, 0, 0)
+    r2a, r2b = O.chunk_records(t2)
+    bad.append(O.pack_chunk(r2a, r2b, 0))
+    for x in bad:
+        rc, t, l, bgo, _ = O.unpack_chunk(x, NMAT)
+        assert rc == _abi.FSG_ERR_INVALID
+        assert not t.any()
