@@ -296,6 +296,83 @@ def bench_islands(fsg, side=4096, nseeds=4096, seed=9):
             "bodies": st.count(3), "removed_small": st.count(2), "attached": st.count(1), "none": st.count(0)}
 
 
+PACK_RAW = 393216 + 65536  # .pack payload per chunk: tiles + layer2 records, background
+
+
+def bench_pack(fsg, side=16384, kind=2, seed=41, reps=3):
+    """Chunk .pack images of every 128x128 chunk of a side^2 basin world
+    (noisy terrain colours, water, lava and oil discs; no layer2 or
+    background, as the benchmark worlds have none): device time of
+    fsg_world_pack (encode, offsets, assembly, the size copy) and of
+    fsg_world_unpack from the device-resident images (decode, checks, cell
+    writes, the status copy); the round trip is checked by the grid checksum."""
+    import numpy as np
+    n = (side // 128) ** 2
+    sizes = np.zeros(n, dtype=np.uintp)
+    status = np.zeros(n, dtype=np.int32)
+    L = fsg.lib()
+    with fsg.Context(0) as ctx, fsg.World(ctx, side, side, seed, fsg.default_config()) as w:
+        w.generate(kind, seed)
+        before = w.stats().checksum
+        pms, ums = [], []
+        for _ in range(reps + 1):
+            ctx.timer_start()
+            fsg.check(L.fsg_world_pack(w.handle, 0, 0, side // 128, side // 128, None, None, None,
+                                       sizes.ctypes.data), "fsg_world_pack")
+            pms.append(ctx.timer_stop())
+            ctx.timer_start()
+            fsg.check(L.fsg_world_unpack(w.handle, 0, 0, side // 128, side // 128, None, None, None, None, None,
+                                         status.ctypes.data), "fsg_world_unpack")
+            ums.append(ctx.timer_stop())
+        ok = bool((status == 0).all()) and w.stats().checksum == before
+    pm, um = float(np.median(pms[1:])), float(np.median(ums[1:]))
+    total = int(sizes.sum())
+    return {"workload": f"{side}x{side} basin world (kind {kind}), {n} chunks of 128x128, "
+                        "no layer2/background",
+            "pack_ms": round(pm, 3), "unpack_ms": round(um, 3),
+            "pack_chunks_per_s": round(n / (pm / 1e3)), "unpack_chunks_per_s": round(n / (um / 1e3)),
+            "pack_raw_GBs": round(n * PACK_RAW / (pm / 1e3) / 1e9, 1),
+            "unpack_raw_GBs": round(n * PACK_RAW / (um / 1e3) / 1e9, 1),
+            "image_bytes": total, "ratio": round(n * PACK_RAW / total, 3), "round_trip_exact": ok}
+
+
+def cpu_pack_baseline(side=2048, kind=2, seed=41, nchunks=256):
+    """ChunkWrite / ChunkRead on the host, 1 thread, on every chunk of a
+    smaller world of the same kind (the same shares of terrain, water and
+    air as the device workload): the records loop (chunk.cpp:238-242, the oracle's
+    restatement) plus the reference's own LZ4_compress_fast(..., 10) and
+    LZ4_decompress_safe (oracle/_ref, built from the reference's lz4.c)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import lz4ref
+    import falling_sand_engine_amd as fsg
+    from oracle import pyoracle
+    if lz4ref.lib() is None:
+        return None
+    cells = fsg.generate(kind, side, side, seed)
+    per = side // 128
+    chunks = [np.ascontiguousarray(cells[(k // per) * 128:(k // per + 1) * 128, (k % per) * 128:(k % per + 1) * 128])
+              for k in range(nchunks)]
+    t0 = time.perf_counter()
+    imgs = []
+    for t in chunks:
+        ra, rb = pyoracle.chunk_records(t)
+        imgs.append((lz4ref.compress_fast(ra, 10), lz4ref.compress_fast(rb, 10)))
+    dp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for za, zb in imgs:
+        ra = lz4ref.decompress_safe(za, 393216)
+        rb = lz4ref.decompress_safe(zb, 65536)
+        assert ra[0] == 393216 and rb[0] == 65536
+    du = time.perf_counter() - t0
+    zbytes = sum(len(a) + len(b) + 17 for a, b in imgs)
+    return {"pack_ref_1_thread_chunks_per_s": round(nchunks / dp, 1),
+            "unpack_ref_1_thread_chunks_per_s": round(nchunks / du, 1),
+            "pack_ref_ratio": round(nchunks * PACK_RAW / zbytes, 3),
+            "pack_sample": f"{nchunks} chunks of a {side}x{side} basin world (kind {kind}); records loop "
+                           "restated, LZ4 from the reference's lz4.c; decode excludes the cell copy"}
+
+
 def cpu_baseline(threads=None, side_w=8192, side_h=4096, ticks=64):
     """refseq (the reference's sequential chunk-checkerboard tick, restated)
     on a bounded sample of the all-active world, on the host's cores; also
@@ -338,6 +415,7 @@ def cpu_baseline(threads=None, side_w=8192, side_h=4096, ticks=64):
     t0 = time.perf_counter()
     pyoracle.physics_check(frag, pyoracle.default_config(), iseeds)
     dti = time.perf_counter() - t0
+    packs = cpu_pack_baseline() or {}
     return {"value": side_w * side_h * ticks / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"refseq restatement of world::tick, all-active striped world {side_w}x{side_h}, "
@@ -350,7 +428,8 @@ def cpu_baseline(threads=None, side_w=8192, side_h=4096, ticks=64):
             "render_sample": "texture pass, three layers + census, 4096x4096 striped world, 1 thread",
             "islands_oracle_1_thread_seeds_per_s": round(64 / dti, 1),
             "islands_sample": "physicsCheck restated, 64 seeds one at a time, 4096x4096 fragment world; "
-                              "the oracle makes two passes (a dry run sizes the body images)"}
+                              "the oracle makes two passes (a dry run sizes the body images)",
+            **packs}
 
 
 def main():
@@ -361,7 +440,7 @@ def main():
     ap.add_argument("--side", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
-    ap.add_argument("--mode", default="all", choices=["all", "c3", "copy", "c5", "render", "islands"],
+    ap.add_argument("--mode", default="all", choices=["all", "c3", "copy", "c5", "render", "islands", "pack"],
                     help="all: the JSON line; c3: only the timed tick loop (for profiling); "
                          "copy: the HBM copy calibration kernel; render: only the texture pass")
     ap.add_argument("--copy-bytes", type=int, default=2 << 30)
@@ -386,6 +465,12 @@ def main():
         return
     if args.mode == "islands":
         print(json.dumps(bench_islands(fsg)))
+        return
+    if args.mode == "pack":
+        r = bench_pack(fsg, side=args.side)
+        if not args.no_cpu:
+            r["cpu"] = cpu_pack_baseline()
+        print(json.dumps(r))
         return
     if args.mode == "render":
         print(json.dumps(bench_render(fsg, side=args.side, reps=max(1, args.steps))))
@@ -423,6 +508,9 @@ def main():
         log("[bench] island check")
         secondary["islands"] = bench_islands(fsg)
         log(f"[bench] {secondary['islands']}")
+        log("[bench] chunk .pack codec")
+        secondary["pack"] = bench_pack(fsg)
+        log(f"[bench] {secondary['pack']}")
         with fsg.Context(0) as ctx:
             cms = ctx.copy_timed(2 << 30, 10)
         secondary["copy_kernel_GBs"] = round(2 * (2 << 30) / (cms / 1e3) / 1e9, 1)

@@ -16,13 +16,13 @@
 //     before copying such a match onto itself (lz4.c:2328-2334);
 //   * length bytes are read only while the input limits of lz4.c's
 //     read_variable_length hold.
-// Every lane of the wave runs the same parse on wave-uniform values; the
-// literal and match copies are split over the lanes.  A match copies from
-// bytes before its own start only (its output is periodic with period
-// `offset` when offset < length), so no lane reads a byte written by the
-// same copy; bytes of earlier copies are visible to every lane of the wave
-// because one wave's vector memory operations go through one L1 in order.
-// The same code runs on the host for tests, with the lanes taken in turn.
+// Every lane of a wave runs the same parse on wave-uniform values; the
+// literal and match copies are split over the lanes.  The parse reaches
+// memory only through a Mem object (a lookahead of the block's next bytes,
+// single byte reads, literal and match copies), so the same parse runs on
+// the host for tests and in the decode kernel, whose Mem (Staged) keeps a
+// window of the block and the recent output in LDS and the lookahead in the
+// lanes' registers: one LDS read per token, its bytes read across lanes.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -57,62 +57,190 @@ __host__ __device__ inline void match_lane(uint8_t *d, int off, int len, int lan
     }
 }
 
-// One lane of a wave of 64.
-struct WaveLane {
-    int lane;
-    __host__ __device__ void copy(uint8_t *d, const uint8_t *s, int len) const { copy_lane(d, s, len, lane, 64); }
-    __host__ __device__ void match(uint8_t *d, int off, int len) const { match_lane(d, off, len, lane, 64); }
+// Block bytes and recent output staged in two small buffers (LDS in the
+// decode kernel).  The window holds IW block bytes starting at a 16-byte
+// aligned address; the ring holds output byte q at ring[q % RING], i.e. the
+// last RING bytes written.  Lanes is the lane policy: each(f) runs f for the
+// lanes this thread stands for (its own lane on the device, all 64 in turn
+// on the host), barrier() orders the lanes' buffer accesses, uniform(x)
+// marks a value that is the same in every lane (it then lives in a scalar
+// register on the device), lane_value(v, k, row) is lane k's v in every lane
+// and own(v, lane, row) this lane's v (row: the bytes the lanes hold, which
+// the host reads instead of the lanes' registers); rcp(x) is 1 / x to within
+// an ulp or so (mod_by corrects its quotient).
+constexpr int IW = 1024, RING = 4096, NL = 64;
+
+// j mod off for 0 <= j < 2^20 and off >= 1, with inv = 1 / off: the float
+// quotient is off by at most one, which the two corrections fix.
+__host__ __device__ inline int mod_by(int j, int off, float inv) {
+    const int q = (int)((float)j * inv);
+    int r = j - q * off;
+    if (r < 0) r += off;
+    else if (r >= off) r -= off;
+    return r;
+}
+
+template <class Lanes>
+struct Staged {
+    const uint8_t *src;  // the block (n bytes)
+    int n;
+    uint8_t *dst;        // the output
+    uint8_t *win;        // IW bytes, 16-byte aligned
+    uint8_t *ring;       // RING bytes
+    int wbase;           // the window holds block bytes [wbase, wbase + IW)
+    Lanes L;
+
+    uint32_t ahead;      // this lane's byte of the lookahead: block byte at0 + lane
+    int at0;
+
+    __host__ __device__ uint32_t byte(int i) {
+        if (i < wbase || i >= wbase + IW) refill(i);
+        return L.uniform(win[i - wbase]);
+    }
+    // The NL block bytes from i on, one per lane (0 past the block's end).
+    __host__ __device__ void look(int i) {
+        if (i < wbase || i + NL > wbase + IW) refill(i);
+        at0 = i;
+        L.each([&](int lane) { ahead = win[i - wbase + lane]; });
+    }
+    // Byte at0 + k (k < NL) of the lookahead, in every lane.
+    __host__ __device__ uint32_t at(int k) { return L.lane_value(ahead, k, win + (at0 - wbase)); }
+    // Reloads the window to start at or just below block byte i: lane k
+    // loads 16 bytes (NL * 16 = IW); bytes outside the block read as 0.
+    __host__ __device__ void refill(int i) {
+        L.barrier();
+        wbase = i - (int)((uintptr_t)(src + i) & 15);
+        L.each([&](int lane) {
+            const int a = wbase + 16 * lane;
+            if (a >= 0 && a + 16 <= n) {
+                *reinterpret_cast<uint4 *>(win + 16 * lane) = *reinterpret_cast<const uint4 *>(src + a);
+            } else {
+                for (int k = 0; k < 16; k++) win[16 * lane + k] = (a + k >= 0 && a + k < n) ? src[a + k] : 0;
+            }
+        });
+        L.barrier();
+    }
+    __host__ __device__ void put(int q, uint8_t b) {
+        dst[q] = b;
+        ring[q & (RING - 1)] = b;
+    }
+    // Literals: from the window when the run lies in it, else from the block.
+    // Output byte q goes to ring slot q % RING; the slots of one lane are
+    // q = lane (mod NL), so a long run's later bytes win in each slot.
+    // k: the literals' index in the lookahead; a run inside it is stored
+    // straight from the lanes that hold it.
+    __host__ __device__ void lit(int op, int ip, int len, int k) {
+        if (k + len <= NL) {
+            L.each([&](int lane) {
+                if (lane >= k && lane < k + len) put(op + lane - k, (uint8_t)L.own(ahead, lane, win + (at0 - wbase)));
+            });
+            return;
+        }
+        const bool inwin = ip >= wbase && ip + len <= wbase + IW;
+        L.each([&](int lane) {
+            for (int j = lane; j < len; j += NL) put(op + j, inwin ? win[ip + j - wbase] : src[ip + j]);
+        });
+    }
+    // Match of len bytes at distance off.  Offsets up to RING - NL read the
+    // ring: byte j < P comes from [op - off, op) (j mod off), byte j >= P from
+    // op + j - P, P the smallest multiple of off that is >= NL, so every
+    // source byte was written before the current group of NL bytes and no
+    // group overwrites a slot it still reads.  Larger offsets read the output
+    // itself (off > NL: the source bytes are from earlier groups).
+    __host__ __device__ void match(int op, int off, int len) {
+        L.barrier();
+        if (off == 0) {
+            L.each([&](int lane) {
+                for (int j = lane; j < len; j += NL) put(op + j, 0);
+            });
+            return;
+        }
+        if (off >= NL) {  // byte j comes from op + j - off, written before j's group
+            const bool near = off <= RING - NL;
+            for (int g = 0; g < len; g += NL) {
+                L.each([&](int lane) {
+                    const int j = g + lane;
+                    if (j < len) put(op + j, near ? ring[(op + j - off) & (RING - 1)] : dst[op + j - off]);
+                });
+                if (g + NL < len) L.barrier();
+            }
+            return;
+        }
+        const float inv = L.rcp((float)off);
+        int P = off * ((int)((float)(NL - 1) * inv) + 1);  // the smallest multiple of off that is >= NL
+        if (P < NL) P += off;
+        else if (P - off >= NL) P -= off;
+        for (int g = 0; g < len; g += NL) {
+            L.each([&](int lane) {
+                const int j = g + lane;
+                if (j < len) put(op + j, ring[(j < P ? op - off + mod_by(j, off, inv) : op + j - P) & (RING - 1)]);
+            });
+            if (g + NL < len) L.barrier();
+        }
+    }
 };
 
-// Decodes the LZ4 block src[0, n) into dst[0, cap).  Returns the decoded
-// size, or -1 for a block that lz4.c's rules reject.  Reads only src[0, n),
-// writes only dst[0, cap).
-template <class Lanes>
-__host__ __device__ inline int decode_block(const uint8_t *src, int n, uint8_t *dst, int cap, const Lanes &L) {
+// Decodes the LZ4 block of n bytes into an output of cap bytes.  Each token
+// starts with M.look(ip), which makes the next NL block bytes readable as
+// M.at(k); length bytes and offsets beyond the lookahead come from M.byte(i).
+// Literals go through M.lit, matches through M.match.  Returns the decoded
+// size, or -1 for a block that lz4.c's rules reject.  Reads only block bytes
+// [0, n) and writes only output bytes [0, cap).
+template <class Mem>
+__host__ __device__ inline int decode_block(int n, int cap, Mem &M) {
     if (n <= 0 || cap < 0) return -1;
-    if (cap == 0) return (n == 1 && src[0] == 0) ? 0 : -1;
+    if (cap == 0) return (n == 1 && M.byte(0) == 0) ? 0 : -1;
     int ip = 0, op = 0;
     for (;;) {
         if (ip >= n) return -1;
-        const int token = src[ip++];
+        M.look(ip);
+        auto next = [&](int &k) -> int {  // the byte at ip + k, then k + 1
+            const int v = (int)(k < NL ? M.at(k) : M.byte(ip + k));
+            k++;
+            return v;
+        };
+        int k = 0;
+        const int token = next(k);
         int ll = token >> 4;
         if (ll == RUN_MASK) {  // read_variable_length(ip, iend - RUN_MASK, initial check)
             const int lim = n - RUN_MASK;
-            if (ip >= lim) return -1;
+            if (ip + k >= lim) return -1;
             int s;
             do {
-                s = src[ip++];
+                s = next(k);
                 ll += s;
-                if (ip > lim || ll > cap) return -1;
+                if (ip + k > lim || ll > cap) return -1;
             } while (s == 255);
         }
+        const int lp = ip + k;  // first literal
         const int cpy = op + ll;
-        if (cpy > cap - MFLIMIT || ip + ll > n - (2 + 1 + LASTLITERALS)) {
+        if (cpy > cap - MFLIMIT || lp + ll > n - (2 + 1 + LASTLITERALS)) {
             // the last sequence: it must end the input exactly and fit
-            if (ip + ll != n || cpy > cap) return -1;
-            L.copy(dst + op, src + ip, ll);
+            if (lp + ll != n || cpy > cap) return -1;
+            M.lit(op, lp, ll, k);
             return cpy;
         }
-        L.copy(dst + op, src + ip, ll);
-        ip += ll;
+        M.lit(op, lp, ll, k);
         op = cpy;
-        const int off = src[ip] | (src[ip + 1] << 8);  // ip + 2 <= n - 6 here
-        ip += 2;
+        k += ll;  // lp + ll + 2 <= n - 6 here
+        const int lo = next(k);
+        const int off = lo | (next(k) << 8);
         int ml = token & 15;
         if (ml == 15) {  // read_variable_length(ip, iend - LASTLITERALS + 1, no initial check)
             const int lim = n - LASTLITERALS + 1;
             int s;
             do {
-                s = src[ip++];
+                s = next(k);
                 ml += s;
-                if (ip > lim || ml > cap) return -1;
+                if (ip + k > lim || ml > cap) return -1;
             } while (s == 255);
         }
         ml += MINMATCH;
         if (off > op) return -1;
         if (op + ml > cap - LASTLITERALS) return -1;
-        L.match(dst + op, off, ml);
+        M.match(op, off, ml);
         op += ml;
+        ip += k;
     }
 }
 

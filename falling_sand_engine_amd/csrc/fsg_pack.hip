@@ -365,16 +365,45 @@ struct DecDesc {
     int32_t n, ok;
 };
 
-// One wave per block (x = chunk * 2 + block); status 0 when the block decoded
-// to its full size.
+// The lane policy of the decode kernel: this thread's own lane.  A decode
+// workgroup is one wave, whose LDS accesses execute in order, so ordering the
+// lanes' window and ring accesses needs only wavefront-scope fences around a
+// wave barrier (which keeps the compiler from moving accesses across it);
+// __syncthreads would also wait for every outstanding output store.
+struct WaveLane {
+    int lane;
+    template <class F>
+    __device__ void each(F f) const {
+        f(lane);
+    }
+    __device__ void barrier() const {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    __device__ uint32_t uniform(uint32_t x) const { return __builtin_amdgcn_readfirstlane(x); }
+    __device__ uint32_t lane_value(uint32_t v, int k, const uint8_t *) const {
+        return (uint32_t)__builtin_amdgcn_readlane((int)v, k);
+    }
+    __device__ uint32_t own(uint32_t v, int, const uint8_t *) const { return v; }
+    __device__ float rcp(float x) const { return __builtin_amdgcn_rcpf(x); }
+};
+
+// One wave per block (x = chunk * 2 + block), the block and the recent
+// output staged in LDS; status 0 when the block decoded to its full size.
 __global__ __launch_bounds__(64) void k_pack_decode(const uint8_t *in, const DecDesc *desc, uint8_t *raw,
                                                     int32_t *status) {
+    __shared__ __align__(16) uint8_t win[fsg_lz4::IW];
+    __shared__ uint8_t ring[fsg_lz4::RING];
     const int b = blockIdx.x, c = b >> 1, which = b & 1;
     const DecDesc d = desc[b];
     const int cap = which ? NB : NA;
     int r = -1;
-    if (d.ok) r = fsg_lz4::decode_block(in + d.off, d.n, raw + (size_t)c * RAW + (which ? NA : 0), cap,
-                                        fsg_lz4::WaveLane{(int)threadIdx.x});
+    if (d.ok) {
+        fsg_lz4::Staged<WaveLane> M{in + d.off, d.n, raw + (size_t)c * RAW + (which ? NA : 0), win, ring,
+                                    -(1 << 30), WaveLane{(int)threadIdx.x}, 0u, 0};
+        r = fsg_lz4::decode_block(d.n, cap, M);
+    }
     if (threadIdx.x == 0) status[b] = r == cap ? 0 : -1;
 }
 

@@ -1,21 +1,54 @@
 // Host build of the library's LZ4 block decoder (falling_sand_engine_amd/
-// csrc/fsg_lz4.hpp) for CPU tests: the same parse and per-lane copy code the
-// decode kernel runs, with the 64 lanes of a wave taken one after another in
-// each copy (a copy reads only bytes written before it starts, so the order
-// of lanes within it does not matter).
+// csrc/fsg_lz4.hpp) for CPU tests: the parse the decode kernel runs, with
+//   * fsg_test_lz4_decode: plain memory, each copy running the 64 lanes'
+//     shares one after another;
+//   * fsg_test_lz4_decode_staged: the kernel's staged access (block window
+//     and output ring, LDS on the device) with the 64 lanes taken in turn
+//     inside each step, as the device's lanes take each step together.
 #include "fsg_lz4.hpp"
 
 namespace {
-struct HostWave {
-    void copy(uint8_t *d, const uint8_t *s, int len) const {
-        for (int lane = 0; lane < 64; lane++) fsg_lz4::copy_lane(d, s, len, lane, 64);
+
+// Lane policy of the host build: all 64 lanes in turn.
+struct HostLanes {
+    template <class F>
+    void each(F f) const {
+        for (int lane = 0; lane < fsg_lz4::NL; lane++) f(lane);
     }
-    void match(uint8_t *d, int off, int len) const {
-        for (int lane = 0; lane < 64; lane++) fsg_lz4::match_lane(d, off, len, lane, 64);
+    void barrier() const {}
+    uint32_t uniform(uint32_t x) const { return x; }
+    // the lanes' registers are the staged row on the host
+    uint32_t lane_value(uint32_t, int k, const uint8_t *row) const { return row[k]; }
+    uint32_t own(uint32_t, int lane, const uint8_t *row) const { return row[lane]; }
+    float rcp(float x) const { return 1.0f / x; }
+};
+
+struct AllLanes {
+    const uint8_t *src;
+    int n;
+    uint8_t *dst;
+    int at0 = 0;
+    uint32_t byte(int i) { return src[i]; }
+    void look(int i) { at0 = i; }
+    uint32_t at(int k) { return at0 + k < n ? src[at0 + k] : 0u; }
+    void lit(int op, int ip, int len, int) {
+        for (int l = 0; l < fsg_lz4::NL; l++) fsg_lz4::copy_lane(dst + op, src + ip, len, l, fsg_lz4::NL);
+    }
+    void match(int op, int off, int len) {
+        for (int l = 0; l < fsg_lz4::NL; l++) fsg_lz4::match_lane(dst + op, off, len, l, fsg_lz4::NL);
     }
 };
+
 }  // namespace
 
 extern "C" int fsg_test_lz4_decode(const uint8_t *src, int n, uint8_t *dst, int cap) {
-    return fsg_lz4::decode_block(src, n, dst, cap, HostWave{});
+    AllLanes m{src, n, dst};
+    return fsg_lz4::decode_block(n, cap, m);
+}
+
+extern "C" int fsg_test_lz4_decode_staged(const uint8_t *src, int n, uint8_t *dst, int cap) {
+    alignas(16) static uint8_t win[fsg_lz4::IW];
+    static uint8_t ring[fsg_lz4::RING];
+    fsg_lz4::Staged<HostLanes> m{src, n, dst, win, ring, -(1 << 30), HostLanes{}, 0u, 0};
+    return fsg_lz4::decode_block(n, cap, m);
 }

@@ -222,24 +222,57 @@ def host_decoder():
         from falling_sand_engine_amd import build as fb
         fb.build_cpp_tests(verbose=False)
     L = C.CDLL(path)
-    L.fsg_test_lz4_decode.restype = C.c_int
-    L.fsg_test_lz4_decode.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    for f in (L.fsg_test_lz4_decode, L.fsg_test_lz4_decode_staged):
+        f.restype = C.c_int
+        f.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
     return L
 
 
+def lz4_seq(lit: bytes, off=None, ml=0) -> bytes:
+    """One LZ4 sequence (the last one when off is None)."""
+    def ext(v):
+        out = bytearray()
+        if v >= 15:
+            v -= 15
+            while v >= 255:
+                out.append(255)
+                v -= 255
+            out.append(v)
+        return bytes(out)
+    m = 0 if off is None else ml - 4
+    b = bytes([(min(len(lit), 15) << 4) | min(m, 15)]) + ext(len(lit)) + lit
+    return b if off is None else b + bytes([off & 255, off >> 8]) + ext(m)
+
+
+def far_match_blocks():
+    """Hand-made blocks whose matches reach past the kernel's output ring
+    (offsets up to 65535) and overlap themselves at small offsets."""
+    rng = np.random.default_rng(21)
+    out = []
+    for off in (1, 3, 12, 63, 64, 65, 1536, 2047, 2048, 2049, 4031, 4032, 4033, 4095, 4096, 5000, 40000, 65535):
+        lit = rng.integers(0, 256, off).astype(np.uint8).tobytes()
+        ml = 300 + off % 7
+        tail = rng.integers(0, 256, 12).astype(np.uint8).tobytes()
+        blk = lz4_seq(lit, off, ml) + lz4_seq(b"", 1, 4 + off % 5) + lz4_seq(tail)
+        out.append((blk, off + ml + 4 + off % 5 + 12))
+    return out
+
+
 def test_library_decoder_matches_oracle():
-    """The decode kernel's code (fsg_lz4.hpp, 64 lanes taken in turn) gives the
+    """The decode kernel's parse (fsg_lz4.hpp) over plain memory and over the
+    kernel's staged window and output ring, 64 lanes taken in turn, gives the
     oracle's result on good and corrupt blocks, writing nothing past the
     capacity (guard bytes after it stay intact)."""
     L = host_decoder()
-    blocks = sample_blocks(lz4ref.lib() is not None)
+    blocks = sample_blocks(lz4ref.lib() is not None) + far_match_blocks()
     for src, cap in blocks + corrupt_cases(blocks, 2500, 12):
         s = np.frombuffer(src, np.uint8) if src else np.zeros(1, np.uint8)
-        out = np.full(cap + 64, 0xA5, np.uint8)
-        r = L.fsg_test_lz4_decode(s.ctypes.data, len(src), out.ctypes.data, cap)
-        mine = out[:cap].tobytes() if r == cap else None
-        assert mine == O.lz4_decode(src, cap), (len(src), cap, r)
-        assert (out[cap:] == 0xA5).all()
+        want = O.lz4_decode(src, cap)
+        for f in (L.fsg_test_lz4_decode, L.fsg_test_lz4_decode_staged):
+            out = np.full(cap + 64, 0xA5, np.uint8)
+            r = f(s.ctypes.data, len(src), out.ctypes.data, cap)
+            assert (out[:cap].tobytes() if r == cap else None) == want, (f.__name__, len(src), cap, r)
+            assert (out[cap:] == 0xA5).all()
 
 
 def test_unpack_rejects_bad_images():
