@@ -10,6 +10,8 @@
 //   world.stampBodies(...)            the stamp loop of game.cpp:1760-1815
 //   world.unstampBodies(...)          the unstamp loop of game.cpp:1896-1983
 //   world.updateTextures(...)         the texture pass of game.cpp:1994-2067
+//   world.writeChunks(...)            Chunk::ChunkWrite (chunk.cpp:219-330) per chunk
+//   world.readChunks(...)             Chunk::ChunkRead (chunk.cpp:74-217) per chunk
 //
 // Error behaviour follows the reference: getTile out of bounds returns the
 // solid wall cell, setTile out of bounds is ignored.  Device failures, which
@@ -144,6 +146,52 @@ class world {
         std::vector<uint32_t> out((size_t)w * h);
         check(fsg_world_layer_download(w_, which, x0, y0, w, h, out.data()), "fsg_world_layer_download");
         return out;
+    }
+
+    // Chunk::ChunkWrite for the ncx x ncy chunks whose top-left cell is
+    // (x0, y0): one .pack file image per chunk, chunk rows in order.  layer2
+    // and background are region-shaped (ncx*128 wide) or null; phase holds
+    // one generation phase per chunk or is null.
+    std::vector<std::vector<uint8_t>> writeChunks(int x0, int y0, int ncx, int ncy, const int8_t *phase = nullptr,
+                                                  const MaterialInstance *layer2 = nullptr,
+                                                  const uint32_t *background = nullptr) {
+        flush();
+        std::vector<size_t> sizes((size_t)ncx * ncy);
+        check(fsg_world_pack(w_, x0, y0, ncx, ncy, phase, layer2, background, sizes.data()), "fsg_world_pack");
+        size_t total = 0;
+        for (size_t v : sizes) total += v;
+        std::vector<uint8_t> all(total);
+        check(fsg_world_pack_download(w_, all.data(), all.size()), "fsg_world_pack_download");
+        std::vector<std::vector<uint8_t>> out;
+        size_t o = 0;
+        for (size_t v : sizes) {
+            out.emplace_back(all.begin() + o, all.begin() + o + v);
+            o += v;
+        }
+        return out;
+    }
+
+    // Chunk::ChunkRead of one image per chunk of the region; returns each
+    // chunk's status (FSG_OK, or FSG_ERR_INVALID for a corrupt image, whose
+    // cells are left as they were).
+    std::vector<int32_t> readChunks(int x0, int y0, int ncx, int ncy, const std::vector<std::vector<uint8_t>> &images,
+                                    int8_t *phase = nullptr, MaterialInstance *layer2 = nullptr,
+                                    uint32_t *background = nullptr) {
+        flush();
+        const size_t n = (size_t)ncx * ncy;
+        if (images.size() != n) check(FSG_ERR_INVALID, "readChunks: one image per chunk");
+        std::vector<size_t> sizes;
+        std::vector<uint8_t> all;
+        for (const auto &im : images) {
+            sizes.push_back(im.size());
+            all.insert(all.end(), im.begin(), im.end());
+        }
+        all.push_back(0);  // a valid pointer when every image is empty
+        std::vector<int32_t> status(n);
+        check(fsg_world_unpack(w_, x0, y0, ncx, ncy, all.data(), sizes.data(), phase, layer2, background,
+                               status.data()),
+              "fsg_world_unpack");
+        return status;
     }
 
     fsg_stats stats() {

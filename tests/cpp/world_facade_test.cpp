@@ -28,8 +28,15 @@ int main() {
         rp.fire_mat = 255;
         world.updateTextures(rp);
         const std::vector<uint32_t> texel = world.layer(FSG_LAYER_CELLS, 50, 50, 1, 1);
-        std::printf("{\"checksum\": %llu, \"wall\": %u, \"tile_50_50\": %llu, \"texel_50_50\": %u, \"counts\": [",
-                    (unsigned long long)s.checksum, (unsigned)wall.mat,
+        // chunk files: write the two 128x128 chunks of the top rows, read them back over other cells
+        const fsg_stats before_files = world.stats();
+        const std::vector<std::vector<uint8_t>> files = world.writeChunks(0, 0, 2, 1);
+        for (int x = 0; x < 128; x++) world.setTile(x, 3, sand);
+        const std::vector<int32_t> st = world.readChunks(0, 0, 2, 1, files);
+        const bool files_ok = st[0] == FSG_OK && st[1] == FSG_OK && world.stats().checksum == before_files.checksum;
+        std::printf("{\"checksum\": %llu, \"files_ok\": %d, \"wall\": %u, \"tile_50_50\": %llu, \"texel_50_50\": %u, "
+                    "\"counts\": [",
+                    (unsigned long long)s.checksum, files_ok ? 1 : 0, (unsigned)wall.mat,
                     (unsigned long long)(t.mat | ((uint64_t)(uint16_t)t.temp << 16) | ((uint64_t)t.rgba << 32)),
                     (unsigned)texel[0]);
         for (int i = 0; i < 8; i++) std::printf("%llu%s", (unsigned long long)s.count[i], i < 7 ? ", " : "");

@@ -146,6 +146,7 @@ def test_cpp_facade_program(orc):
     orc.run(cells, orc.default_config(), 7, 0, 10)
     assert got["checksum"] == orc.checksum(cells)
     assert got["wall"] == 15 and got["tile_50_50"] == int(cells[50, 50])
+    assert got["files_ok"] == 1  # writeChunks / readChunks round trip over edited cells
     assert got["counts"] == [int(v) for v in orc.counts(cells)[:8]]
     layers, _ = orc.render(cells, orc.default_config(), _cells_layer_params())
     assert got["texel_50_50"] == int(layers[1][50, 50])
