@@ -22,8 +22,9 @@
 // (material, temperature, alpha, padding) all become matches.  The last 12
 // bytes of a block are literals, as the format requires.
 //
-// Decoding: one wave per block (fsg_lz4.hpp), into a record staging area,
-// then one workgroup per chunk checks every record and writes the cells.
+// Decoding: one wave per block (fsg_lz4.hpp), four waves per workgroup, into
+// a record staging area; then one workgroup per chunk checks every record and
+// writes the cells.
 #include "fsg_internal.hpp"
 #include "fsg_lz4.hpp"
 
@@ -365,11 +366,12 @@ struct DecDesc {
     int32_t n, ok;
 };
 
-// The lane policy of the decode kernel: this thread's own lane.  A decode
-// workgroup is one wave, whose LDS accesses execute in order, so ordering the
-// lanes' window and ring accesses needs only wavefront-scope fences around a
-// wave barrier (which keeps the compiler from moving accesses across it);
-// __syncthreads would also wait for every outstanding output store.
+// The lane policy of the decode kernel: this thread's own lane.  A wave's
+// window and ring are its own and a wave's LDS accesses execute in order, so
+// ordering the lanes' accesses needs only wavefront-scope fences around a
+// wave barrier (which keeps the compiler from moving accesses across it); a
+// workgroup barrier would tie the workgroup's waves together and also wait
+// for every outstanding output store.
 struct WaveLane {
     int lane;
     template <class F>
@@ -389,22 +391,30 @@ struct WaveLane {
     __device__ float rcp(float x) const { return __builtin_amdgcn_rcpf(x); }
 };
 
-// One wave per block (x = chunk * 2 + block), the block and the recent
-// output staged in LDS; status 0 when the block decoded to its full size.
-__global__ __launch_bounds__(64) void k_pack_decode(const uint8_t *in, const DecDesc *desc, uint8_t *raw,
-                                                    int32_t *status) {
-    __shared__ __align__(16) uint8_t win[fsg_lz4::IW];
-    __shared__ uint8_t ring[fsg_lz4::RING];
-    const int b = blockIdx.x, c = b >> 1, which = b & 1;
+// One wave per block, four blocks per workgroup (a workgroup per wave would
+// leave CUs short of waves: their workgroup slots run out first); block
+// index = blockIdx.x * 4 + wave, chunk = index / 2.  Each wave keeps its own
+// block window and output ring in LDS and decodes alone (no workgroup
+// barrier); status 0 when the block decoded to its full size.
+constexpr int DW = 4;  // waves (blocks) per decode workgroup
+
+__global__ __launch_bounds__(64 * DW) __attribute__((amdgpu_waves_per_eu(6))) void k_pack_decode(
+    const uint8_t *in, const DecDesc *desc, uint8_t *raw, int32_t *status, int nblocks) {
+    __shared__ __align__(16) uint8_t win[DW][fsg_lz4::IW];
+    __shared__ uint8_t ring[DW][fsg_lz4::RING];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = blockIdx.x * DW + wv;
+    if (b >= nblocks) return;
+    const int c = b >> 1, which = b & 1;
     const DecDesc d = desc[b];
     const int cap = which ? NB : NA;
     int r = -1;
     if (d.ok) {
-        fsg_lz4::Staged<WaveLane> M{in + d.off, d.n, raw + (size_t)c * RAW + (which ? NA : 0), win, ring,
-                                    -(1 << 30), WaveLane{(int)threadIdx.x}, 0u, 0};
+        fsg_lz4::Staged<WaveLane> M{in + d.off, d.n, raw + (size_t)c * RAW + (which ? NA : 0), win[wv], ring[wv],
+                                    -(1 << 30), WaveLane{lane}, 0u, 0};
         r = fsg_lz4::decode_block(d.n, cap, M);
     }
-    if (threadIdx.x == 0) status[b] = r == cap ? 0 : -1;
+    if (lane == 0) status[b] = r == cap ? 0 : -1;
 }
 
 __device__ __forceinline__ uint32_t ld32(const uint8_t *p) {
@@ -621,7 +631,7 @@ extern "C" int fsg_world_unpack(fsg_world *w, int32_t x0, int32_t y0, int32_t nc
     HIPCHK(hipMemcpyAsync(d_desc, desc.data(), 2 * n * sizeof(DecDesc), hipMemcpyHostToDevice, c->stream));
     if (b_l2 + b_bg) HIPCHK(hipMemsetAsync(dec + b_desc + b_st, 0, b_l2 + b_bg, c->stream));  // failed chunks: 0
     uint8_t *raw = static_cast<uint8_t *>(w->d_pack_raw);
-    k_pack_decode<<<(unsigned)(2 * n), 64, 0, c->stream>>>(src, d_desc, raw, d_bst);
+    k_pack_decode<<<(unsigned)((2 * n + DW - 1) / DW), 64 * DW, 0, c->stream>>>(src, d_desc, raw, d_bst, (int)(2 * n));
     k_pack_cells<<<(unsigned)n, 256, 0, c->stream>>>(raw, d_bst, d, d_cst);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(status, d_cst, n * 4, hipMemcpyDeviceToHost, c->stream));
