@@ -399,10 +399,11 @@ extern "C" int fsg_world_create(fsg_ctx *c, int W, int H, uint64_t seed, const f
     const uint64_t air = conf.mat_air;
     fsg::k_fill<<<grid_for(cells, 256), 256, 0, c->stream>>>(w->d.buf[0], cells, air);
     fsg::k_fill<<<grid_for(cells, 256), 256, 0, c->stream>>>(w->d.buf[1], cells, air);
+    // no wait: everything later on this world is ordered behind these on the
+    // context's stream (a serving loop creates worlds while a step runs)
     if (hipMemsetAsync(w->d.flags[0], 0, (size_t)tx * ty * 4, c->stream) != hipSuccess ||
         hipMemsetAsync(w->d.flags[1], 0, (size_t)tx * ty * 4, c->stream) != hipSuccess ||
-        hipMemsetAsync(w->d_stats, 0, 18 * 8, c->stream) != hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess)
+        hipMemsetAsync(w->d_stats, 0, 18 * 8, c->stream) != hipSuccess || hipGetLastError() != hipSuccess)
         return fail(FSG_ERR_DEVICE);
     *out = w;
     return FSG_OK;
@@ -412,7 +413,15 @@ extern "C" int fsg_world_destroy(fsg_world *w) {
     if (!w) return FSG_ERR_INVALID;
     fsg_ctx *c = w->ctx;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    // The world's device blocks go back to the context's cache, whose reuse is
+    // ordered behind any pending work on the one stream, so destroying needs no
+    // wait -- except for a captured graph over this world, which may be running.
+    bool in_graph;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        in_graph = std::find(c->graph_key.begin(), c->graph_key.end(), w) != c->graph_key.end();
+    }
+    if (in_graph) (void)hipStreamSynchronize(c->stream);
     {
         std::lock_guard<std::mutex> lk(c->mu);
         c->used -= w->bytes;

@@ -411,7 +411,8 @@ int ensure_staging(fsg_server *S, int nworlds, size_t nbodies) {
     return FSG_OK;
 }
 
-// Advances every active request by T ticks.
+// Enqueues T ticks of every active request (stamp, batched tick, unstamp)
+// and steps the host body model for the next step; the caller synchronises.
 int run_step(fsg_server *S, int T) {
     std::vector<fsg_world *> ws;
     std::vector<Resident *> bw;  // worlds with bodies
@@ -444,10 +445,14 @@ int run_step(fsg_server *S, int T) {
         if (int rc = stamp_batch(S, bw, true)) return rc;
         for (Resident *r : bw) step_bodies(r->hb, r->w->d.W, r->w->d.H);
     }
-    HIPCHK(hipStreamSynchronize(c->stream));
     return FSG_OK;
 }
 
+// One loop iteration: enqueue a step of the active requests, admit new ones
+// while it runs (their worlds are created and generated on the same stream,
+// behind the step; admission never touches a world of the running step, which
+// is busy), then wait for the step and account for the requests that ran in
+// it -- the first n_run entries of `active`, since admission only appends.
 void worker(fsg_server *S) {
     for (;;) {
         std::vector<fsg_serve_result> out;
@@ -466,51 +471,63 @@ void worker(fsg_server *S) {
                 }
             }
             S->corruptions.clear();
-            admit(S, out);
-            if (S->active.empty() && out.empty()) {
-                S->cv.wait_for(lk, std::chrono::milliseconds(2));
-                continue;
+            if (S->active.empty()) {
+                admit(S, out);
+                if (S->active.empty() && out.empty()) {
+                    S->cv.wait_for(lk, std::chrono::milliseconds(2));
+                    continue;
+                }
             }
         }
         if (!S->active.empty()) {
             // cancelled or late active requests leave before the step
             const uint64_t now = fsg_now_ns();
-            {
-                std::lock_guard<std::mutex> lk(S->mu);
-                for (auto it = S->active.begin(); it != S->active.end();) {
-                    if (it->cancelled || (it->r.deadline_ns && now > it->r.deadline_ns)) {
-                        finish(S, *it, it->cancelled ? FSG_ERR_CANCELLED : FSG_ERR_TIMEOUT, out);
-                        it = S->active.erase(it);
-                    } else {
-                        ++it;
-                    }
+            std::lock_guard<std::mutex> lk(S->mu);
+            for (auto it = S->active.begin(); it != S->active.end();) {
+                if (it->cancelled || (it->r.deadline_ns && now > it->r.deadline_ns)) {
+                    finish(S, *it, it->cancelled ? FSG_ERR_CANCELLED : FSG_ERR_TIMEOUT, out);
+                    it = S->active.erase(it);
+                } else {
+                    ++it;
                 }
             }
         }
-        if (!S->active.empty()) {
+        const size_t n_run = S->active.size();
+        int T = 0, rc = FSG_OK;
+        if (n_run) {
             bool bodies = false;
-            int T = S->cfg.block_ticks > 0 ? S->cfg.block_ticks : 16;
+            T = S->cfg.block_ticks > 0 ? S->cfg.block_ticks : 16;
             for (Req &q : S->active) {
                 T = std::min(T, q.r.ticks - q.done_ticks);
                 bodies = bodies || !q.res->hb.empty();
             }
             if (bodies) T = 1;
-            const int rc = run_step(S, T);
+            rc = run_step(S, T);
+        }
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            admit(S, out);  // overlaps the step
+        }
+        if (n_run) {
+            // always wait: the pinned staging of this step is rewritten by the next one
+            if (hipStreamSynchronize(S->ctx->stream) != hipSuccess && rc == FSG_OK) rc = FSG_ERR_DEVICE;
             uint64_t cu = 0;
             std::lock_guard<std::mutex> lk(S->mu);
-            for (auto it = S->active.begin(); it != S->active.end();) {
+            size_t i = 0;
+            for (size_t k = 0; k < n_run; k++) {
+                Req &q = S->active[i];
                 if (rc != FSG_OK) {
-                    finish(S, *it, rc, out);
-                    it = S->active.erase(it);
+                    finish(S, q, rc, out);
+                    S->active.erase(S->active.begin() + (long)i);
                     continue;
                 }
-                it->done_ticks += T;
-                cu += (uint64_t)T * it->res->w->d.W * it->res->w->d.H;
-                if (it->done_ticks >= it->r.ticks) {
-                    finish(S, *it, FSG_OK, out);
-                    it = S->active.erase(it);
+                q.done_ticks += T;
+                cu += (uint64_t)T * q.res->w->d.W * q.res->w->d.H;
+                if (q.done_ticks >= q.r.ticks) {
+                    finish(S, q, FSG_OK, out);
+                    S->active.erase(S->active.begin() + (long)i);
                 } else {
-                    ++it;
+                    ++i;
                 }
             }
             S->st.steps++;
