@@ -248,7 +248,8 @@ extern "C" int fsg_ctx_create(const fsg_ctx_config *cfg, fsg_ctx **out) {
     c->budget = cfg ? cfg->hbm_budget : 0;
     if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess ||
-        hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) {
+        hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->staged, hipEventDisableTiming) != hipSuccess) {
         delete c;
         return FSG_ERR_DEVICE;
     }
@@ -270,6 +271,9 @@ extern "C" int fsg_ctx_destroy(fsg_ctx *c) {
     (void)hipEventDestroy(c->e1);
     (void)hipEventDestroy(c->t0);
     (void)hipEventDestroy(c->t1);
+    (void)hipEventDestroy(c->staged);
+    if (c->h_worlds) (void)hipHostFree(c->h_worlds);
+    if (c->h_tile_start) (void)hipHostFree(c->h_tile_start);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return FSG_OK;
@@ -855,23 +859,36 @@ namespace fsg_impl {
 int batch_prepare(fsg_ctx *c, fsg_world *const *ws, int n, bool *graph_ok) {
     for (int i = 0; i < n; i++)
         if (!ws[i] || ws[i]->ctx != c) return FSG_ERR_INVALID;
+    // the pinned staging may still feed the previous batch's upload
+    if (c->staged_pending) {
+        HIPCHK(hipEventSynchronize(c->staged));
+        c->staged_pending = false;
+    }
     if (n > c->batch_cap) {
         dev_free(c->d_worlds);
         dev_free(c->d_tile_start);
+        if (c->h_worlds) (void)hipHostFree(c->h_worlds);
+        if (c->h_tile_start) (void)hipHostFree(c->h_tile_start);
         c->d_worlds = nullptr;
         c->d_tile_start = nullptr;
+        c->h_worlds = nullptr;
+        c->h_tile_start = nullptr;
         c->batch_cap = 0;
         void *p;
         if (dev_alloc(c, &p, (size_t)n * sizeof(DevWorld))) return FSG_ERR_NOMEM;
         c->d_worlds = (DevWorld *)p;
         if (dev_alloc(c, &p, (size_t)(n + 1) * 4)) return FSG_ERR_NOMEM;
         c->d_tile_start = (uint32_t *)p;
+        if (hipHostMalloc(&p, (size_t)n * sizeof(DevWorld)) != hipSuccess) return FSG_ERR_NOMEM;
+        c->h_worlds = (DevWorld *)p;
+        if (hipHostMalloc(&p, (size_t)(n + 1) * 4) != hipSuccess) return FSG_ERR_NOMEM;
+        c->h_tile_start = (uint32_t *)p;
         c->batch_cap = n;
         if (c->graph) (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
         c->graph_key.clear();
     }
-    std::vector<DevWorld> dw(n);
+    DevWorld *dw = c->h_worlds;
     std::vector<uint32_t> start(n + 1);
     uint64_t total = 0;
     for (int i = 0; i < n; i++) {
@@ -884,10 +901,12 @@ int batch_prepare(fsg_ctx *c, fsg_world *const *ws, int n, bool *graph_ok) {
     }
     if (total >= 0x7FFFFFFFull) return FSG_ERR_INVALID;
     start[n] = (uint32_t)total;
-    HIPCHK(hipMemcpyAsync(c->d_worlds, dw.data(), (size_t)n * sizeof(DevWorld), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_tile_start, start.data(), (size_t)(n + 1) * 4, hipMemcpyHostToDevice, c->stream));
-    // host staging buffers must outlive the async copies
-    HIPCHK(hipStreamSynchronize(c->stream));
+    std::copy(start.begin(), start.end(), c->h_tile_start);
+    // no wait for the upload: the staging is pinned and reused only after `staged`
+    HIPCHK(hipMemcpyAsync(c->d_worlds, dw, (size_t)n * sizeof(DevWorld), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_tile_start, c->h_tile_start, (size_t)(n + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->staged, c->stream));
+    c->staged_pending = true;
     const std::vector<fsg_world *> key(ws, ws + n);
     *graph_ok = c->graph && key == c->graph_key && start == c->graph_tiles && c->graph_ticks == c->tune.graph_ticks &&
                 c->graph_occ == c->tune.occupancy;

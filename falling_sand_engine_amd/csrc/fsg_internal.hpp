@@ -32,6 +32,12 @@ struct fsg_ctx {
     DevWorld *d_worlds = nullptr;
     uint32_t *d_tile_start = nullptr;
     int batch_cap = 0;
+    // their pinned host staging, rewritten only after `staged` (recorded after
+    // the last upload from it) has completed
+    DevWorld *h_worlds = nullptr;
+    uint32_t *h_tile_start = nullptr;
+    hipEvent_t staged = nullptr;
+    bool staged_pending = false;
     std::vector<fsg_world *> graph_key;
     std::vector<uint32_t> graph_tiles;
     hipGraphExec_t graph = nullptr;
