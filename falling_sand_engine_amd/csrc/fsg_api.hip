@@ -29,7 +29,7 @@ namespace fsg_impl {
 // Device memory with a per-context cache of freed blocks.  hipFree waits for
 // the whole device, which would stall a serving loop that creates and evicts
 // worlds while other worlds tick; a freed block instead goes back to its
-// context and is handed out again for the same rounded size.  A context's
+// context and is handed out again for the same size class.  A context's
 // work runs on its one stream, so a reused block is only touched after the
 // work that used it before.  The cache is trimmed past cache_cap and emptied
 // by fsg_ctx_destroy.
@@ -52,6 +52,17 @@ void trim_cache(fsg_ctx *c, size_t keep) {
     trim_cache_locked(c, keep);
 }
 
+// Rounded block size: 256-byte steps up to 1 MiB, above that 8 size classes
+// per power of two (at most 12.5% over).  Worlds of many sizes come and go in
+// a serving loop; with exact sizes almost no freed block was ever reused, and
+// the cache overflowed into hipFree calls (~0.4 ms each, device-wide waits).
+size_t size_class(size_t bytes) {
+    if (bytes <= ((size_t)1 << 20)) return (bytes + 255) & ~(size_t)255;
+    const int k = 63 - __builtin_clzll(bytes - 1);  // 2^k < bytes <= 2^(k+1)
+    const size_t step = (size_t)1 << (k - 3);
+    return (bytes + step - 1) & ~(step - 1);
+}
+
 int dev_alloc(fsg_ctx *c, void **p, size_t bytes) {
     *p = nullptr;
     if (c->inject > 0) {
@@ -59,7 +70,7 @@ int dev_alloc(fsg_ctx *c, void **p, size_t bytes) {
         return FSG_ERR_NOMEM;
     }
     if (bytes == 0) bytes = 8;
-    bytes = (bytes + 255) & ~(size_t)255;
+    bytes = size_class(bytes);
     {
         std::lock_guard<std::mutex> lk(g_alloc_mu);
         auto it = c->cache.find(bytes);
@@ -253,6 +264,9 @@ extern "C" int fsg_ctx_create(const fsg_ctx_config *cfg, fsg_ctx **out) {
         delete c;
         return FSG_ERR_DEVICE;
     }
+    // freed blocks kept for reuse: up to a quarter of the device's memory
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b / 4 > c->cache_cap) c->cache_cap = total_b / 4;
     *out = c;
     return FSG_OK;
 }

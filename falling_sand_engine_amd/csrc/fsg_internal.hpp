@@ -46,7 +46,8 @@ struct fsg_ctx {
     // the cached graph was captured with
     fsg_tuning tune{16, -1, 5, 0};
     int graph_ticks = 0, graph_occ = 0;
-    // freed device blocks by rounded size (dev_alloc / dev_free)
+    // freed device blocks by size class (dev_alloc / dev_free); the cap is at
+    // least 16 GiB and a quarter of the device's memory
     std::multimap<size_t, void *> cache;
     size_t cached = 0, cache_cap = (size_t)16 << 30;
     // stamp claim rounds: pixels still open after round 0 and their count
