@@ -606,17 +606,27 @@ extern "C" int fsg_world_sync(fsg_world *w) {
     return FSG_OK;
 }
 
-extern "C" int fsg_world_stats(fsg_world *w, fsg_stats *out) {
-    if (!w || !out) return FSG_ERR_INVALID;
+namespace fsg_impl {
+// Enqueues the census of w's grid (counts [0, 16), checksum [16], processed
+// chunks [17]) into host[18]; valid after the stream is synchronised.
+int stats_enqueue(fsg_world *w, unsigned long long *host) {
     fsg_ctx *c = w->ctx;
-    HIPCHK(hipSetDevice(c->device));
     const size_t n = (size_t)w->d.W * w->d.H;
     HIPCHK(hipMemsetAsync(w->d_stats, 0, 17 * 8, c->stream));
     const unsigned blocks = (unsigned)std::min<size_t>(grid_for(n, fsg::NT), 4096);
     fsg::k_stats<<<blocks, fsg::NT, 0, c->stream>>>(w->d.buf[w->d.tick & 1], n, w->d_stats);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host, w->d_stats, 18 * 8, hipMemcpyDeviceToHost, c->stream));
+    return FSG_OK;
+}
+}  // namespace fsg_impl
+
+extern "C" int fsg_world_stats(fsg_world *w, fsg_stats *out) {
+    if (!w || !out) return FSG_ERR_INVALID;
+    fsg_ctx *c = w->ctx;
+    HIPCHK(hipSetDevice(c->device));
     unsigned long long h[18];
-    HIPCHK(hipMemcpyAsync(h, w->d_stats, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    if (int rc = fsg_impl::stats_enqueue(w, h)) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     std::memset(out, 0, sizeof(*out));
     out->tick = w->d.tick;

@@ -118,6 +118,9 @@ fsg::StampWorld stamp_world_of(fsg_world *w, uint32_t pix_start);
 // max_npix: the largest pixel count of one world (the launch's x extent).
 int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t total, uint32_t max_npix,
                  const MatTab &M, bool unstamp);
+// Enqueues w's census (counts [0,16), checksum [16], processed chunks [17])
+// into host[18], valid after the context's stream is synchronised.
+int stats_enqueue(fsg_world *w, unsigned long long *host);
 // Uploads pixels/bodies of a world for stamping (validates them).
 int set_bodies(fsg_world *w, const fsg_body *bodies, int nb, const fsg_cell *pixels, size_t npix);
 }  // namespace fsg_impl

@@ -196,13 +196,18 @@ struct fsg_server {
     int table_cap = 0;
     fsg_body *d_pose = nullptr, *h_pose = nullptr;
     size_t pose_cap = 0;
+    // census results of the requests a step completes (18 words each, pinned)
+    unsigned long long *h_sum = nullptr;
+    size_t sum_cap = 0;
     std::vector<size_t> pose_off;
     std::thread th;
 };
 
 namespace {
 
-void finish(fsg_server *S, Req &q, int status, std::vector<fsg_serve_result> &out) {
+// sum: the world's census words from stats_enqueue, or null to compute them.
+void finish(fsg_server *S, Req &q, int status, std::vector<fsg_serve_result> &out,
+            const unsigned long long *sum = nullptr) {
     fsg_serve_result r{};
     r.request_id = q.id;
     r.world_id = q.r.world_id;
@@ -219,7 +224,11 @@ void finish(fsg_server *S, Req &q, int status, std::vector<fsg_serve_result> &ou
         r.cells = (uint64_t)W * H;
         if (status == FSG_OK) {
             fsg_stats s;
-            if (fsg_world_stats(q.res->w, &s) == FSG_OK) {
+            if (sum) {
+                r.checksum = sum[16];
+                q.res->checksum = sum[16];
+                q.res->checksum_known = true;
+            } else if (fsg_world_stats(q.res->w, &s) == FSG_OK) {
                 r.checksum = s.checksum;
                 q.res->checksum = s.checksum;
                 q.res->checksum_known = true;
@@ -509,11 +518,33 @@ void worker(fsg_server *S) {
             admit(S, out);  // overlaps the step
         }
         if (n_run) {
-            // always wait: the pinned staging of this step is rewritten by the next one
+            // the census of every request this step completes, behind the step,
+            // read back with the step's one wait (the pinned staging of this
+            // step is rewritten by the next one, so the wait is always made)
+            size_t ndone = 0;
+            if (rc == FSG_OK) {
+                for (size_t k = 0; k < n_run; k++)
+                    if (S->active[k].done_ticks + T >= S->active[k].r.ticks) ndone++;
+                if (ndone > S->sum_cap) {
+                    if (S->h_sum) (void)hipHostFree(S->h_sum);
+                    S->h_sum = nullptr;
+                    S->sum_cap = 0;
+                    void *p = nullptr;
+                    if (hipHostMalloc(&p, std::max<size_t>(ndone, 64) * 18 * 8) != hipSuccess) rc = FSG_ERR_NOMEM;
+                    else {
+                        S->h_sum = static_cast<unsigned long long *>(p);
+                        S->sum_cap = std::max<size_t>(ndone, 64);
+                    }
+                }
+                size_t j = 0;
+                for (size_t k = 0; k < n_run && rc == FSG_OK; k++)
+                    if (S->active[k].done_ticks + T >= S->active[k].r.ticks)
+                        rc = stats_enqueue(S->active[k].res->w, S->h_sum + 18 * j++);
+            }
             if (hipStreamSynchronize(S->ctx->stream) != hipSuccess && rc == FSG_OK) rc = FSG_ERR_DEVICE;
             uint64_t cu = 0;
             std::lock_guard<std::mutex> lk(S->mu);
-            size_t i = 0;
+            size_t i = 0, j = 0;
             for (size_t k = 0; k < n_run; k++) {
                 Req &q = S->active[i];
                 if (rc != FSG_OK) {
@@ -524,7 +555,7 @@ void worker(fsg_server *S) {
                 q.done_ticks += T;
                 cu += (uint64_t)T * q.res->w->d.W * q.res->w->d.H;
                 if (q.done_ticks >= q.r.ticks) {
-                    finish(S, q, FSG_OK, out);
+                    finish(S, q, FSG_OK, out, S->h_sum + 18 * j++);
                     S->active.erase(S->active.begin() + (long)i);
                 } else {
                     ++i;
@@ -582,6 +613,7 @@ extern "C" int fsg_serve_destroy(fsg_server *S) {
     dev_free(S->d_pose);
     if (S->h_table) (void)hipHostFree(S->h_table);
     if (S->h_pose) (void)hipHostFree(S->h_pose);
+    if (S->h_sum) (void)hipHostFree(S->h_sum);
     delete S;
     return FSG_OK;
 }
