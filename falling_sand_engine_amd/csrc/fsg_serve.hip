@@ -189,9 +189,10 @@ struct fsg_server {
     // worker-owned
     std::unordered_map<uint64_t, std::unique_ptr<Resident>> res;
     std::vector<Req> active;
-    // per-step staging in pinned host memory, one upload each: the stamp
-    // tables of the step (stamp at [0, n), unstamp at [n, 2n)) and every
-    // world's body poses; reused after the step's final synchronisation
+    // per-step staging in pinned host memory: the stamp tables of the step
+    // (tick t: stamp at [2tn, (2t+1)n), unstamp at [(2t+1)n, (2t+2)n)) and
+    // every world's body poses of every tick (one upload); reused after the
+    // step's final synchronisation
     fsg::StampWorld *d_table = nullptr, *h_table = nullptr;
     int table_cap = 0;
     fsg_body *d_pose = nullptr, *h_pose = nullptr;
@@ -368,20 +369,23 @@ void admit(fsg_server *S, std::vector<fsg_serve_result> &out) {
     }
 }
 
-// Claim rounds of the worlds with bodies: stamp (half 0) or unstamp (half 1)
-// table, bodies read from the step's pose staging.
-int stamp_batch(fsg_server *S, const std::vector<Resident *> &bw, bool unstamp) {
+// Claim rounds of the worlds with bodies for tick t of the step: stamp or
+// unstamp table slice 2t + unstamp, bodies read from tick t's poses.  The
+// table is built when the rounds are enqueued, after the ticks before it, so
+// it names the world's buffers of that tick.
+int stamp_batch(fsg_server *S, const std::vector<Resident *> &bw, bool unstamp, int t) {
     fsg_ctx *c = S->ctx;
     const int n = (int)bw.size();
-    fsg::StampWorld *h = S->h_table + (unstamp ? n : 0);
+    const size_t slice = (size_t)(2 * t + (unstamp ? 1 : 0)) * n;
+    fsg::StampWorld *h = S->h_table + slice;
     uint32_t total = 0, max_npix = 0;
     for (int i = 0; i < n; i++) {
         h[i] = stamp_world_of(bw[i]->w, total);
-        h[i].bodies = S->d_pose + S->pose_off[(size_t)i];
+        h[i].bodies = S->d_pose + S->pose_off[(size_t)t * n + i];
         total += (uint32_t)bw[i]->w->npix;
         max_npix = std::max(max_npix, (uint32_t)bw[i]->w->npix);
     }
-    fsg::StampWorld *d = S->d_table + (unstamp ? n : 0);
+    fsg::StampWorld *d = S->d_table + slice;
     HIPCHK(hipMemcpyAsync(d, h, (size_t)n * sizeof(fsg::StampWorld), hipMemcpyHostToDevice, c->stream));
     return stamp_rounds(c, d, n, total, max_npix, bw[0]->w->d.mat, unstamp);
 }
@@ -420,8 +424,11 @@ int ensure_staging(fsg_server *S, int nworlds, size_t nbodies) {
     return FSG_OK;
 }
 
-// Enqueues T ticks of every active request (stamp, batched tick, unstamp)
-// and steps the host body model for the next step; the caller synchronises.
+// Enqueues T ticks of every active request.  Without bodies: one batched
+// launch sequence of T ticks.  With bodies, each tick is stamp, batched tick,
+// unstamp; the host body model (which reads nothing back from the device)
+// gives the poses of all T ticks up front, uploaded once, so a step with
+// bodies also needs only one wait.  The caller synchronises.
 int run_step(fsg_server *S, int T) {
     std::vector<fsg_world *> ws;
     std::vector<Resident *> bw;  // worlds with bodies
@@ -430,29 +437,31 @@ int run_step(fsg_server *S, int T) {
         if (!q.res->hb.empty()) bw.push_back(q.res);
     }
     fsg_ctx *c = S->ctx;
-    if (!bw.empty()) {
-        size_t nb = 0;
+    bool graph_ok = false;
+    if (bw.empty()) {
+        if (int rc = batch_prepare(c, ws.data(), (int)ws.size(), &graph_ok)) return rc;
+        return batch_run(c, ws.data(), (int)ws.size(), T, graph_ok);
+    }
+    size_t nb = 0;  // bodies per tick
+    for (Resident *r : bw) nb += r->hb.size();
+    if (int rc = ensure_staging(S, (int)bw.size() * T, nb * T)) return rc;
+    S->pose_off.clear();
+    size_t off = 0;
+    for (int t = 0; t < T; t++) {  // poses of tick t at pose_off[t * n + i]
         for (Resident *r : bw) {
             poses_of(r->hb, r->poses);
-            nb += r->poses.size();
-        }
-        if (int rc = ensure_staging(S, (int)bw.size(), nb)) return rc;
-        S->pose_off.clear();
-        size_t off = 0;
-        for (Resident *r : bw) {
             std::memcpy(S->h_pose + off, r->poses.data(), r->poses.size() * sizeof(fsg_body));
             S->pose_off.push_back(off);
             off += r->poses.size();
+            step_bodies(r->hb, r->w->d.W, r->w->d.H);
         }
-        HIPCHK(hipMemcpyAsync(S->d_pose, S->h_pose, nb * sizeof(fsg_body), hipMemcpyHostToDevice, c->stream));
-        if (int rc = stamp_batch(S, bw, false)) return rc;
     }
-    bool graph_ok = false;
+    HIPCHK(hipMemcpyAsync(S->d_pose, S->h_pose, off * sizeof(fsg_body), hipMemcpyHostToDevice, c->stream));
     if (int rc = batch_prepare(c, ws.data(), (int)ws.size(), &graph_ok)) return rc;
-    if (int rc = batch_run(c, ws.data(), (int)ws.size(), T, graph_ok)) return rc;
-    if (!bw.empty()) {
-        if (int rc = stamp_batch(S, bw, true)) return rc;
-        for (Resident *r : bw) step_bodies(r->hb, r->w->d.W, r->w->d.H);
+    for (int t = 0; t < T; t++) {
+        if (int rc = stamp_batch(S, bw, false, t)) return rc;
+        if (int rc = batch_run(c, ws.data(), (int)ws.size(), 1, graph_ok)) return rc;
+        if (int rc = stamp_batch(S, bw, true, t)) return rc;
     }
     return FSG_OK;
 }
@@ -504,13 +513,8 @@ void worker(fsg_server *S) {
         const size_t n_run = S->active.size();
         int T = 0, rc = FSG_OK;
         if (n_run) {
-            bool bodies = false;
             T = S->cfg.block_ticks > 0 ? S->cfg.block_ticks : 16;
-            for (Req &q : S->active) {
-                T = std::min(T, q.r.ticks - q.done_ticks);
-                bodies = bodies || !q.res->hb.empty();
-            }
-            if (bodies) T = 1;
+            for (Req &q : S->active) T = std::min(T, q.r.ticks - q.done_ticks);
             rc = run_step(S, T);
         }
         {

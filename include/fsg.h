@@ -413,7 +413,7 @@ typedef struct fsg_server fsg_server;
 
 typedef struct fsg_serve_config {
     int32_t max_active;      /* requests stepped together (0 = 1024)            */
-    int32_t block_ticks;     /* ticks per step when no active world has bodies  */
+    int32_t block_ticks;     /* ticks per step (at most; 0 = 16)                */
     int32_t verify_resident; /* 1: check a resident world's checksum on reuse   */
     int32_t gen_threads;     /* host threads generating new worlds (0 = 4)      */
 } fsg_serve_config;

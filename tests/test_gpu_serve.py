@@ -38,6 +38,24 @@ def test_result_is_the_same_alone_and_in_the_mix(ctx):
     assert st["steps"] < 37 + 50 + 3 + 20   # requests shared steps
 
 
+def test_bodies_same_result_one_or_many_ticks_per_step(ctx):
+    """A world with bodies stepped one tick per step (stamp, tick, unstamp,
+    wait) and sixteen ticks per step (poses of all sixteen ticks uploaded up
+    front) ends in the same state."""
+    req = dict(world_id=4, ticks=37, kind=4, width=512, height=384, seed=6, n_bodies=40)
+    out = []
+    for bt in (1, 16):
+        with fsg.Server(ctx, block_ticks=bt) as s:
+            s.submit(**req)
+            assert s.drain(120) == 0
+            (r,) = s.poll()
+            st = s.stats()
+        assert r["status"] == OK and r["tick"] == 37
+        out.append((r["checksum"], st["steps"]))
+    assert out[0][0] == out[1][0]
+    assert out[0][1] == 37 and out[1][1] == 3   # 16 + 16 + 5 ticks
+
+
 def test_served_world_matches_direct_stepping(ctx):
     r = run_alone(ctx, world_id=3, ticks=45, kind=4, width=384, height=256, seed=9)
     with fsg.World(ctx, 384, 256, 9) as w:
