@@ -705,7 +705,10 @@ int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t tot
     const int propose = unstamp ? fsg::OP_UNPROPOSE : fsg::OP_PROPOSE;
     const int resolve = unstamp ? fsg::OP_UNRESOLVE : fsg::OP_RESOLVE;
     const dim3 g(grid_for(max_npix, 256), (unsigned)n);  // y: world, x: its pixels
-    fsg::k_claims_init<<<g, 256, 0, c->stream>>>(a);
+    // small batches clear every target slot up front (fewer launches), large
+    // ones only the slots their open pixels reach in each round (less work)
+    const bool all_slots = total <= (1u << 20);
+    fsg::k_claims_init<<<g, 256, 0, c->stream>>>(a, all_slots ? 1 : 0);
     fsg::k_stamp_step<<<g, 256, 0, c->stream>>>(a, propose, 0);
     fsg::k_stamp_step<<<g, 256, 0, c->stream>>>(a, resolve, 0);
     fsg::k_zero_u32<<<1, 1, 0, c->stream>>>(a.nopen);
@@ -714,7 +717,7 @@ int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t tot
     // the list holds at most `total` entries; the loop kernels stride over it
     const unsigned gl = std::min(grid_for(total, 256), 1024u);
     for (int d = 1; d < 5; d++) {
-        fsg::k_stamp_step_list<<<gl, 256, 0, c->stream>>>(a, fsg::OP_CLAIMS, d);
+        if (!all_slots) fsg::k_stamp_step_list<<<gl, 256, 0, c->stream>>>(a, fsg::OP_CLAIMS, d);
         fsg::k_stamp_step_list<<<gl, 256, 0, c->stream>>>(a, propose, d);
         fsg::k_stamp_step_list<<<gl, 256, 0, c->stream>>>(a, resolve, d);
     }

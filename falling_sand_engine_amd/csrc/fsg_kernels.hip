@@ -714,28 +714,53 @@ __device__ __forceinline__ uint64_t own_cell(const StampArgs &a, const StampWorl
     return (uint64_t)(uint32_t)(int)(fx >> 16) | ((uint64_t)(uint32_t)(int)(fy >> 16) << 32);
 }
 
-// Chunks whose cells a stamp round touches must run on the next tick (their
-// spare-buffer claim slots were written); flags is null without skipping.
-__device__ __forceinline__ void mark_tile(const StampWorld &W, int x, int y) {
-    if (W.flags) atomicOr(&W.flags[(y / TH) * W.tiles_x + (x / TW)], F_WAKE_ALL);
+// Chunks whose spare-buffer cells serve as claim slots must run on the next
+// tick (the slots were written); flags is null without skipping.  Takes the
+// wave's lanes together (flags is the same in every lane): one atomic per
+// distinct chunk of the wave, not one per lane -- a wave's pixels fall in one
+// or two chunks, and same-address atomics serialise.
+__device__ __forceinline__ void mark_tiles_wave(uint32_t *flags, bool on, uint32_t tile) {
+    const int lane = threadIdx.x & 63;
+    uint64_t m = __ballot(on);
+    while (m) {
+        const int leader = __builtin_ctzll(m);
+        const uint32_t t0 = (uint32_t)__shfl((int)tile, leader, 64);
+        if (lane == leader) atomicOr(&flags[t0], F_WAKE_ALL);
+        on = on && tile != t0;
+        m = __ballot(on);
+    }
 }
 
-// Round 0: every live pixel's own cell and its claim slot.  Each later round
-// d first clears the slots of its open pixels' round-d targets (OP_CLAIMS):
-// a slot a winner of an earlier round holds may be reset,
-// because the propose kernels also require the cell to be free (stamp) or to
-// hold the pixel's material (unstamp), so the earliest round and then the
-// lowest pixel still win.
-__global__ void k_claims_init(StampArgs a) {
-    PixelRef ref;
-    if (!pixel_ref(a, ref)) return;
-    const StampWorld &W = *ref.w;
-    W.state[ref.p] = 0;
-    W.tgt[ref.p] = own_cell(a, W, ref.p);
-    int wx, wy;
-    if (!pixel_target(ref, 0, wx, wy)) return;
-    W.claims[(size_t)wy * W.W + wx] = ~0ull;
-    mark_tile(W, wx, wy);
+// Every live pixel's own cell, and the claim slots it may target, cleared
+// before any round: with all_slots the slots of all five cells (the own cell
+// and its four neighbours), else only the own cell's, and each later round
+// clears its open pixels' slots first (OP_CLAIMS over the list).  Both give
+// the same result: a slot that receives proposals in some round gets a
+// winner, which fills the cell (stamp) or empties it (unstamp), and a later
+// round's proposal to that cell fails the cell check of its propose step
+// whatever the slot holds -- so the earliest round and then the lowest pixel
+// win, as a round-by-round pass over the pixels gives.  Clearing all five up
+// front saves the per-round launches (small batches: latency); clearing per
+// round writes only the open pixels' slots (large batches: work).
+__global__ void k_claims_init(StampArgs a, int all_slots) {
+    const StampWorld &W = a.worlds[blockIdx.y];
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = p < W.npix;
+    uint64_t t = NOT_LIVE;
+    if (in) {
+        W.state[p] = 0;
+        t = own_cell(a, W, p);
+        W.tgt[p] = t;
+    }
+    const bool live = in && (uint32_t)t != NOT_LIVE;
+    const int ox = (int)(uint32_t)t, oy = (int)(uint32_t)(t >> 32);
+    const int nd = all_slots ? 5 : 1;
+    for (int d = 0; d < nd; d++) {
+        const int wx = ox + kDX[d], wy = oy + kDY[d];
+        const bool on = live && wx >= 0 && wy >= 0 && wx < W.W && wy < W.H;
+        if (on) W.claims[(size_t)wy * W.W + wx] = ~0ull;
+        if (W.flags) mark_tiles_wave(W.flags, on, on ? (uint32_t)((wy / TH) * W.tiles_x + wx / TW) : 0u);
+    }
 }
 
 // The per-pixel steps of a claim round (d: round).  Each runs in two launch
@@ -752,9 +777,9 @@ __device__ __forceinline__ void stamp_step(const StampArgs &a, const PixelRef &r
     const size_t ci = (size_t)wy * W.W + wx;
     const uint64_t mine = ((uint64_t)d << 32) | ref.p;
     switch (op) {
-    case OP_CLAIMS:
+    case OP_CLAIMS:  // lanes may be of different worlds here: one atomic each
         W.claims[ci] = ~0ull;
-        mark_tile(W, wx, wy);
+        if (W.flags) atomicOr(&W.flags[(wy / TH) * W.tiles_x + (wx / TW)], F_WAKE_ALL);
         break;
     case OP_PROPOSE: {
         const uint32_t m = cid(W.cells[ci]);

@@ -44,7 +44,7 @@ __global__ void k_stats(const uint64_t *__restrict__ cells, size_t n, unsigned l
 __global__ void k_fill(uint64_t *cells, size_t n, uint64_t v);
 __global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16);
 __global__ void k_set_cells(uint64_t *cells, int W, int H, const int32_t *xy, const uint64_t *vals, size_t n);
-__global__ void k_claims_init(StampArgs a);
+__global__ void k_claims_init(StampArgs a, int all_slots);
 // Per-pixel steps of a stamp claim round (k_stamp_step, k_stamp_step_list).
 enum StampOp { OP_CLAIMS = 0, OP_PROPOSE, OP_RESOLVE, OP_UNPROPOSE, OP_UNRESOLVE, OP_UNFINISH };
 __global__ void k_stamp_step(StampArgs a, int op, int d);

@@ -10,12 +10,16 @@ from bodies import random_bodies
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("seed,nb,skip", [(1, 30, 0), (2, 120, 0), (3, 60, 1)])
-def test_stamp_tick_unstamp_matches_oracle(ctx, orc, seed, nb, skip):
-    W, H = 320, 256
+@pytest.mark.parametrize("seed,nb,skip,W,H,max_side", [
+    (1, 30, 0, 320, 256, 16), (2, 120, 0, 320, 256, 16), (3, 60, 1, 320, 256, 16),
+    # over 2^20 body pixels: the claim slots are cleared round by round, not all up front
+    (5, 1100, 1, 1024, 1024, 64)])
+def test_stamp_tick_unstamp_matches_oracle(ctx, orc, seed, nb, skip, W, H, max_side):
     rng = np.random.default_rng(seed)
     cells = fsg.generate(1, W, H, seed)
-    bodies, pixels = random_bodies(rng, W, H, nb)
+    bodies, pixels = random_bodies(rng, W, H, nb, max_side=max_side)
+    assert (pixels.size > 1 << 20) == (max_side == 64)
+    print(f"bodies seed={seed} nb={nb} skip={skip} {W}x{H} pixels={pixels.size}", flush=True)
     cfg = orc.default_config(chunk_skip=skip)
     # oracle: stamp, 3 ticks, unstamp
     ref = cells.copy()
