@@ -28,7 +28,10 @@ struct fsg_ctx {
     uint64_t budget = 0, used = 0;
     int inject = 0;
     std::mutex mu;
-    // batch launch state: device world table + chunk prefix sums + graph
+    // batch launch state: device world table + chunk prefix sums + graph,
+    // shared by every batched launch of the context (held under batch_mu from
+    // batch_prepare to the end of the batch_run that follows it)
+    std::mutex batch_mu;
     DevWorld *d_worlds = nullptr;
     uint32_t *d_tile_start = nullptr;
     int batch_cap = 0;

@@ -437,6 +437,7 @@ int run_step(fsg_server *S, int T) {
         if (!q.res->hb.empty()) bw.push_back(q.res);
     }
     fsg_ctx *c = S->ctx;
+    std::lock_guard<std::mutex> lk(c->batch_mu);  // the context's batch staging
     bool graph_ok = false;
     if (bw.empty()) {
         if (int rc = batch_prepare(c, ws.data(), (int)ws.size(), &graph_ok)) return rc;
