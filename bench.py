@@ -347,6 +347,14 @@ def bench_pack(fsg, side=16384, kind=2, seed=41, reps=3):
             "pack_raw_GBs": round(n * PACK_RAW / (pm / 1e3) / 1e9, 1),
             "unpack_raw_GBs": round(n * PACK_RAW / (um / 1e3) / 1e9, 1),
             "image_bytes": total, "ratio": round(n * PACK_RAW / total, 3), "round_trip_exact": ok,
+            # algorithmic HBM bytes per call: the 8-byte cells read and the images written (pack), the
+            # images read and the cells written (unpack), over the whole call's device time
+            "roofline": {"bound": "issue (serial parse per record row / per LZ4 block)", "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s",
+                         "pack_achieved": round((side * side * 8 + total) / (pm / 1e3) / 1e9, 1),
+                         "unpack_achieved": round((side * side * 8 + total) / (um / 1e3) / 1e9, 1),
+                         "pack_frac": round((side * side * 8 + total) / (pm / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "unpack_frac": round((side * side * 8 + total) / (um / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
             "host_pack_download_ms": round(hp, 1), "host_unpack_from_host_ms": round(hu, 1),
             "host_note": "wall time through pageable host memory (PCIe copies of the images included)"}
 
