@@ -123,15 +123,20 @@ __device__ int walk_row(int t, int n, const Rec &rec, Sink &sink) {
             len += R;
             continue;
         }
-        for (int b = 0; b < R; b++) {
+        // runs of one kind inside the record, found with bit scans
+        const uint32_t isX = ~(isL | isU) & FULL;
+        for (int b = 0; b < R;) {
             const int k = (isL >> b & 1u) ? KL : (isU >> b & 1u) ? KU : KX;
+            const uint32_t m = k == KL ? isL : k == KU ? isU : isX;
+            const int run = min(R - b, __builtin_ctz(~(m >> b)));  // bytes of kind k from b on
             if (k == kind) {
-                len++;
-                continue;
+                len += run;
+            } else {
+                if (kind >= 0) close(p0 + b);
+                kind = k;
+                len = run;
             }
-            if (kind >= 0) close(p0 + b);
-            kind = k;
-            len = 1;
+            b += run;
         }
     }
     close((i0 + CW) * R);
