@@ -422,9 +422,10 @@ __global__ __launch_bounds__(64 * DW) __attribute__((amdgpu_waves_per_eu(6))) vo
     if (lane == 0) status[b] = r == cap ? 0 : -1;
 }
 
-__device__ __forceinline__ uint32_t ld32(const uint8_t *p) {
-    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
-}
+// Little-endian u32 at a 4-byte aligned staging address (records are 12 or
+// 4 bytes and each chunk's staging starts at a multiple of 4).
+__device__ __forceinline__ uint32_t ld32(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
+static_assert(RAW % 4 == 0 && NA % 4 == 0 && RA % 4 == 0 && RB % 4 == 0, "aligned record words");
 
 struct UnpackDst {
     uint64_t *cells;  // world grid
