@@ -80,6 +80,9 @@ def test_round_trip_on_device_and_through_host(ctx, orc):
     print("round trip 768x512", flush=True)
     with fsg.World(ctx, 768, 512, 1, orc.default_config()) as w:
         w.upload(cells)
+        w.tick(7)  # ticked cells: moved bits set, temperatures changed
+        cells = w.download()
+        assert (cells & np.uint64(0x100)).any()
         imgs = w.pack(0, 0, 6, 4, layer2=l2, background=bg)
         w.upload(other)
         st, _ph, gl2, gbg = w.unpack(0, 0, 6, 4, None, want_layer2=True, want_background=True)  # device images
