@@ -434,21 +434,20 @@ extern "C" int fsg_world_destroy(fsg_world *w) {
     // The world's device blocks go back to the context's cache, whose reuse is
     // ordered behind any pending work on the one stream, so destroying needs no
     // wait -- except for a captured graph over this world, which may be running.
-    bool in_graph;
     {
-        std::lock_guard<std::mutex> lk(c->mu);
-        in_graph = std::find(c->graph_key.begin(), c->graph_key.end(), w) != c->graph_key.end();
-    }
-    if (in_graph) (void)hipStreamSynchronize(c->stream);
-    {
-        std::lock_guard<std::mutex> lk(c->mu);
-        c->used -= w->bytes;
-        // a graph captured over this world must not be replayed
+        // the graph state belongs to the batch launches (batch_mu)
+        std::lock_guard<std::mutex> bl(c->batch_mu);
         if (std::find(c->graph_key.begin(), c->graph_key.end(), w) != c->graph_key.end()) {
+            (void)hipStreamSynchronize(c->stream);  // the graph may be running
+            // a graph captured over this world must not be replayed
             if (c->graph) (void)hipGraphExecDestroy(c->graph);
             c->graph = nullptr;
             c->graph_key.clear();
         }
+    }
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->used -= w->bytes;
     }
     dev_free(w->d.buf[0]);
     dev_free(w->d.buf[1]);
@@ -1168,6 +1167,7 @@ extern "C" int fsg_autotune_batch(fsg_ctx *c, fsg_world *const *ws, int n, int s
             r.best = cand[k];
         }
     }
+    blk.unlock();  // fsg_world_destroy takes batch_mu
     for (fsg_world *w : cl)
         if (w) fsg_world_destroy(w);
     if (rc) {
