@@ -65,10 +65,8 @@ size_t size_class(size_t bytes) {
 
 int dev_alloc(fsg_ctx *c, void **p, size_t bytes) {
     *p = nullptr;
-    if (c->inject > 0) {
-        c->inject--;
-        return FSG_ERR_NOMEM;
-    }
+    for (int k = c->inject.load(); k > 0;)
+        if (c->inject.compare_exchange_weak(k, k - 1)) return FSG_ERR_NOMEM;
     if (bytes == 0) bytes = 8;
     bytes = size_class(bytes);
     {
@@ -328,6 +326,7 @@ extern "C" int fsg_ctx_copy_timed(fsg_ctx *c, size_t bytes, int reps, double *ms
     if (!c || bytes < 16 || reps < 1) return FSG_ERR_INVALID;
     HIPCHK(hipSetDevice(c->device));
     const size_t n16 = bytes / 16;
+    if (n16 > (size_t)0x7FFFFFFF * 256) return FSG_ERR_INVALID;
     void *a = nullptr, *b = nullptr;
     if (dev_alloc(c, &a, n16 * 16)) return FSG_ERR_NOMEM;
     if (dev_alloc(c, &b, n16 * 16)) {
@@ -335,10 +334,10 @@ extern "C" int fsg_ctx_copy_timed(fsg_ctx *c, size_t bytes, int reps, double *ms
         return FSG_ERR_NOMEM;
     }
     int rc = FSG_OK;
-    if (n16 > (size_t)0x7FFFFFFF * 256) return FSG_ERR_INVALID;
     const unsigned grid = grid_for(n16, 256);
     (void)hipMemsetAsync(a, 1, n16 * 16, c->stream);
     fsg::k_copy<<<grid, 256, 0, c->stream>>>((const uint4 *)a, (uint4 *)b, n16);  // warm-up
+    std::unique_lock<std::mutex> tl(c->time_mu);
     if (hipEventRecord(c->e0, c->stream) != hipSuccess) rc = FSG_ERR_DEVICE;
     for (int r = 0; r < reps && !rc; r++)
         fsg::k_copy<<<grid, 256, 0, c->stream>>>((const uint4 *)a, (uint4 *)b, n16);
@@ -582,6 +581,7 @@ extern "C" int fsg_world_step_timed(fsg_world *w, int ticks, double *ms) {
     if (!w || ticks < 0) return FSG_ERR_INVALID;
     fsg_ctx *c = w->ctx;
     HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> tl(c->time_mu);
     HIPCHK(hipEventRecord(c->e0, c->stream));
     if (int rc = step_world(w, ticks)) return rc;
     HIPCHK(hipEventRecord(c->e1, c->stream));
@@ -687,6 +687,7 @@ fsg::StampWorld stamp_world_of(fsg_world *w, uint32_t pix_start) {
 int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t total, uint32_t max_npix,
                  const MatTab &M, bool unstamp) {
     if (total == 0 || max_npix == 0) return FSG_OK;
+    std::lock_guard<std::mutex> lk(c->stamp_mu);
     if (!c->d_nopen) {
         void *p;
         if (dev_alloc(c, &p, sizeof(uint32_t))) return FSG_ERR_NOMEM;
@@ -1016,6 +1017,7 @@ extern "C" int fsg_batch_step_timed(fsg_ctx *c, fsg_world *const *ws, int n, int
     bool graph_ok = false;
     if (n > 0)
         if (int rc = batch_prepare(c, ws, n, &graph_ok)) return rc;
+    std::lock_guard<std::mutex> tl(c->time_mu);
     HIPCHK(hipEventRecord(c->e0, c->stream));
     if (n > 0 && ticks > 0)
         if (int rc = batch_run(c, ws, n, ticks, graph_ok)) return rc;
@@ -1032,6 +1034,7 @@ extern "C" int fsg_perworld_step_timed(fsg_ctx *c, fsg_world *const *ws, int n, 
     for (int i = 0; i < n; i++)
         if (!ws[i] || ws[i]->ctx != c) return FSG_ERR_INVALID;
     HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> tl(c->time_mu);
     HIPCHK(hipEventRecord(c->e0, c->stream));
     for (int t = 0; t < ticks; t++)
         for (int i = 0; i < n; i++)
@@ -1139,6 +1142,7 @@ extern "C" int fsg_autotune_batch(fsg_ctx *c, fsg_world *const *ws, int n, int s
         if (!rc) rc = copy_state(cl[i], ws[i]);
     }
     std::unique_lock<std::mutex> blk(c->batch_mu);
+    std::unique_lock<std::mutex> tl(c->time_mu);
     const fsg_tuning before = c->tune;
     std::vector<fsg_tuning> cand;
     for (int occ : {5, 4})
@@ -1167,6 +1171,7 @@ extern "C" int fsg_autotune_batch(fsg_ctx *c, fsg_world *const *ws, int n, int s
             r.best = cand[k];
         }
     }
+    tl.unlock();
     blk.unlock();  // fsg_world_destroy takes batch_mu
     for (fsg_world *w : cl)
         if (w) fsg_world_destroy(w);

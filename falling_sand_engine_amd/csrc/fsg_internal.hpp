@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -23,10 +24,13 @@ using fsg::MatTab;
 struct fsg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // events of the *_timed calls, held under time_mu from the first record
+    // to the elapsed-time read
+    std::mutex time_mu;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;  // fsg_ctx_timer_start/stop
     uint64_t budget = 0, used = 0;
-    int inject = 0;
+    std::atomic<int> inject{0};  // allocations still to fail (fsg_ctx_inject_alloc_failures)
     std::mutex mu;
     // batch launch state: device world table + chunk prefix sums + graph,
     // shared by every batched launch of the context (held under batch_mu from
@@ -53,7 +57,10 @@ struct fsg_ctx {
     // least 16 GiB and a quarter of the device's memory
     std::multimap<size_t, void *> cache;
     size_t cached = 0, cache_cap = (size_t)16 << 30;
-    // stamp claim rounds: pixels still open after round 0 and their count
+    // stamp claim rounds: pixels still open after round 0 and their count,
+    // held under stamp_mu from the list's sizing to the last launch reading it
+    // (one claim sequence at a time on the stream, so no two interleave)
+    std::mutex stamp_mu;
     uint64_t *d_open = nullptr;
     uint32_t *d_nopen = nullptr;
     size_t open_cap = 0;
