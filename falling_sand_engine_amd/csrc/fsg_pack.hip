@@ -397,10 +397,14 @@ struct WaveLane {
 };
 
 // One wave per block, four blocks per workgroup (a workgroup per wave would
-// leave CUs short of waves: their workgroup slots run out first); block
-// index = blockIdx.x * 4 + wave, chunk = index / 2.  Each wave keeps its own
-// block window and output ring in LDS and decodes alone (no workgroup
-// barrier); status 0 when the block decoded to its full size.
+// leave CUs short of waves: their workgroup slots run out first).  Wave
+// v = blockIdx.x * 4 + wave of the 2n takes chunk v's tiles-and-layer2 block
+// (384 KiB decoded) for v < n and chunk v - n's background block (64 KiB)
+// after: a workgroup holds blocks of one kind, so its waves end together
+// (a mixed one stays resident until its long blocks end), and the long
+// blocks start first.  Each wave keeps its own block window and output ring
+// in LDS and decodes alone (no workgroup barrier); status[2c + which] is 0
+// when the block decoded to its full size.
 constexpr int DW = 4;  // waves (blocks) per decode workgroup
 
 __global__ __launch_bounds__(64 * DW) __attribute__((amdgpu_waves_per_eu(6))) void k_pack_decode(
@@ -408,9 +412,9 @@ __global__ __launch_bounds__(64 * DW) __attribute__((amdgpu_waves_per_eu(6))) vo
     __shared__ __align__(16) uint8_t win[DW][fsg_lz4::IW];
     __shared__ uint8_t ring[DW][fsg_lz4::RING];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int b = blockIdx.x * DW + wv;
-    if (b >= nblocks) return;
-    const int c = b >> 1, which = b & 1;
+    const int v = blockIdx.x * DW + wv, n = nblocks >> 1;
+    if (v >= nblocks) return;
+    const int which = v >= n ? 1 : 0, c = v - which * n, b = 2 * c + which;
     const DecDesc d = desc[b];
     const int cap = which ? NB : NA;
     int r = -1;
