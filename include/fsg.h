@@ -130,6 +130,9 @@ typedef struct fsg_ctx_config {
 } fsg_ctx_config;
 
 int fsg_ctx_create(const fsg_ctx_config *cfg, fsg_ctx **out);
+/* Waits for the context's stream and frees it.  Destroy the context's
+ * serving engines and worlds first: they keep a pointer to it, so no call
+ * on them (fsg_world_destroy included) is valid after this one. */
 int fsg_ctx_destroy(fsg_ctx *ctx);
 /* Bytes of world state currently held by the context. */
 int fsg_ctx_usage(fsg_ctx *ctx, uint64_t *bytes_used);
