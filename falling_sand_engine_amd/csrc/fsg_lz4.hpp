@@ -66,18 +66,17 @@ __host__ __device__ inline void match_lane(uint8_t *d, int off, int len, int lan
 // marks a value that is the same in every lane (it then lives in a scalar
 // register on the device), lane_value(v, k, row) is lane k's v in every lane
 // and own(v, lane, row) this lane's v (row: the bytes the lanes hold, which
-// the host reads instead of the lanes' registers); rcp(x) is 1 / x to within
-// an ulp or so (mod_by corrects its quotient).
+// the host reads instead of the lanes' registers); mod(i) is match table
+// entry i (LDS on the device, filled before the decode).
 constexpr int IW = 1024, RING = 4096, NL = 64;
 
-// j mod off for 0 <= j < 2^20 and off >= 1, with inv = 1 / off: the float
-// quotient is off by at most one, which the two corrections fix.
-__host__ __device__ inline int mod_by(int j, int off, float inv) {
-    const int q = (int)((float)j * inv);
-    int r = j - q * off;
-    if (r < 0) r += off;
-    else if (r >= off) r -= off;
-    return r;
+// The match table: entry off * NL + j is j mod off for 1 <= off < NL and
+// j < NL (MODTAB bytes; entry i is modtab_entry(i)).  A short-offset match
+// reads it instead of dividing.
+constexpr int MODTAB = NL * NL;
+__host__ __device__ inline uint8_t modtab_entry(int i) {
+    const int off = i / NL, j = i % NL;
+    return off ? (uint8_t)(j % off) : (uint8_t)0;
 }
 
 template <class Lanes>
@@ -166,14 +165,16 @@ struct Staged {
             }
             return;
         }
-        const float inv = L.rcp((float)off);
-        int P = off * ((int)((float)(NL - 1) * inv) + 1);  // the smallest multiple of off that is >= NL
-        if (P < NL) P += off;
-        else if (P - off >= NL) P -= off;
+        // P, the smallest multiple of off that is >= NL (only matches longer
+        // than NL need it): NL - 1 = q * off + r gives P = NL - 1 - r + off.
+        // The first group's bytes j = lane < P come from op - off + (j mod off)
+        // (the table); a later byte j from op + j - P, which for j < P is the
+        // same as op - off + (j mod off), since then j mod off = j - (P - off).
+        const int P = len > NL ? NL - 1 - (int)L.uniform(L.mod(off * NL + NL - 1)) + off : 0;
         for (int g = 0; g < len; g += NL) {
             L.each([&](int lane) {
                 const int j = g + lane;
-                if (j < len) put(op + j, ring[(j < P ? op - off + mod_by(j, off, inv) : op + j - P) & (RING - 1)]);
+                if (j < len) put(op + j, ring[(g == 0 ? op - off + (int)L.mod(off * NL + lane) : op + j - P) & (RING - 1)]);
             });
             if (g + NL < len) L.barrier();
         }

@@ -371,6 +371,10 @@ struct DecDesc {
     int32_t n, ok;
 };
 
+// The match table of the decode kernel (LDS, filled by each workgroup before
+// its waves decode; read only after that).
+__shared__ uint8_t dec_modtab[fsg_lz4::MODTAB];
+
 // The lane policy of the decode kernel: this thread's own lane.  A wave's
 // window and ring are its own and a wave's LDS accesses execute in order, so
 // ordering the lanes' accesses needs only wavefront-scope fences around a
@@ -393,7 +397,7 @@ struct WaveLane {
         return (uint32_t)__builtin_amdgcn_readlane((int)v, k);
     }
     __device__ uint32_t own(uint32_t v, int, const uint8_t *) const { return v; }
-    __device__ float rcp(float x) const { return __builtin_amdgcn_rcpf(x); }
+    __device__ uint32_t mod(int i) const { return dec_modtab[i]; }
 };
 
 // One wave per block, four blocks per workgroup (a workgroup per wave would
@@ -411,6 +415,8 @@ __global__ __launch_bounds__(64 * DW) __attribute__((amdgpu_waves_per_eu(6))) vo
     const uint8_t *in, const DecDesc *desc, uint8_t *raw, int32_t *status, int nblocks) {
     __shared__ __align__(16) uint8_t win[DW][fsg_lz4::IW];
     __shared__ uint8_t ring[DW][fsg_lz4::RING];
+    for (int i = threadIdx.x; i < fsg_lz4::MODTAB; i += 64 * DW) dec_modtab[i] = fsg_lz4::modtab_entry(i);
+    __syncthreads();
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int v = blockIdx.x * DW + wv, n = nblocks >> 1;
     if (v >= nblocks) return;

@@ -20,7 +20,7 @@ struct HostLanes {
     // the lanes' registers are the staged row on the host
     uint32_t lane_value(uint32_t, int k, const uint8_t *row) const { return row[k]; }
     uint32_t own(uint32_t, int lane, const uint8_t *row) const { return row[lane]; }
-    float rcp(float x) const { return 1.0f / x; }
+    uint32_t mod(int i) const { return fsg_lz4::modtab_entry(i); }
 };
 
 struct AllLanes {
