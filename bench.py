@@ -26,16 +26,25 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE = 16
 
 
+PROFILE = os.path.join(ROOT, "profiles", "k_tick_world_16384_all_active.md")
+
+
+def profile_field(side, key):
+    """A `key: value` line of the committed rocprofv3 summary of the tick
+    kernel on the 16384^2 world, or None."""
+    if side != 16384 or not os.path.exists(PROFILE):
+        return None
+    for line in open(PROFILE):
+        if line.startswith(key + ":"):
+            return line.split(":", 1)[1].strip()
+    return None
+
+
 def profiled_traffic(side):
     """HBM bytes per tick launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE
     (copy-kernel calibrated) in the committed profile summary, or None."""
-    path = os.path.join(ROOT, "profiles", "k_tick_world_16384_all_active.md")
-    if side != 16384 or not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return int(line.split(":")[1])
-    return None
+    v = profile_field(side, "traffic_bytes_per_launch")
+    return None if v is None else int(v)
 
 
 def log(*a):
@@ -132,42 +141,60 @@ def uneven_batch_sizes(n=256, seed=4):
 def bench_c4(fsg, ticks=256, n=256, tune_ticks=32):
     """256 worlds of uneven sizes: one launch per tick over all of their
     chunks (HIP graphs of ticks, launch shape picked by the autotuner on
-    copies of the same worlds) vs one launch per world per tick."""
+    copies of the same worlds) vs one launch per world per tick.  Per-world
+    launches run with the same dirty-chunk skipping as the tuned batch, so
+    the headline ratio is batching alone; the batch with skipping off is
+    timed against per-world launches without it as well."""
     sizes = uneven_batch_sizes(n)
     with fsg.Context(0) as ctx:
-        sets = []
-        for _ in range(2):
+        def world_set(skip):
             ws = []
             for i, (W, H) in enumerate(sizes):
-                w = fsg.World(ctx, W, H, 1000 + i, fsg.default_config())
-                w.upload(fsg.generate(4, W, H, 1000 + i))
+                w = fsg.World(ctx, W, H, 1000 + i, fsg.default_config(chunk_skip=skip))
+                w.generate(4, 1000 + i)
                 ws.append(w)
-            sets.append(ws)
+            return ws
+        tune_set = world_set(0)
         t0 = time.perf_counter()
-        tune = ctx.autotune(sets[0], sample_ticks=tune_ticks)
+        tune = ctx.autotune(tune_set, sample_ticks=tune_ticks)
         tune_s = time.perf_counter() - t0
-        fsg.batch_step(ctx, sets[0], 16)        # warm-up + graph capture
-        for w in sets[1]:
-            w.tick(16)
-        sets[1][0].sync()
-        t0 = time.perf_counter()
-        ms_b = fsg.batch_step_timed(ctx, sets[0], ticks)
-        wall_b = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        ms_p = fsg.perworld_step_timed(ctx, sets[1], ticks)
-        wall_p = time.perf_counter() - t0
-        exact = all(a.stats().checksum == b.stats().checksum for a, b in zip(*sets))
-        for ws in sets:
+        for w in tune_set:
+            w.close()
+        best = tune["best"]
+        skip = 1 if best["chunk_skip"] == 1 else 0
+        runs = {}
+        # batched with the tuned choice; batched with skipping off; per-world
+        # with the batch's skipping; per-world without skipping
+        for name, batched, sk in (("batched", True, None), ("batched_noskip", True, 0),
+                                  ("perworld", False, skip), ("perworld_noskip", False, 0)):
+            ws = world_set(skip if sk is None else sk)
+            if batched:
+                ctx.set_tuning(best["graph_ticks"], best["chunk_skip"] if sk is None else 0, best["occupancy"])
+                fsg.batch_step(ctx, ws, 16)  # warm-up + graph capture
+            else:
+                for w in ws:
+                    w.tick(16)
+            ws[0].sync()
+            t0 = time.perf_counter()
+            ms = fsg.batch_step_timed(ctx, ws, ticks) if batched else fsg.perworld_step_timed(ctx, ws, ticks)
+            wall = time.perf_counter() - t0
+            runs[name] = {"wall": wall, "ms": ms, "sums": [w.stats().checksum for w in ws]}
             for w in ws:
                 w.close()
+        ctx.set_tuning(best["graph_ticks"], best["chunk_skip"], best["occupancy"])
+    exact = all(r["sums"] == runs["batched"]["sums"] for r in runs.values())
     cells = sum(W * H for W, H in sizes)
     cand = sorted(tune["candidates"], key=lambda c: c["ms"])
+    g = {k: round(cells * ticks / r["wall"] / 1e9, 2) for k, r in runs.items()}
     return {"workload": f"{n} worlds, sides log-uniform [256,2048], aspect [0.5,2], {ticks} ticks",
             "cells": cells,
-            "batched_gcell_s": round(cells * ticks / wall_b / 1e9, 2),
-            "perworld_gcell_s": round(cells * ticks / wall_p / 1e9, 2),
-            "ratio": round(wall_p / wall_b, 3),
-            "batched_dev_ms": round(ms_b, 2), "perworld_dev_ms": round(ms_p, 2),
+            "batched_gcell_s": g["batched"], "perworld_gcell_s": g["perworld"],
+            "ratio": round(runs["perworld"]["wall"] / runs["batched"]["wall"], 3),
+            "ratio_note": f"per-world launches with the batch's chunk skipping ({skip}): batching alone",
+            "batched_noskip_gcell_s": g["batched_noskip"], "perworld_noskip_gcell_s": g["perworld_noskip"],
+            "ratio_noskip": round(runs["perworld_noskip"]["wall"] / runs["batched_noskip"]["wall"], 3),
+            "ratio_tuned_vs_perworld_noskip": round(runs["perworld_noskip"]["wall"] / runs["batched"]["wall"], 3),
+            "dev_ms": {k: round(r["ms"], 2) for k, r in runs.items()},
             "bit_exact": exact,
             "tuning": tune["best"],
             "autotune": {"sample_ticks": tune_ticks, "seconds": round(tune_s, 2),
@@ -560,7 +587,10 @@ def main():
         "config": {"workload": f"all-active striped world {args.side}x{args.side}, reactions off, "
                                f"temperature on, {args.steps} ticks",
                    "cells": r["cells"], "cell_iter": 3, "chunk": "64x64", "parallelism": "none (1 GPU)"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        # bound: what the kernel's counters say limits it (the summary's "bound" line: "valu" while
+        # VALU issue takes most of the launch); achieved / peak are against the HBM roofline
+        "roofline": {"bound": profile_field(args.side, "bound") or "hbm", "roofline": "hbm",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.side),
                      "traffic_source": "profiles/k_tick_world_16384_all_active.md (rocprofv3 FETCH_SIZE x2.00 "
                                        "+ WRITE_SIZE x1.00, bytes per launch)",

@@ -16,8 +16,8 @@ import numpy as np
 
 from . import _abi
 from ._abi import (AIR, GAS, LAYER_CELLS, LAYER_EMISSION, LAYER_FIRE, MOVED, SAND, SOLID, SOUP,  # noqa: F401
-                   Body, Cell, CellEdit, Config, FsgError, RenderParams, RenderStats, Stats, StampResult,
-                   check, lib)
+                   Body, Cell, CellEdit, Config, FsgError, PackMap, RenderParams, RenderStats, Stats,
+                   StampResult, check, lib)
 
 __all__ = ["Context", "World", "default_config", "generate", "make_cell", "cell_mat",
            "cell_temp", "cell_rgba", "batch_step", "FsgError", "Body", "Config"]
@@ -38,6 +38,57 @@ def default_config(**overrides) -> Config:
     for k, v in overrides.items():
         setattr(cfg, k, v)
     return cfg
+
+
+# Material ids of the reference's own material list, in registration order
+# (game_datastruct.cpp:72-110): the indices its .pack records hold.
+REFERENCE_MATERIAL_IDS = {
+    "GENERIC_AIR": 0, "GENERIC_SOLID": 1, "GENERIC_SAND": 2, "GENERIC_LIQUID": 3, "GENERIC_GAS": 4,
+    "GENERIC_PASSABLE": 5, "GENERIC_OBJECT": 6, "STONE": 7, "GRASS": 8, "DIRT": 9, "SMOOTH_STONE": 10,
+    "COBBLE_STONE": 11, "SMOOTH_DIRT": 12, "COBBLE_DIRT": 13, "SOFT_DIRT": 14, "WATER": 15, "LAVA": 16,
+    "CLOUD": 17, "GOLD_ORE": 18, "GOLD_MOLTEN": 19, "GOLD_SOLID": 20, "IRON_ORE": 21, "OBSIDIAN": 22,
+    "STEAM": 23, "SOFT_DIRT_SAND": 24, "FIRE": 25, "FLAT_COBBLE_STONE": 26, "FLAT_COBBLE_DIRT": 27,
+}
+
+
+def pack_map_default(cfg: Config = None) -> PackMap:
+    """index = material id (fsg_pack_map_default)."""
+    m = PackMap()
+    check(lib().fsg_pack_map_default(C.byref(cfg if cfg is not None else default_config()), C.byref(m)),
+          "fsg_pack_map_default")
+    return m
+
+
+def reference_pack_map(keep_moved: bool = True) -> PackMap:
+    """Map between the benchmark materials and the reference's material
+    list, so .pack files carry the reference's indices.  Written: air ->
+    GENERIC_AIR, stone -> STONE, sand -> GENERIC_SAND, water -> WATER, oil ->
+    GENERIC_LIQUID (the reference has no oil), steam -> STEAM, lava -> LAVA,
+    wood -> GENERIC_SOLID (no wood either).  Read: those indices back, and
+    every other reference material by its PhysicsType (game_datastruct.cpp:
+    72-110): SOLID and OBJECT kinds as stone (OBSIDIAN is the stone the
+    water+lava reaction makes here), SAND kinds as sand, GOLD_MOLTEN as lava,
+    GENERIC_GAS as steam, PASSABLE kinds (FIRE) as air."""
+    R = REFERENCE_MATERIAL_IDS
+    m = PackMap()
+    write = {MAT_AIR: "GENERIC_AIR", MAT_STONE: "STONE", MAT_SAND: "GENERIC_SAND", MAT_WATER: "WATER",
+             MAT_OIL: "GENERIC_LIQUID", MAT_STEAM: "STEAM", MAT_LAVA: "LAVA", MAT_WOOD: "GENERIC_SOLID"}
+    for i in range(256):
+        m.from_file[i] = -1
+    for mid, name in write.items():
+        m.to_file[mid] = R[name]
+        m.from_file[R[name]] = mid
+    for name in ("SMOOTH_STONE", "COBBLE_STONE", "SMOOTH_DIRT", "COBBLE_DIRT", "SOFT_DIRT", "CLOUD",
+                 "GOLD_SOLID", "OBSIDIAN", "FLAT_COBBLE_STONE", "FLAT_COBBLE_DIRT", "GENERIC_OBJECT"):
+        m.from_file[R[name]] = MAT_STONE
+    for name in ("GRASS", "DIRT", "GOLD_ORE", "IRON_ORE", "SOFT_DIRT_SAND"):
+        m.from_file[R[name]] = MAT_SAND
+    m.from_file[R["GOLD_MOLTEN"]] = MAT_LAVA
+    m.from_file[R["GENERIC_GAS"]] = MAT_STEAM
+    m.from_file[R["GENERIC_PASSABLE"]] = MAT_AIR
+    m.from_file[R["FIRE"]] = MAT_AIR
+    m.flags = 0 if keep_moved else _abi.PACK_NO_MOVED
+    return m
 
 
 def make_cell(mat, temp=320, rgba=0):
@@ -319,6 +370,12 @@ class World:
             out.append(buf[o:o + int(v)].tobytes())
             o += int(v)
         return out
+
+    def set_pack_map(self, pack_map=None):
+        """Material index map of later pack()/unpack() calls (a PackMap, e.g.
+        reference_pack_map(); None restores index = material id)."""
+        check(lib().fsg_world_set_pack_map(self.handle, None if pack_map is None else C.byref(pack_map)),
+              "fsg_world_set_pack_map")
 
     def unpack(self, x0, y0, ncx, ncy, images=None, want_layer2=False, want_background=False):
         """Chunk::ChunkRead (chunk.cpp:74-217) of ncx x ncy images into the grid

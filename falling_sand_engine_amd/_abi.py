@@ -122,6 +122,10 @@ class RenderStats(C.Structure):
                 ("reserved", C.c_uint32)]
 
 
+class PackMap(C.Structure):
+    _fields_ = [("to_file", C.c_uint32 * 16), ("from_file", C.c_int16 * 256), ("flags", C.c_uint32)]
+
+
 P = C.c_void_p
 u64p = C.POINTER(C.c_uint64)
 
@@ -146,6 +150,8 @@ SIGNATURES = {
     "fsg_world_pack": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, P, P]),
     "fsg_world_pack_download": (C.c_int, [P, P, C.c_size_t]),
     "fsg_world_unpack": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, P, P, P, P]),
+    "fsg_world_set_pack_map": (C.c_int, [P, C.POINTER(PackMap)]),
+    "fsg_pack_map_default": (C.c_int, [C.POINTER(Config), C.POINTER(PackMap)]),
     "fsg_world_set_cells": (C.c_int, [P, P, C.c_size_t]),
     "fsg_world_step": (C.c_int, [P, C.c_int]),
     "fsg_world_step_timed": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
@@ -217,3 +223,4 @@ def check(status, what=""):
 
 
 CHUNK = 128  # FSG_CHUNK: chunk side of the .pack images (const.h:19-20)
+PACK_NO_MOVED = 1  # FSG_PACK_NO_MOVED

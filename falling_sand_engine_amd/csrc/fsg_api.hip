@@ -460,6 +460,7 @@ extern "C" int fsg_world_destroy(fsg_world *w) {
     dev_free(w->d_disp);
     dev_free(w->d_dispxy);
     dev_free(w->d_tgt);
+    dev_free(w->d_save);
     for (uint32_t *l : w->d_layer) dev_free(l);
     dev_free(w->d_tally);
     dev_free(w->d_isl);
@@ -671,6 +672,7 @@ fsg::StampWorld stamp_world_of(fsg_world *w, uint32_t pix_start) {
     sw.disp = w->d_disp;
     sw.disp_xy = w->d_dispxy;
     sw.tgt = w->d_tgt;
+    sw.save = nullptr;
     sw.W = w->d.W;
     sw.H = w->d.H;
     sw.tiles_x = w->d.tiles_x;
@@ -685,7 +687,7 @@ fsg::StampWorld stamp_world_of(fsg_world *w, uint32_t pix_start) {
 // leaves open are then compacted into one list that rounds 1-4 (and the
 // unstamp's last step) walk, so those rounds cost what is left, not npix.
 int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t total, uint32_t max_npix,
-                 const MatTab &M, bool unstamp) {
+                 const MatTab &M, bool unstamp, bool save) {
     if (total == 0 || max_npix == 0) return FSG_OK;
     std::lock_guard<std::mutex> lk(c->stamp_mu);
     if (!c->d_nopen) {
@@ -712,10 +714,10 @@ int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t tot
     const int propose = unstamp ? fsg::OP_UNPROPOSE : fsg::OP_PROPOSE;
     const int resolve = unstamp ? fsg::OP_UNRESOLVE : fsg::OP_RESOLVE;
     const dim3 g(grid_for(max_npix, 256), (unsigned)n);  // y: world, x: its pixels
-    // small batches clear every target slot up front (fewer launches), large
-    // ones only the slots their open pixels reach in each round (less work)
-    const bool all_slots = total <= (1u << 20);
-    fsg::k_claims_init<<<g, 256, 0, c->stream>>>(a, all_slots ? 1 : 0);
+    // every target slot is cleared up front; with save its old contents are
+    // copied out first and put back after the last round
+    fsg::k_claims_init<<<g, 256, 0, c->stream>>>(a, save ? 1 : 0);
+    if (save) fsg::k_claims_clear<<<g, 256, 0, c->stream>>>(a);
     fsg::k_stamp_step<<<g, 256, 0, c->stream>>>(a, propose, 0);
     fsg::k_stamp_step<<<g, 256, 0, c->stream>>>(a, resolve, 0);
     fsg::k_zero_u32<<<1, 1, 0, c->stream>>>(a.nopen);
@@ -724,11 +726,11 @@ int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t tot
     // the list holds at most `total` entries; the loop kernels stride over it
     const unsigned gl = std::min(grid_for(total, 256), 1024u);
     for (int d = 1; d < 5; d++) {
-        if (!all_slots) fsg::k_stamp_step_list<<<gl, 256, 0, c->stream>>>(a, fsg::OP_CLAIMS, d);
         fsg::k_stamp_step_list<<<gl, 256, 0, c->stream>>>(a, propose, d);
         fsg::k_stamp_step_list<<<gl, 256, 0, c->stream>>>(a, resolve, d);
     }
     if (unstamp) fsg::k_stamp_step_list<<<gl, 256, 0, c->stream>>>(a, fsg::OP_UNFINISH, 0);
+    if (save) fsg::k_claims_restore<<<g, 256, 0, c->stream>>>(a);
     HIPCHK(hipGetLastError());
     return FSG_OK;
 }
@@ -741,9 +743,21 @@ int stamp_single(fsg_world *w, bool unstamp) {
         if (dev_alloc(c, &p, sizeof(fsg::StampWorld))) return FSG_ERR_NOMEM;
         w->d_sw = (fsg::StampWorld *)p;
     }
+    if (w->save_cap < w->npix) {  // five saved claim slots per pixel
+        dev_free(w->d_save);
+        w->d_save = nullptr;
+        w->save_cap = 0;
+        void *p;
+        if (dev_alloc(c, &p, w->npix * 5 * 8)) return FSG_ERR_NOMEM;
+        w->d_save = (uint64_t *)p;
+        w->save_cap = w->npix;
+    }
     w->sw_host = stamp_world_of(w, 0);
+    // the spare buffer's claim slots are put back afterwards, so it still
+    // holds the state before the last tick (the texture pass's census)
+    w->sw_host.save = w->d_save;
     HIPCHK(hipMemcpyAsync(w->d_sw, &w->sw_host, sizeof(fsg::StampWorld), hipMemcpyHostToDevice, c->stream));
-    return stamp_rounds(c, w->d_sw, 1, (uint32_t)w->npix, (uint32_t)w->npix, w->d.mat, unstamp);
+    return stamp_rounds(c, w->d_sw, 1, (uint32_t)w->npix, (uint32_t)w->npix, w->d.mat, unstamp, true);
 }
 
 int ensure_stamp_buffers(fsg_world *w, size_t npix, int nb) {

@@ -90,6 +90,8 @@ struct fsg_world {
     size_t pack_n = 0, pack_total = 0;     // chunks and bytes of the last fsg_world_pack
     std::vector<int32_t> pack_sizes;       // its block sizes (A, B per chunk)
     std::vector<int8_t> pack_phase;        // its generation phases
+    bool pack_map_set = false;             // pack_map holds a caller's map (else the default)
+    fsg_pack_map pack_map{};
     size_t bytes = 0;
     uint64_t offered = 0;
     unsigned long long *d_stats = nullptr;  // [17] census scratch + [1] processed counter
@@ -100,6 +102,8 @@ struct fsg_world {
     uint8_t *d_state = nullptr;
     uint64_t *d_disp = nullptr, *d_dispxy = nullptr;
     uint64_t *d_tgt = nullptr;  // per pixel: own cell under the pose (stamp kernels)
+    uint64_t *d_save = nullptr; // per pixel x 5: claim slot contents kept across a stamp/unstamp
+    size_t save_cap = 0;        // pixels d_save holds (single-world stamps only)
     size_t pix_cap = 0;
     int body_cap = 0;
     size_t npix = 0;
@@ -126,8 +130,9 @@ int batch_run(fsg_ctx *c, fsg_world *const *ws, int n, int ticks, bool graph_ok)
 fsg::StampWorld stamp_world_of(fsg_world *w, uint32_t pix_start);
 // Stamp (or unstamp) claim rounds over a device table of n worlds.
 // max_npix: the largest pixel count of one world (the launch's x extent).
+// save: the claim slots of the spare buffer are restored afterwards.
 int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t total, uint32_t max_npix,
-                 const MatTab &M, bool unstamp);
+                 const MatTab &M, bool unstamp, bool save);
 // Enqueues w's census (counts [0,16), checksum [16], processed chunks [17])
 // into host[18], valid after the context's stream is synchronised.
 int stats_enqueue(fsg_world *w, unsigned long long *host);

@@ -699,8 +699,12 @@ __device__ __forceinline__ bool pixel_target(const PixelRef &ref, int d, int &wx
     return wx >= 0 && wy >= 0 && wx < ref.w->W && wy < ref.w->H;
 }
 
+// Q16.16 value to int, rounding toward zero like the reference's (int) cast.
+__device__ __forceinline__ int q16_trunc(int64_t v) { return (int)(v >= 0 ? v >> 16 : -((-v) >> 16)); }
+
 // Own cell of pixel p under its body's pose (Q16.16 rotation of the pixel's
-// lower-left corner, as the reference rotates (tx, ty + 1)), or NOT_LIVE.
+// lower-left corner, as the reference rotates (tx, ty + 1), truncated toward
+// zero: game.cpp:1774-1775), or NOT_LIVE.
 __device__ __forceinline__ uint64_t own_cell(const StampArgs &a, const StampWorld &W, uint32_t p) {
     if (bit(a.mat.air, cid(W.pixels[p]))) return NOT_LIVE;
     const uint32_t b = W.pix_body[p];
@@ -711,7 +715,7 @@ __device__ __forceinline__ uint64_t own_cell(const StampArgs &a, const StampWorl
     const int tx = (int)(k % (uint32_t)B.mat_w), ty = (int)(k / (uint32_t)B.mat_w);
     const int64_t fx = (int64_t)tx * B.cos_q16 - (int64_t)(ty + 1) * B.sin_q16 + B.pos_x_q16;
     const int64_t fy = (int64_t)tx * B.sin_q16 + (int64_t)(ty + 1) * B.cos_q16 + B.pos_y_q16;
-    return (uint64_t)(uint32_t)(int)(fx >> 16) | ((uint64_t)(uint32_t)(int)(fy >> 16) << 32);
+    return (uint64_t)(uint32_t)q16_trunc(fx) | ((uint64_t)(uint32_t)q16_trunc(fy) << 32);
 }
 
 // Chunks whose spare-buffer cells serve as claim slots must run on the next
@@ -731,18 +735,25 @@ __device__ __forceinline__ void mark_tiles_wave(uint32_t *flags, bool on, uint32
     }
 }
 
-// Every live pixel's own cell, and the claim slots it may target, cleared
-// before any round: with all_slots the slots of all five cells (the own cell
-// and its four neighbours), else only the own cell's, and each later round
-// clears its open pixels' slots first (OP_CLAIMS over the list).  Both give
-// the same result: a slot that receives proposals in some round gets a
-// winner, which fills the cell (stamp) or empties it (unstamp), and a later
-// round's proposal to that cell fails the cell check of its propose step
-// whatever the slot holds -- so the earliest round and then the lowest pixel
-// win, as a round-by-round pass over the pixels gives.  Clearing all five up
-// front saves the per-round launches (small batches: latency); clearing per
-// round writes only the open pixels' slots (large batches: work).
-__global__ void k_claims_init(StampArgs a, int all_slots) {
+// Every live pixel's own cell, and the claim slots of its five targets (the
+// own cell and its four neighbours) cleared before any round.  A slot that
+// receives proposals in some round gets a winner, which fills the cell
+// (stamp) or empties it (unstamp); a later round's proposal to that cell
+// then either fails the cell check of its propose step or, for a SAND or
+// SOUP typed body pixel (which leaves the cell a stamping target), loses
+// the atomicMin to the earlier round's claim still in the slot -- so the
+// earliest round and then the lowest pixel win, as a round-by-round pass
+// over the pixels gives.  With save the slots' contents (the state before
+// the last tick) are first copied out, and k_claims_clear clears them in a
+// launch of its own, after every copy; k_claims_restore puts them back.
+__device__ __forceinline__ void slot_at(const StampWorld &W, uint64_t t, int d, bool live, bool &on, int &wx,
+                                        int &wy) {
+    wx = (int)(uint32_t)t + kDX[d];
+    wy = (int)(uint32_t)(t >> 32) + kDY[d];
+    on = live && wx >= 0 && wy >= 0 && wx < W.W && wy < W.H;
+}
+
+__global__ void k_claims_init(StampArgs a, int save) {
     const StampWorld &W = a.worlds[blockIdx.y];
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = p < W.npix;
@@ -753,13 +764,47 @@ __global__ void k_claims_init(StampArgs a, int all_slots) {
         W.tgt[p] = t;
     }
     const bool live = in && (uint32_t)t != NOT_LIVE;
-    const int ox = (int)(uint32_t)t, oy = (int)(uint32_t)(t >> 32);
-    const int nd = all_slots ? 5 : 1;
-    for (int d = 0; d < nd; d++) {
-        const int wx = ox + kDX[d], wy = oy + kDY[d];
-        const bool on = live && wx >= 0 && wy >= 0 && wx < W.W && wy < W.H;
+    for (int d = 0; d < 5; d++) {
+        bool on;
+        int wx, wy;
+        slot_at(W, t, d, live, on, wx, wy);
+        if (save) {
+            if (on) W.save[5 * (size_t)p + d] = W.claims[(size_t)wy * W.W + wx];
+            continue;
+        }
         if (on) W.claims[(size_t)wy * W.W + wx] = ~0ull;
         if (W.flags) mark_tiles_wave(W.flags, on, on ? (uint32_t)((wy / TH) * W.tiles_x + wx / TW) : 0u);
+    }
+}
+
+__global__ void k_claims_clear(StampArgs a) {
+    const StampWorld &W = a.worlds[blockIdx.y];
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = p < W.npix;
+    const uint64_t t = in ? W.tgt[p] : NOT_LIVE;
+    const bool live = in && (uint32_t)t != NOT_LIVE;
+    for (int d = 0; d < 5; d++) {
+        bool on;
+        int wx, wy;
+        slot_at(W, t, d, live, on, wx, wy);
+        if (on) W.claims[(size_t)wy * W.W + wx] = ~0ull;
+        if (W.flags) mark_tiles_wave(W.flags, on, on ? (uint32_t)((wy / TH) * W.tiles_x + wx / TW) : 0u);
+    }
+}
+
+// Every pixel that saved a slot writes the saved value back; pixels sharing
+// a slot saved the same value, so the writes agree.
+__global__ void k_claims_restore(StampArgs a) {
+    const StampWorld &W = a.worlds[blockIdx.y];
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= W.npix || !W.save) return;
+    const uint64_t t = W.tgt[p];
+    const bool live = (uint32_t)t != NOT_LIVE;
+    for (int d = 0; d < 5; d++) {
+        bool on;
+        int wx, wy;
+        slot_at(W, t, d, live, on, wx, wy);
+        if (on) W.claims[(size_t)wy * W.W + wx] = W.save[5 * (size_t)p + d];
     }
 }
 
@@ -777,10 +822,6 @@ __device__ __forceinline__ void stamp_step(const StampArgs &a, const PixelRef &r
     const size_t ci = (size_t)wy * W.W + wx;
     const uint64_t mine = ((uint64_t)d << 32) | ref.p;
     switch (op) {
-    case OP_CLAIMS:  // lanes may be of different worlds here: one atomic each
-        W.claims[ci] = ~0ull;
-        if (W.flags) atomicOr(&W.flags[(wy / TH) * W.tiles_x + (wx / TW)], F_WAKE_ALL);
-        break;
     case OP_PROPOSE: {
         const uint32_t m = cid(W.cells[ci]);
         if (bit(a.mat.air, m) || bit(a.mat.sand, m) || bit(a.mat.soup, m))

@@ -18,6 +18,7 @@ struct StampWorld {
     uint64_t *disp;            // displaced cell per pixel
     uint64_t *disp_xy;         // its position (x | y << 32)
     uint64_t *tgt;             // per pixel: its own cell under the pose, x | y << 32 (x = INT32_MIN: not live)
+    uint64_t *save;            // per pixel x 5: the claim slots' contents before the claims (null: not kept)
     int32_t W, H, tiles_x;
     uint32_t npix, pix_start, pad;
 };
@@ -44,9 +45,11 @@ __global__ void k_stats(const uint64_t *__restrict__ cells, size_t n, unsigned l
 __global__ void k_fill(uint64_t *cells, size_t n, uint64_t v);
 __global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16);
 __global__ void k_set_cells(uint64_t *cells, int W, int H, const int32_t *xy, const uint64_t *vals, size_t n);
-__global__ void k_claims_init(StampArgs a, int all_slots);
+__global__ void k_claims_init(StampArgs a, int save);
+__global__ void k_claims_clear(StampArgs a);
+__global__ void k_claims_restore(StampArgs a);
 // Per-pixel steps of a stamp claim round (k_stamp_step, k_stamp_step_list).
-enum StampOp { OP_CLAIMS = 0, OP_PROPOSE, OP_RESOLVE, OP_UNPROPOSE, OP_UNRESOLVE, OP_UNFINISH };
+enum StampOp { OP_PROPOSE = 1, OP_RESOLVE, OP_UNPROPOSE, OP_UNRESOLVE, OP_UNFINISH };
 __global__ void k_stamp_step(StampArgs a, int op, int d);
 __global__ void k_stamp_step_list(StampArgs a, int op, int d);
 constexpr int STAMP_CPT = 8;  // pixels per thread of k_stamp_compact (grid x: npix / (256 * STAMP_CPT))

@@ -7,8 +7,11 @@
 //   LZ4 block of the 12-byte MaterialInstanceData records {u32 material
 //   index, u32 colour, i16 temperature, 2 pad bytes} of tiles[] then layer2[],
 //   LZ4 block of the u32 background.
-// The cell's material field (with its moved bit), colour and temperature map
-// one to one onto a record, so an image round-trips a chunk exactly.
+// A record's index is the file's material index of the cell's material id
+// (the world's pack map; by default the id itself), its colour and
+// temperature are the cell's, and the cell's MOVED flag travels in bit 0 of
+// pad byte 10, which ChunkRead ignores; so an image round-trips a chunk
+// exactly, and every index it holds is a material the reader knows.
 //
 // Encoding: one workgroup per chunk and block.  The reference asks LZ4 for a
 // hash-chain search (LZ4_compress_fast, acceleration 10); any parse of the
@@ -51,30 +54,40 @@ struct PackSrc {
     const uint64_t *l2;     // region layer2 cells, row width rw (null: empty)
     const uint32_t *bg;     // region background (null: 0)
     int rw, x0, y0, ncx;
+    uint64_t keep;          // cell bits a record carries: ~0x1FF, plus MOVED unless FSG_PACK_NO_MOVED
+    uint8_t to_file[16];    // file index of each material id
 };
+
+// A cell in record form: the material id replaced by its file index (< 256,
+// bits 0-7) and MOVED kept or dropped (bit 8); temperature and colour as
+// stored.  Record bytes are then plain bytes of this word (rbyte).
+__device__ __forceinline__ uint64_t rec_form(const PackSrc &s, uint64_t v) {
+    return (v & s.keep) | s.to_file[v & 15u];
+}
 
 // Record i of block A (tiles rows 0..127, then layer2) or B of chunk (cx, cy).
 __device__ __forceinline__ uint64_t rec_a(const PackSrc &s, int cx, int cy, int i) {
     const int row = (i >> 7) & (CW - 1), col = i & (CW - 1);
-    if (i < CW * CW) return s.cells[(size_t)(s.y0 + cy * CW + row) * s.W + (s.x0 + cx * CW + col)];
-    return s.l2 ? s.l2[(size_t)(cy * CW + row) * s.rw + cx * CW + col] : 0ull;
+    if (i < CW * CW) return rec_form(s, s.cells[(size_t)(s.y0 + cy * CW + row) * s.W + (s.x0 + cx * CW + col)]);
+    return rec_form(s, s.l2 ? s.l2[(size_t)(cy * CW + row) * s.rw + cx * CW + col] : 0ull);
 }
 __device__ __forceinline__ uint64_t rec_b(const PackSrc &s, int cx, int cy, int i) {
     const int row = i >> 7, col = i & (CW - 1);
     return s.bg ? s.bg[(size_t)(cy * CW + row) * s.rw + cx * CW + col] : 0u;
 }
 
-// Byte b of the record of cell value v: material index (u32), colour (u32),
-// temperature (i16), padding; for block B the u32 itself.
+// Byte b of the record of record-form value v: material index (u32, < 256),
+// colour (u32), temperature (i16), pad byte 10 (MOVED), pad byte 11 (0); for
+// block B the u32 itself.
 template <int R>
 __device__ __forceinline__ uint32_t rbyte(uint64_t v, int b) {
     if (R == RB) return (uint32_t)(v >> (8 * b)) & 0xFFu;
     switch (b) {
     case 0: return (uint32_t)v & 0xFFu;
-    case 1: return (uint32_t)(v >> 8) & 0xFFu;
     case 4: case 5: case 6: case 7: return (uint32_t)(v >> (32 + 8 * (b - 4))) & 0xFFu;
     case 8: return (uint32_t)(v >> 16) & 0xFFu;
     case 9: return (uint32_t)(v >> 24) & 0xFFu;
+    case 10: return (uint32_t)(v >> 8) & 0xFFu;
     default: return 0u;
     }
 }
@@ -442,11 +455,13 @@ struct UnpackDst {
     int W, x0, y0, ncx, rw;
     uint64_t *l2;     // region layer2 out (null: not kept)
     uint32_t *bg;     // region background out (null: not kept)
-    uint32_t n_materials;
+    uint32_t moved;   // MOVED taken from pad byte 10 (else clear)
+    int16_t from_file[256];  // material id of each file index, or -1
 };
 
 // One workgroup per chunk: records -> cells, after checking every material
-// index (a bad one fails the chunk and leaves its cells as they were).
+// index against the map (a bad one fails the chunk and leaves its cells as
+// they were).
 __global__ __launch_bounds__(256) void k_pack_cells(const uint8_t *raw, const int32_t *status, UnpackDst d,
                                                     int32_t *chunk_status) {
     __shared__ int bad;
@@ -461,8 +476,7 @@ __global__ __launch_bounds__(256) void k_pack_cells(const uint8_t *raw, const in
     int mybad = 0;
     for (int i = t; i < 2 * CW * CW; i += 256) {
         const uint32_t idx = ld32(r + (size_t)i * RA);
-        if (i < CW * CW) mybad |= (idx & 0xFFFFFE00u) != 0 || (idx & 0xFFu) >= d.n_materials;
-        else mybad |= idx > 0xFFFFu;
+        mybad |= idx > 0xFFu || d.from_file[idx & 0xFFu] < 0;
     }
     if (mybad) bad = 1;
     __syncthreads();
@@ -472,8 +486,9 @@ __global__ __launch_bounds__(256) void k_pack_cells(const uint8_t *raw, const in
     }
     for (int i = t; i < 2 * CW * CW; i += 256) {
         const uint8_t *q = r + (size_t)i * RA;
-        const uint64_t v = (uint64_t)(ld32(q) & 0xFFFFu) | ((uint64_t)(ld32(q + 8) & 0xFFFFu) << 16) |
-                           ((uint64_t)ld32(q + 4) << 32);
+        const uint32_t w2 = ld32(q + 8);  // temperature, pad byte 10, pad byte 11
+        const uint64_t v = (uint64_t)(uint16_t)d.from_file[ld32(q)] | (d.moved ? (w2 >> 8) & 0x100u : 0u) |
+                           ((uint64_t)(w2 & 0xFFFFu) << 16) | ((uint64_t)ld32(q + 4) << 32);
         const int row = (i >> 7) & (CW - 1), col = i & (CW - 1);
         if (i < CW * CW) d.cells[(size_t)(d.y0 + cy * CW + row) * d.W + (d.x0 + cx * CW + col)] = v;
         else if (d.l2) d.l2[(size_t)(cy * CW + row) * d.rw + cx * CW + col] = v;
@@ -509,14 +524,52 @@ int32_t rd_i32(const uint8_t *p) {
     return v;
 }
 
+void default_map(const fsg_config &cfg, fsg_pack_map *m) {
+    std::memset(m, 0, sizeof(*m));
+    for (int i = 0; i < 16; i++) m->to_file[i] = (uint32_t)i;
+    for (int i = 0; i < 256; i++) m->from_file[i] = (int16_t)((uint32_t)i < cfg.n_materials ? i : -1);
+}
+
+// The map the world's pack calls use.
+const fsg_pack_map &world_map(fsg_world *w) {
+    if (!w->pack_map_set) default_map(w->cfg, &w->pack_map);
+    return w->pack_map;
+}
+
 }  // namespace
+
+extern "C" int fsg_pack_map_default(const fsg_config *cfg, fsg_pack_map *map) {
+    if (!cfg || !map || cfg->n_materials > FSG_MAX_MATERIALS) return FSG_ERR_INVALID;
+    default_map(*cfg, map);
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_set_pack_map(fsg_world *w, const fsg_pack_map *map) {
+    if (!w) return FSG_ERR_INVALID;
+    if (!map) {
+        w->pack_map_set = false;
+        return FSG_OK;
+    }
+    for (uint32_t m = 0; m < w->cfg.n_materials; m++)
+        if (map->to_file[m] > 0xFFu) return FSG_ERR_INVALID;
+    for (int i = 0; i < 256; i++)
+        if (map->from_file[i] < -1 || map->from_file[i] >= (int)w->cfg.n_materials) return FSG_ERR_INVALID;
+    if (map->flags & ~FSG_PACK_NO_MOVED) return FSG_ERR_INVALID;
+    w->pack_map = *map;
+    w->pack_map_set = true;
+    return FSG_OK;
+}
 
 extern "C" int fsg_world_pack(fsg_world *w, int32_t x0, int32_t y0, int32_t ncx, int32_t ncy, const int8_t *phase,
                               const fsg_cell *layer2, const uint32_t *background, size_t *image_bytes) {
     if (!w || !image_bytes || !region_ok(w, x0, y0, ncx, ncy)) return FSG_ERR_INVALID;
+    const size_t n = (size_t)ncx * ncy, rcells = n * CW * CW;
+    if (layer2)  // layer2 records are cells of the world's materials too
+        for (size_t i = 0; i < rcells; i++)
+            if ((layer2[i].mat & 0xFE00u) || (uint32_t)(layer2[i].mat & 0xFFu) >= w->cfg.n_materials)
+                return FSG_ERR_INVALID;
     fsg_ctx *c = w->ctx;
     HIPCHK(hipSetDevice(c->device));
-    const size_t n = (size_t)ncx * ncy, rcells = n * CW * CW;
     // per-chunk device state: encoder slots, images (worst case), sizes, offsets, phases;
     // the region's layer2 / background when given
     const size_t b_small = ((n * 8 + (n + 1) * 8 + n + 255) & ~(size_t)255);
@@ -539,6 +592,9 @@ extern "C" int fsg_world_pack(fsg_world *w, int32_t x0, int32_t y0, int32_t ncx,
     s.x0 = x0;
     s.y0 = y0;
     s.ncx = ncx;
+    const fsg_pack_map &map = world_map(w);
+    s.keep = (map.flags & FSG_PACK_NO_MOVED) ? ~0x1FFull : ~0xFFull;
+    for (int m = 0; m < 16; m++) s.to_file[m] = (uint8_t)map.to_file[m];
     char *aux = static_cast<char *>(w->d_pack_aux);
     if (layer2) {
         HIPCHK(hipMemcpyAsync(aux, layer2, b_l2, hipMemcpyHostToDevice, c->stream));
@@ -643,7 +699,9 @@ extern "C" int fsg_world_unpack(fsg_world *w, int32_t x0, int32_t y0, int32_t nc
     d.rw = ncx * CW;
     d.l2 = layer2 ? reinterpret_cast<uint64_t *>(dec + b_desc + b_st) : nullptr;
     d.bg = background ? reinterpret_cast<uint32_t *>(dec + b_desc + b_st + b_l2) : nullptr;
-    d.n_materials = w->cfg.n_materials;
+    const fsg_pack_map &map = world_map(w);
+    d.moved = (map.flags & FSG_PACK_NO_MOVED) ? 0u : 1u;
+    std::memcpy(d.from_file, map.from_file, sizeof(d.from_file));
     HIPCHK(hipMemcpyAsync(d_desc, desc.data(), 2 * n * sizeof(DecDesc), hipMemcpyHostToDevice, c->stream));
     if (b_l2 + b_bg) HIPCHK(hipMemsetAsync(dec + b_desc + b_st, 0, b_l2 + b_bg, c->stream));  // failed chunks: 0
     uint8_t *raw = static_cast<uint8_t *>(w->d_pack_raw);

@@ -382,12 +382,13 @@ int stamp_batch(fsg_server *S, const std::vector<Resident *> &bw, bool unstamp, 
     for (int i = 0; i < n; i++) {
         h[i] = stamp_world_of(bw[i]->w, total);
         h[i].bodies = S->d_pose + S->pose_off[(size_t)t * n + i];
+        bw[i]->w->prev_valid = false;  // claim slots stay in the spare buffer (nothing renders served worlds)
         total += (uint32_t)bw[i]->w->npix;
         max_npix = std::max(max_npix, (uint32_t)bw[i]->w->npix);
     }
     fsg::StampWorld *d = S->d_table + slice;
     HIPCHK(hipMemcpyAsync(d, h, (size_t)n * sizeof(fsg::StampWorld), hipMemcpyHostToDevice, c->stream));
-    return stamp_rounds(c, d, n, total, max_npix, bw[0]->w->d.mat, unstamp);
+    return stamp_rounds(c, d, n, total, max_npix, bw[0]->w->d.mat, unstamp, false);
 }
 
 // Grows the pinned host / device staging of a step.

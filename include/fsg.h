@@ -9,7 +9,7 @@
  * directly.  Every entry point below names the reference member or loop it
  * replaces.  A C++ facade with the reference's method names (init, tick,
  * tickTemperature, getTile, setTile) sits on top of this ABI in
- * falling_sand_engine_amd/csrc/fsg_world.hpp.
+ * include/fsg_world.hpp.
  *
  * Conventions (all entry points):
  *   - extern "C", plain pointers and sizes, no C++ or torch types;
@@ -208,6 +208,34 @@ int fsg_world_pack_download(fsg_world *w, void *out, size_t cap);
 int fsg_world_unpack(fsg_world *w, int32_t x0, int32_t y0, int32_t ncx, int32_t ncy, const void *images,
                      const size_t *image_bytes, int8_t *phase, fsg_cell *layer2, uint32_t *background,
                      int32_t *status);
+
+/* Material index of the records.  ChunkWrite stores (u16)mat->id in the
+ * record's index (chunk.cpp:241-242) and ChunkRead indexes materials_array
+ * with it (chunk.cpp:167-177), so a file's indices are the reference's own
+ * material ids (GENERIC_AIR 0, STONE 7, WATER 15, LAVA 16, OBSIDIAN 22,
+ * STEAM 23: game_datastruct.cpp:72-110), not a world's.  to_file[m] is the
+ * index written for material id m (< 256); from_file[i] is the material id
+ * a record with index i reads as, or -1 for none (the chunk then fails, as
+ * does any index >= 256).  MOVED (MaterialInstance::moved) has no record
+ * field: it travels in bit 0 of the record's pad byte 10, which ChunkRead
+ * ignores; FSG_PACK_NO_MOVED writes the pad as zero and reads every cell
+ * with MOVED clear, as ChunkRead does.  The default map (and map = NULL) is
+ * index = material id for ids < n_materials, MOVED kept; layer2 records
+ * follow the same map.  Replaces materials_array / Material::id lookups of
+ * chunk.cpp:167-177 and 241-242. */
+#define FSG_PACK_NO_MOVED 1u
+typedef struct fsg_pack_map {
+    uint32_t to_file[16];
+    int16_t from_file[256];
+    uint32_t flags;
+} fsg_pack_map;
+/* Sets the map of the world's later fsg_world_pack / fsg_world_unpack calls
+ * (copied; NULL restores the default).  FSG_ERR_INVALID if an id in use
+ * (< n_materials) maps to an index >= 256 or a from_file entry names an id
+ * >= n_materials. */
+int fsg_world_set_pack_map(fsg_world *w, const fsg_pack_map *map);
+/* The default map of a config (identity on its n_materials ids). */
+int fsg_pack_map_default(const fsg_config *cfg, fsg_pack_map *map);
 
 /* setTile (world.cpp:1004-1008) batched: edits are applied in array order
  * before the next tick; out-of-bounds edits are ignored like setTile's. */

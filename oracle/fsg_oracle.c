@@ -487,16 +487,19 @@ int orc_region_unstable(const uint64_t *cells, int W, int H, const fsg_config *c
 /* ----------------------------------------------------------------------- */
 /* Rigid-body pixels                                                        */
 /* ----------------------------------------------------------------------- */
+/* Q16.16 to int, toward zero: the (int) cast of game.cpp:1774-1775. */
+static int q16_trunc(int64_t v) { return (int)(v >= 0 ? v >> 16 : -((-v) >> 16)); }
+
 /* Pixel (tx, ty) of a body lands on
- *   wx = floor((tx*c - (ty+1)*s + x) / 2^16),  wy = floor((tx*s + (ty+1)*c + y) / 2^16)
- * in Q16.16 integer arithmetic (the float rotate of game.cpp:1774-1775,
- * without its FMA sensitivity). */
+ *   wx = (int)((tx*c - (ty+1)*s + x) / 2^16),  wy = (int)((tx*s + (ty+1)*c + y) / 2^16)
+ * (quotients truncated toward zero) in Q16.16 integer arithmetic: the float
+ * rotate of game.cpp:1774-1775 with exact products. */
 int orc_body_pixel_xy(const fsg_body *b, int tx, int ty, int *wx, int *wy) {
     const int64_t X = b->pos_x_q16, Y = b->pos_y_q16, C = b->cos_q16, S = b->sin_q16;
     const int64_t fx = (int64_t)tx * C - (int64_t)(ty + 1) * S + X;
     const int64_t fy = (int64_t)tx * S + (int64_t)(ty + 1) * C + Y;
-    *wx = (int)(fx >> 16);
-    *wy = (int)(fy >> 16);
+    *wx = q16_trunc(fx);
+    *wy = q16_trunc(fy);
     return 0;
 }
 
@@ -628,6 +631,93 @@ int orc_unstamp(uint64_t *cells, int W, int H, const fsg_config *cfg, const fsg_
     }
     free(claimed);
     free(found);
+    return 0;
+}
+
+/* The reference's own stamp loop (game.cpp:1768-1812), as a pin for the
+ * claim-round form above: bodies in order, and inside a body tx outer, ty
+ * inner (pixel-major, column by column); each non-air pixel takes the first
+ * checkDirs offset whose cell is in the world and AIR, SAND or SOUP, at
+ * once, before the next pixel is tried.  Nothing marks a cell as taken, so a
+ * later pixel may land on an earlier SAND or SOUP typed body pixel and eject
+ * it.  placed[p] = 1 for a placed pixel; displaced cells are reported in
+ * ejection order. */
+int orc_refstamp(uint64_t *cells, int W, int H, const fsg_config *cfg, const fsg_body *bodies, int nbodies,
+                 const uint64_t *pixels, size_t npixels, uint8_t *placed, uint32_t *n_placed,
+                 uint32_t *n_displaced, fsg_cell_edit *displaced_out, uint32_t displaced_cap) {
+    props P;
+    build_props(cfg, &P);
+    memset(placed, 0, npixels);
+    uint32_t np = 0, nd = 0;
+    for (int b = 0; b < nbodies; b++) {
+        const fsg_body *B = &bodies[b];
+        if (!B->enabled) continue;
+        for (int tx = 0; tx < B->mat_w; tx++)
+            for (int ty = 0; ty < B->mat_h; ty++) {
+                const size_t p = B->pixel_offset + (size_t)ty * B->mat_w + tx;
+                if (p >= npixels || is_air(&P, mid(pixels[p]))) continue;
+                int wx, wy;
+                orc_body_pixel_xy(B, tx, ty, &wx, &wy);
+                for (int d = 0; d < 5; d++) {
+                    const int x = wx + DIRX[d], y = wy + DIRY[d];
+                    if (x < 0 || y < 0 || x >= W || y >= H) continue;
+                    const size_t ci = (size_t)y * W + x;
+                    const int tt = P.type[mid(cells[ci])];
+                    if (tt != FSG_AIR && tt != FSG_SAND && tt != FSG_SOUP) continue;
+                    if (tt != FSG_AIR) {
+                        if (displaced_out && nd < displaced_cap) {
+                            displaced_out[nd].x = x;
+                            displaced_out[nd].y = y;
+                            memcpy(&displaced_out[nd].cell, &cells[ci], 8);
+                        }
+                        nd++;
+                    }
+                    cells[ci] = pixels[p];
+                    if (!placed[p]) np++;
+                    placed[p] = 1;
+                    break;
+                }
+            }
+    }
+    if (n_placed) *n_placed = np;
+    if (n_displaced) *n_displaced = nd;
+    return 0;
+}
+
+/* The reference's unstamp loop (game.cpp:1908-1965) in its own order: tx
+ * outer, ty inner; a pixel takes the first checkDirs cell in the world with
+ * its material id (MaterialInstance::id is the material id,
+ * game_datastruct.cpp:304-309), which becomes air keeping its temperature
+ * (the build's air, as orc_unstamp); a pixel that finds none is lost when
+ * the cell at its own position is air. */
+int orc_refunstamp(uint64_t *cells, int W, int H, const fsg_config *cfg, const fsg_body *bodies, int nbodies,
+                   uint64_t *pixels, size_t npixels) {
+    props P;
+    build_props(cfg, &P);
+    const uint64_t air_id = cfg->mat_air;
+    for (int b = 0; b < nbodies; b++) {
+        const fsg_body *B = &bodies[b];
+        if (!B->enabled) continue;
+        for (int tx = 0; tx < B->mat_w; tx++)
+            for (int ty = 0; ty < B->mat_h; ty++) {
+                const size_t p = B->pixel_offset + (size_t)ty * B->mat_w + tx;
+                if (p >= npixels || is_air(&P, mid(pixels[p]))) continue;
+                int wx, wy;
+                orc_body_pixel_xy(B, tx, ty, &wx, &wy);
+                int found = 0;
+                for (int d = 0; d < 5 && !found; d++) {
+                    const int x = wx + DIRX[d], y = wy + DIRY[d];
+                    if (x < 0 || y < 0 || x >= W || y >= H) continue;
+                    const size_t ci = (size_t)y * W + x;
+                    if (mid(cells[ci]) != mid(pixels[p])) continue;
+                    pixels[p] = cells[ci];
+                    cells[ci] = air_id | (cells[ci] & 0xFFFF0000ull);
+                    found = 1;
+                }
+                if (!found && wx >= 0 && wy >= 0 && wx < W && wy < H && is_air(&P, mid(cells[(size_t)wy * W + wx])))
+                    pixels[p] = air_id;
+            }
+    }
     return 0;
 }
 
@@ -886,21 +976,28 @@ static uint32_t get_le(const uint8_t *p, int nbytes) {
     return v;
 }
 
-/* chunk.cpp:238-242: buf[i] = {mat->id, color, temperature} for tiles, then
- * layer2 at CHUNK_W * CHUNK_H + i; the record's last two bytes are padding. */
+/* chunk.cpp:238-242: buf[i] = {(u16)mat->id, color, temperature} for tiles,
+ * then layer2 at CHUNK_W * CHUNK_H + i.  The index is the material's id in
+ * the file's material list (map->to_file), never the cell's MOVED bit; the
+ * record's two trailing bytes are padding, which ChunkRead never reads, and
+ * byte 10 carries MOVED in its bit 0. */
 void orc_chunk_records(const uint64_t *tiles, size_t tstride, const uint64_t *layer2, size_t lstride,
-                       const uint32_t *bg, size_t bstride, uint8_t *raw_a, uint8_t *raw_b) {
+                       const uint32_t *bg, size_t bstride, uint8_t *raw_a, uint8_t *raw_b,
+                       const fsg_pack_map *map) {
+    const int keep_moved = !map || !(map->flags & FSG_PACK_NO_MOVED);
     for (int layer = 0; layer < 2; layer++)
         for (int y = 0; y < 128; y++)
             for (int x = 0; x < 128; x++) {
                 uint64_t c = 0;
                 if (layer == 0) c = tiles[(size_t)y * tstride + x];
                 else if (layer2) c = layer2[(size_t)y * lstride + x];
+                const uint32_t id = (uint32_t)(c & 0xFFu);
                 uint8_t *r = raw_a + ((size_t)layer * 16384 + (size_t)y * 128 + x) * 12;
-                put_le(r, (uint32_t)(c & 0xFFFFu), 4);            /* index */
-                put_le(r + 4, (uint32_t)(c >> 32), 4);            /* color */
-                put_le(r + 8, (uint32_t)((c >> 16) & 0xFFFFu), 2); /* temperature */
-                r[10] = r[11] = 0;
+                put_le(r, map ? map->to_file[id & 15u] : id, 4);   /* index */
+                put_le(r + 4, (uint32_t)(c >> 32), 4);              /* color */
+                put_le(r + 8, (uint32_t)((c >> 16) & 0xFFFFu), 2);  /* temperature */
+                r[10] = (uint8_t)(keep_moved && (c & 0x100u) ? 1 : 0);
+                r[11] = 0;
             }
     for (int y = 0; y < 128; y++)
         for (int x = 0; x < 128; x++) put_le(raw_b + ((size_t)y * 128 + x) * 4, bg ? bg[(size_t)y * bstride + x] : 0u, 4);
@@ -1016,8 +1113,19 @@ size_t orc_pack_chunk(const uint8_t *raw_a, const uint8_t *raw_b, int8_t phase, 
     return (size_t)ORC_PACK_HDR + (size_t)ca + (size_t)cb;
 }
 
+/* The material id a record index names: from_file, or the id itself below
+ * n_materials; -1 for none (chunk.cpp:167-177 would index materials_array
+ * out of range). */
+static int file_material(const fsg_pack_map *map, uint32_t n_materials, uint32_t idx) {
+    if (idx > 0xFFu) return -1;
+    if (map) return map->from_file[idx];
+    return idx < n_materials ? (int)idx : -1;
+}
+
 int orc_unpack_chunk(const uint8_t *img, size_t n, uint32_t n_materials, uint64_t *tiles, size_t tstride,
-                     uint64_t *layer2, size_t lstride, uint32_t *bg, size_t bstride, int8_t *phase) {
+                     uint64_t *layer2, size_t lstride, uint32_t *bg, size_t bstride, int8_t *phase,
+                     const fsg_pack_map *map) {
+    const int keep_moved = !map || !(map->flags & FSG_PACK_NO_MOVED);
     if (n < ORC_PACK_HDR) return FSG_ERR_INVALID;
     if (phase) *phase = (int8_t)img[0];
     const int32_t sa = (int32_t)get_le(img + 1, 4), ca = (int32_t)get_le(img + 5, 4);
@@ -1031,14 +1139,13 @@ int orc_unpack_chunk(const uint8_t *img, size_t n, uint32_t n_materials, uint64_
     if (!ra || !rb) goto done;
     if (orc_lz4_decode(img + ORC_PACK_HDR, ca, ra, ORC_PACK_NA) != ORC_PACK_NA) goto done;
     if (orc_lz4_decode(img + ORC_PACK_HDR + ca, cb, rb, ORC_PACK_NB) != ORC_PACK_NB) goto done;
-    for (int i = 0; i < 2 * 16384; i++) { /* chunk.cpp:167-171: index -> materials_array */
-        const uint32_t idx = get_le(ra + (size_t)i * 12, 4);
-        if (i < 16384 ? ((idx & 0xFFFFFE00u) || (idx & 0xFFu) >= n_materials) : idx > 0xFFFFu) goto done;
-    }
+    for (int i = 0; i < 2 * 16384; i++) /* chunk.cpp:167-177: index -> materials_array */
+        if (file_material(map, n_materials, get_le(ra + (size_t)i * 12, 4)) < 0) goto done;
     for (int i = 0; i < 2 * 16384; i++) {
         const uint8_t *r = ra + (size_t)i * 12;
-        const uint64_t c = (uint64_t)(get_le(r, 4) & 0xFFFFu) | ((uint64_t)get_le(r + 8, 2) << 16) |
-                           ((uint64_t)get_le(r + 4, 4) << 32);
+        const uint64_t mat = (uint64_t)file_material(map, n_materials, get_le(r, 4)) |
+                             (keep_moved && (r[10] & 1u) ? 0x100u : 0u);
+        const uint64_t c = mat | ((uint64_t)get_le(r + 8, 2) << 16) | ((uint64_t)get_le(r + 4, 4) << 32);
         const int y = (i >> 7) & 127, x = i & 127;
         if (i < 16384) tiles[(size_t)y * tstride + x] = c;
         else if (layer2) layer2[(size_t)y * lstride + x] = c;

@@ -81,11 +81,14 @@ int orc_render(const uint64_t *cells, const uint64_t *prev, int W, int H, const 
 #define ORC_LZ4_BOUND(n) ((n) + (n) / 255 + 16) /* LZ4_compressBound (lz4.h) */
 
 /* The records of one chunk (chunk.cpp:238-242): raw_a = {u32 index, u32
- * colour, i16 temperature, 2 zero bytes} of tiles then layer2, raw_b = the
- * background.  Inputs are chunk windows with row strides in elements;
- * layer2 / bg may be NULL (zero records). */
+ * colour, i16 temperature, pad byte 10 = MOVED, pad byte 11 = 0} of tiles
+ * then layer2, raw_b = the background.  index = map->to_file[material id]
+ * (map NULL: the id itself); FSG_PACK_NO_MOVED writes pad byte 10 as 0.
+ * Inputs are chunk windows with row strides in elements; layer2 / bg may be
+ * NULL (zero records). */
 void orc_chunk_records(const uint64_t *tiles, size_t tstride, const uint64_t *layer2, size_t lstride,
-                       const uint32_t *bg, size_t bstride, uint8_t *raw_a, uint8_t *raw_b);
+                       const uint32_t *bg, size_t bstride, uint8_t *raw_a, uint8_t *raw_b,
+                       const fsg_pack_map *map);
 /* LZ4 block decoding with the acceptance rules of lz4.c's
  * LZ4_decompress_safe for a full block (lz4.c:1949-2360): decoded size, or -1. */
 int orc_lz4_decode(const uint8_t *src, int n, uint8_t *dst, int cap);
@@ -100,10 +103,23 @@ int orc_lz4_encode_rows(const uint8_t *raw, int n, int rec, uint8_t *dst, int ca
 size_t orc_pack_chunk(const uint8_t *raw_a, const uint8_t *raw_b, int8_t phase, uint8_t *img);
 /* ChunkRead of one image into chunk windows (layer2 / bg may be NULL):
  * FSG_OK, or FSG_ERR_INVALID for bad header sizes, a block that does not
- * decode to its full size or a bad material index (nothing written then;
- * *phase is still set once the header has been read). */
+ * decode to its full size or a record index that names no material (index
+ * >= 256 or map->from_file[index] < 0; map NULL: index >= n_materials)
+ * (nothing written then; *phase is still set once the header has been
+ * read).  MOVED comes from bit 0 of pad byte 10 unless FSG_PACK_NO_MOVED. */
 int orc_unpack_chunk(const uint8_t *img, size_t n, uint32_t n_materials, uint64_t *tiles, size_t tstride,
-                     uint64_t *layer2, size_t lstride, uint32_t *bg, size_t bstride, int8_t *phase);
+                     uint64_t *layer2, size_t lstride, uint32_t *bg, size_t bstride, int8_t *phase,
+                     const fsg_pack_map *map);
+
+/* The reference's own pixel-major stamp and unstamp loops (game.cpp:1768-1812,
+ * 1908-1965): each pixel, tx outer and ty inner, takes its first free
+ * checkDirs offset before the next pixel is tried.  Test pins of the
+ * claim-round forms above, not the build's definition. */
+int orc_refstamp(uint64_t *cells, int W, int H, const fsg_config *cfg, const fsg_body *bodies, int nbodies,
+                 const uint64_t *pixels, size_t npixels, uint8_t *placed, uint32_t *n_placed,
+                 uint32_t *n_displaced, fsg_cell_edit *displaced_out, uint32_t displaced_cap);
+int orc_refunstamp(uint64_t *cells, int W, int H, const fsg_config *cfg, const fsg_body *bodies, int nbodies,
+                   uint64_t *pixels, size_t npixels);
 
 void orc_counts(const uint64_t *cells, size_t n, uint64_t counts[16]);
 uint64_t orc_checksum(const uint64_t *cells, size_t n);

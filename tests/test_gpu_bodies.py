@@ -10,16 +10,23 @@ from bodies import random_bodies
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("seed,nb,skip,W,H,max_side", [
-    (1, 30, 0, 320, 256, 16), (2, 120, 0, 320, 256, 16), (3, 60, 1, 320, 256, 16),
-    # over 2^20 body pixels: the claim slots are cleared round by round, not all up front
-    (5, 1100, 1, 1024, 1024, 64)])
-def test_stamp_tick_unstamp_matches_oracle(ctx, orc, seed, nb, skip, W, H, max_side):
+# stone/wood bodies, and bodies with SAND and SOUP typed pixels (a stamped
+# SAND/SOUP pixel leaves its cell a stamping target; later rounds must not
+# land on it)
+SOLID_MATS, MIXED_MATS = (1, 7), (1, 7, 2, 3)
+
+
+@pytest.mark.parametrize("seed,nb,skip,W,H,max_side,mats", [
+    (1, 30, 0, 320, 256, 16, SOLID_MATS), (2, 120, 0, 320, 256, 16, SOLID_MATS),
+    (3, 60, 1, 320, 256, 16, SOLID_MATS), (4, 160, 1, 320, 256, 16, MIXED_MATS),
+    # over 2^20 body pixels
+    (5, 1100, 1, 1024, 1024, 64, SOLID_MATS), (6, 1100, 0, 1024, 1024, 64, MIXED_MATS)])
+def test_stamp_tick_unstamp_matches_oracle(ctx, orc, seed, nb, skip, W, H, max_side, mats):
     rng = np.random.default_rng(seed)
     cells = fsg.generate(1, W, H, seed)
-    bodies, pixels = random_bodies(rng, W, H, nb, max_side=max_side)
+    bodies, pixels = random_bodies(rng, W, H, nb, max_side=max_side, mats=mats)
     assert (pixels.size > 1 << 20) == (max_side == 64)
-    print(f"bodies seed={seed} nb={nb} skip={skip} {W}x{H} pixels={pixels.size}", flush=True)
+    print(f"bodies seed={seed} nb={nb} skip={skip} {W}x{H} pixels={pixels.size} mats={mats}", flush=True)
     cfg = orc.default_config(chunk_skip=skip)
     # oracle: stamp, 3 ticks, unstamp
     ref = cells.copy()
@@ -85,3 +92,47 @@ def test_stamp_state_errors(ctx, orc):
         b = fsg.body_from_pose(10, 10, 0, 4, 4, 0)
         with pytest.raises(fsg.FsgError):   # pixel array too short for the body
             w.stamp_bodies([b], np.zeros(3, dtype=np.uint64))
+
+
+def test_tick_after_unstamp_and_census(ctx, orc):
+    """The reference's order, stamp -> tick -> unstamp -> texture pass
+    (game.cpp:1760-2067), then more ticks with chunk skipping: the grid
+    equals the oracle's, and the census compares with the state before the
+    tick (the stamp and unstamp put the claim slots they borrow from the
+    spare buffer back)."""
+    W, H, seed = 384, 256, 12
+    rng = np.random.default_rng(seed)
+    cells = fsg.generate(2, W, H, seed)
+    bodies, pixels = random_bodies(rng, W, H, 70, max_side=16, mats=MIXED_MATS)
+    cfg = orc.default_config(chunk_skip=1)
+    print(f"stamp/tick/unstamp/render {W}x{H} pixels={pixels.size}", flush=True)
+    ref = cells.copy()
+    orc.stamp(ref, cfg, bodies, pixels)
+    pre_tick = ref.copy()
+    orc.run(ref, cfg, seed, 0, 1)
+    ref_px = orc.unstamp(ref, cfg, bodies, pixels)
+    _layers, ref_st = orc.render(ref, cfg, _render_params(), prev=pre_tick)
+    after = orc.run(ref.copy(), cfg, seed, 1, 20)
+    with fsg.World(ctx, W, H, seed, cfg) as w:
+        w.upload(cells)
+        w.stamp_bodies(bodies, pixels)
+        w.tick(1)
+        got_px = w.unstamp_bodies(pixels.size)
+        st = w.render(fsg.LAYER_CELLS, stats=True)
+        assert np.array_equal(w.download(), ref)
+        w.tick(20)
+        got_after = w.download()
+    assert np.array_equal(got_px, ref_px)
+    assert st["moved"] == [int(v) for v in ref_st.moved] and sum(st["moved"]) > 0
+    assert np.array_equal(got_after, after), int(np.count_nonzero(got_after != after))
+
+
+def _render_params():
+    from falling_sand_engine_amd._abi import RenderParams
+    p = RenderParams()
+    p.layers = fsg.LAYER_CELLS
+    p.fire_mat = 255
+    p.count_moved = 1
+    for m, e in enumerate(fsg.EMIT_DEFAULT):
+        p.emit[m] = e
+    return p

@@ -16,7 +16,7 @@ from test_pack import NMAT, load_golden
 pytestmark = pytest.mark.gpu
 
 
-def oracle_images(orc, cells, x0, y0, ncx, ncy, phase=None, l2=None, bg=None):
+def oracle_images(orc, cells, x0, y0, ncx, ncy, phase=None, l2=None, bg=None, pack_map=None):
     out = []
     for cy in range(ncy):
         for cx in range(ncx):
@@ -24,7 +24,7 @@ def oracle_images(orc, cells, x0, y0, ncx, ncy, phase=None, l2=None, bg=None):
             lw = None if l2 is None else l2[cy * 128:(cy + 1) * 128, cx * 128:(cx + 1) * 128]
             bw = None if bg is None else bg[cy * 128:(cy + 1) * 128, cx * 128:(cx + 1) * 128]
             ra, rb = orc.chunk_records(np.ascontiguousarray(t), None if lw is None else np.ascontiguousarray(lw),
-                                       None if bw is None else np.ascontiguousarray(bw))
+                                       None if bw is None else np.ascontiguousarray(bw), pack_map)
             out.append(orc.pack_chunk(ra, rb, 0 if phase is None else int(phase[cy * ncx + cx])))
     return out
 
@@ -52,24 +52,55 @@ def test_pack_matches_oracle(ctx, orc, kind, W, H, seed, x0, y0, ncx, ncy, aux):
         assert got[k] == ref[k], k
 
 
+@pytest.mark.parametrize("refmap", [False, True])
+def test_pack_ticked_world_with_maps(ctx, orc, refmap):
+    """Ticked worlds hold MOVED sand: the device images equal the oracle's
+    (index = material, MOVED in pad byte 10), with the default map and with
+    the reference's material indices, and read back to the same cells."""
+    print(f"pack ticked refmap={refmap}", flush=True)
+    cfg = orc.default_config()
+    cells = fsg.generate(3, 512, 256, 7)
+    want = orc.run(cells.copy(), cfg, 7, 0, 20)
+    assert (want & np.uint64(0x100)).any()
+    pm = fsg.reference_pack_map() if refmap else None
+    l2 = packcases.layer2_pattern(256, 512, 2)
+    with fsg.World(ctx, 512, 256, 7, cfg) as w:
+        w.upload(cells)
+        w.tick(20)
+        if refmap:
+            w.set_pack_map(pm)
+        got = w.pack(0, 0, 4, 2, layer2=l2)
+        w.upload(np.zeros_like(cells))
+        st, _ph, gl2, _bg = w.unpack(0, 0, 4, 2, got, want_layer2=True)
+        back = w.download()
+    ref = oracle_images(orc, want, 0, 0, 4, 2, None, l2, None, pm)
+    for k in range(8):
+        assert got[k] == ref[k], k
+    assert (st == 0).all() and np.array_equal(back, want) and np.array_equal(gl2, l2)
+
+
 def test_unpack_golden_reference_images(ctx, orc):
     """Images compressed by the reference's LZ4_compress_fast read back
-    exactly: cells, layer2, background and phase."""
+    exactly: cells, layer2, background and phase (the reference-index image
+    through the reference map)."""
     gold = load_golden()
-    names = sorted(gold)
-    print("unpack golden", names, flush=True)
-    imgs = [gold[k][0] for k in names]
-    with fsg.World(ctx, 128 * len(names), 128, 1, orc.default_config()) as w:
-        st, ph, l2, bg = w.unpack(0, 0, len(names), 1, imgs, want_layer2=True, want_background=True)
-        grid = w.download()
-    assert (st == 0).all(), st
-    for k, name in enumerate(names):
-        _img, c, tiles, tl2, tbg = gold[name]
-        sl = np.s_[:, k * 128:(k + 1) * 128]
-        assert ph[k] == c["phase"]
-        assert np.array_equal(grid[sl], tiles), name
-        assert np.array_equal(l2[sl], np.zeros_like(tiles) if tl2 is None else tl2), name
-        assert np.array_equal(bg[sl], np.zeros((128, 128), np.uint32) if tbg is None else tbg), name
+    for refmap in (False, True):
+        names = sorted(k for k in gold if bool(gold[k][1].get("map")) == refmap)
+        print("unpack golden", names, flush=True)
+        imgs = [gold[k][0] for k in names]
+        with fsg.World(ctx, 128 * len(names), 128, 1, orc.default_config()) as w:
+            if refmap:
+                w.set_pack_map(fsg.reference_pack_map())
+            st, ph, l2, bg = w.unpack(0, 0, len(names), 1, imgs, want_layer2=True, want_background=True)
+            grid = w.download()
+        assert (st == 0).all(), st
+        for k, name in enumerate(names):
+            _img, c, tiles, tl2, tbg = gold[name]
+            sl = np.s_[:, k * 128:(k + 1) * 128]
+            assert ph[k] == c["phase"]
+            assert np.array_equal(grid[sl], tiles), name
+            assert np.array_equal(l2[sl], np.zeros_like(tiles) if tl2 is None else tl2), name
+            assert np.array_equal(bg[sl], np.zeros((128, 128), np.uint32) if tbg is None else tbg), name
 
 
 def test_round_trip_on_device_and_through_host(ctx, orc):

@@ -41,6 +41,8 @@ def load_golden():
             cells = np.zeros((c["H"], c["W"]), np.uint64)
         else:
             cells = fsg.generate(c["kind"], c["W"], c["H"], c["seed"])
+        if c.get("ticks"):
+            cells = O.run(cells, O.default_config(), c["seed"], 0, c["ticks"])
         tiles = packcases.chunk_of(cells, c["cx"], c["cy"])
         l2 = None if c["l2"] is None else packcases.layer2_pattern(128, 128, c["l2"])
         bg = None if c["bg"] is None else packcases.background_pattern(128, 128, c["bg"])
@@ -61,33 +63,117 @@ def test_record_layout_is_material_instance_data():
 
     assert C.sizeof(MID) == 12 and MID.color.offset == 4 and MID.temperature.offset == 8
     tiles = np.zeros((128, 128), np.uint64)
-    tiles[0, 0] = cell(0x107, -5, 0x11223344)  # moved sand-like id with its moved bit
+    tiles[0, 0] = cell(0x107, -5, 0x11223344)  # material 7 with its MOVED bit
     tiles[127, 127] = cell(3, 320, 0xFFAA0000)
     l2 = np.zeros((128, 128), np.uint64)
-    l2[5, 9] = cell(1, 7, 0x01020304)
+    l2[5, 9] = cell(0x101, 7, 0x01020304)
     bg = np.arange(128 * 128, dtype=np.uint32).reshape(128, 128) * np.uint32(2654435761)
     ra, rb = O.chunk_records(tiles, l2, bg)
     recs = np.frombuffer(ra.tobytes(), dtype=np.dtype(MID))
     assert recs.shape == (2 * 128 * 128,)
-    assert (recs[0]["index"], recs[0]["color"], recs[0]["temperature"]) == (0x107, 0x11223344, -5)
+    # the index is (u16)mat->id alone (chunk.cpp:241-242); MOVED is bit 0 of pad byte 10
+    assert (recs[0]["index"], recs[0]["color"], recs[0]["temperature"]) == (7, 0x11223344, -5)
+    assert (ra[10], ra[11]) == (1, 0)
     last = recs[128 * 128 - 1]
     assert (last["index"], last["color"], last["temperature"]) == (3, 0xFFAA0000, 320)
-    r = recs[128 * 128 + 5 * 128 + 9]
+    k = 128 * 128 + 5 * 128 + 9
+    r = recs[k]
     assert (r["index"], r["color"], r["temperature"]) == (1, 0x01020304, 7)
-    assert all(ra[12 * i + 10] == 0 and ra[12 * i + 11] == 0 for i in range(0, 2 * 128 * 128, 977))
+    assert (ra[12 * k + 10], ra[12 * k + 11]) == (1, 0)
+    assert all(ra[12 * i + 10] == 0 and ra[12 * i + 11] == 0 for i in range(1, 2 * 128 * 128, 977) if i != k)
     assert np.array_equal(np.frombuffer(rb.tobytes(), np.uint32), bg.ravel())
+    # FSG_PACK_NO_MOVED: zero padding, as ChunkWrite leaves it
+    m = fsg.pack_map_default(O.default_config())
+    m.flags = _abi.PACK_NO_MOVED
+    ra2, _ = O.chunk_records(tiles, l2, bg, m)
+    assert ra2[10] == 0 and ra2[12 * k + 10] == 0 and recs[0]["index"] == 7
+    rc, t, l, _, _ = O.unpack_chunk(O.pack_chunk(ra, rb), NMAT, m)
+    assert rc == 0 and t[0, 0] == cell(7, -5, 0x11223344) and l[5, 9] == cell(1, 7, 0x01020304)
+    rc, t, l, _, _ = O.unpack_chunk(O.pack_chunk(ra, rb), NMAT)
+    assert rc == 0 and np.array_equal(t, tiles) and np.array_equal(l, l2)
+
+
+def ticked_world(W=256, H=256, seed=6, ticks=24):
+    return O.run(fsg.generate(3, W, H, seed), O.default_config(), seed, 0, ticks)
+
+
+def decoded_records(img):
+    """Block A of an image decoded by the reference's own LZ4_decompress_safe
+    (as chunk.cpp:151 calls it), as MaterialInstanceData fields."""
+    _, sa, ca, _, _ = struct.unpack("<biiii", img[:17])
+    r, raw = lz4ref.decompress_safe(img[17:17 + ca], sa)
+    assert r == sa
+    a = np.frombuffer(raw, np.uint8).reshape(-1, 12)
+    return a[:, 0:4].copy().view(np.uint32).ravel(), a[:, 10], a[:, 11]
+
+
+@needs_ref
+def test_ticked_images_hold_only_material_indices():
+    """A ticked chunk holds moving sand (MOVED set); the reference's ChunkRead
+    indexes materials_array[index] (chunk.cpp:167-177), so every index of the
+    image must be a material of the list it was written for: the world's 8
+    (default map) or the reference's 28 (reference map).  MOVED travels in
+    pad byte 10 only."""
+    gold = load_golden()
+    for name, refmap in (("ticked", False), ("ticked_refmap", True)):
+        img, c, tiles, l2, bg = gold[name]
+        moved = (tiles & np.uint64(0x100)) != 0
+        assert moved.sum() > 100, "the golden chunk must hold moving grains"
+        idx, pad10, pad11 = decoded_records(img)
+        allowed = set(fsg.REFERENCE_MATERIAL_IDS.values()) if refmap else set(range(NMAT))
+        assert set(np.unique(idx).tolist()) <= allowed
+        assert np.array_equal(pad10[:128 * 128], moved.ravel().astype(np.uint8))
+        assert not pad11.any()
+        pm = fsg.reference_pack_map() if refmap else None
+        rc, t, l, b, ph = O.unpack_chunk(img, NMAT, pm)
+        assert rc == 0 and ph == c["phase"] and np.array_equal(t, tiles)
+        assert np.array_equal(l, np.zeros_like(l) if l2 is None else l2)
+    # the reference ids of the benchmark materials, read off game_datastruct.cpp:72-110
+    img = gold["ticked_refmap"][0]
+    idx, _, _ = decoded_records(img)
+    t = gold["ticked_refmap"][2]
+    R = fsg.REFERENCE_MATERIAL_IDS
+    want = {fsg.MAT_AIR: R["GENERIC_AIR"], fsg.MAT_SAND: R["GENERIC_SAND"], fsg.MAT_WATER: R["WATER"],
+            fsg.MAT_STEAM: R["STEAM"], fsg.MAT_STONE: R["STONE"]}
+    mats = (t & np.uint64(0xFF)).ravel()
+    for m, r in want.items():
+        if (mats == m).any():
+            assert set(np.unique(idx[:128 * 128][mats == m]).tolist()) == {r}
+
+
+def test_reference_map_reads_reference_materials():
+    """Records naming the reference's other materials read as the benchmark
+    material of the same PhysicsType (game_datastruct.cpp:72-110); an index
+    naming no material fails the chunk."""
+    R = fsg.REFERENCE_MATERIAL_IDS
+    pm = fsg.reference_pack_map()
+    ra, rb = O.chunk_records(np.zeros((128, 128), np.uint64))
+    a = ra.reshape(-1, 12)
+    for i, (name, want) in enumerate((("OBSIDIAN", fsg.MAT_STONE), ("LAVA", fsg.MAT_LAVA),
+                                      ("WATER", fsg.MAT_WATER), ("STEAM", fsg.MAT_STEAM),
+                                      ("DIRT", fsg.MAT_SAND), ("FIRE", fsg.MAT_AIR))):
+        a[i, 0:4] = np.frombuffer(struct.pack("<I", R[name]), np.uint8)
+    rc, t, _, _, _ = O.unpack_chunk(O.pack_chunk(ra, rb), NMAT, pm)
+    assert rc == 0
+    assert [int(v) & 0xFF for v in t[0, :6]] == [fsg.MAT_STONE, fsg.MAT_LAVA, fsg.MAT_WATER, fsg.MAT_STEAM,
+                                                fsg.MAT_SAND, fsg.MAT_AIR]
+    a[7, 0:4] = np.frombuffer(struct.pack("<I", 28), np.uint8)  # past the reference's list
+    assert O.unpack_chunk(O.pack_chunk(ra, rb), NMAT, pm)[0] == _abi.FSG_ERR_INVALID
+    a[7, 0:4] = np.frombuffer(struct.pack("<I", 15), np.uint8)  # WATER: no material of the default map
+    assert O.unpack_chunk(O.pack_chunk(ra, rb), NMAT)[0] == _abi.FSG_ERR_INVALID
+    assert O.unpack_chunk(O.pack_chunk(ra, rb), NMAT, pm)[0] == 0
 
 
 def test_golden_images_read_back():
     """Images made by the reference's own compressor read back to the chunk."""
     for name, (img, c, tiles, l2, bg) in load_golden().items():
-        rc, t, l, b, ph = O.unpack_chunk(img, NMAT)
+        rc, t, l, b, ph = O.unpack_chunk(img, NMAT, fsg.reference_pack_map() if c.get("map") else None)
         assert rc == 0, name
         assert ph == c["phase"]
         assert np.array_equal(t, tiles), name
         assert np.array_equal(l, np.zeros_like(l) if l2 is None else l2), name
         assert np.array_equal(b, np.zeros_like(b) if bg is None else bg), name
-        ra, rb = O.chunk_records(tiles, l2, bg)
+        ra, rb = O.chunk_records(tiles, l2, bg, fsg.reference_pack_map() if c.get("map") else None)
         assert hashlib.sha256(ra.tobytes()).hexdigest() == c["raw_a_sha256"]
         assert hashlib.sha256(rb.tobytes()).hexdigest() == c["raw_b_sha256"]
 
