@@ -153,6 +153,7 @@ SIGNATURES = {
     "fsg_world_set_pack_map": (C.c_int, [P, C.POINTER(PackMap)]),
     "fsg_pack_map_default": (C.c_int, [C.POINTER(Config), C.POINTER(PackMap)]),
     "fsg_world_set_cells": (C.c_int, [P, P, C.c_size_t]),
+    "fsg_world_set_cells_noalloc": (C.c_int, [P, P, C.c_size_t]),
     "fsg_world_step": (C.c_int, [P, C.c_int]),
     "fsg_world_step_timed": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
     "fsg_world_tick_temperature": (C.c_int, [P]),

@@ -262,6 +262,12 @@ extern "C" int fsg_ctx_create(const fsg_ctx_config *cfg, fsg_ctx **out) {
         delete c;
         return FSG_ERR_DEVICE;
     }
+    // the fixed device array of fsg_world_set_cells_noalloc: its address on this device
+    if (fsg_impl::noalloc_setup(dev) != FSG_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return FSG_ERR_DEVICE;
+    }
     // freed blocks kept for reuse: up to a quarter of the device's memory
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b / 4 > c->cache_cap) c->cache_cap = total_b / 4;

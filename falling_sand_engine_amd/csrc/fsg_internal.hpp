@@ -138,4 +138,7 @@ int stamp_rounds(fsg_ctx *c, const fsg::StampWorld *d_table, int n, uint32_t tot
 int stats_enqueue(fsg_world *w, unsigned long long *host);
 // Uploads pixels/bodies of a world for stamping (validates them).
 int set_bodies(fsg_world *w, const fsg_body *bodies, int nb, const fsg_cell *pixels, size_t npix);
+// Looks up the device address of fsg_world_set_cells_noalloc's array on the
+// current device (once per device).
+int noalloc_setup(int device);
 }  // namespace fsg_impl

@@ -244,6 +244,18 @@ typedef struct fsg_cell_edit {
     fsg_cell cell;
 } fsg_cell_edit;
 int fsg_world_set_cells(fsg_world *w, const fsg_cell_edit *edits, size_t n);
+/* fsg_world_set_cells for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference: it is the same setTile batch
+ * (same edits, same last-edit-wins result, same OOB rule), staged in a fixed
+ * device array that every context on a GPU shares, instead of device blocks
+ * allocated per call; it calls no hipMalloc, hipFree or other allocating
+ * routine.  At most FSG_SET_CELLS_NOALLOC_MAX edits per call
+ * (FSG_ERR_INVALID beyond).  Synchronous.  Calls on one GPU use the one
+ * array and are serialised by a per-device lock inside the library (calls
+ * from other processes have their own copy). */
+#define FSG_SET_CELLS_NOALLOC_MAX 65536
+int fsg_world_set_cells_noalloc(fsg_world *w, const fsg_cell_edit *edits, size_t n);
 
 /* world::tick (world.cpp:1036-1948) ticks times, each followed by the
  * temperature step when the cadence says so (game.cpp:2157-2159).  Enqueued
