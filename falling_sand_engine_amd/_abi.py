@@ -122,6 +122,16 @@ class RenderStats(C.Structure):
                 ("reserved", C.c_uint32)]
 
 
+class MeshRegion(C.Structure):
+    _fields_ = [("x0", C.c_int32), ("y0", C.c_int32), ("w", C.c_int32), ("h", C.c_int32), ("status", C.c_int32),
+                ("first_loop", C.c_uint32), ("nloops", C.c_uint32)]
+
+
+class MeshLoop(C.Structure):
+    _fields_ = [("first", C.c_uint32), ("count", C.c_uint32), ("hole", C.c_int32), ("start_x", C.c_int32),
+                ("start_y", C.c_int32), ("region", C.c_uint32)]
+
+
 class PackMap(C.Structure):
     _fields_ = [("to_file", C.c_uint32 * 16), ("from_file", C.c_int16 * 256), ("flags", C.c_uint32)]
 
@@ -164,6 +174,9 @@ SIGNATURES = {
     "fsg_world_unstamp_bodies": (C.c_int, [P, P, C.c_size_t]),
     "fsg_world_physics_check": (C.c_int, [P, P, C.c_int, C.POINTER(C.c_size_t)]),
     "fsg_world_island_pixels": (C.c_int, [P, P, C.c_size_t]),
+    "fsg_world_mesh": (C.c_int, [P, P, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "fsg_world_mesh_islands": (C.c_int, [P, P, C.c_int, P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "fsg_world_mesh_download": (C.c_int, [P, P, C.c_uint32, P, C.c_uint32]),
     "fsg_world_render": (C.c_int, [P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "fsg_world_layer": (C.c_int, [P, C.c_uint32, C.POINTER(P)]),
     "fsg_world_layer_download": (C.c_int, [P, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int, P]),
@@ -225,3 +238,5 @@ def check(status, what=""):
 
 CHUNK = 128  # FSG_CHUNK: chunk side of the .pack images (const.h:19-20)
 PACK_NO_MOVED = 1  # FSG_PACK_NO_MOVED
+MESH_MAX_CELLS = 16384  # FSG_MESH_MAX_CELLS
+MESH_EDGE = 1  # FSG_MESH_EDGE

@@ -189,7 +189,12 @@ int launch_tick(fsg_world *w) {
     const unsigned tiles = (unsigned)(w->d.tiles_x * w->d.tiles_y);
     // chunk flags left stale by a tick without skipping: process every chunk once
     if (w->d.skip && !w->flags_valid) w->d.force_tick = w->d.tick;
-    fsg::k_tick_world<<<tiles, fsg::NT, 0, w->ctx->stream>>>(w->d, 0u);
+    const bool sk = w->d.skip != 0, re = w->d.mat.reactions != 0;
+    hipStream_t st = w->ctx->stream;
+    if (sk && re) fsg::k_tick_world<1, 1><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
+    else if (sk) fsg::k_tick_world<1, 0><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
+    else if (re) fsg::k_tick_world<0, 1><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
+    else fsg::k_tick_world<0, 0><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
     HIPCHK(hipGetLastError());
     w->d.tick++;
     w->offered += tiles;

@@ -122,7 +122,7 @@ constexpr int QCAP = ((36 * 36 + NT - 1) / NT) * 64;
 // of a phase are disjoint and a wave's LDS accesses stay in order, so the
 // passes need no barrier.  With TRACK, returns the band bits of chunk cells
 // that changed.
-template <bool TRACK>
+template <bool TRACK, bool REACT>
 __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW], const uint32_t *T, int it,
                                               uint64_t key, int ox, int oy, int need, int gx0, int gy0,
                                               bool has_wood, uint16_t *queue, const uint32_t *perm) {
@@ -154,7 +154,7 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
             const uint32_t HT = T[(i0 << 4) | i1], HB = T[(i2 << 4) | i3];
             act = !block_inert(VL, VR, DL, DR, HT, HB, (q0 >> 8) & 1u, (q1 >> 8) & 1u);
             plain = ((((VL | VR) & (K_GRISE | K_REACT)) | ((DL | DR) & K_GRISE) | ((HT | HB) & K_HSWAP)) == 0);
-            if (has_wood && (wood_hot(M, q0) || wood_hot(M, q1) || wood_hot(M, q2) || wood_hot(M, q3))) {
+            if (REACT && has_wood && (wood_hot(M, q0) || wood_hot(M, q1) || wood_hot(M, q2) || wood_hot(M, q3))) {
                 act = true;
                 plain = false;
             }
@@ -221,10 +221,10 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
         uint32_t r, s;
         block_hash(key, gx0 + lx, gy0 + ly, r, s);
         bool hot = false;
-        if (has_wood)
+        if (REACT && has_wood)
             hot = wood_hot(M, q[0]) || wood_hot(M, q[1]) || wood_hot(M, q[2]) || wood_hot(M, q[3]);
         bool wrote = false;
-        if (hot || ((VL | VR) & K_REACT)) {
+        if (REACT && (hot || ((VL | VR) & K_REACT))) {
             if (block_react(M, q)) {
                 p0[0] = q[0];
                 p0[1] = q[1];
@@ -302,16 +302,20 @@ constexpr int KC = (LH - 2) / RS;  // sweeps over rows 1 .. LH-2 (10)
 static_assert(RS * KC == LH - 2 && HALO - MAX_IT >= 1, "sweeps cover rows 1 .. LH-2 exactly");
 static_assert((uint64_t)FSG_MAX_SIDE * 8u * LH < (1ull << 32), "32-bit byte offsets within a region");
 
+// SK / RE: chunk skipping / reactions compiled in (1), out (0), or read
+// from the world at run time (2, the batch kernels, whose worlds may differ).
+template <int SK, int RE>
 __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, int tile, uint32_t (*L)[LW],
                                             uint32_t (*lut)[256], uint32_t *rgba, int32_t *addt,
                                             uint16_t *queue, uint32_t *red) {
     const int tid = threadIdx.x;
     const MatTab &M = w.mat;
+    const bool react = RE == 2 ? M.reactions != 0 : RE == 1;
     const int tx = tile % w.tiles_x, ty = tile / w.tiles_x;
     const int gx0 = tx * TW - HALO, gy0 = ty * TH - HALO;  // world coords of L[0][0]
     const uint64_t *__restrict__ in = w.buf[tick & 1];
     uint64_t *__restrict__ out = w.buf[(tick + 1) & 1];
-    const bool skip = w.skip != 0;
+    const bool skip = SK == 2 ? w.skip != 0 : SK == 1;
     bool prev_changed = true;
 
     if (skip) {
@@ -401,7 +405,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         const uint32_t org = (((uint32_t)(2 * ec2) & 7u) << 9) | (((uint32_t)ey & 7u) << 12);
         *reinterpret_cast<uint2 *>(&L[ey][2 * ec2]) = make_uint2(ve.x | org, ve.z | org | (1u << 9));
     }
-    if (M.reactions) {
+    if (RE != 0 && react) {
         if (mine) {
 #pragma unroll
             for (int k = 0; k < KC; k++) wood |= (v[k].x & 0xFFu) == M.wood || (v[k].z & 0xFFu) == M.wood;
@@ -425,8 +429,11 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         rgba[tid] = mrgba;
         addt[tid] = madd;
     }
-    // burning wood is the one reaction the pair table cannot see
-    const bool has_wood = __syncthreads_or(wood && M.reactions) != 0;
+    // burning wood is the one reaction the pair table cannot see (the
+    // reduction is also the barrier after the staging)
+    bool has_wood = false;
+    if (RE == 0) __syncthreads();
+    else has_wood = __syncthreads_or(wood && react) != 0;
 
     if (skip && !prev_changed) {
         // the chunk did not change last tick: if nothing in its region can
@@ -448,10 +455,12 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
             const uint64_t g = tick * (uint64_t)M.cell_iter + (uint64_t)it;
             const uint64_t key = splitmix64(seedmix ^ g);
             const int ox = kOX[g & 3], oy = kOY[g & 3];
-            if (skip)
-                bands |= run_phase<true>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue, lut[MAX_IT]);
-            else
-                run_phase<false>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue, lut[MAX_IT]);
+            if (SK != 0 && skip)
+                bands |= run_phase<true, RE != 0>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue,
+                                                  lut[MAX_IT]);
+            else if (SK != 1)
+                run_phase<false, RE != 0>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue,
+                                          lut[MAX_IT]);
             __syncthreads();
         }
     }
@@ -496,7 +505,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     auto colour = [&](uint32_t wd, int x, int y) -> uint32_t {
         return *((wd & C_XF) ? rgba + (wd & 15u) : Lf + origin(wd, x, y));
     };
-    const bool xf_on = M.reactions != 0;  // only reactions make XF cells
+    const bool xf_on = RE != 0 && react;  // only reactions make XF cells
     if (interior) {
         // a uniform base plus a 32-bit offset per row
         char *__restrict__ dstb = reinterpret_cast<char *>(out + (size_t)(gy0 + HALO) * w.W + (gx0 + HALO));
@@ -542,7 +551,10 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     }
 }
 
-// Five workgroups per CU (95 VGPRs, 24 KB of LDS each).
+// Five workgroups per CU (at most 96 VGPRs, 28 KB of LDS each).  One
+// instantiation per (chunk skipping, reactions) setting of the world, so
+// the branches and paths a setting never takes are not compiled in.
+template <int SK, int RE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_tick_world(DevWorld w, uint32_t k) {
     __shared__ __align__(16) uint32_t Lmem[LH * LW + 16];  // the region, then the 16 base colours
     __shared__ uint32_t lut[MAX_IT + 1][256];               // pair tables + scatter offsets
@@ -551,8 +563,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
     __shared__ int32_t addt[16];
     __shared__ uint16_t queue[(NT / 64) * QCAP];
     __shared__ uint32_t red[NT / 64];
-    tile_update(w, w.tick + k, blockIdx.x, L, lut, rgba, addt, queue, red);
+    tile_update<SK, RE>(w, w.tick + k, blockIdx.x, L, lut, rgba, addt, queue, red);
 }
+template __global__ void k_tick_world<0, 0>(DevWorld, uint32_t);
+template __global__ void k_tick_world<0, 1>(DevWorld, uint32_t);
+template __global__ void k_tick_world<1, 0>(DevWorld, uint32_t);
+template __global__ void k_tick_world<1, 1>(DevWorld, uint32_t);
 
 // Many worlds in one launch: tile_start[i] is the first global chunk index of
 // world i (prefix sums, tile_start[n] = total).
@@ -573,7 +589,7 @@ __device__ __forceinline__ void tick_batch_chunk(const DevWorld *__restrict__ wo
         else hi = mid - 1;
     }
     const DevWorld &w = worlds[lo];
-    tile_update(w, w.tick + k, (int)(gt - tile_start[lo]), L, lut, rgba, addt, queue, red);
+    tile_update<2, 2>(w, w.tick + k, (int)(gt - tile_start[lo]), L, lut, rgba, addt, queue, red);
 }
 
 // Two register budgets of the batch tick, one of the autotuner's choices:

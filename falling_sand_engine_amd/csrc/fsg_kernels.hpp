@@ -33,6 +33,7 @@ struct StampArgs {
     uint32_t *nopen;           // their count
 };
 
+template <int SK, int RE>
 __global__ void k_tick_world(DevWorld w, uint32_t k);
 __global__ void k_tick_batch(const DevWorld *__restrict__ worlds, const uint32_t *__restrict__ tile_start,
                              int nworlds, uint32_t k);

@@ -354,6 +354,59 @@ int fsg_world_physics_check(fsg_world *w, fsg_island *seeds, int n, size_t *pixe
 int fsg_world_island_pixels(fsg_world *w, fsg_cell *out, size_t cap);
 
 /* ------------------------------------------------------------------------ */
+/* Outlines of solid cells for the host physics engine: the marching-squares */
+/* step of world::updateChunkMesh (world.cpp:722-895, which the mesh zone     */
+/* runs for every chunk, world.cpp:961-998) and of updateRigidBodyHitbox      */
+/* (world.cpp:288-510).  For each region the reference scans the cells in    */
+/* row order for the next solid cell whose right, lower and lower-right     */
+/* neighbours are not all solid and that no earlier loop marked (edgeSeen),  */
+/* traces its perimeter with MarchingSquares::FindPerimeter                 */
+/* (physics_math.cpp:1872-1965), marks the cells along it, simplifies the    */
+/* loop's run-end vertices with Ramer-Douglas-Peucker at tolerance 1         */
+/* (simplify, physics_math.cpp:1766-1812) and drops loops left with fewer    */
+/* than 3 vertices; a loop that TPPLPoly::GetOrientation calls clockwise is  */
+/* a hole.  The device does all of that; the host keeps TPPLPartition's      */
+/* RemoveHoles / Triangulate_EC and the Box2D shapes.                        */
+/* ------------------------------------------------------------------------ */
+#define FSG_MESH_MAX_CELLS 16384 /* cells of one region (a 128x128 chunk) */
+#define FSG_MESH_EDGE 1          /* region status: not meshed (see below) */
+
+typedef struct fsg_mesh_region {
+    int32_t x0, y0, w, h;        /* in: grid cells (world) or image size (islands) */
+    int32_t status;              /* out: FSG_OK, FSG_MESH_EDGE, or FSG_ERR_INVALID */
+    uint32_t first_loop, nloops; /* out: its loops in the loop array */
+} fsg_mesh_region;
+
+typedef struct fsg_mesh_loop {
+    uint32_t first;           /* first vertex: (x, y) float pairs, region coordinates */
+    uint32_t count;           /* vertices (>= 3), in the order of the reference's worldMesh;
+                               * the reference fills its TPPLPoly in reverse */
+    int32_t hole;             /* 1: clockwise by GetOrientation (SetHole(true)) */
+    int32_t start_x, start_y; /* the cell the perimeter was traced from */
+    uint32_t region;          /* index of its region */
+} fsg_mesh_loop;
+
+/* Outlines the SOLID cells of n regions of the grid (a region is meshed as
+ * updateChunkMesh meshes a chunk: chunks are 128x128, at most
+ * FSG_MESH_MAX_CELLS cells).  A region that reaches the world's last row or
+ * column or lies outside it gets FSG_MESH_EDGE and no loops (the reference
+ * skips chunks with chTx + CHUNK_W >= width, world.cpp:734-736).  The loops
+ * stay in device memory owned by the world until the next mesh call;
+ * *nloops / *nverts (may be NULL) get their totals. */
+int fsg_world_mesh(fsg_world *w, fsg_mesh_region *regions, int n, uint32_t *nloops, uint32_t *nverts);
+/* The same for the body images of the world's last island check
+ * (fsg_world_physics_check): islands[i] with status FSG_ISLAND_BODY are
+ * outlined over their w x h image (non-air pixels; the images are already
+ * cropped to the island, as updateRigidBodyHitbox crops its surface);
+ * regions[i] (n entries) receives each island's result, FSG_MESH_EDGE for
+ * other statuses and for images 1 pixel wide or high (world.cpp:331, 365). */
+int fsg_world_mesh_islands(fsg_world *w, const fsg_island *islands, int n, fsg_mesh_region *regions,
+                           uint32_t *nloops, uint32_t *nverts);
+/* Copies the loops and vertices (2 floats each) of the last mesh call. */
+int fsg_world_mesh_download(fsg_world *w, fsg_mesh_loop *loops, uint32_t loop_cap, float *verts,
+                            uint32_t vert_cap);
+
+/* ------------------------------------------------------------------------ */
 /* Texture pass (game.cpp:1994-2067): the grid as RGBA8 layers in device     */
 /* memory, byte order R,G,B,A per texel, row-major with pitch width texels.  */
 /* The reference rewrites the texels of dirty cells after every tick; every  */

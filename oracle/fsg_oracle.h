@@ -121,6 +121,15 @@ int orc_refstamp(uint64_t *cells, int W, int H, const fsg_config *cfg, const fsg
 int orc_refunstamp(uint64_t *cells, int W, int H, const fsg_config *cfg, const fsg_body *bodies, int nbodies,
                    uint64_t *pixels, size_t npixels);
 
+/* Marching-squares outline of a binary mask (fsg_mesh_oracle.c): the
+ * vertex value of MarchingSquares::value, FindPerimeter's runs, simplify,
+ * and updateChunkMesh's loop scan over a whole mask. */
+int orc_ms_value(const uint8_t *m, int W, int H, int x, int y);
+int orc_find_perimeter(const uint8_t *m, int W, int H, int x0, int y0, int32_t *dirs, int cap);
+size_t orc_simplify(const float *pts, size_t n, float tol, float *out);
+int orc_mesh_mask(const uint8_t *m, int W, int H, fsg_mesh_loop *loops, uint32_t loop_cap, float *verts,
+                  uint32_t vert_cap, uint32_t *nloops, uint32_t *nverts);
+
 void orc_counts(const uint64_t *cells, size_t n, uint64_t counts[16]);
 uint64_t orc_checksum(const uint64_t *cells, size_t n);
 

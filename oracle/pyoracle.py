@@ -9,7 +9,7 @@ import subprocess
 
 import numpy as np
 
-from falling_sand_engine_amd._abi import Body, CellEdit, Config, Island, PackMap, RenderParams, RenderStats
+from falling_sand_engine_amd._abi import Body, CellEdit, Config, Island, MeshLoop, PackMap, RenderParams, RenderStats
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
@@ -48,6 +48,10 @@ def lib():
             "orc_refstamp": (i32, [P, i32, i32, cfgp, P, i32, P, C.c_size_t, P, C.POINTER(u32),
                                    C.POINTER(u32), P, u32]),
             "orc_refunstamp": (i32, [P, i32, i32, cfgp, P, i32, P, C.c_size_t]),
+            "orc_ms_value": (i32, [P, i32, i32, i32, i32]),
+            "orc_find_perimeter": (i32, [P, i32, i32, i32, i32, P, i32]),
+            "orc_simplify": (C.c_size_t, [P, C.c_size_t, C.c_float, P]),
+            "orc_mesh_mask": (i32, [P, i32, i32, P, u32, P, u32, C.POINTER(u32), C.POINTER(u32)]),
             "orc_physics_check": (i32, [P, i32, i32, cfgp, P, i32, P, C.c_size_t, C.POINTER(C.c_size_t)]),
             "orc_render": (i32, [P, P, i32, i32, cfgp, C.POINTER(RenderParams), P, P, P,
                                  C.POINTER(RenderStats)]),
@@ -326,3 +330,50 @@ def unpack_chunk(img: bytes, n_materials: int, pack_map=None):
                                 l2.ctypes.data, 128, bg.ctypes.data, 128, C.byref(ph),
                                 None if pack_map is None else C.byref(pack_map))
     return rc, tiles, l2, bg, ph.value
+
+
+def _mask(m):
+    m = np.ascontiguousarray(m, dtype=np.uint8)
+    assert m.ndim == 2
+    return m, m.shape[1], m.shape[0]
+
+
+def ms_value(mask, x, y) -> int:
+    """MarchingSquares::value at vertex (x, y) of a 0/1 mask (H, W)."""
+    m, W, H = _mask(mask)
+    return lib().orc_ms_value(m.ctypes.data, W, H, x, y)
+
+
+def find_perimeter(mask, x0, y0):
+    """FindPerimeter's runs [(dx, dy), ...] from vertex (x0, y0); [] off a perimeter."""
+    m, W, H = _mask(mask)
+    cap = 4 * (W + 1) * (H + 1) + 8
+    d = np.zeros(2 * cap, np.int32)
+    n = lib().orc_find_perimeter(m.ctypes.data, W, H, x0, y0, d.ctypes.data, cap)
+    assert n >= 0, n
+    return [(int(d[2 * i]), int(d[2 * i + 1])) for i in range(n)]
+
+
+def simplify(points, tol=1.0):
+    """simplify (Ramer-Douglas-Peucker in f32) of [(x, y), ...]."""
+    p = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 2))
+    out = np.zeros_like(p)
+    n = lib().orc_simplify(p.ctypes.data, len(p), tol, out.ctypes.data)
+    return [(float(a), float(b)) for a, b in out[:n]]
+
+
+def mesh_mask(mask):
+    """updateChunkMesh's outline of a 0/1 mask: [(start_x, start_y, hole, [(x, y), ...]), ...]."""
+    m, W, H = _mask(mask)
+    lcap, vcap = W * H + 8, 4 * (W + 1) * (H + 1) + 8
+    loops = (MeshLoop * lcap)()
+    verts = np.zeros(2 * vcap, np.float32)
+    nl, nv = C.c_uint32(), C.c_uint32()
+    rc = lib().orc_mesh_mask(m.ctypes.data, W, H, loops, lcap, verts.ctypes.data, vcap, C.byref(nl), C.byref(nv))
+    assert rc == 0, rc
+    out = []
+    for i in range(nl.value):
+        L = loops[i]
+        v = verts[2 * L.first:2 * (L.first + L.count)].reshape(-1, 2)
+        out.append((L.start_x, L.start_y, bool(L.hole), [(float(a), float(b)) for a, b in v]))
+    return out
