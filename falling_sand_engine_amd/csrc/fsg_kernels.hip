@@ -266,6 +266,24 @@ __device__ __forceinline__ bool final_words(const MatTab &M, const uint32_t (*L)
         }
         return false;
     }
+    if (!WALLS && !M.any_add_temp) {
+        // T + ((Tu + Td + Tl + Tr - 4T) >> 3) is T when the four neighbours
+        // share T, and no material adds heat: a wave whose strips and their
+        // neighbours all hold one temperature keeps its words as they are.
+        const uint32_t w0 = L[ly0][lx];
+        uint32_t acc = 0;  // OR of the differences from w0; temperatures in bits 16-31
+#pragma unroll
+        for (int r = 0; r < TH / 8; r++) {
+            const uint2 c = *reinterpret_cast<const uint2 *>(&L[ly0 + r][lx]);
+            fw[2 * r] = c.x;
+            fw[2 * r + 1] = c.y;
+            acc |= (c.x ^ w0) | (c.y ^ w0) | (L[ly0 + r][lx - 1] ^ w0) | (L[ly0 + r][lx + 2] ^ w0);
+        }
+        const uint2 up = *reinterpret_cast<const uint2 *>(&L[ly0 - 1][lx]);
+        const uint2 dn = *reinterpret_cast<const uint2 *>(&L[ly0 + TH / 8][lx]);
+        acc |= (up.x ^ w0) | (up.y ^ w0) | (dn.x ^ w0) | (dn.y ^ w0);
+        if (__all((acc >> 16) == 0)) return false;
+    }
     uint2 up = *reinterpret_cast<const uint2 *>(&L[ly0 - 1][lx]);
     uint2 cur = *reinterpret_cast<const uint2 *>(&L[ly0][lx]);
 #pragma unroll
