@@ -324,6 +324,44 @@ def bench_islands(fsg, side=4096, nseeds=4096, seed=9):
             "bodies": st.count(3), "removed_small": st.count(2), "attached": st.count(1), "none": st.count(0)}
 
 
+def bench_mesh(fsg, side=4096, seed=12, reps=3):
+    """Outlines of the SOLID cells of every 128x128 chunk that ends before
+    the world's last row and column (world::updateChunkMesh for a whole mesh
+    zone, world.cpp:722-998) on a basin world and on a world of stone
+    fragments; the whole synchronous call (launch, pools, host ordering).
+    The oracle's restatement of the same step, one thread, on 64 chunks of
+    each world is the CPU figure."""
+    import numpy as np
+    from oracle import pyoracle
+    out = {"workload": f"every 128x128 chunk of {side}x{side} worlds (basin kind 2; stone fragments p=0.5)"}
+    n = side // 128
+    regions = [(cx * 128, cy * 128, 128, 128) for cy in range(n) for cx in range(n)]
+    cfg = fsg.default_config()
+    for name, cells in (("basin", fsg.generate(2, side, side, seed)), ("fragments", fragments(side, seed))):
+        with fsg.Context(0) as ctx, fsg.World(ctx, side, side, seed, cfg) as w:
+            w.upload(cells)
+            w.mesh(regions)  # pools sized
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                res = w.mesh(regions)
+                ts.append(time.perf_counter() - t0)
+        loops = sum(len(r[1]) for r in res)
+        meshed = sum(1 for r in res if r[0] == 0)
+        types = np.array([cfg.materials[i].type for i in range(cfg.n_materials)] + [0] * 16, np.uint8)
+        solid = (types[(cells & np.uint64(15)).astype(np.int64)] == fsg.SOLID).astype(np.uint8)
+        sample = [r for r in regions if r[0] + 128 < side and r[1] + 128 < side][:64]
+        t0 = time.perf_counter()
+        for x0, y0, _w, _h in sample:
+            pyoracle.mesh_mask(solid[y0:y0 + 128, x0:x0 + 128])
+        cpu = (time.perf_counter() - t0) / len(sample)
+        ms = min(ts) * 1e3
+        out[name] = {"chunks_meshed": meshed, "loops": loops, "ms": round(ms, 2),
+                     "chunks_per_s": round(meshed / (ms / 1e3)),
+                     "cpu_oracle_1_thread_chunks_per_s": round(1.0 / cpu, 1)}
+    return out
+
+
 PACK_RAW = 393216 + 65536  # .pack payload per chunk: tiles + layer2 records, background
 
 
@@ -490,7 +528,7 @@ def main():
     ap.add_argument("--side", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
-    ap.add_argument("--mode", default="all", choices=["all", "c3", "copy", "c5", "render", "islands", "pack"],
+    ap.add_argument("--mode", default="all", choices=["all", "c3", "copy", "c5", "render", "islands", "pack", "mesh"],
                     help="all: the JSON line; c3: only the timed tick loop (for profiling); "
                          "copy: the HBM copy calibration kernel; render: only the texture pass")
     ap.add_argument("--copy-bytes", type=int, default=2 << 30)
@@ -515,6 +553,9 @@ def main():
         return
     if args.mode == "islands":
         print(json.dumps(bench_islands(fsg)))
+        return
+    if args.mode == "mesh":
+        print(json.dumps(bench_mesh(fsg)))
         return
     if args.mode == "pack":
         r = bench_pack(fsg, side=args.side)
@@ -558,6 +599,9 @@ def main():
         log("[bench] island check")
         secondary["islands"] = bench_islands(fsg)
         log(f"[bench] {secondary['islands']}")
+        log("[bench] chunk outlines")
+        secondary["mesh"] = bench_mesh(fsg)
+        log(f"[bench] {secondary['mesh']}")
         log("[bench] chunk .pack codec")
         secondary["pack"] = bench_pack(fsg)
         log(f"[bench] {secondary['pack']}")

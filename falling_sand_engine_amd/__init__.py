@@ -309,6 +309,7 @@ class World:
             arr[i].x, arr[i].y = int(x), int(y)
         need = C.c_size_t()
         check(lib().fsg_world_physics_check(self.handle, arr, n, C.byref(need)), "fsg_world_physics_check")
+        self._islands = (arr, n)
         pix = np.empty(need.value, dtype=np.uint64)
         if need.value:
             check(lib().fsg_world_island_pixels(self.handle, pix.ctypes.data, pix.size), "fsg_world_island_pixels")
@@ -370,6 +371,44 @@ class World:
             out.append(buf[o:o + int(v)].tobytes())
             o += int(v)
         return out
+
+    def _mesh_result(self, regs, nl, nv):
+        loops = (_abi.MeshLoop * max(nl.value, 1))()
+        verts = np.zeros(max(2 * nv.value, 2), np.float32)
+        check(lib().fsg_world_mesh_download(self.handle, loops, nl.value, verts.ctypes.data, nv.value),
+              "fsg_world_mesh_download")
+        out = []
+        for R in regs:
+            ls = []
+            for k in range(R.first_loop, R.first_loop + R.nloops):
+                L = loops[k]
+                v = verts[2 * L.first:2 * (L.first + L.count)].reshape(-1, 2)
+                ls.append((L.start_x, L.start_y, bool(L.hole), [(float(a), float(b)) for a, b in v]))
+            out.append((R.status, ls))
+        return out
+
+    def mesh(self, regions):
+        """Outlines of the SOLID cells of grid regions [(x0, y0, w, h), ...]
+        (world::updateChunkMesh's marching squares, world.cpp:722-895): per
+        region (status, [(start_x, start_y, hole, [(x, y), ...]), ...])."""
+        self._flush()
+        regs = (_abi.MeshRegion * max(len(regions), 1))()
+        for i, (x0, y0, w, h) in enumerate(regions):
+            regs[i].x0, regs[i].y0, regs[i].w, regs[i].h = x0, y0, w, h
+        nl, nv = C.c_uint32(), C.c_uint32()
+        check(lib().fsg_world_mesh(self.handle, regs, len(regions), C.byref(nl), C.byref(nv)), "fsg_world_mesh")
+        return self._mesh_result(list(regs)[:len(regions)], nl, nv)
+
+    def mesh_islands(self):
+        """Outlines of the body images of the last physicsCheck (the marching
+        squares of updateRigidBodyHitbox, world.cpp:288-510), one entry per
+        seed as mesh() returns them."""
+        islands, n = self._islands
+        regs = (_abi.MeshRegion * max(n, 1))()
+        nl, nv = C.c_uint32(), C.c_uint32()
+        check(lib().fsg_world_mesh_islands(self.handle, islands, n, regs, C.byref(nl), C.byref(nv)),
+              "fsg_world_mesh_islands")
+        return self._mesh_result(list(regs)[:n], nl, nv)
 
     def set_pack_map(self, pack_map=None):
         """Material index map of later pack()/unpack() calls (a PackMap, e.g.

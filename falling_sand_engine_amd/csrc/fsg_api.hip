@@ -472,6 +472,9 @@ extern "C" int fsg_world_destroy(fsg_world *w) {
     dev_free(w->d_dispxy);
     dev_free(w->d_tgt);
     dev_free(w->d_save);
+    dev_free(w->d_mesh_regs);
+    dev_free(w->d_mesh_scratch);
+    dev_free(w->d_mesh_pool);
     for (uint32_t *l : w->d_layer) dev_free(l);
     dev_free(w->d_tally);
     dev_free(w->d_isl);

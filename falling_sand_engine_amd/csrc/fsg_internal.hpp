@@ -90,6 +90,13 @@ struct fsg_world {
     size_t pack_n = 0, pack_total = 0;     // chunks and bytes of the last fsg_world_pack
     std::vector<int32_t> pack_sizes;       // its block sizes (A, B per chunk)
     std::vector<int8_t> pack_phase;        // its generation phases
+    // outlines (fsg_mesh.hip): region table, per-workgroup loop scratch, output
+    // pools, their sizes from the last call, and the ordered result
+    void *d_mesh_regs = nullptr, *d_mesh_scratch = nullptr, *d_mesh_pool = nullptr;
+    size_t mesh_regs_cap = 0, mesh_scratch_cap = 0, mesh_pool_cap = 0;
+    uint32_t mesh_loop_hint = 0, mesh_vert_hint = 0;
+    std::vector<fsg_mesh_loop> mesh_loops;
+    std::vector<float> mesh_verts;
     bool pack_map_set = false;             // pack_map holds a caller's map (else the default)
     fsg_pack_map pack_map{};
     size_t bytes = 0;
