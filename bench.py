@@ -362,6 +362,39 @@ def bench_mesh(fsg, side=4096, seed=12, reps=3):
     return out
 
 
+def bench_particles(fsg, side=4096, n=100000, passes=20, seed=21):
+    """world::tickCells over a spray of n particles (cells carried by bodies'
+    displaced cells in the reference) above a basin world: device time per
+    pass of the whole synchronous call; the oracle's restatement, one thread,
+    on the same list is the CPU figure."""
+    import numpy as np
+    from oracle import pyoracle
+    rng = np.random.default_rng(seed)
+    cfg = fsg.default_config()
+    cells = fsg.generate(2, side, side, seed)
+    ps = [fsg.make_particle(fsg.make_cell(int(rng.choice([2, 3, 4])), 320, 0xFF808080),
+                            float(rng.uniform(1, side - 1)), float(rng.uniform(1, side * 0.5)),
+                            float(rng.uniform(-1, 1)), float(rng.uniform(-1, 1)), 0.0, 0.1) for _ in range(n)]
+    with fsg.Context(0) as ctx, fsg.World(ctx, side, side, seed, cfg) as w:
+        w.upload(cells)
+        w.particles_add(ps)
+        w.particles_tick()
+        ts, landed = [], 0
+        for _ in range(passes):
+            t0 = time.perf_counter()
+            alive, nl = w.particles_tick()
+            ts.append(time.perf_counter() - t0)
+            landed += nl
+    ref = cells.copy()
+    t0 = time.perf_counter()
+    pyoracle.particles_tick(ref, pyoracle.default_config(), ps)
+    cpu = time.perf_counter() - t0
+    ms = float(np.median(ts)) * 1e3
+    return {"workload": f"{n} particles over a {side}x{side} basin world, {passes} passes",
+            "ms_per_pass": round(ms, 3), "particles_per_s": round(n / (ms / 1e3)), "landed": landed,
+            "alive_after": alive, "cpu_oracle_1_thread_particles_per_s": round(n / cpu)}
+
+
 PACK_RAW = 393216 + 65536  # .pack payload per chunk: tiles + layer2 records, background
 
 
@@ -528,7 +561,7 @@ def main():
     ap.add_argument("--side", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
-    ap.add_argument("--mode", default="all", choices=["all", "c3", "copy", "c5", "render", "islands", "pack", "mesh"],
+    ap.add_argument("--mode", default="all", choices=["all", "c3", "copy", "c5", "render", "islands", "pack", "mesh", "particles"],
                     help="all: the JSON line; c3: only the timed tick loop (for profiling); "
                          "copy: the HBM copy calibration kernel; render: only the texture pass")
     ap.add_argument("--copy-bytes", type=int, default=2 << 30)
@@ -556,6 +589,9 @@ def main():
         return
     if args.mode == "mesh":
         print(json.dumps(bench_mesh(fsg)))
+        return
+    if args.mode == "particles":
+        print(json.dumps(bench_particles(fsg)))
         return
     if args.mode == "pack":
         r = bench_pack(fsg, side=args.side)
@@ -602,6 +638,9 @@ def main():
         log("[bench] chunk outlines")
         secondary["mesh"] = bench_mesh(fsg)
         log(f"[bench] {secondary['mesh']}")
+        log("[bench] particles")
+        secondary["particles"] = bench_particles(fsg)
+        log(f"[bench] {secondary['particles']}")
         log("[bench] chunk .pack codec")
         secondary["pack"] = bench_pack(fsg)
         log(f"[bench] {secondary['pack']}")

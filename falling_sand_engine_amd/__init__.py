@@ -91,6 +91,33 @@ def reference_pack_map(keep_moved: bool = True) -> PackMap:
     return m
 
 
+def make_particle(cell, x, y, vx=0.0, vy=0.0, ax=0.0, ay=0.0, lifetime=0, flags=0, target=None) -> _abi.Particle:
+    """A CellData (game_utils/cells.h:15-36) carrying `cell` (a uint64 cell)."""
+    p = _abi.Particle()
+    p.x, p.y, p.vx, p.vy, p.ax, p.ay = x, y, vx, vy, ax, ay
+    if target is not None:
+        p.target_x, p.target_y, p.target_force = target
+    p.lifetime, p.flags = lifetime, flags
+    c = int(cell)
+    p.tile.mat, p.tile.temp, p.tile.rgba = c & 0xFFFF, ((c >> 16) & 0xFFFF) - (0x10000 if (c >> 31) & 1 else 0), c >> 32
+    return p
+
+
+def particles_from_displaced(displaced, seed: int, tick: int):
+    """Particles for the cells a body stamp displaced, as game.cpp:1789-1791
+    ejects them: at (x, y - 3) with vx = (r % 10 - 5) / 10, vy = -(r' % 5 + 5)
+    / 10 and gravity 0.1, r and r' from a counter hash of (seed, tick, index)
+    in place of rand()."""
+    out = []
+    key = lib().fsg_phase_key(seed, tick)
+    for k, (x, y, cell) in enumerate(displaced):
+        r = lib().fsg_splitmix64(key ^ (k + 1))
+        vx = ((r & 0xFFFFFFFF) % 10 - 5) / 10.0
+        vy = -(((r >> 32) % 5) + 5) / 10.0
+        out.append(make_particle(cell, float(x), float(y - 3), vx, vy, 0.0, 0.1))
+    return out
+
+
 def make_cell(mat, temp=320, rgba=0):
     return (int(mat) & 0xFFFF) | ((int(temp) & 0xFFFF) << 16) | ((int(rgba) & 0xFFFFFFFF) << 32)
 
@@ -409,6 +436,28 @@ class World:
         check(lib().fsg_world_mesh_islands(self.handle, islands, n, regs, C.byref(nl), C.byref(nv)),
               "fsg_world_mesh_islands")
         return self._mesh_result(list(regs)[:n], nl, nv)
+
+    # --- free-fall particles (CellData, world::tickCells) --------------------
+    def particles_add(self, particles):
+        """Appends Particle structs (see make_particle) to the world's list."""
+        self._flush()
+        arr = (_abi.Particle * max(len(particles), 1))(*particles)
+        check(lib().fsg_world_particles_add(self.handle, arr, len(particles)), "fsg_world_particles_add")
+
+    def particles_tick(self):
+        """One world::tickCells pass; returns (particles left, cells landed)."""
+        self._flush()
+        alive, landed = C.c_size_t(), C.c_size_t()
+        check(lib().fsg_world_particles_tick(self.handle, C.byref(alive), C.byref(landed)),
+              "fsg_world_particles_tick")
+        return alive.value, landed.value
+
+    def particles(self):
+        n = C.c_size_t()
+        check(lib().fsg_world_particles_get(self.handle, None, 0, C.byref(n)), "fsg_world_particles_get")
+        arr = (_abi.Particle * max(n.value, 1))()
+        check(lib().fsg_world_particles_get(self.handle, arr, n.value, C.byref(n)), "fsg_world_particles_get")
+        return [arr[i] for i in range(n.value)]
 
     def set_pack_map(self, pack_map=None):
         """Material index map of later pack()/unpack() calls (a PackMap, e.g.

@@ -132,6 +132,12 @@ class MeshLoop(C.Structure):
                 ("start_y", C.c_int32), ("region", C.c_uint32)]
 
 
+class Particle(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("x", "y", "vx", "vy", "ax", "ay", "target_x", "target_y",
+                                         "target_force")] + \
+               [("lifetime", C.c_int32), ("flags", C.c_uint32), ("in_object_state", C.c_uint32), ("tile", Cell)]
+
+
 class PackMap(C.Structure):
     _fields_ = [("to_file", C.c_uint32 * 16), ("from_file", C.c_int16 * 256), ("flags", C.c_uint32)]
 
@@ -177,6 +183,9 @@ SIGNATURES = {
     "fsg_world_mesh": (C.c_int, [P, P, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "fsg_world_mesh_islands": (C.c_int, [P, P, C.c_int, P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "fsg_world_mesh_download": (C.c_int, [P, P, C.c_uint32, P, C.c_uint32]),
+    "fsg_world_particles_add": (C.c_int, [P, P, C.c_size_t]),
+    "fsg_world_particles_tick": (C.c_int, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "fsg_world_particles_get": (C.c_int, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fsg_world_render": (C.c_int, [P, C.POINTER(RenderParams), C.POINTER(RenderStats)]),
     "fsg_world_layer": (C.c_int, [P, C.c_uint32, C.POINTER(P)]),
     "fsg_world_layer_download": (C.c_int, [P, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int, P]),
@@ -240,3 +249,4 @@ CHUNK = 128  # FSG_CHUNK: chunk side of the .pack images (const.h:19-20)
 PACK_NO_MOVED = 1  # FSG_PACK_NO_MOVED
 MESH_MAX_CELLS = 16384  # FSG_MESH_MAX_CELLS
 MESH_EDGE = 1  # FSG_MESH_EDGE
+PARTICLE_PHASE, PARTICLE_TEMPORARY = 1, 2  # FSG_PARTICLE_*

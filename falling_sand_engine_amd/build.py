@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libfsg.so")
 
-SOURCES = ["fsg_kernels.hip", "fsg_api.hip", "fsg_serve.hip", "fsg_render.hip", "fsg_island.hip", "fsg_generate.hip", "fsg_pack.hip", "fsg_mesh.hip", "fsg_noalloc.hip", "fsg_host.cpp"]
+SOURCES = ["fsg_kernels.hip", "fsg_api.hip", "fsg_serve.hip", "fsg_render.hip", "fsg_island.hip", "fsg_generate.hip", "fsg_pack.hip", "fsg_mesh.hip", "fsg_particles.hip", "fsg_noalloc.hip", "fsg_host.cpp"]
 HEADERS = ["fsg_device.hpp", "fsg_kernels.hpp", "fsg_internal.hpp", "fsg_generate.hpp", "fsg_lz4.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

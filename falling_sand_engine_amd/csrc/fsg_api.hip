@@ -475,6 +475,10 @@ extern "C" int fsg_world_destroy(fsg_world *w) {
     dev_free(w->d_mesh_regs);
     dev_free(w->d_mesh_scratch);
     dev_free(w->d_mesh_pool);
+    dev_free(w->d_part);
+    dev_free(w->d_part2);
+    dev_free(w->d_part_alive);
+    dev_free(w->d_part_cnt);
     for (uint32_t *l : w->d_layer) dev_free(l);
     dev_free(w->d_tally);
     dev_free(w->d_isl);

@@ -50,6 +50,16 @@ struct DevWorld {
     uint16_t pair[MAX_IT][256];  // pair_code(mat, it, t, u) at [it][t * 16 + u], built on the host
 };
 
+// Chunk activity flags (one u32 per chunk per tick parity):
+//   F_POT   the chunk may change next tick even if no neighbour changes;
+//   F_ANY   the chunk changed this tick;
+//   F_N..F_SE  it changed within HALO cells of that edge / corner, the band
+//           a neighbour's region (chunk + HALO ring) overlaps.
+constexpr uint32_t F_POT = 1u, F_ANY = 2u;
+constexpr uint32_t F_N = 4u, F_S = 8u, F_W = 16u, F_E = 32u;
+constexpr uint32_t F_NW = 64u, F_NE = 128u, F_SW = 256u, F_SE = 512u;
+constexpr uint32_t F_WAKE_ALL = 0x3FFu;  // changed everywhere: wakes the chunk and its 8 neighbours
+
 __host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     uint64_t z = x + GOLDEN;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;

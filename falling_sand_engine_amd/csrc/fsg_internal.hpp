@@ -97,6 +97,12 @@ struct fsg_world {
     uint32_t mesh_loop_hint = 0, mesh_vert_hint = 0;
     std::vector<fsg_mesh_loop> mesh_loops;
     std::vector<float> mesh_verts;
+    // free-fall particles (fsg_particles.hip): the list, the compaction
+    // target, per-particle kept flags, counters
+    fsg_particle *d_part = nullptr, *d_part2 = nullptr;
+    uint8_t *d_part_alive = nullptr;
+    unsigned long long *d_part_cnt = nullptr;
+    size_t part_cap = 0, npart = 0;
     bool pack_map_set = false;             // pack_map holds a caller's map (else the default)
     fsg_pack_map pack_map{};
     size_t bytes = 0;

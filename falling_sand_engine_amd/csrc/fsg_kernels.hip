@@ -30,16 +30,6 @@ __device__ __forceinline__ bool wood_hot(const MatTab &M, uint32_t c) {
     return cbyte(c) == M.wood && ctemp(c) > M.burn;
 }
 
-// Chunk activity flags (one u32 per chunk per tick parity):
-//   F_POT   the chunk may change next tick even if no neighbour changes;
-//   F_ANY   the chunk changed this tick;
-//   F_N..F_SE  it changed within HALO cells of that edge / corner, the band
-//           a neighbour's region (chunk + HALO ring) overlaps.
-constexpr uint32_t F_POT = 1u, F_ANY = 2u;
-constexpr uint32_t F_N = 4u, F_S = 8u, F_W = 16u, F_E = 32u;
-constexpr uint32_t F_NW = 64u, F_NE = 128u, F_SW = 256u, F_SE = 512u;
-constexpr uint32_t F_WAKE_ALL = 0x3FFu;  // changed everywhere: wakes the chunk and its 8 neighbours
-
 // Bands touched by a change covering chunk cells [x0, x1] x [y0, y1]
 // (chunk coordinates, already clipped to the chunk).
 __device__ __forceinline__ uint32_t band_bits(int x0, int x1, int y0, int y1) {

@@ -407,6 +407,41 @@ int fsg_world_mesh_download(fsg_world *w, fsg_mesh_loop *loops, uint32_t loop_ca
                             uint32_t vert_cap);
 
 /* ------------------------------------------------------------------------ */
+/* Free-fall particles: CellData (game_utils/cells.h:15-36) and              */
+/* world::tickCells (world.cpp:2030-2189).  A particle carries one cell and  */
+/* moves in f32 steps; when it enters a cell that is not AIR it lands: in    */
+/* the cell it started the tick in if that is AIR, else in the first AIR     */
+/* cell of a 33x33 square spiral around it; with none it is thrown up 16    */
+/* cells.  Particles move in list order, and each sees the cells earlier     */
+/* particles landed in during the same tick, as the reference's loop does.   */
+/* The reference spawns them for cells a rigid body displaces (game.cpp:     */
+/* 1788-1812) and for falling sand and liquid (world.cpp:1218-1225,          */
+/* 1283-1305); this build's tick keeps falling cells on the grid, so the     */
+/* body stamp's displaced cells are the source (fsg_stamp_result).           */
+/* ------------------------------------------------------------------------ */
+#define FSG_PARTICLE_PHASE 1u     /* CellData::phase: passes through cells     */
+#define FSG_PARTICLE_TEMPORARY 2u /* CellData::temporary: dies at lifetime 0   */
+
+typedef struct fsg_particle {
+    float x, y, vx, vy, ax, ay;     /* CellData fields                          */
+    float target_x, target_y, target_force;
+    int32_t lifetime;
+    uint32_t flags;                 /* FSG_PARTICLE_*                           */
+    uint32_t in_object_state;       /* CellData::inObjectState (0..2)           */
+    fsg_cell tile;                  /* the cell it carries                      */
+} fsg_particle;
+
+/* Appends n particles to the world's list (the reference's cells vector). */
+int fsg_world_particles_add(fsg_world *w, const fsg_particle *p, size_t n);
+/* One world::tickCells pass over the list, landing into the grid; the list
+ * keeps its order, without the particles that landed, left the world or
+ * ran out of lifetime.  *alive (may be NULL) gets the new count, *landed
+ * (may be NULL) the cells written. */
+int fsg_world_particles_tick(fsg_world *w, size_t *alive, size_t *landed);
+/* Copies the list (at most cap particles) and its length. */
+int fsg_world_particles_get(fsg_world *w, fsg_particle *out, size_t cap, size_t *n);
+
+/* ------------------------------------------------------------------------ */
 /* Texture pass (game.cpp:1994-2067): the grid as RGBA8 layers in device     */
 /* memory, byte order R,G,B,A per texel, row-major with pitch width texels.  */
 /* The reference rewrites the texels of dirty cells after every tick; every  */

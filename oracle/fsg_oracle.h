@@ -130,6 +130,12 @@ size_t orc_simplify(const float *pts, size_t n, float tol, float *out);
 int orc_mesh_mask(const uint8_t *m, int W, int H, fsg_mesh_loop *loops, uint32_t loop_cap, float *verts,
                   uint32_t vert_cap, uint32_t *nloops, uint32_t *nverts);
 
+/* world::tickCells (fsg_particle_oracle.c): one pass over the particle list
+ * in order, landing into cells; the list is compacted in place. */
+int orc_spiral(int32_t *offs);
+int orc_particles_tick(uint64_t *cells, int W, int H, const fsg_config *cfg, fsg_particle *p, size_t n,
+                       size_t *alive, size_t *landed);
+
 void orc_counts(const uint64_t *cells, size_t n, uint64_t counts[16]);
 uint64_t orc_checksum(const uint64_t *cells, size_t n);
 

@@ -9,7 +9,8 @@ import subprocess
 
 import numpy as np
 
-from falling_sand_engine_amd._abi import Body, CellEdit, Config, Island, MeshLoop, PackMap, RenderParams, RenderStats
+from falling_sand_engine_amd._abi import (Body, CellEdit, Config, Island, MeshLoop, PackMap, Particle, RenderParams,
+                                          RenderStats)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
@@ -48,6 +49,9 @@ def lib():
             "orc_refstamp": (i32, [P, i32, i32, cfgp, P, i32, P, C.c_size_t, P, C.POINTER(u32),
                                    C.POINTER(u32), P, u32]),
             "orc_refunstamp": (i32, [P, i32, i32, cfgp, P, i32, P, C.c_size_t]),
+            "orc_spiral": (i32, [P]),
+            "orc_particles_tick": (i32, [P, i32, i32, cfgp, P, C.c_size_t, C.POINTER(C.c_size_t),
+                                         C.POINTER(C.c_size_t)]),
             "orc_ms_value": (i32, [P, i32, i32, i32, i32]),
             "orc_find_perimeter": (i32, [P, i32, i32, i32, i32, P, i32]),
             "orc_simplify": (C.c_size_t, [P, C.c_size_t, C.c_float, P]),
@@ -377,3 +381,21 @@ def mesh_mask(mask):
         v = verts[2 * L.first:2 * (L.first + L.count)].reshape(-1, 2)
         out.append((L.start_x, L.start_y, bool(L.hole), [(float(a), float(b)) for a, b in v]))
     return out
+
+
+def spiral():
+    """tickCells' landing spiral offsets [(x, y), ...] in search order."""
+    o = np.zeros(2 * 1024, np.int32)
+    n = lib().orc_spiral(o.ctypes.data)
+    return [(int(o[2 * i]), int(o[2 * i + 1])) for i in range(n)]
+
+
+def particles_tick(cells, cfg, particles):
+    """world::tickCells over a list of Particle structs, in place on cells;
+    returns (surviving particles, cells landed)."""
+    ptr, W, H = _grid(cells)
+    arr = (Particle * max(len(particles), 1))(*particles)
+    alive, landed = C.c_size_t(), C.c_size_t()
+    rc = lib().orc_particles_tick(ptr, W, H, C.byref(cfg), arr, len(particles), C.byref(alive), C.byref(landed))
+    assert rc == 0
+    return [arr[i] for i in range(alive.value)], landed.value
