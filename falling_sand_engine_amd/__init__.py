@@ -206,6 +206,9 @@ class Context:
         self.close()
 
 
+MIRROR_PAGE = 64  # side of the getTile mirror's pages
+
+
 class World:
     """Device-resident world with the reference's world API (world.hpp:63-193)."""
 
@@ -218,28 +221,83 @@ class World:
               "fsg_world_create")
         self.handle = h
         self._edits = []
+        self._pages = {}  # host mirror for getTile: (px, py) -> MIRROR_PAGE^2 cells
 
     # --- reference method names -------------------------------------------
     def tick(self, n: int = 1):
         """world::tick (world.cpp:1036) n times, temperature by cadence."""
+        self._drop_mirror()
         self._flush()
         check(lib().fsg_world_step(self.handle, n), "fsg_world_step")
 
     def tickTemperature(self):  # noqa: N802 (reference name)
         """world::tickTemperature (world.cpp:1950)."""
+        self._drop_mirror()
         self._flush()
         check(lib().fsg_world_tick_temperature(self.handle), "fsg_world_tick_temperature")
 
+    @property
+    def tickZone(self):  # noqa: N802 (reference name)
+        """World::tickZone (world.hpp:138) as (x, y, w, h): the rectangle that
+        ticks.  A block acts iff one of its cells lies inside; only zone cells
+        take the temperature step and only particles in it move."""
+        v = [C.c_int32() for _ in range(4)]
+        check(lib().fsg_world_get_tick_zone(self.handle, *[C.byref(x) for x in v]), "fsg_world_get_tick_zone")
+        return tuple(x.value for x in v)
+
+    @tickZone.setter
+    def tickZone(self, zone):  # noqa: N802
+        x, y, w, h = (int(v) for v in zone)
+        self._flush()
+        check(lib().fsg_world_set_tick_zone(self.handle, x, y, w, h), "fsg_world_set_tick_zone")
+
     def getTile(self, x: int, y: int) -> int:  # noqa: N802
-        """world::getTile (world.cpp:999): out of bounds reads as a solid wall."""
+        """world::getTile (world.cpp:999): out of bounds reads as a solid wall.
+        Reads come from a host mirror of MIRROR_PAGE^2-cell pages: a page is
+        downloaded on its first read (one stream sync) and kept until a call
+        changes the grid on the device (tick, stamp, upload, ...), so the
+        reference's per-cell readers do not wait on the device per cell."""
         if x < 0 or y < 0 or x >= self.width or y >= self.height:
             return 15
-        self._flush()
-        return int(self.download(x, y, 1, 1)[0, 0])
+        key = (x // MIRROR_PAGE, y // MIRROR_PAGE)
+        page = self._pages.get(key)
+        if page is None:
+            x0, y0 = key[0] * MIRROR_PAGE, key[1] * MIRROR_PAGE
+            page = self.download(x0, y0, min(MIRROR_PAGE, self.width - x0), min(MIRROR_PAGE, self.height - y0))
+            self._pages[key] = page
+        return int(page[y % MIRROR_PAGE, x % MIRROR_PAGE])
 
     def setTile(self, x: int, y: int, cell: int):  # noqa: N802
-        """world::setTile (world.cpp:1004): queued, applied before the next tick."""
+        """world::setTile (world.cpp:1004): queued, applied before the next
+        device call; a mirrored page takes the write at once, so getTile sees
+        it as the reference's does.  Out of bounds is ignored."""
+        if x < 0 or y < 0 or x >= self.width or y >= self.height:
+            return
         self._edits.append((x, y, int(cell)))
+        page = self._pages.get((x // MIRROR_PAGE, y // MIRROR_PAGE))
+        if page is not None:
+            page[y % MIRROR_PAGE, x % MIRROR_PAGE] = np.uint64(int(cell) & 0xFFFFFFFFFFFFFFFF)
+
+    def prefetch(self, x0: int = 0, y0: int = 0, w: int = None, h: int = None):
+        """Fills the getTile mirror for the pages that meet the rectangle, in
+        one download."""
+        w = self.width - x0 if w is None else w
+        h = self.height - y0 if h is None else h
+        if w <= 0 or h <= 0:
+            return
+        P = MIRROR_PAGE
+        px0, py0 = max(x0, 0) // P, max(y0, 0) // P
+        px1, py1 = (min(x0 + w, self.width) - 1) // P, (min(y0 + h, self.height) - 1) // P
+        ax, ay = px0 * P, py0 * P
+        bw, bh = min((px1 + 1) * P, self.width) - ax, min((py1 + 1) * P, self.height) - ay
+        blk = self.download(ax, ay, bw, bh)
+        for py in range(py0, py1 + 1):
+            for px in range(px0, px1 + 1):
+                self._pages[(px, py)] = blk[py * P - ay:min((py + 1) * P, self.height) - ay,
+                                            px * P - ax:min((px + 1) * P, self.width) - ax].copy()
+
+    def _drop_mirror(self):
+        self._pages.clear()
 
     # --- bulk / state -----------------------------------------------------------
     def _flush(self):
@@ -255,12 +313,14 @@ class World:
         check(lib().fsg_world_set_cells(self.handle, arr, len(arr)), "fsg_world_set_cells")
 
     def upload(self, cells: np.ndarray):
+        self._drop_mirror()
         a = np.ascontiguousarray(cells, dtype=np.uint64)
         assert a.size == self.width * self.height
         check(lib().fsg_world_upload(self.handle, a.ctypes.data, a.size), "fsg_world_upload")
 
     def generate(self, kind: int, seed: int):
         """Fills the grid with generate(kind, width, height, seed), on the device."""
+        self._drop_mirror()
         self._edits = []
         check(lib().fsg_world_generate(self.handle, kind, seed), "fsg_world_generate")
 
@@ -273,6 +333,7 @@ class World:
         return out
 
     def step_timed(self, n: int) -> float:
+        self._drop_mirror()
         self._flush()
         ms = C.c_double()
         check(lib().fsg_world_step_timed(self.handle, n, C.byref(ms)), "fsg_world_step_timed")
@@ -288,6 +349,7 @@ class World:
         return t.value
 
     def set_tick(self, t: int):
+        self._drop_mirror()
         check(lib().fsg_world_set_tick(self.handle, t))
 
     def stats(self) -> Stats:
@@ -300,6 +362,7 @@ class World:
 
     # --- rigid bodies -----------------------------------------------------------
     def stamp_bodies(self, bodies, pixels: np.ndarray, displaced_cap: int = 0):
+        self._drop_mirror()
         nb = len(bodies)
         barr = (Body * max(nb, 1))(*bodies)
         px = np.ascontiguousarray(pixels, dtype=np.uint64)
@@ -320,6 +383,7 @@ class World:
                 "displaced_cells": displaced, "hits_sand": list(hs)[:nb], "hits_soup": list(hw)[:nb]}
 
     def unstamp_bodies(self, npixels: int) -> np.ndarray:
+        self._drop_mirror()
         out = np.empty(npixels, dtype=np.uint64)
         check(lib().fsg_world_unstamp_bodies(self.handle, out.ctypes.data, npixels), "fsg_world_unstamp_bodies")
         return out
@@ -329,6 +393,7 @@ class World:
         """Runs the island check for seed cells [(x, y), ...] in order.  Returns
         one dict per seed: status (ISLAND_*), count, box (min_x, min_y, w, h) and,
         for a new body, its (h, w) uint64 pixel image (air outside the island)."""
+        self._drop_mirror()
         self._flush()
         n = len(seeds)
         arr = (_abi.Island * max(n, 1))()
@@ -446,6 +511,7 @@ class World:
 
     def particles_tick(self):
         """One world::tickCells pass; returns (particles left, cells landed)."""
+        self._drop_mirror()
         self._flush()
         alive, landed = C.c_size_t(), C.c_size_t()
         check(lib().fsg_world_particles_tick(self.handle, C.byref(alive), C.byref(landed)),
@@ -470,6 +536,7 @@ class World:
         at (x0, y0); images None reads the device-resident images of the last
         pack().  Returns (status per chunk, phases, layer2 or None,
         background or None)."""
+        self._drop_mirror()
         self._flush()
         n = ncx * ncy
         status = np.zeros(n, dtype=np.int32)
@@ -518,6 +585,7 @@ def _handles(worlds):
     arr = (C.c_void_p * len(worlds))()
     for i, w in enumerate(worlds):
         w._flush()
+        w._drop_mirror()
         arr[i] = w.handle
     return arr
 

@@ -173,6 +173,9 @@ SIGNATURES = {
     "fsg_world_step": (C.c_int, [P, C.c_int]),
     "fsg_world_step_timed": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
     "fsg_world_tick_temperature": (C.c_int, [P]),
+    "fsg_world_set_tick_zone": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "fsg_world_get_tick_zone": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_int32)]),
     "fsg_world_sync": (C.c_int, [P]),
     "fsg_world_stats": (C.c_int, [P, C.POINTER(Stats)]),
     "fsg_world_reset_activity": (C.c_int, [P]),

@@ -190,11 +190,34 @@ int launch_tick(fsg_world *w) {
     // chunk flags left stale by a tick without skipping: process every chunk once
     if (w->d.skip && !w->flags_valid) w->d.force_tick = w->d.tick;
     const bool sk = w->d.skip != 0, re = w->d.mat.reactions != 0;
+    // a zone smaller than the world takes the kernels that test it
+    const bool zoned = w->d.zx0 > 0 || w->d.zy0 > 0 || w->d.zx1 < w->d.W || w->d.zy1 < w->d.H;
     hipStream_t st = w->ctx->stream;
-    if (sk && re) fsg::k_tick_world<1, 1><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
-    else if (sk) fsg::k_tick_world<1, 0><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
-    else if (re) fsg::k_tick_world<0, 1><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
-    else fsg::k_tick_world<0, 0><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
+    if (sk) {
+        // the awake chunks listed, then one workgroup per listed chunk; the
+        // count of this tick's parity must start at zero (a tick run another
+        // way, or a jump of the tick counter, leaves it stale)
+        if (!w->list_any && w->list_next != w->d.tick) {
+            fsg::k_zero_u32<<<1, 1, 0, st>>>(w->d.awake_n);
+            fsg::k_zero_u32<<<1, 1, 0, st>>>(w->d.awake_n + 1);
+        }
+        fsg::k_wake<<<grid_for(tiles, 256), 256, 0, st>>>(w->d, 0u);
+        if (zoned) {
+            if (re) fsg::k_tick_list<1, 1><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
+            else fsg::k_tick_list<0, 1><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
+        } else {
+            if (re) fsg::k_tick_list<1, 0><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
+            else fsg::k_tick_list<0, 0><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
+        }
+        w->list_any = false;
+        w->list_next = w->d.tick + 1;
+    } else if (zoned) {
+        if (re) fsg::k_tick_world<0, 1, 1><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
+        else fsg::k_tick_world<0, 0, 1><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
+    } else {
+        if (re) fsg::k_tick_world<0, 1, 0><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
+        else fsg::k_tick_world<0, 0, 0><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
+    }
     HIPCHK(hipGetLastError());
     w->d.tick++;
     w->offered += tiles;
@@ -273,6 +296,9 @@ extern "C" int fsg_ctx_create(const fsg_ctx_config *cfg, fsg_ctx **out) {
         delete c;
         return FSG_ERR_DEVICE;
     }
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+        c->n_cu = ncu;
     // freed blocks kept for reuse: up to a quarter of the device's memory
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b / 4 > c->cache_cap) c->cache_cap = total_b / 4;
@@ -377,7 +403,7 @@ extern "C" int fsg_world_create(fsg_ctx *c, int W, int H, uint64_t seed, const f
 
     const int tx = (W + fsg::TW - 1) / fsg::TW, ty = (H + fsg::TH - 1) / fsg::TH;
     const size_t cells = (size_t)W * H;
-    const size_t bytes = 2 * cells * 8 + 2 * (size_t)tx * ty * 4 + 18 * 8;
+    const size_t bytes = 2 * cells * 8 + 3 * (size_t)tx * ty * 4 + 16 + 18 * 8;
     {
         std::lock_guard<std::mutex> lk(c->mu);
         if (c->budget && c->used + bytes > c->budget) return FSG_ERR_BUDGET;
@@ -390,6 +416,8 @@ extern "C" int fsg_world_create(fsg_ctx *c, int W, int H, uint64_t seed, const f
             dev_free(w->d.buf[1]);
             dev_free(w->d.flags[0]);
             dev_free(w->d.flags[1]);
+            dev_free(w->d.awake);
+            dev_free(w->d.awake_n);
             dev_free(w->d_stats);
             delete w;
         }
@@ -411,6 +439,10 @@ extern "C" int fsg_world_create(fsg_ctx *c, int W, int H, uint64_t seed, const f
     w->d.flags[0] = (uint32_t *)p;
     if (dev_alloc(c, &p, (size_t)tx * ty * 4)) return fail(FSG_ERR_NOMEM);
     w->d.flags[1] = (uint32_t *)p;
+    if (dev_alloc(c, &p, (size_t)tx * ty * 4)) return fail(FSG_ERR_NOMEM);
+    w->d.awake = (uint32_t *)p;
+    if (dev_alloc(c, &p, 16)) return fail(FSG_ERR_NOMEM);
+    w->d.awake_n = (uint32_t *)p;
     if (dev_alloc(c, &p, 18 * 8)) return fail(FSG_ERR_NOMEM);
     w->d_stats = (unsigned long long *)p;
     w->d.W = W;
@@ -422,6 +454,10 @@ extern "C" int fsg_world_create(fsg_ctx *c, int W, int H, uint64_t seed, const f
     w->d.force_tick = 0;
     w->d.skip = conf.chunk_skip ? 1 : 0;
     w->d.processed = w->d_stats + 17;
+    w->d.zx0 = 0;
+    w->d.zy0 = 0;
+    w->d.zx1 = W;
+    w->d.zy1 = H;
     w->d.mat = build_mattab(conf);
     fill_pair_table(w->d);
     const uint64_t air = conf.mat_air;
@@ -431,6 +467,7 @@ extern "C" int fsg_world_create(fsg_ctx *c, int W, int H, uint64_t seed, const f
     // context's stream (a serving loop creates worlds while a step runs)
     if (hipMemsetAsync(w->d.flags[0], 0, (size_t)tx * ty * 4, c->stream) != hipSuccess ||
         hipMemsetAsync(w->d.flags[1], 0, (size_t)tx * ty * 4, c->stream) != hipSuccess ||
+        hipMemsetAsync(w->d.awake_n, 0, 16, c->stream) != hipSuccess ||
         hipMemsetAsync(w->d_stats, 0, 18 * 8, c->stream) != hipSuccess || hipGetLastError() != hipSuccess)
         return fail(FSG_ERR_DEVICE);
     *out = w;
@@ -463,6 +500,8 @@ extern "C" int fsg_world_destroy(fsg_world *w) {
     dev_free(w->d.buf[1]);
     dev_free(w->d.flags[0]);
     dev_free(w->d.flags[1]);
+    dev_free(w->d.awake);
+    dev_free(w->d.awake_n);
     dev_free(w->d_stats);
     dev_free(w->d_pixels);
     dev_free(w->d_pixbody);
@@ -611,13 +650,35 @@ extern "C" int fsg_world_step_timed(fsg_world *w, int ticks, double *ms) {
     return FSG_OK;
 }
 
+extern "C" int fsg_world_set_tick_zone(fsg_world *w, int32_t x, int32_t y, int32_t width, int32_t height) {
+    if (!w || x < 0 || y < 0 || width < 1 || height < 1 || width > w->d.W - x || height > w->d.H - y)
+        return FSG_ERR_INVALID;
+    w->d.zx0 = x;
+    w->d.zy0 = y;
+    w->d.zx1 = x + width;
+    w->d.zy1 = y + height;
+    // a grown zone wakes chunks that slept outside it
+    w->d.force_tick = w->d.tick;
+    return FSG_OK;
+}
+
+extern "C" int fsg_world_get_tick_zone(fsg_world *w, int32_t *x, int32_t *y, int32_t *width, int32_t *height) {
+    if (!w) return FSG_ERR_INVALID;
+    if (x) *x = w->d.zx0;
+    if (y) *y = w->d.zy0;
+    if (width) *width = w->d.zx1 - w->d.zx0;
+    if (height) *height = w->d.zy1 - w->d.zy0;
+    return FSG_OK;
+}
+
 extern "C" int fsg_world_tick_temperature(fsg_world *w) {
     if (!w) return FSG_ERR_INVALID;
     fsg_ctx *c = w->ctx;
     HIPCHK(hipSetDevice(c->device));
     const size_t n = (size_t)w->d.W * w->d.H;
     uint64_t *cur = w->d.buf[w->d.tick & 1], *spare = w->d.buf[(w->d.tick + 1) & 1];
-    fsg::k_temperature<<<grid_for(n, fsg::NT), fsg::NT, 0, c->stream>>>(cur, spare, w->d.W, w->d.H, w->d.mat);
+    fsg::k_temperature<<<grid_for(n, fsg::NT), fsg::NT, 0, c->stream>>>(
+        cur, spare, w->d.W, w->d.H, w->d.mat, make_int4(w->d.zx0, w->d.zy0, w->d.zx1, w->d.zy1));
     HIPCHK(hipGetLastError());
     // the result becomes the current state without advancing the tick
     std::swap(w->d.buf[0], w->d.buf[1]);

@@ -46,6 +46,13 @@ struct DevWorld {
     uint32_t skip;          // exact dirty-rect skipping on
     uint32_t pad;
     unsigned long long *processed; // chunk updates run (skip mode)
+    // tick zone: cells [zx0, zx1) x [zy0, zy1) tick; a block acts iff one of
+    // its cells lies inside, and only zone cells take the temperature step
+    int32_t zx0, zy0, zx1, zy1;
+    // awake-chunk list of the skip path: awake[0 .. awake_n[t & 1]) are the
+    // chunks tick t updates (k_wake, k_tick_list)
+    uint32_t *awake;
+    uint32_t *awake_n;
     MatTab mat;
     uint16_t pair[MAX_IT][256];  // pair_code(mat, it, t, u) at [it][t * 16 + u], built on the host
 };

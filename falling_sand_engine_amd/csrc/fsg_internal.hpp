@@ -23,6 +23,7 @@ using fsg::MatTab;
 
 struct fsg_ctx {
     int device = 0;
+    int n_cu = 256;  // compute units of the device
     hipStream_t stream = nullptr;
     // events of the *_timed calls, held under time_mu from the first record
     // to the elapsed-time read
@@ -72,6 +73,10 @@ struct fsg_world {
     DevWorld d{};
     bool flags_valid = false;  // the chunk flags describe the last tick (it ran with skipping on)
     bool prev_valid = false;   // buf[(tick + 1) & 1] holds the state before the last tick
+    // the awake-list counts are zero for the tick list_next (after a listed
+    // tick) or for any tick (list_any: both zero)
+    bool list_any = true;
+    uint64_t list_next = 0;
     uint32_t *d_layer[3] = {nullptr, nullptr, nullptr};  // texture layers (fsg_world_render)
     unsigned long long *d_tally = nullptr;                 // render census replicas
     void *d_isl = nullptr;                                 // island check scratch

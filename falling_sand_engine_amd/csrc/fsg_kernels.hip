@@ -112,10 +112,19 @@ constexpr int QCAP = ((36 * 36 + NT - 1) / NT) * 64;
 // of a phase are disjoint and a wave's LDS accesses stay in order, so the
 // passes need no barrier.  With TRACK, returns the band bits of chunk cells
 // that changed.
-template <bool TRACK, bool REACT>
+// Block origins (LDS coordinates) whose block holds a tick-zone cell:
+// [x0, x0 + w) x [y0, y0 + h); `on` is false when every block does.
+struct ZoneBlocks {
+    bool on;
+    int x0, y0;
+    uint32_t w, h;
+};
+
+template <bool TRACK, bool REACT, bool ZONE>
 __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW], const uint32_t *T, int it,
                                               uint64_t key, int ox, int oy, int need, int gx0, int gy0,
-                                              bool has_wood, uint16_t *queue, const uint32_t *perm) {
+                                              bool has_wood, uint16_t *queue, const uint32_t *perm,
+                                              const ZoneBlocks &zb) {
     // first/last block origin (chunk coordinates) with the phase's parity
     const int x0f = -need - ((need ^ ox) & 1), y0f = -need - ((need ^ oy) & 1);
     const int x0l = (TW - 1 + need) - (((TW - 1 + need) ^ ox) & 1);
@@ -128,6 +137,11 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
         const int by = (int)(__umul24((uint32_t)b, magic) >> 16);
         const int bx = b - (int)__umul24((uint32_t)by, (uint32_t)nbx);
         return &L[0][0] + __umul24((uint32_t)(ly0 + 2 * by), (uint32_t)LW) + lx0 + 2 * bx;
+    };
+    auto in_zone = [&](int b) -> bool {
+        const int by = (int)(__umul24((uint32_t)b, magic) >> 16);
+        const int bx = b - (int)__umul24((uint32_t)by, (uint32_t)nbx);
+        return (uint32_t)(lx0 + 2 * bx - zb.x0) < zb.w && (uint32_t)(ly0 + 2 * by - zb.y0) < zb.h;
     };
     const int lane = threadIdx.x & 63;
     uint16_t *Q = queue + (threadIdx.x >> 6) * QCAP;
@@ -148,6 +162,7 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
                 act = true;
                 plain = false;
             }
+            if (ZONE) act = act && in_zone(b);  // workgroups on the zone's edge only
             plain = plain && act;
         }
         const uint64_t mp = __ballot(plain), mo = __ballot(act && !plain);
@@ -242,10 +257,13 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
 // Final compact words of a thread's 2 x 8 output cells (columns lx, lx + 1,
 // rows ly0 .. ly0 + 7), with the temperature step applied when temp_now; the
 // stencil slides down the strip.  With TCHG, returns whether a temperature
-// changed (only chunk skipping needs to know).
-template <bool WALLS, bool TCHG = true>
+// changed (only chunk skipping needs to know).  With ZONE, only cells of the
+// tick zone take the step: LDS rows [zr0, zr1), the thread's columns whose
+// bits (1: lx, 2: lx + 1) are set in zc.
+template <bool WALLS, bool TCHG = true, bool ZONE = false>
 __device__ __forceinline__ bool final_words(const MatTab &M, const uint32_t (*L)[LW], const int32_t *addt,
-                                            bool temp_now, int lx, int ly0, uint32_t (&fw)[2 * (TH / 8)]) {
+                                            bool temp_now, int lx, int ly0, uint32_t (&fw)[2 * (TH / 8)],
+                                            int zr0 = 0, int zr1 = 0, uint32_t zc = 0) {
     bool tchg = false;
     if (!temp_now) {
 #pragma unroll
@@ -288,6 +306,11 @@ __device__ __forceinline__ bool final_words(const MatTab &M, const uint32_t (*L)
         if (!WALLS || cid(cur.y) != WALL)
             v1 = temp_of(M, addt, t1, cid(cur.y), ntemp<WALLS>(t1, up.y), ntemp<WALLS>(t1, dn.y),
                          ntemp<WALLS>(t1, cur.x), ntemp<WALLS>(t1, cr));
+        if (ZONE) {
+            const bool rin = ly0 + r >= zr0 && ly0 + r < zr1;
+            if (!(rin && (zc & 1u))) v0 = t0;
+            if (!(rin && (zc & 2u))) v1 = t1;
+        }
         if (TCHG) tchg |= (v0 != t0) | (v1 != t1);
         fw[2 * r] = (cur.x & 0xFFFFu) | ((uint32_t)v0 << 16);
         fw[2 * r + 1] = (cur.y & 0xFFFFu) | ((uint32_t)v1 << 16);
@@ -310,9 +333,48 @@ constexpr int KC = (LH - 2) / RS;  // sweeps over rows 1 .. LH-2 (10)
 static_assert(RS * KC == LH - 2 && HALO - MAX_IT >= 1, "sweeps cover rows 1 .. LH-2 exactly");
 static_assert((uint64_t)FSG_MAX_SIDE * 8u * LH < (1ull << 32), "32-bit byte offsets within a region");
 
+// The tick zone against a workgroup's region [gx0, gx0 + LW) x [gy0, gy0 + LH):
+// dead when no cell of the region lies in the zone (then no block the
+// workgroup computes acts and no chunk cell takes the temperature step);
+// blocks.on when the region's in-world part is not wholly inside the zone.
+struct ZoneTest {
+    bool dead;
+    ZoneBlocks blocks;
+};
+__device__ __forceinline__ ZoneTest zone_test(const DevWorld &w, int gx0, int gy0) {
+    ZoneTest z;
+    z.dead = gx0 + LW <= w.zx0 || gx0 >= w.zx1 || gy0 + LH <= w.zy0 || gy0 >= w.zy1;
+    const bool inside = max(gx0, 0) >= w.zx0 && min(gx0 + LW, w.W) <= w.zx1 && max(gy0, 0) >= w.zy0 &&
+                        min(gy0 + LH, w.H) <= w.zy1;
+    z.blocks.on = !z.dead && !inside;
+    // a block at LDS origin (lx, ly) holds a zone cell iff
+    // zx0 - 1 <= gx0 + lx < zx1 and zy0 - 1 <= gy0 + ly < zy1
+    z.blocks.x0 = w.zx0 - 1 - gx0;
+    z.blocks.y0 = w.zy0 - 1 - gy0;
+    z.blocks.w = (uint32_t)(w.zx1 - w.zx0 + 1);
+    z.blocks.h = (uint32_t)(w.zy1 - w.zy0 + 1);
+    return z;
+}
+
+// Wake test of the skip path: the chunk may change by itself (F_POT), changed
+// last tick (F_ANY: its two state buffers differ), a neighbour changed inside
+// the band the chunk's region overlaps, or the tick is forced.
+__device__ __forceinline__ bool chunk_awake(const DevWorld &w, const uint32_t *fr, int tx, int ty, bool forced) {
+    auto fl = [&](int dx, int dy) -> uint32_t {
+        const int nx = tx + dx, ny = ty + dy;
+        return (nx < 0 || ny < 0 || nx >= w.tiles_x || ny >= w.tiles_y) ? 0u : fr[ny * w.tiles_x + nx];
+    };
+    return forced || (fl(0, 0) & (F_POT | F_ANY)) || (fl(0, -1) & F_S) || (fl(0, 1) & F_N) || (fl(-1, 0) & F_E) ||
+           (fl(1, 0) & F_W) || (fl(-1, -1) & F_SE) || (fl(1, -1) & F_SW) || (fl(-1, 1) & F_NE) ||
+           (fl(1, 1) & F_NW);
+}
+
 // SK / RE: chunk skipping / reactions compiled in (1), out (0), or read
 // from the world at run time (2, the batch kernels, whose worlds may differ).
-template <int SK, int RE>
+// SK == 3: skipping on, and the chunk is known to be awake (it was taken from
+// the awake list k_wake built).  ZN: the tick zone is tested (1), or the
+// world's zone covers it and no zone code is compiled in (0).
+template <int SK, int RE, int ZN>
 __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, int tile, uint32_t (*L)[LW],
                                             uint32_t (*lut)[256], uint32_t *rgba, int32_t *addt,
                                             uint16_t *queue, uint32_t *red) {
@@ -323,28 +385,29 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     const int gx0 = tx * TW - HALO, gy0 = ty * TH - HALO;  // world coords of L[0][0]
     const uint64_t *__restrict__ in = w.buf[tick & 1];
     uint64_t *__restrict__ out = w.buf[(tick + 1) & 1];
-    const bool skip = SK == 2 ? w.skip != 0 : SK == 1;
+    const bool skip = SK == 2 ? w.skip != 0 : SK != 0;
     bool prev_changed = true;
+    // Tick zone: no block of the region holds a zone cell (the chunk cannot
+    // change), or some do and some do not (blocks are tested one by one).
+    ZoneTest zt{};
+    if (ZN) zt = zone_test(w, gx0, gy0);
 
     if (skip) {
         // Exact skip: a chunk sleeps unless it may change by itself (F_POT)
         // or a neighbour changed inside the band its region overlaps.
         const uint32_t *fr = w.flags[(tick + 1) & 1];
-        auto fl = [&](int dx, int dy) -> uint32_t {
-            const int nx = tx + dx, ny = ty + dy;
-            return (nx < 0 || ny < 0 || nx >= w.tiles_x || ny >= w.tiles_y) ? 0u : fr[ny * w.tiles_x + nx];
-        };
-        const uint32_t self = fl(0, 0);
+        const uint32_t self = fr[tile];
         const bool forced = tick == w.force_tick;
-        const bool awake = forced || (self & (F_POT | F_ANY)) || (fl(0, -1) & F_S) || (fl(0, 1) & F_N) ||
-                           (fl(-1, 0) & F_E) || (fl(1, 0) & F_W) || (fl(-1, -1) & F_SE) ||
-                           (fl(1, -1) & F_SW) || (fl(-1, 1) & F_NE) || (fl(1, 1) & F_NW);
-        if (!awake) {
+        if (SK != 3 && !chunk_awake(w, fr, tx, ty, forced)) {
             if (tid == 0) w.flags[tick & 1][tile] = 0;
             return;
         }
         // both state buffers hold this chunk unless it changed last tick
         prev_changed = forced || (self & F_ANY);
+        if (zt.dead && !prev_changed) {
+            if (tid == 0) w.flags[tick & 1][tile] = 0;
+            return;
+        }
     }
 
     // Every global load (pair table, colour and add_temp tables, the region
@@ -446,6 +509,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     if (skip && !prev_changed) {
         // the chunk did not change last tick: if nothing in its region can
         // change now it keeps its state, and both buffers already hold it
+        // (the test ignores the tick zone: it may only wake too many)
         const bool pot = __syncthreads_or(tile_unstable(M, L, lut[0], addt, M.temp_every != 0)) != 0;
         if (!pot) {
             if (tid == 0) {
@@ -457,31 +521,43 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     }
 
     uint32_t bands = 0;
-    {
+    if (!zt.dead) {
         const uint64_t seedmix = splitmix64(w.seed);
         for (int it = 0; it < ci; it++) {
             const uint64_t g = tick * (uint64_t)M.cell_iter + (uint64_t)it;
             const uint64_t key = splitmix64(seedmix ^ g);
             const int ox = kOX[g & 3], oy = kOY[g & 3];
-            if (SK != 0 && skip)
-                bands |= run_phase<true, RE != 0>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue,
-                                                  lut[MAX_IT]);
-            else if (SK != 1)
-                run_phase<false, RE != 0>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue,
-                                          lut[MAX_IT]);
+            if (zt.blocks.on)  // on the zone's edge: one form for every setting (the bands may go unused)
+                bands |= run_phase<true, RE != 0, true>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood,
+                                                        queue, lut[MAX_IT], zt.blocks);
+            else if (SK != 0 && skip)
+                bands |= run_phase<true, RE != 0, false>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood,
+                                                         queue, lut[MAX_IT], zt.blocks);
+            else if (SK != 1 && SK != 3)
+                run_phase<false, RE != 0, false>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue,
+                                                 lut[MAX_IT], zt.blocks);
             __syncthreads();
         }
     }
 
     // Write-back.  Thread = 2-column strip x 8 rows of the chunk.
-    const bool temp_now = M.temp_every != 0 && (tick % M.temp_every) == M.temp_phase;
+    const bool temp_now = !zt.dead && M.temp_every != 0 && (tick % M.temp_every) == M.temp_phase;
     const int j = tid & 31, rg = tid >> 5;
     const int lx = HALO + 2 * j, ly0 = HALO + rg * (TH / 8);
     uint32_t fw[2 * (TH / 8)];
-    const bool tchg = skip ? (interior ? final_words<false>(M, L, addt, temp_now, lx, ly0, fw)
-                                       : final_words<true>(M, L, addt, temp_now, lx, ly0, fw))
-                           : (interior ? final_words<false, false>(M, L, addt, temp_now, lx, ly0, fw)
-                                       : final_words<true, false>(M, L, addt, temp_now, lx, ly0, fw));
+    bool tchg;
+    if (zt.blocks.on) {
+        // on the zone's edge: cells outside it keep their temperature (the
+        // wall-checking form serves interior chunks too)
+        const int gx = gx0 + lx;
+        const uint32_t zc = (gx >= w.zx0 && gx < w.zx1 ? 1u : 0u) | (gx + 1 >= w.zx0 && gx + 1 < w.zx1 ? 2u : 0u);
+        tchg = final_words<true, true, true>(M, L, addt, temp_now, lx, ly0, fw, w.zy0 - gy0, w.zy1 - gy0, zc);
+    } else {
+        tchg = skip ? (interior ? final_words<false>(M, L, addt, temp_now, lx, ly0, fw)
+                                : final_words<true>(M, L, addt, temp_now, lx, ly0, fw))
+                    : (interior ? final_words<false, false>(M, L, addt, temp_now, lx, ly0, fw)
+                                : final_words<true, false>(M, L, addt, temp_now, lx, ly0, fw));
+    }
     if (skip && tchg) bands |= band_bits(2 * j, 2 * j + 1, rg * (TH / 8), rg * (TH / 8) + TH / 8 - 1);
     __syncthreads();
     // the colour plane: each cell's colour at its origin (rows 1 .. LH-2)
@@ -553,7 +629,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         if (tid == 0) {
             uint32_t all = 0;
             for (int k = 0; k < NT / 64; k++) all |= red[k];
-            w.flags[tick & 1][tile] = F_POT | all;
+            w.flags[tick & 1][tile] = (zt.dead ? 0u : F_POT) | all;  // a dead chunk cannot change by itself
             atomicAdd(w.processed, 1ull);
         }
     }
@@ -562,7 +638,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
 // Five workgroups per CU (at most 96 VGPRs, 28 KB of LDS each).  One
 // instantiation per (chunk skipping, reactions) setting of the world, so
 // the branches and paths a setting never takes are not compiled in.
-template <int SK, int RE>
+template <int SK, int RE, int ZN>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_tick_world(DevWorld w, uint32_t k) {
     __shared__ __align__(16) uint32_t Lmem[LH * LW + 16];  // the region, then the 16 base colours
     __shared__ uint32_t lut[MAX_IT + 1][256];               // pair tables + scatter offsets
@@ -571,12 +647,75 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
     __shared__ int32_t addt[16];
     __shared__ uint16_t queue[(NT / 64) * QCAP];
     __shared__ uint32_t red[NT / 64];
-    tile_update<SK, RE>(w, w.tick + k, blockIdx.x, L, lut, rgba, addt, queue, red);
+    tile_update<SK, RE, ZN>(w, w.tick + k, blockIdx.x, L, lut, rgba, addt, queue, red);
 }
-template __global__ void k_tick_world<0, 0>(DevWorld, uint32_t);
-template __global__ void k_tick_world<0, 1>(DevWorld, uint32_t);
-template __global__ void k_tick_world<1, 0>(DevWorld, uint32_t);
-template __global__ void k_tick_world<1, 1>(DevWorld, uint32_t);
+template __global__ void k_tick_world<0, 0, 0>(DevWorld, uint32_t);
+template __global__ void k_tick_world<0, 1, 0>(DevWorld, uint32_t);
+template __global__ void k_tick_world<0, 0, 1>(DevWorld, uint32_t);
+template __global__ void k_tick_world<0, 1, 1>(DevWorld, uint32_t);
+
+// The awake chunks of tick t = w.tick + k, listed for k_tick_list: one thread
+// per chunk runs the wake test of the skip path (and writes the flags of the
+// chunks that sleep), then each workgroup reserves its slice of the list
+// with one atomic on the count awake_n[t & 1], which is zero on entry: this
+// launch clears awake_n[(t + 1) & 1] for the next tick (tick t - 1, the
+// last to read it, has finished).  The list's order is that of the
+// reservations; the result does not depend on it (the chunks of one tick
+// are independent).
+__global__ __launch_bounds__(256) void k_wake(DevWorld w, uint32_t k) {
+    __shared__ uint32_t wave_n[4], base;
+    const uint64_t tick = w.tick + k;
+    const int ntiles = w.tiles_x * w.tiles_y;
+    const int tile = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (tile == 0) w.awake_n[(tick + 1) & 1] = 0;
+    bool awake = false;
+    if (tile < ntiles) {
+        const int tx = tile % w.tiles_x, ty = tile / w.tiles_x;
+        const uint32_t *fr = w.flags[(tick + 1) & 1];
+        const bool forced = tick == w.force_tick;
+        awake = chunk_awake(w, fr, tx, ty, forced);
+        // a chunk outside the zone's reach only needs a tick to copy itself
+        if (awake && !forced && !(fr[tile] & F_ANY) && zone_test(w, tx * TW - HALO, ty * TH - HALO).dead)
+            awake = false;
+        if (!awake) w.flags[tick & 1][tile] = 0;
+    }
+    const uint64_t m = __ballot(awake);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) wave_n[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t n = wave_n[0] + wave_n[1] + wave_n[2] + wave_n[3];
+        base = n ? atomicAdd(&w.awake_n[tick & 1], n) : 0u;
+    }
+    __syncthreads();
+    if (awake) {
+        uint32_t off = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        for (int v = 0; v < wv; v++) off += wave_n[v];
+        w.awake[off] = (uint32_t)tile;
+    }
+}
+
+// Skip-path tick over the awake list: workgroup i updates the i-th listed
+// chunk, and workgroups past the list's end exit at once.  The listed
+// chunks take the first workgroups, so they start back to back and no
+// sleeping chunk holds a workgroup slot while they run.
+template <int RE, int ZN>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_tick_list(DevWorld w, uint32_t k) {
+    __shared__ __align__(16) uint32_t Lmem[LH * LW + 16];  // the region, then the 16 base colours
+    __shared__ uint32_t lut[MAX_IT + 1][256];               // pair tables + scatter offsets
+    uint32_t(*L)[LW] = reinterpret_cast<uint32_t(*)[LW]>(Lmem);
+    uint32_t *rgba = Lmem + LH * LW;
+    __shared__ int32_t addt[16];
+    __shared__ uint16_t queue[(NT / 64) * QCAP];
+    __shared__ uint32_t red[NT / 64];
+    const uint64_t tick = w.tick + k;
+    if (blockIdx.x >= w.awake_n[tick & 1]) return;
+    tile_update<3, RE, ZN>(w, tick, (int)w.awake[blockIdx.x], L, lut, rgba, addt, queue, red);
+}
+template __global__ void k_tick_list<0, 0>(DevWorld, uint32_t);
+template __global__ void k_tick_list<1, 0>(DevWorld, uint32_t);
+template __global__ void k_tick_list<0, 1>(DevWorld, uint32_t);
+template __global__ void k_tick_list<1, 1>(DevWorld, uint32_t);
 
 // Many worlds in one launch: tile_start[i] is the first global chunk index of
 // world i (prefix sums, tile_start[n] = total).
@@ -597,7 +736,7 @@ __device__ __forceinline__ void tick_batch_chunk(const DevWorld *__restrict__ wo
         else hi = mid - 1;
     }
     const DevWorld &w = worlds[lo];
-    tile_update<2, 2>(w, w.tick + k, (int)(gt - tile_start[lo]), L, lut, rgba, addt, queue, red);
+    tile_update<2, 2, 1>(w, w.tick + k, (int)(gt - tile_start[lo]), L, lut, rgba, addt, queue, red);
 }
 
 // Two register budgets of the batch tick, one of the autotuner's choices:
@@ -620,11 +759,15 @@ __global__ void k_advance(DevWorld *worlds, int nworlds, uint32_t ticks) {
 // buf[(t+1)&1]; the caller advances the buffer parity by swapping roles.
 __global__ __launch_bounds__(NT) void k_temperature(const uint64_t *__restrict__ in,
                                                      uint64_t *__restrict__ out, int W, int H,
-                                                     MatTab M) {
+                                                     MatTab M, int4 zone) {
     const size_t i = (size_t)blockIdx.x * NT + threadIdx.x;
     if (i >= (size_t)W * H) return;
     const int x = (int)(i % (size_t)W), y = (int)(i / (size_t)W);
     const uint64_t c = in[i];
+    if (x < zone.x || y < zone.y || x >= zone.z || y >= zone.w) {  // outside the tick zone
+        out[i] = c;
+        return;
+    }
     const int T = ctemp(c);
     const int tu = y > 0 ? ctemp(in[i - W]) : T;
     const int td = y < H - 1 ? ctemp(in[i + W]) : T;

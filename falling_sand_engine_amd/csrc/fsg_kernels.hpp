@@ -33,7 +33,7 @@ struct StampArgs {
     uint32_t *nopen;           // their count
 };
 
-template <int SK, int RE>
+template <int SK, int RE, int ZN>
 __global__ void k_tick_world(DevWorld w, uint32_t k);
 __global__ void k_tick_batch(const DevWorld *__restrict__ worlds, const uint32_t *__restrict__ tile_start,
                              int nworlds, uint32_t k);
@@ -41,7 +41,10 @@ __global__ void k_tick_batch4(const DevWorld *__restrict__ worlds, const uint32_
                               int nworlds, uint32_t k);
 __global__ void k_advance(DevWorld *worlds, int nworlds, uint32_t ticks);
 __global__ void k_temperature(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int W, int H,
-                              MatTab M);
+                              MatTab M, int4 zone);
+__global__ void k_wake(DevWorld w, uint32_t k);
+template <int RE, int ZN>
+__global__ void k_tick_list(DevWorld w, uint32_t k);
 __global__ void k_stats(const uint64_t *__restrict__ cells, size_t n, unsigned long long *out);
 __global__ void k_fill(uint64_t *cells, size_t n, uint64_t v);
 __global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16);

@@ -47,7 +47,7 @@ __device__ __forceinline__ bool is_air(const uint64_t *cells, uint32_t airmask, 
 }
 
 // One particle's tick against the grid (the reference's loop body).
-__device__ void move_particle(PState &s, const uint64_t *cells, int W, int H, uint32_t airmask) {
+__device__ void move_particle(PState &s, const uint64_t *cells, int W, int H, uint32_t airmask, int4 zone) {
 #pragma clang fp contract(off)
     fsg_particle c = s.p;
     s.kind = 0;
@@ -71,6 +71,10 @@ __device__ void move_particle(PState &s, const uint64_t *cells, int W, int H, ui
     const int lx = (int)c.x, ly = (int)c.y;
     if (c.x < 0.0f || (int)c.x >= W || c.y < 0.0f || (int)c.y >= H) {
         s.kind = 1;
+        return;
+    }
+    if (lx < zone.x || ly < zone.y || lx >= zone.z || ly >= zone.w) {  // outside the tick zone: kept as it is
+        s.p = c;
         return;
     }
     c.vx += c.ax;
@@ -126,6 +130,7 @@ struct PArgs {
     uint64_t *cells;
     int W, H, tiles_x;
     uint32_t airmask;
+    int4 zone;             // tick zone [x, z) x [y, w)
     uint32_t *flags;       // chunk flags the next tick reads (null: no skipping)
     unsigned long long *landed;
 };
@@ -142,7 +147,7 @@ __global__ __launch_bounds__(PT) void k_particles(PArgs a) {
         const bool mine = i < a.n;
         if (mine) {
             s.p = a.p[i];
-            move_particle(s, a.cells, a.W, a.H, a.airmask);
+            move_particle(s, a.cells, a.W, a.H, a.airmask, a.zone);
         }
         land_at[t] = (mine && s.kind == 2) ? s.land : -1;
         if (t == 0) first_bad = PT;
@@ -315,6 +320,7 @@ extern "C" int fsg_world_particles_tick(fsg_world *w, size_t *alive, size_t *lan
         a.W = w->d.W;
         a.H = w->d.H;
         a.tiles_x = w->d.tiles_x;
+        a.zone = make_int4(w->d.zx0, w->d.zy0, w->d.zx1, w->d.zy1);
         a.airmask = 0;
         for (uint32_t m = 0; m < w->cfg.n_materials; m++)
             if (w->cfg.materials[m].type == FSG_AIR) a.airmask |= 1u << m;

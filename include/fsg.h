@@ -267,6 +267,19 @@ int fsg_world_step_timed(fsg_world *w, int ticks, double *elapsed_ms);
 /* world::tickTemperature (world.cpp:1950-2004): one standalone temperature
  * step for callers that drive the cadence themselves (temp_every = 0). */
 int fsg_world_tick_temperature(fsg_world *w);
+/* The tick zone, World::tickZone (world.hpp:138), which game.cpp:1629-1636
+ * sets to the world less a one-chunk frame: only the rectangle
+ * [x, x + width) x [y, y + height) ticks.  A Margolus block acts iff one of
+ * its four cells lies in the zone, so cells leave the zone across its edges
+ * (world.cpp:1073-1074 visits zone chunks, whose cells move anywhere) and a
+ * cell of the one-cell ring outside moves only together with a zone cell.
+ * Only zone cells take the temperature step (world.cpp:1953-1954), in the
+ * tick and in fsg_world_tick_temperature, and fsg_world_particles_tick moves
+ * only particles whose cell lies in the zone (world.cpp:2061).  The zone
+ * must lie inside the world, width, height >= 1; a new world's zone, and
+ * (0, 0, W, H), is the whole world.  The next tick processes every chunk. */
+int fsg_world_set_tick_zone(fsg_world *w, int32_t x, int32_t y, int32_t width, int32_t height);
+int fsg_world_get_tick_zone(fsg_world *w, int32_t *x, int32_t *y, int32_t *width, int32_t *height);
 int fsg_world_sync(fsg_world *w);
 
 /* Per-material counts and activity (the movingTiles[] census,

@@ -17,9 +17,18 @@
 // solid wall cell, setTile out of bounds is ignored.  Device failures, which
 // the reference cannot have, throw fsg::error from the facade (the C-ABI
 // itself never throws).
+//
+// getTile reads a host mirror of the grid in 64 x 64 pages: a page is
+// downloaded on its first read (or by region / prefetch) and stays valid
+// until a call of this facade changes the grid on the device (tick, stamp,
+// load, readChunks, ...).  setTile writes through to a mirrored page and
+// queues the edit for the device, so a getTile after a setTile sees the new
+// cell as in the reference.  A caller that changes the grid through
+// handle() calls invalidate().
 #ifndef FSG_WORLD_HPP
 #define FSG_WORLD_HPP
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -76,10 +85,15 @@ class world {
         width = w;
         height = h;
         edits_.clear();
+        npx_ = (w + kPage - 1) / kPage;
+        npy_ = (h + kPage - 1) / kPage;
+        pages_.assign((size_t)npx_ * npy_, Page{});
+        invalidate();
     }
 
     // Whole-grid load, e.g. from a generator or a saved world.
     void load(const std::vector<MaterialInstance> &cells) {
+        invalidate();
         check(fsg_world_upload(w_, cells.data(), cells.size()), "fsg_world_upload");
     }
 
@@ -88,38 +102,78 @@ class world {
             MaterialInstance wall{15, 0, 0};  // Tiles_TEST_SOLID
             return wall;
         }
-        flush();
-        MaterialInstance c;
-        check(fsg_world_download(w_, &c, x, y, 1, 1), "fsg_world_download");
-        return c;
+        Page &p = page_at(x / kPage, y / kPage);
+        if (p.gen != gen_) fetch(x / kPage, y / kPage, x / kPage, y / kPage);
+        return p.cells[(size_t)(y % kPage) * kPage + (x % kPage)];
     }
 
-    // Queued like the reference's direct real_tiles writes; applied before the next tick.
+    // Queued like the reference's direct real_tiles writes; applied before the
+    // next device call, and at once to a mirrored page.
     void setTile(int x, int y, MaterialInstance type) {
         if (x < 0 || x >= width || y < 0 || y >= height) return;
         edits_.push_back(fsg_cell_edit{x, y, type});
+        Page &p = page_at(x / kPage, y / kPage);
+        if (p.gen == gen_) p.cells[(size_t)(y % kPage) * kPage + (x % kPage)] = type;
     }
+
+    // Mirrors every page that meets [x0, x0+w) x [y0, y0+h), in one download.
+    void prefetch(int x0, int y0, int w, int h) {
+        const int xa = std::max(x0, 0), ya = std::max(y0, 0);
+        const int xb = std::min(x0 + w, width), yb = std::min(y0 + h, height);
+        if (xa >= xb || ya >= yb) return;
+        fetch(xa / kPage, ya / kPage, (xb - 1) / kPage, (yb - 1) / kPage);
+    }
+
+    // Forgets the mirror (the grid changed on the device).
+    void invalidate() { ++gen_; }
 
     void tick(int n = 1) {
         flush();
+        invalidate();
         check(fsg_world_step(w_, n), "fsg_world_step");
     }
 
     void tickTemperature() {
         flush();
+        invalidate();
         check(fsg_world_tick_temperature(w_), "fsg_world_tick_temperature");
     }
 
-    // Downloads the rectangle [x0, x0+w) x [y0, y0+h) (row-major).
+    // World::tickZone (world.hpp:138): only [x, x+w) x [y, y+h) ticks.
+    void setTickZone(int x, int y, int w, int h) {
+        flush();
+        check(fsg_world_set_tick_zone(w_, x, y, w, h), "fsg_world_set_tick_zone");
+    }
+    void tickZone(int &x, int &y, int &w, int &h) const {
+        int32_t a, b, c, d;
+        check(fsg_world_get_tick_zone(w_, &a, &b, &c, &d), "fsg_world_get_tick_zone");
+        x = a, y = b, w = c, h = d;
+    }
+
+    // Downloads the rectangle [x0, x0+w) x [y0, y0+h) (row-major); the pages
+    // it covers whole are mirrored for getTile.
     std::vector<MaterialInstance> region(int x0, int y0, int w, int h) {
         flush();
         std::vector<MaterialInstance> out((size_t)w * h);
         check(fsg_world_download(w_, out.data(), x0, y0, w, h), "fsg_world_download");
+        for (int py = (y0 + kPage - 1) / kPage; py < npy_ && py * kPage < y0 + h; py++) {
+            for (int px = (x0 + kPage - 1) / kPage; px < npx_ && px * kPage < x0 + w; px++) {
+                const int pw = std::min(kPage, width - px * kPage), ph = std::min(kPage, height - py * kPage);
+                if (px * kPage + pw > x0 + w || py * kPage + ph > y0 + h) continue;
+                Page &p = page_at(px, py);
+                p.cells.resize((size_t)kPage * kPage);
+                for (int r = 0; r < ph; r++)
+                    std::copy_n(out.data() + (size_t)(py * kPage + r - y0) * w + (px * kPage - x0), pw,
+                                p.cells.data() + (size_t)r * kPage);
+                p.gen = gen_;
+            }
+        }
         return out;
     }
 
     fsg_stamp_result stampBodies(const std::vector<fsg_body> &bodies, const std::vector<MaterialInstance> &pixels) {
         flush();
+        invalidate();
         fsg_stamp_result r{};
         check(fsg_world_stamp_bodies(w_, bodies.data(), (int)bodies.size(), pixels.data(), pixels.size(), &r),
               "fsg_world_stamp_bodies");
@@ -128,6 +182,8 @@ class world {
 
     // Pulls the pixels back into `pixels` (RigidBody::tiles).
     void unstampBodies(std::vector<MaterialInstance> &pixels) {
+        flush();
+        invalidate();
         check(fsg_world_unstamp_bodies(w_, pixels.data(), pixels.size()), "fsg_world_unstamp_bodies");
     }
 
@@ -188,6 +244,7 @@ class world {
         }
         all.push_back(0);  // a valid pointer when every image is empty
         std::vector<int32_t> status(n);
+        invalidate();
         check(fsg_world_unpack(w_, x0, y0, ncx, ncy, all.data(), sizes.data(), phase, layer2, background,
                                status.data()),
               "fsg_world_unpack");
@@ -204,15 +261,46 @@ class world {
     fsg_world *handle() const { return w_; }
 
   private:
+    static constexpr int kPage = 64;
+    struct Page {
+        uint64_t gen = 0;  // valid while equal to gen_
+        std::vector<MaterialInstance> cells;  // kPage x kPage, row-major
+    };
+
     void flush() {
         if (edits_.empty()) return;
         check(fsg_world_set_cells(w_, edits_.data(), edits_.size()), "fsg_world_set_cells");
         edits_.clear();
     }
 
+    Page &page_at(int px, int py) { return pages_[(size_t)py * npx_ + px]; }
+
+    // Downloads pages [px0, px1] x [py0, py1] in one call.
+    void fetch(int px0, int py0, int px1, int py1) {
+        flush();
+        const int x0 = px0 * kPage, y0 = py0 * kPage;
+        const int w = std::min((px1 + 1) * kPage, width) - x0, h = std::min((py1 + 1) * kPage, height) - y0;
+        std::vector<MaterialInstance> blk((size_t)w * h);
+        check(fsg_world_download(w_, blk.data(), x0, y0, w, h), "fsg_world_download");
+        for (int py = py0; py <= py1; py++) {
+            for (int px = px0; px <= px1; px++) {
+                Page &p = page_at(px, py);
+                p.cells.resize((size_t)kPage * kPage);
+                const int pw = std::min(kPage, width - px * kPage), ph = std::min(kPage, height - py * kPage);
+                for (int r = 0; r < ph; r++)
+                    std::copy_n(blk.data() + (size_t)(py * kPage + r - y0) * w + (px * kPage - x0), pw,
+                                p.cells.data() + (size_t)r * kPage);
+                p.gen = gen_;
+            }
+        }
+    }
+
     context &ctx_;
     fsg_world *w_ = nullptr;
     std::vector<fsg_cell_edit> edits_;
+    std::vector<Page> pages_;
+    int npx_ = 0, npy_ = 0;
+    uint64_t gen_ = 1;
 };
 
 }  // namespace fsg

@@ -342,8 +342,22 @@ static inline uint64_t cell_at(const uint64_t *c, int W, int H, int x, int y) {
     return c[(size_t)y * W + x];
 }
 
+/* Tick zone (World::tickZone, world.hpp:138, set in game.cpp:1629-1636):
+ * world::tick visits only the chunks whose origin lies in the zone
+ * (world.cpp:1073-1074), and a cell it visits may move into cells outside.
+ * Here a block acts iff at least one of its four cells lies in the zone
+ * zone[0..3] = x, y, w, h, so cells leave the zone across any edge, cells
+ * of the one-cell ring outside it move only together with a zone cell, and
+ * a zone covering the world is the same as no zone (NULL). */
+static int block_in_zone(const int32_t *zone, int x0, int y0) {
+    return !zone || (x0 + 1 >= zone[0] && x0 < zone[0] + zone[2] && y0 + 1 >= zone[1] && y0 < zone[1] + zone[3]);
+}
+static int cell_in_zone(const int32_t *zone, int x, int y) {
+    return !zone || (x >= zone[0] && x < zone[0] + zone[2] && y >= zone[1] && y < zone[1] + zone[3]);
+}
+
 static void run_phase(uint64_t *cells, int W, int H, const props *P, const fsg_config *cfg,
-                      uint64_t seed, uint64_t tick, int it) {
+                      uint64_t seed, uint64_t tick, int it, const int32_t *zone) {
     const uint64_t g = tick * (uint64_t)cfg->cell_iter + (uint64_t)it;
     const uint64_t key = orc_phase_key(seed, g);
     int ox, oy;
@@ -353,6 +367,7 @@ static void run_phase(uint64_t *cells, int W, int H, const props *P, const fsg_c
             uint64_t q[4] = {cell_at(cells, W, H, x0, y0), cell_at(cells, W, H, x0 + 1, y0),
                              cell_at(cells, W, H, x0, y0 + 1),
                              cell_at(cells, W, H, x0 + 1, y0 + 1)};
+            if (!block_in_zone(zone, x0, y0)) continue;
             uint32_t r, s;
             orc_block_hash(key, x0, y0, &r, &s);
             if (!block_rule(P, cfg, it, r, s, q)) continue;
@@ -366,16 +381,22 @@ static void run_phase(uint64_t *cells, int W, int H, const props *P, const fsg_c
     }
 }
 
-static void temperature_step(uint64_t *cells, int W, int H, const props *P) {
+static void temperature_step(uint64_t *cells, int W, int H, const props *P, const int32_t *zone) {
     /* Integer 5-point diffusion with coefficient 1/8, floor rounding
      * (arithmetic shift); an out-of-world neighbour counts as the cell
      * itself.  Replaces world::tickTemperature's float 9-point mean
-     * (world.cpp:1950-2004); add_temp is Material::addTemp (1988). */
+     * (world.cpp:1950-2004); add_temp is Material::addTemp (1988).  Only
+     * zone cells change (the loops of world.cpp:1953-1954); their stencil
+     * reads neighbours outside the zone as they are. */
     int16_t *nt = (int16_t *)malloc((size_t)W * H * sizeof(int16_t));
     for (int y = 0; y < H; y++) {
         for (int x = 0; x < W; x++) {
             const uint64_t c = cells[(size_t)y * W + x];
             const int T = temp_of(c);
+            if (!cell_in_zone(zone, x, y)) {
+                nt[(size_t)y * W + x] = (int16_t)T;
+                continue;
+            }
             const int tu = y > 0 ? temp_of(cells[(size_t)(y - 1) * W + x]) : T;
             const int td = y < H - 1 ? temp_of(cells[(size_t)(y + 1) * W + x]) : T;
             const int tl = x > 0 ? temp_of(cells[(size_t)y * W + x - 1]) : T;
@@ -395,29 +416,49 @@ static int temp_runs(const fsg_config *cfg, uint64_t tick) {
     return cfg->temp_every > 0 && (tick % cfg->temp_every) == cfg->temp_phase;
 }
 
-int orc_temperature(uint64_t *cells, int W, int H, const fsg_config *cfg) {
+static int zone_ok(const int32_t *zone, int W, int H) {
+    return !zone || (zone[0] >= 0 && zone[1] >= 0 && zone[2] >= 1 && zone[3] >= 1 && zone[2] <= W - zone[0] &&
+                     zone[3] <= H - zone[1]);
+}
+
+int orc_temperature_zone(uint64_t *cells, int W, int H, const fsg_config *cfg, const int32_t *zone) {
+    if (!cells || W <= 0 || H <= 0 || !zone_ok(zone, W, H)) return FSG_ERR_INVALID;
     props P;
     build_props(cfg, &P);
-    temperature_step(cells, W, H, &P);
+    temperature_step(cells, W, H, &P, zone);
+    return 0;
+}
+
+int orc_temperature(uint64_t *cells, int W, int H, const fsg_config *cfg) {
+    return orc_temperature_zone(cells, W, H, cfg, NULL);
+}
+
+int orc_tick_zone(uint64_t *cells, int W, int H, const fsg_config *cfg, uint64_t seed, uint64_t tick,
+                  const int32_t *zone) {
+    if (!cells || W <= 0 || H <= 0 || cfg->cell_iter == 0 || !zone_ok(zone, W, H)) return FSG_ERR_INVALID;
+    props P;
+    build_props(cfg, &P);
+    for (int it = 0; it < (int)cfg->cell_iter; it++) run_phase(cells, W, H, &P, cfg, seed, tick, it, zone);
+    if (temp_runs(cfg, tick)) temperature_step(cells, W, H, &P, zone);
     return 0;
 }
 
 int orc_tick(uint64_t *cells, int W, int H, const fsg_config *cfg, uint64_t seed, uint64_t tick) {
-    if (!cells || W <= 0 || H <= 0 || cfg->cell_iter == 0) return FSG_ERR_INVALID;
-    props P;
-    build_props(cfg, &P);
-    for (int it = 0; it < (int)cfg->cell_iter; it++) run_phase(cells, W, H, &P, cfg, seed, tick, it);
-    if (temp_runs(cfg, tick)) temperature_step(cells, W, H, &P);
+    return orc_tick_zone(cells, W, H, cfg, seed, tick, NULL);
+}
+
+int orc_run_zone(uint64_t *cells, int W, int H, const fsg_config *cfg, uint64_t seed, uint64_t tick0,
+                 int nticks, const int32_t *zone) {
+    for (int t = 0; t < nticks; t++) {
+        int rc = orc_tick_zone(cells, W, H, cfg, seed, tick0 + (uint64_t)t, zone);
+        if (rc) return rc;
+    }
     return 0;
 }
 
 int orc_run(uint64_t *cells, int W, int H, const fsg_config *cfg, uint64_t seed, uint64_t tick0,
             int nticks) {
-    for (int t = 0; t < nticks; t++) {
-        int rc = orc_tick(cells, W, H, cfg, seed, tick0 + (uint64_t)t);
-        if (rc) return rc;
-    }
-    return 0;
+    return orc_run_zone(cells, W, H, cfg, seed, tick0, nticks, NULL);
 }
 
 /* ----------------------------------------------------------------------- */

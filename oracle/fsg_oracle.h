@@ -40,8 +40,16 @@ int orc_block_rule(const fsg_config *cfg, int it, uint32_t r, uint32_t s, uint64
 int orc_tick(uint64_t *cells, int W, int H, const fsg_config *cfg, uint64_t seed, uint64_t tick);
 int orc_run(uint64_t *cells, int W, int H, const fsg_config *cfg, uint64_t seed,
             uint64_t tick0, int nticks);
+/* The same, limited to a tick zone zone[0..3] = x, y, w, h (World::tickZone):
+ * a block acts iff one of its cells lies in the zone, and only zone cells
+ * take the temperature step.  NULL = the whole world. */
+int orc_tick_zone(uint64_t *cells, int W, int H, const fsg_config *cfg, uint64_t seed, uint64_t tick,
+                  const int32_t *zone);
+int orc_run_zone(uint64_t *cells, int W, int H, const fsg_config *cfg, uint64_t seed, uint64_t tick0,
+                 int nticks, const int32_t *zone);
 /* One temperature step (5-point, 1/8, floor; out-of-world neighbour = self). */
 int orc_temperature(uint64_t *cells, int W, int H, const fsg_config *cfg);
+int orc_temperature_zone(uint64_t *cells, int W, int H, const fsg_config *cfg, const int32_t *zone);
 
 /* Dirty-rect predicate: 1 if any cell of the rectangle [x0,x1)x[y0,y1) or
  * any pair of cells within it could change under some RNG draw, or any
@@ -135,6 +143,10 @@ int orc_mesh_mask(const uint8_t *m, int W, int H, fsg_mesh_loop *loops, uint32_t
 int orc_spiral(int32_t *offs);
 int orc_particles_tick(uint64_t *cells, int W, int H, const fsg_config *cfg, fsg_particle *p, size_t n,
                        size_t *alive, size_t *landed);
+/* The same with a tick zone (NULL = the whole world): a particle whose cell
+ * lies outside it keeps its place (world.cpp:2061). */
+int orc_particles_tick_zone(uint64_t *cells, int W, int H, const fsg_config *cfg, fsg_particle *p, size_t n,
+                            size_t *alive, size_t *landed, const int32_t *zone);
 
 void orc_counts(const uint64_t *cells, size_t n, uint64_t counts[16]);
 uint64_t orc_checksum(const uint64_t *cells, size_t n);

@@ -44,8 +44,11 @@ int orc_spiral(int32_t *offs) {
     return n;
 }
 
-int orc_particles_tick(uint64_t *cells, int W, int H, const fsg_config *cfg, fsg_particle *p, size_t n,
-                       size_t *alive, size_t *landed) {
+int orc_particles_tick_zone(uint64_t *cells, int W, int H, const fsg_config *cfg, fsg_particle *p, size_t n,
+                            size_t *alive, size_t *landed, const int32_t *zone) {
+    if (zone && (zone[0] < 0 || zone[1] < 0 || zone[2] < 1 || zone[3] < 1 || zone[2] > W - zone[0] ||
+                 zone[3] > H - zone[1]))
+        return FSG_ERR_INVALID;
     static int32_t spiral[2 * 1024];
     static int nspiral = -1;
     if (nspiral < 0) nspiral = orc_spiral(spiral);
@@ -66,6 +69,11 @@ int orc_particles_tick(uint64_t *cells, int W, int H, const fsg_config *cfg, fsg
         }
         const int lx = (int)c.x, ly = (int)c.y;
         if (c.x < 0 || (int)c.x >= W || c.y < 0 || (int)c.y >= H) continue;
+        /* outside the tick zone: kept as it is, the target pull included */
+        if (zone && !(lx >= zone[0] && ly >= zone[1] && lx < zone[0] + zone[2] && ly < zone[1] + zone[3])) {
+            p[out++] = c;
+            continue;
+        }
         c.vx += c.ax;
         c.vy += c.ay;
         const int div = (int)((fabsf(c.vx) + fabsf(c.vy)) + 1);
@@ -125,4 +133,9 @@ int orc_particles_tick(uint64_t *cells, int W, int H, const fsg_config *cfg, fsg
     if (alive) *alive = m;
     if (landed) *landed = nl;
     return FSG_OK;
+}
+
+int orc_particles_tick(uint64_t *cells, int W, int H, const fsg_config *cfg, fsg_particle *p, size_t n,
+                       size_t *alive, size_t *landed) {
+    return orc_particles_tick_zone(cells, W, H, cfg, p, n, alive, landed, NULL);
 }

@@ -40,7 +40,9 @@ def lib():
             "orc_block_rule": (i32, [cfgp, i32, u32, u32, P]),
             "orc_tick": (i32, [P, i32, i32, cfgp, u64, u64]),
             "orc_run": (i32, [P, i32, i32, cfgp, u64, u64, i32]),
+            "orc_run_zone": (i32, [P, i32, i32, cfgp, u64, u64, i32, P]),
             "orc_temperature": (i32, [P, i32, i32, cfgp]),
+            "orc_temperature_zone": (i32, [P, i32, i32, cfgp, P]),
             "orc_region_unstable": (i32, [P, i32, i32, cfgp] + [i32] * 8),
             "orc_stamp": (i32, [P, i32, i32, cfgp, P, i32, P, C.c_size_t, P, C.POINTER(u32),
                                 C.POINTER(u32), P, u32, P, P]),
@@ -52,6 +54,8 @@ def lib():
             "orc_spiral": (i32, [P]),
             "orc_particles_tick": (i32, [P, i32, i32, cfgp, P, C.c_size_t, C.POINTER(C.c_size_t),
                                          C.POINTER(C.c_size_t)]),
+            "orc_particles_tick_zone": (i32, [P, i32, i32, cfgp, P, C.c_size_t, C.POINTER(C.c_size_t),
+                                              C.POINTER(C.c_size_t), P]),
             "orc_ms_value": (i32, [P, i32, i32, i32, i32]),
             "orc_find_perimeter": (i32, [P, i32, i32, i32, i32, P, i32]),
             "orc_simplify": (C.c_size_t, [P, C.c_size_t, C.c_float, P]),
@@ -91,17 +95,29 @@ def _grid(cells):
     return cells.ctypes.data, cells.shape[1], cells.shape[0]
 
 
-def run(cells: np.ndarray, cfg: Config, seed: int, tick0: int, nticks: int) -> np.ndarray:
-    """Det-mode ticks tick0 .. tick0+nticks-1, in place on an (H, W) uint64 grid."""
+def _zone(zone):
+    if zone is None:
+        return None, None
+    z = np.ascontiguousarray(zone, dtype=np.int32)
+    assert z.shape == (4,)
+    return z, z.ctypes.data
+
+
+def run(cells: np.ndarray, cfg: Config, seed: int, tick0: int, nticks: int, zone=None) -> np.ndarray:
+    """Det-mode ticks tick0 .. tick0+nticks-1, in place on an (H, W) uint64
+    grid; zone = (x, y, w, h) limits them to a tick zone."""
     p, W, H = _grid(cells)
-    rc = lib().orc_run(p, W, H, C.byref(cfg), seed, tick0, nticks)
+    z, zp = _zone(zone)
+    rc = lib().orc_run_zone(p, W, H, C.byref(cfg), seed, tick0, nticks, zp)
     assert rc == 0, rc
     return cells
 
 
-def temperature(cells: np.ndarray, cfg: Config) -> np.ndarray:
+def temperature(cells: np.ndarray, cfg: Config, zone=None) -> np.ndarray:
     p, W, H = _grid(cells)
-    lib().orc_temperature(p, W, H, C.byref(cfg))
+    z, zp = _zone(zone)
+    rc = lib().orc_temperature_zone(p, W, H, C.byref(cfg), zp)
+    assert rc == 0, rc
     return cells
 
 
@@ -390,12 +406,15 @@ def spiral():
     return [(int(o[2 * i]), int(o[2 * i + 1])) for i in range(n)]
 
 
-def particles_tick(cells, cfg, particles):
+def particles_tick(cells, cfg, particles, zone=None):
     """world::tickCells over a list of Particle structs, in place on cells;
-    returns (surviving particles, cells landed)."""
+    returns (surviving particles, cells landed).  zone = (x, y, w, h) is the
+    tick zone."""
     ptr, W, H = _grid(cells)
     arr = (Particle * max(len(particles), 1))(*particles)
     alive, landed = C.c_size_t(), C.c_size_t()
-    rc = lib().orc_particles_tick(ptr, W, H, C.byref(cfg), arr, len(particles), C.byref(alive), C.byref(landed))
+    z, zp = _zone(zone)
+    rc = lib().orc_particles_tick_zone(ptr, W, H, C.byref(cfg), arr, len(particles), C.byref(alive),
+                                       C.byref(landed), zp)
     assert rc == 0
     return [arr[i] for i in range(alive.value)], landed.value

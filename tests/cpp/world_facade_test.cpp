@@ -34,9 +34,28 @@ int main() {
         for (int x = 0; x < 128; x++) world.setTile(x, 3, sand);
         const std::vector<int32_t> st = world.readChunks(0, 0, 2, 1, files);
         const bool files_ok = st[0] == FSG_OK && st[1] == FSG_OK && world.stats().checksum == before_files.checksum;
-        std::printf("{\"checksum\": %llu, \"files_ok\": %d, \"wall\": %u, \"tile_50_50\": %llu, \"texel_50_50\": %u, "
-                    "\"counts\": [",
-                    (unsigned long long)s.checksum, files_ok ? 1 : 0, (unsigned)wall.mat,
+        // getTile's mirror: cell by cell equal to region(), setTile seen at
+        // once, and a tick drops it (reads after the tick see the new grid)
+        auto same = [](const fsg::MaterialInstance &a, const fsg::MaterialInstance &b) {
+            return a.mat == b.mat && a.temp == b.temp && a.rgba == b.rgba;
+        };
+        bool mirror_ok = true;
+        for (int pass = 0; pass < 2; pass++) {
+            const std::vector<fsg::MaterialInstance> reg = world.region(0, 0, W, H);
+            for (int y = 0; y < H; y += (pass ? 1 : 3))
+                for (int x = 0; x < W; x += (pass ? 3 : 1))
+                    mirror_ok = mirror_ok && same(world.getTile(x, y), reg[(size_t)y * W + x]);
+            world.setTile(131, 67, sand);
+            mirror_ok = mirror_ok && same(world.getTile(131, 67), sand);
+            world.tick(5);
+        }
+        world.prefetch(60, 60, 100, 50);
+        const std::vector<fsg::MaterialInstance> reg3 = world.region(60, 60, 100, 50);
+        for (int y = 0; y < 50; y++)
+            for (int x = 0; x < 100; x++) mirror_ok = mirror_ok && same(world.getTile(60 + x, 60 + y), reg3[(size_t)y * 100 + x]);
+        std::printf("{\"checksum\": %llu, \"files_ok\": %d, \"mirror_ok\": %d, \"wall\": %u, \"tile_50_50\": %llu, "
+                    "\"texel_50_50\": %u, \"counts\": [",
+                    (unsigned long long)s.checksum, files_ok ? 1 : 0, mirror_ok ? 1 : 0, (unsigned)wall.mat,
                     (unsigned long long)(t.mat | ((uint64_t)(uint16_t)t.temp << 16) | ((uint64_t)t.rgba << 32)),
                     (unsigned)texel[0]);
         for (int i = 0; i < 8; i++) std::printf("%llu%s", (unsigned long long)s.count[i], i < 7 ? ", " : "");

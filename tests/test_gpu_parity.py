@@ -118,6 +118,31 @@ def test_set_tile_and_get_tile(ctx, orc):
     assert first_diff(got, ref) is None, first_diff(got, ref)
 
 
+def test_get_tile_mirror(ctx, orc):
+    """getTile reads a host mirror: every cell equals the oracle's, a setTile
+    is seen at once, and ticks, stamps and batched steps drop the mirror."""
+    W, H = 200, 130  # pages clipped at the right and bottom edges
+    cells = fsg.generate(2, W, H, 4)
+    cfg = orc.default_config()
+    ref = cells.copy()
+    with fsg.World(ctx, W, H, 4, cfg) as w:
+        w.upload(cells)
+        w.tick(6)
+        orc.run(ref, cfg, 4, 0, 6)
+        assert all(w.getTile(x, y) == int(ref[y, x]) for y in range(H) for x in range(W))
+        sand = fsg.make_cell(fsg.MAT_SAND, 320, 0x11223344)
+        w.setTile(150, 100, sand)
+        ref[100, 150] = sand
+        assert w.getTile(150, 100) == sand
+        fsg.batch_step(ctx, [w], 4)
+        orc.run(ref, cfg, 4, 6, 4)
+        w.prefetch(100, 50, 100, 80)
+        assert all(w.getTile(x, y) == int(ref[y, x]) for y in range(50, H) for x in range(100, W, 3))
+        w.tick(3)
+        orc.run(ref, cfg, 4, 10, 3)
+        assert all(w.getTile(x, y) == int(ref[y, x]) for y in range(0, H, 2) for x in range(W))
+
+
 def test_stats_match_oracle(ctx, orc):
     cells = fsg.generate(2, 320, 192, 8)
     cfg = orc.default_config()
@@ -147,6 +172,7 @@ def test_cpp_facade_program(orc):
     assert got["checksum"] == orc.checksum(cells)
     assert got["wall"] == 15 and got["tile_50_50"] == int(cells[50, 50])
     assert got["files_ok"] == 1  # writeChunks / readChunks round trip over edited cells
+    assert got["mirror_ok"] == 1  # getTile's host mirror against region(), setTile, tick
     assert got["counts"] == [int(v) for v in orc.counts(cells)[:8]]
     layers, _ = orc.render(cells, orc.default_config(), _cells_layer_params())
     assert got["texel_50_50"] == int(layers[1][50, 50])
