@@ -518,6 +518,7 @@ extern "C" int fsg_world_destroy(fsg_world *w) {
     dev_free(w->d_part2);
     dev_free(w->d_part_alive);
     dev_free(w->d_part_cnt);
+    dev_free(w->d_pround);
     for (uint32_t *l : w->d_layer) dev_free(l);
     dev_free(w->d_tally);
     dev_free(w->d_isl);

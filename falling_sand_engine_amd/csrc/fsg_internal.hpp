@@ -108,6 +108,8 @@ struct fsg_world {
     uint8_t *d_part_alive = nullptr;
     unsigned long long *d_part_cnt = nullptr;
     size_t part_cap = 0, npart = 0;
+    void *d_pround = nullptr;  // the particle pass's round buffers
+    size_t pround_cap = 0;
     bool pack_map_set = false;             // pack_map holds a caller's map (else the default)
     fsg_pack_map pack_map{};
     size_t bytes = 0;
