@@ -328,9 +328,10 @@ def bench_mesh(fsg, side=4096, seed=12, reps=3):
     """Outlines of the SOLID cells of every 128x128 chunk that ends before
     the world's last row and column (world::updateChunkMesh for a whole mesh
     zone, world.cpp:722-998) on a basin world and on a world of stone
-    fragments; the whole synchronous call (launch, pools, host ordering).
-    The oracle's restatement of the same step, one thread, on 64 chunks of
-    each world is the CPU figure."""
+    fragments; the whole synchronous call (launch, pools, host ordering,
+    download of the loop and vertex arrays).  The oracle's restatement of
+    the same step in C, one thread, on 64 chunks of each world is the CPU
+    figure."""
     import numpy as np
     from oracle import pyoracle
     out = {"workload": f"every 128x128 chunk of {side}x{side} worlds (basin kind 2; stone fragments p=0.5)"}
@@ -340,20 +341,21 @@ def bench_mesh(fsg, side=4096, seed=12, reps=3):
     for name, cells in (("basin", fsg.generate(2, side, side, seed)), ("fragments", fragments(side, seed))):
         with fsg.Context(0) as ctx, fsg.World(ctx, side, side, seed, cfg) as w:
             w.upload(cells)
-            w.mesh(regions)  # pools sized
+            w.mesh(regions, raw=True)  # pools sized
             ts = []
             for _ in range(reps):
                 t0 = time.perf_counter()
-                res = w.mesh(regions)
+                regs, lps, vs = w.mesh(regions, raw=True)
                 ts.append(time.perf_counter() - t0)
-        loops = sum(len(r[1]) for r in res)
-        meshed = sum(1 for r in res if r[0] == 0)
+        loops = sum(regs[i].nloops for i in range(len(regions)))
+        meshed = sum(1 for i in range(len(regions)) if regs[i].status == 0)
         types = np.array([cfg.materials[i].type for i in range(cfg.n_materials)] + [0] * 16, np.uint8)
         solid = (types[(cells & np.uint64(15)).astype(np.int64)] == fsg.SOLID).astype(np.uint8)
         sample = [r for r in regions if r[0] + 128 < side and r[1] + 128 < side][:64]
+        masks = [np.ascontiguousarray(solid[y0:y0 + 128, x0:x0 + 128]) for x0, y0, _w, _h in sample]
         t0 = time.perf_counter()
-        for x0, y0, _w, _h in sample:
-            pyoracle.mesh_mask(solid[y0:y0 + 128, x0:x0 + 128])
+        for m in masks:
+            pyoracle.mesh_mask(m, raw=True)
         cpu = (time.perf_counter() - t0) / len(sample)
         ms = min(ts) * 1e3
         out[name] = {"chunks_meshed": meshed, "loops": loops, "ms": round(ms, 2),

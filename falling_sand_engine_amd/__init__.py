@@ -479,16 +479,24 @@ class World:
             out.append((R.status, ls))
         return out
 
-    def mesh(self, regions):
+    def mesh(self, regions, raw=False):
         """Outlines of the SOLID cells of grid regions [(x0, y0, w, h), ...]
         (world::updateChunkMesh's marching squares, world.cpp:722-895): per
-        region (status, [(start_x, start_y, hole, [(x, y), ...]), ...])."""
+        region (status, [(start_x, start_y, hole, [(x, y), ...]), ...]).
+        raw=True returns the C-ABI's arrays instead: (regions, loops,
+        vertices as an (n, 2) float32 array), with no per-vertex objects."""
         self._flush()
         regs = (_abi.MeshRegion * max(len(regions), 1))()
         for i, (x0, y0, w, h) in enumerate(regions):
             regs[i].x0, regs[i].y0, regs[i].w, regs[i].h = x0, y0, w, h
         nl, nv = C.c_uint32(), C.c_uint32()
         check(lib().fsg_world_mesh(self.handle, regs, len(regions), C.byref(nl), C.byref(nv)), "fsg_world_mesh")
+        if raw:
+            loops = (_abi.MeshLoop * max(nl.value, 1))()
+            verts = np.zeros((max(nv.value, 1), 2), np.float32)
+            check(lib().fsg_world_mesh_download(self.handle, loops, nl.value, verts.ctypes.data, nv.value),
+                  "fsg_world_mesh_download")
+            return regs, loops, verts[:nv.value]
         return self._mesh_result(list(regs)[:len(regions)], nl, nv)
 
     def mesh_islands(self):

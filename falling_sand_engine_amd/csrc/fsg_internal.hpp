@@ -100,8 +100,10 @@ struct fsg_world {
     void *d_mesh_regs = nullptr, *d_mesh_scratch = nullptr, *d_mesh_pool = nullptr;
     size_t mesh_regs_cap = 0, mesh_scratch_cap = 0, mesh_pool_cap = 0;
     uint32_t mesh_loop_hint = 0, mesh_vert_hint = 0;
-    std::vector<fsg_mesh_loop> mesh_loops;
-    std::vector<float> mesh_verts;
+    // the last mesh call's ordered result, on the device (in d_mesh_pool)
+    fsg_mesh_loop *d_mesh_loops = nullptr;
+    float *d_mesh_verts = nullptr;
+    uint32_t mesh_nloops = 0, mesh_nverts = 0;
     // free-fall particles (fsg_particles.hip): the list, the compaction
     // target, per-particle kept flags, counters
     fsg_particle *d_part = nullptr, *d_part2 = nullptr;

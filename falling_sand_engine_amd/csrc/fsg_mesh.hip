@@ -27,6 +27,7 @@ constexpr int MW = 64;                                  // threads per workgroup
 constexpr int MBITS = FSG_MESH_MAX_CELLS;               // mask bits per region
 constexpr int MWORDS = MBITS / 32;
 constexpr int MAXV = (128 + 1) * (128 + 1) + 2;         // runs of one loop of a 16384-cell region, bound
+constexpr int LCAP = 2048;                              // loops up to this many runs stay in LDS
 
 struct MeshSrc {
     const uint64_t *cells;  // region r's cell (x, y) is cells[off[r] + y * pitch[r] + x]
@@ -46,6 +47,7 @@ struct MeshOut {
     uint32_t loop_cap, vert_cap;
     unsigned int *counters; // [0] loops, [1] vertices, [2] overflow, [3] walk failures
     uint32_t *scratch;      // per workgroup: MAXV vertices + MAXV stack entries
+    uint32_t *regcnt;       // per caller's region: its loops (zero for regions not traced)
 };
 
 __device__ __forceinline__ bool bit_at(const uint32_t *b, int i) { return (b[i >> 5] >> (i & 31)) & 1u; }
@@ -89,9 +91,13 @@ __device__ __forceinline__ float vy(uint32_t v) { return (float)(v >> 16); }
 __global__ __launch_bounds__(MW) void k_mesh(MeshSrc src, const MeshRegionDev *regs, int nreg, MeshOut out) {
     __shared__ uint32_t mask[MWORDS], cand[MWORDS], seen[MWORDS];
     __shared__ uint32_t keep[MAXV / 32 + 1];
+    __shared__ uint32_t lmesh[LCAP], lstack[LCAP];
     const int lane = threadIdx.x;
-    uint32_t *mesh = out.scratch + (size_t)blockIdx.x * 2 * MAXV;  // vertices of the loop being traced
-    uint32_t *stack = mesh + MAXV;                                   // RDP sections (i | j << 16)
+    // the loop being traced: its run ends (vertices) and the RDP sections
+    // (i | j << 16), in LDS for loops of up to LCAP runs, else in the
+    // workgroup's global scratch (the walk writes both until it knows)
+    uint32_t *gmesh = out.scratch + (size_t)blockIdx.x * 2 * MAXV;
+    uint32_t *gstack = gmesh + MAXV;
     for (int ri = blockIdx.x; ri < nreg; ri += gridDim.x) {
         const MeshRegionDev R = regs[ri];
         const int w = R.w, h = R.h, size = w * h, nwords = (size + 31) >> 5;
@@ -174,7 +180,11 @@ __global__ __launch_bounds__(MW) void k_mesh(MeshSrc src, const MeshRegionDev *r
                 }
                 sx += rdx;
                 sy -= rdy;
-                if (nr < MAXV && lane == 0) mesh[nr] = (uint32_t)sx | ((uint32_t)sy << 16);
+                if (lane == 0) {
+                    const uint32_t v = (uint32_t)sx | ((uint32_t)sy << 16);
+                    if (nr < LCAP) lmesh[nr] = v;
+                    if (nr < MAXV) gmesh[nr] = v;
+                }
                 nr++;
             };
             for (int step = 0;; step++) {
@@ -213,6 +223,8 @@ __global__ __launch_bounds__(MW) void k_mesh(MeshSrc src, const MeshRegionDev *r
                 if (lane == 0) atomicAdd(&out.counters[3], 1u);
                 continue;
             }
+            uint32_t *mesh = nr <= LCAP ? lmesh : gmesh;
+            uint32_t *stack = nr <= LCAP ? lstack : gstack;
             // Ramer-Douglas-Peucker, tolerance 1, over mesh[0 .. nr)
             for (int k = lane; k < (nr + 31) / 32; k += 64) keep[k] = ~0u;
             __syncthreads();
@@ -265,11 +277,19 @@ __global__ __launch_bounds__(MW) void k_mesh(MeshSrc src, const MeshRegionDev *r
             for (int k = lane; k < nr; k += 64) ns += bit_at(keep, k);
             for (int off = 32; off > 0; off >>= 1) ns += __shfl_xor(ns, off, 64);
             if (ns < 3) continue;
-            // compact the kept vertices to the front of mesh[] (lane 0, in order)
-            if (lane == 0) {
+            // compact the kept vertices to the front of mesh[], in order, 64 at
+            // a time (every lane reads before any writes, and writes land at or
+            // before the positions read)
+            {
                 int m = 0;
-                for (int k = 0; k < nr; k++)
-                    if (bit_at(keep, k)) mesh[m++] = mesh[k];
+                for (int k0 = 0; k0 < nr; k0 += 64) {
+                    const int k = k0 + lane;
+                    const bool kp = k < nr && bit_at(keep, k);
+                    const uint32_t v = kp ? mesh[k] : 0u;
+                    const uint64_t b = __ballot(kp);
+                    if (kp) mesh[m + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u))] = v;
+                    m += __popcll(b);
+                }
             }
             __syncthreads();
             long long area = 0;
@@ -305,7 +325,59 @@ __global__ __launch_bounds__(MW) void k_mesh(MeshSrc src, const MeshRegionDev *r
             nloops_region++;
             __syncthreads();
         }
+        if (lane == 0) out.regcnt[R.region] = nloops_region;
         __syncthreads();
+    }
+}
+
+// ---- ordering on the device: regions in the caller's order, loops in trace
+// order within each ---------------------------------------------------------
+// Exclusive scan of n counts in one workgroup (1024 at a time, carried).
+__global__ __launch_bounds__(1024) void k_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n) {
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t b0 = 0; b0 < n; b0 += 1024) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t v = i < n ? in[i] : 0u;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (uint32_t off = 1; off < 1024; off <<= 1) {
+            const uint32_t add = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+            __syncthreads();
+            part[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < n) out[i] = carry + part[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += part[1023];
+        __syncthreads();
+    }
+}
+
+// Pooled loop l goes to place regbase[region] + seq: its pool index and count.
+__global__ void k_mesh_place(const fsg_mesh_loop *loops, const uint32_t *seq, uint32_t nl, const uint32_t *regbase,
+                             uint32_t *perm, uint32_t *cnt) {
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nl) return;
+    const uint32_t f = regbase[loops[l].region] + seq[l];
+    perm[f] = l;
+    cnt[f] = loops[l].count;
+}
+
+// The ordered loops and their vertices as float pairs.
+__global__ void k_mesh_emit(const fsg_mesh_loop *loops, const uint32_t *verts, uint32_t nl, const uint32_t *perm,
+                            const uint32_t *vbase, fsg_mesh_loop *oloops, float2 *overts) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nl) return;
+    fsg_mesh_loop L = loops[perm[f]];
+    const uint32_t from = L.first;
+    L.first = vbase[f];
+    oloops[f] = L;
+    for (uint32_t v = 0; v < L.count; v++) {
+        const uint32_t p = verts[from + v];
+        overts[L.first + v] = make_float2((float)(p & 0xFFFFu), (float)(p >> 16));
     }
 }
 
@@ -328,8 +400,8 @@ int run_mesh(fsg_world *w, const MeshSrc &src, std::vector<MeshRegionDev> &regs,
              int nout, uint32_t *nloops, uint32_t *nverts) {
     fsg_ctx *c = w->ctx;
     HIPCHK(hipSetDevice(c->device));
-    w->mesh_loops.clear();
-    w->mesh_verts.clear();
+    w->mesh_nloops = 0;
+    w->mesh_nverts = 0;
     for (int i = 0; i < nout; i++) {
         out_regions[i].first_loop = 0;
         out_regions[i].nloops = 0;
@@ -337,10 +409,13 @@ int run_mesh(fsg_world *w, const MeshSrc &src, std::vector<MeshRegionDev> &regs,
     const int n = (int)regs.size();
     if (n > 0) {
         const int grid = std::min(n, 1024);
-        const size_t b_regs = ((size_t)n * sizeof(MeshRegionDev) + 255) & ~(size_t)255;
-        if (grow_buf(c, &w->d_mesh_regs, &w->mesh_regs_cap, b_regs) ||
+        auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        const size_t b_regs = al((size_t)n * sizeof(MeshRegionDev));
+        if (grow_buf(c, &w->d_mesh_regs, &w->mesh_regs_cap, b_regs + 2 * al((size_t)nout * 4)) ||
             grow_buf(c, &w->d_mesh_scratch, &w->mesh_scratch_cap, (size_t)grid * 2 * MAXV * 4))
             return FSG_ERR_NOMEM;
+        uint32_t *regcnt = reinterpret_cast<uint32_t *>(static_cast<char *>(w->d_mesh_regs) + b_regs);
+        uint32_t *regbase = regcnt + al((size_t)nout * 4) / 4;
         HIPCHK(hipMemcpyAsync(w->d_mesh_regs, regs.data(), (size_t)n * sizeof(MeshRegionDev), hipMemcpyHostToDevice,
                               c->stream));
         // pools sized from the last call, grown and the launch repeated on overflow
@@ -348,20 +423,37 @@ int run_mesh(fsg_world *w, const MeshSrc &src, std::vector<MeshRegionDev> &regs,
         uint32_t vert_cap = std::max<uint32_t>(w->mesh_vert_hint, 65536u);
         bool done = false;
         for (int attempt = 0; attempt < 2 && !done; attempt++) {
-            const size_t b_loops = ((size_t)loop_cap * sizeof(fsg_mesh_loop) + 255) & ~(size_t)255;
-            const size_t b_seq = ((size_t)loop_cap * 4 + 255) & ~(size_t)255;
-            if (grow_buf(c, &w->d_mesh_pool, &w->mesh_pool_cap, b_loops + b_seq + (size_t)vert_cap * 4 + 256))
-                return FSG_ERR_NOMEM;
+            // the pool: traced loops, their seq, vertices; then the ordered
+            // result: place -> pool index, counts, vertex bases, loops, vertices
+            const size_t b_loops = al((size_t)loop_cap * sizeof(fsg_mesh_loop)), b_l4 = al((size_t)loop_cap * 4);
+            const size_t b_v4 = al((size_t)vert_cap * 4);
+            const size_t total = 2 * b_loops + 4 * b_l4 + b_v4 + al((size_t)vert_cap * 8) + 256;
+            if (grow_buf(c, &w->d_mesh_pool, &w->mesh_pool_cap, total)) return FSG_ERR_NOMEM;
             char *pool = static_cast<char *>(w->d_mesh_pool);
             MeshOut o;
             o.loops = reinterpret_cast<fsg_mesh_loop *>(pool);
-            o.seq = reinterpret_cast<uint32_t *>(pool + b_loops);
-            o.verts = reinterpret_cast<uint32_t *>(pool + b_loops + b_seq);
-            o.counters = reinterpret_cast<unsigned int *>(pool + b_loops + b_seq + (size_t)vert_cap * 4);
+            pool += b_loops;
+            o.seq = reinterpret_cast<uint32_t *>(pool);
+            pool += b_l4;
+            o.verts = reinterpret_cast<uint32_t *>(pool);
+            pool += b_v4;
+            o.counters = reinterpret_cast<unsigned int *>(pool);
+            pool += 256;
+            uint32_t *perm = reinterpret_cast<uint32_t *>(pool);
+            pool += b_l4;
+            uint32_t *lcnt = reinterpret_cast<uint32_t *>(pool);
+            pool += b_l4;
+            uint32_t *vbase = reinterpret_cast<uint32_t *>(pool);
+            pool += b_l4;
+            fsg_mesh_loop *oloops = reinterpret_cast<fsg_mesh_loop *>(pool);
+            pool += b_loops;
+            float2 *overts = reinterpret_cast<float2 *>(pool);
             o.loop_cap = loop_cap;
             o.vert_cap = vert_cap;
             o.scratch = static_cast<uint32_t *>(w->d_mesh_scratch);
+            o.regcnt = regcnt;
             HIPCHK(hipMemsetAsync(o.counters, 0, 16, c->stream));
+            HIPCHK(hipMemsetAsync(regcnt, 0, (size_t)nout * 4, c->stream));
             k_mesh<<<grid, MW, 0, c->stream>>>(src, static_cast<const MeshRegionDev *>(w->d_mesh_regs), n, o);
             HIPCHK(hipGetLastError());
             unsigned int cnt[4];
@@ -375,43 +467,33 @@ int run_mesh(fsg_world *w, const MeshSrc &src, std::vector<MeshRegionDev> &regs,
             }
             w->mesh_loop_hint = cnt[0];
             w->mesh_vert_hint = cnt[1];
-            std::vector<fsg_mesh_loop> loops(cnt[0]);
-            std::vector<uint32_t> seq(cnt[0]), verts(cnt[1]);
+            std::vector<uint32_t> rc((size_t)nout);
             if (cnt[0]) {
-                HIPCHK(hipMemcpyAsync(loops.data(), o.loops, (size_t)cnt[0] * sizeof(fsg_mesh_loop),
-                                      hipMemcpyDeviceToHost, c->stream));
-                HIPCHK(hipMemcpyAsync(seq.data(), o.seq, (size_t)cnt[0] * 4, hipMemcpyDeviceToHost, c->stream));
-                HIPCHK(hipMemcpyAsync(verts.data(), o.verts, (size_t)cnt[1] * 4, hipMemcpyDeviceToHost, c->stream));
+                k_scan_u32<<<1, 1024, 0, c->stream>>>(regcnt, regbase, (uint32_t)nout);
+                k_mesh_place<<<(cnt[0] + 255) / 256, 256, 0, c->stream>>>(o.loops, o.seq, cnt[0], regbase, perm, lcnt);
+                k_scan_u32<<<1, 1024, 0, c->stream>>>(lcnt, vbase, cnt[0]);
+                k_mesh_emit<<<(cnt[0] + 255) / 256, 256, 0, c->stream>>>(o.loops, o.verts, cnt[0], perm, vbase,
+                                                                         oloops, overts);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(rc.data(), regcnt, (size_t)nout * 4, hipMemcpyDeviceToHost, c->stream));
                 HIPCHK(hipStreamSynchronize(c->stream));
             }
-            // regions in the caller's order, loops in trace order within each
-            std::vector<uint32_t> order(cnt[0]);
-            for (uint32_t k = 0; k < cnt[0]; k++) order[k] = k;
-            std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-                return loops[a].region != loops[b].region ? loops[a].region < loops[b].region : seq[a] < seq[b];
-            });
-            w->mesh_loops.reserve(cnt[0]);
-            w->mesh_verts.reserve((size_t)cnt[1] * 2);
-            for (uint32_t k : order) {
-                fsg_mesh_loop L = loops[k];
-                const uint32_t from = L.first;
-                L.first = (uint32_t)(w->mesh_verts.size() / 2);
-                for (uint32_t v = 0; v < L.count; v++) {
-                    const uint32_t p = verts[from + v];
-                    w->mesh_verts.push_back((float)(p & 0xFFFFu));
-                    w->mesh_verts.push_back((float)(p >> 16));
-                }
-                fsg_mesh_region &R = out_regions[L.region];
-                if (R.nloops == 0) R.first_loop = (uint32_t)w->mesh_loops.size();
-                R.nloops++;
-                w->mesh_loops.push_back(L);
+            uint32_t first = 0;
+            for (int r = 0; r < nout; r++) {
+                out_regions[r].first_loop = rc[r] ? first : 0;
+                out_regions[r].nloops = rc[r];
+                first += rc[r];
             }
+            w->d_mesh_loops = oloops;
+            w->d_mesh_verts = reinterpret_cast<float *>(overts);
+            w->mesh_nloops = cnt[0];
+            w->mesh_nverts = cnt[1];
             done = true;
         }
         if (!done) return FSG_ERR_NOMEM;  // the second launch had pools of the first one's totals
     }
-    if (nloops) *nloops = (uint32_t)w->mesh_loops.size();
-    if (nverts) *nverts = (uint32_t)(w->mesh_verts.size() / 2);
+    if (nloops) *nloops = w->mesh_nloops;
+    if (nverts) *nverts = w->mesh_nverts;
     return FSG_OK;
 }
 
@@ -479,9 +561,12 @@ extern "C" int fsg_world_mesh_islands(fsg_world *w, const fsg_island *islands, i
 extern "C" int fsg_world_mesh_download(fsg_world *w, fsg_mesh_loop *loops, uint32_t loop_cap, float *verts,
                                        uint32_t vert_cap) {
     if (!w) return FSG_ERR_INVALID;
-    const size_t nl = w->mesh_loops.size(), nv = w->mesh_verts.size() / 2;
+    const size_t nl = w->mesh_nloops, nv = w->mesh_nverts;
     if ((nl && (!loops || loop_cap < nl)) || (nv && (!verts || vert_cap < nv))) return FSG_ERR_INVALID;
-    if (nl) std::memcpy(loops, w->mesh_loops.data(), nl * sizeof(fsg_mesh_loop));
-    if (nv) std::memcpy(verts, w->mesh_verts.data(), nv * 2 * sizeof(float));
+    fsg_ctx *c = w->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    if (nl) HIPCHK(hipMemcpyAsync(loops, w->d_mesh_loops, nl * sizeof(fsg_mesh_loop), hipMemcpyDeviceToHost, c->stream));
+    if (nv) HIPCHK(hipMemcpyAsync(verts, w->d_mesh_verts, nv * 2 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return FSG_OK;
 }

@@ -382,8 +382,9 @@ def simplify(points, tol=1.0):
     return [(float(a), float(b)) for a, b in out[:n]]
 
 
-def mesh_mask(mask):
-    """updateChunkMesh's outline of a 0/1 mask: [(start_x, start_y, hole, [(x, y), ...]), ...]."""
+def mesh_mask(mask, raw=False):
+    """updateChunkMesh's outline of a 0/1 mask: [(start_x, start_y, hole, [(x, y), ...]), ...]
+    (raw=True: the loop count and vertex count only, with no Python objects)."""
     m, W, H = _mask(mask)
     lcap, vcap = W * H + 8, 4 * (W + 1) * (H + 1) + 8
     loops = (MeshLoop * lcap)()
@@ -391,6 +392,8 @@ def mesh_mask(mask):
     nl, nv = C.c_uint32(), C.c_uint32()
     rc = lib().orc_mesh_mask(m.ctypes.data, W, H, loops, lcap, verts.ctypes.data, vcap, C.byref(nl), C.byref(nv))
     assert rc == 0, rc
+    if raw:
+        return nl.value, nv.value
     out = []
     for i in range(nl.value):
         L = loops[i]
