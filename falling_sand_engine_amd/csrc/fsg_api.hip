@@ -245,12 +245,12 @@ extern "C" int fsg_rule_eval(const fsg_config *cfg, int it, uint32_t r, uint32_t
     bool react = ((VL | VR) & fsg::K_REACT) != 0;
     for (int k = 0; k < 4; k++)
         react = react || (M.reactions && fsg::cbyte(c[k]) == M.wood && fsg::ctemp(c[k]) > M.burn);
-    const uint32_t movA = (c[0] >> 8) & 1u, movB = (c[1] >> 8) & 1u;
+    const uint32_t movA = (c[0] >> 8) & 1u, movB = (c[1] >> 8) & 1u, movC = (c[2] >> 8) & 1u, movD = (c[3] >> 8) & 1u;
     if (react) {
         if (!fsg::block_react(M, c)) return FSG_ERR_STATE;  // the kernel queues only blocks that react
     } else {
         if (fsg::block_inert(VL, VR, DL, DR, HT, HB, movA, movB)) return 0;
-        fsg::apply_decision(fsg::block_decide(VL, VR, DL, DR, HT, HTR, HB, movA, movB, r, s), c);
+        fsg::apply_decision(fsg::block_decide(VL, VR, DL, DR, HT, HTR, HB, movA, movB, movC, movD, r, s), c);
     }
     uint64_t o[4];
     bool ch = false;

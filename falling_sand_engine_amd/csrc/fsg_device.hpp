@@ -281,14 +281,27 @@ __host__ __device__ __forceinline__ bool block_rule(const MatTab &M, int it, uin
             q[col] &= ~MOVED;
     }
 
-    // H: per row, top then bottom; soup into lighter non-solid, gas into air, p=1/2
+    // H: per row, top then bottom.  Soup flows into a lighter non-solid cell
+    // when it faces it (MOVED set: heading right), or turns with p=1/16
+    // (r[24:27] top, r[28:31] bottom) and takes the new heading; otherwise
+    // gas spreads into air with p=1/2
 #pragma unroll
     for (int row = 0; row < 2; row++) {
         const int i = 2 * row, j = i + 1;
         const uint32_t ml = cid(q[i]), mr = cid(q[j]);
-        const bool f = nib(pass, mr) < nib(flowt, ml) || nib(pass, ml) < nib(flowt, mr) ||
-                       (bit(gasm, ml) && bit(airm, mr)) || (bit(gasm, mr) && bit(airm, ml));
-        if (f && ((s >> (23 + row)) & 1u)) {
+        const bool fl = nib(pass, mr) < nib(flowt, ml), fr = nib(pass, ml) < nib(flowt, mr);
+        bool go;
+        if (fl || fr) {
+            const bool facing = fl ? (q[i] & MOVED) != 0 : (q[j] & MOVED) == 0;
+            go = facing || ((r >> (24 + 4 * row)) & 15u) == 0;
+            if (go) {
+                if (fl) q[i] |= MOVED;
+                else q[j] &= ~MOVED;
+            }
+        } else {
+            go = ((bit(gasm, ml) && bit(airm, mr)) || (bit(gasm, mr) && bit(airm, ml))) && ((s >> (23 + row)) & 1u);
+        }
+        if (go) {
             const Cell t = q[i];
             q[i] = q[j];
             q[j] = t;
@@ -319,6 +332,8 @@ constexpr uint32_t K_USAND = 1u << 5;
 constexpr uint32_t K_TAIR = 1u << 6;
 constexpr uint32_t K_UAIR = 1u << 7;
 constexpr uint32_t K_TSANDACT = 1u << 8; // t is sand active in this iteration
+constexpr uint32_t K_FLOWTU = 1u << 9;   // t, an active SOUP cell, flows sideways into u
+constexpr uint32_t K_FLOWUT = 1u << 10;  // u flows sideways into t
 
 __host__ __device__ __forceinline__ uint32_t pair_code(const MatTab &M, int it, uint32_t t, uint32_t u) {
     const int pt = nib(M.pass, t), pu = nib(M.pass, u);
@@ -336,6 +351,8 @@ __host__ __device__ __forceinline__ uint32_t pair_code(const MatTab &M, int it, 
     if (at) c |= K_TAIR;
     if (au) c |= K_UAIR;
     if (bit(M.sand, t) && st > 0) c |= K_TSANDACT;
+    if (pu < ft) c |= K_FLOWTU;
+    if (pt < fu) c |= K_FLOWUT;
     return c;
 }
 
@@ -351,8 +368,8 @@ __host__ __device__ __forceinline__ uint32_t pair_code(const MatTab &M, int it, 
 template <bool GAS_H = true>
 __host__ __device__ __forceinline__ uint32_t block_decide(uint32_t VL, uint32_t VR, uint32_t DL, uint32_t DR,
                                                           uint32_t HT, uint32_t HTR, uint32_t HB,
-                                                          uint32_t movA, uint32_t movB, uint32_t r,
-                                                          uint32_t s) {
+                                                          uint32_t movA, uint32_t movB, uint32_t movC,
+                                                          uint32_t movD, uint32_t r, uint32_t s) {
     if (!GAS_H) {
         VL &= ~K_GRISE;
         VR &= ~K_GRISE;
@@ -414,9 +431,18 @@ __host__ __device__ __forceinline__ uint32_t block_decide(uint32_t VL, uint32_t 
 
     // F: an active sand top cell with no straight or in-block diagonal move stops
     clr |= (rest & actA & (sL ^ 1u) & (dL ^ 1u)) | ((rest & actB & (sR ^ 1u) & (dR ^ 1u)) << 1);
-    // H: horizontal swaps, top row and bottom row
-    y ^= (rest & hT & S7) * 0x05u;
-    y ^= (rest & hB & S8) * 0x50u;
+    // H: per row, a SOUP cell that can flow sideways moves if it faces the
+    // other cell (MOVED set: heading right), else turns and moves with
+    // p=1/16, taking the new heading; otherwise gas spreads with p=1/2
+    const uint32_t fAB = (HT >> 9) & 1u, fBA = (HT >> 10) & 1u, fCD = (HB >> 9) & 1u, fDC = (HB >> 10) & 1u;
+    const uint32_t lT = fAB | fBA, lB = fCD | fDC;
+    const uint32_t TT = (r & 0xF000000u) == 0, TB = (r & 0xF0000000u) == 0;
+    const uint32_t goT = rest & ((lT & ((fAB & movA) | (fBA & (movB ^ 1u)) | TT)) | ((lT ^ 1u) & hT & S7));
+    const uint32_t goB = rest & ((lB & ((fCD & movC) | (fDC & (movD ^ 1u)) | TB)) | ((lB ^ 1u) & hB & S8));
+    y ^= goT * 0x05u;
+    y ^= goB * 0x50u;
+    set |= (goT & fAB) | ((goB & fCD) << 2);
+    clr |= ((goT & fBA) << 1) | ((goB & fDC) << 3);
     return y | (set << 8) | (clr << 12);
 }
 
@@ -456,7 +482,8 @@ __device__ __forceinline__ bool pair_horz(const MatTab &M, uint64_t l, uint64_t 
            (bit(M.gas[0], ml) && bit(M.air, mr)) || (bit(M.gas[0], mr) && bit(M.air, ml));
 }
 __device__ __forceinline__ bool single_unstable(const MatTab &M, uint64_t c) {
-    return (c & MOVED) || (M.reactions && cbyte(c) == M.wood && ctemp(c) > M.burn);
+    // a SOUP cell's heading (its MOVED bit) changes only with a move
+    return ((c & MOVED) && bit(M.sand, cid(c))) || (M.reactions && cbyte(c) == M.wood && ctemp(c) > M.burn);
 }
 
 } // namespace fsg

@@ -212,7 +212,7 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
         block_hash(key, gx0 + lx, gy0 + ly, r, s);
         const uint32_t dec = block_decide<false>(T[(i0 << 4) | i2], T[(i1 << 4) | i3], T[(i0 << 4) | i3],
                                                  T[(i1 << 4) | i2], 0u, T[(i1 << 4) | i0], 0u,
-                                                 (q[0] >> 8) & 1u, (q[1] >> 8) & 1u, r, s);
+                                                 (q[0] >> 8) & 1u, (q[1] >> 8) & 1u, 0u, 0u, r, s);
         if (dec != 0) scatter(p0, q, dec);
         track(dec != 0, lx, ly);
     }
@@ -240,7 +240,7 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
         } else {
             const uint32_t dec = block_decide<true>(VL, VR, T[(i0 << 4) | i3], T[(i1 << 4) | i2], T[(i0 << 4) | i1],
                                                     T[(i1 << 4) | i0], T[(i2 << 4) | i3], (q[0] >> 8) & 1u,
-                                                    (q[1] >> 8) & 1u, r, s);
+                                                    (q[1] >> 8) & 1u, (q[2] >> 8) & 1u, (q[3] >> 8) & 1u, r, s);
             if (dec != 0) scatter(p0, q, dec);
             wrote = dec != 0;
         }

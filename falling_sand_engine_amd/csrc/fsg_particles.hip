@@ -219,7 +219,7 @@ __device__ void move_particle(PState &s, const uint64_t *cells, int W, int H, ui
                     const int cx = (int)(c.x + (float)sp.x);
                     const int cy = (int)(c.y + (float)sp.y);
                     sp.next();
-                    bool in = cx >= 0 && cy >= 0 && cx < W && cy < H;
+                    bool in = cx >= 0 && cy >= 0 && cx < W && cy < H;  // the reference reads out of range here
                     if (masked && in) in = (m25 >> (((cy >> 3) - by0) * 5 + ((cx >> 3) - bx0))) & 1u;
                     at[u] = in ? cx + cy * W : -1;
                     m[u] = in ? (uint32_t)cells[cx + cy * W] : 0u;
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(PT) void k_particles(PArgs a) {
                 }
                 nl++;
             }
-            a.alive[i] = s.kind == 0 && !(s.p.y > (float)a.H);  // erase_if(y > height), world.cpp:2183
+            a.alive[i] = s.kind == 0 && !(s.p.y > (float)a.H);  // erase_if(y > height), world.cpp:2190
             if (s.kind == 0) a.p[i] = s.p;
         }
         if (t == 0) next = (int)min((uint32_t)fb, a.n - base);
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(256) void k_ph_apply(ApplyArgs a) {
                 }
             }
         }
-        a.alive[i] = r.kind == 0 && !(r.p.y > (float)a.H);  // erase_if(y > height), world.cpp:2183
+        a.alive[i] = r.kind == 0 && !(r.p.y > (float)a.H);  // erase_if(y > height), world.cpp:2190
         if (r.kind == 0) a.p[i] = r.p;
     }
     for (int off = 32; off > 0; off >>= 1) nl += __shfl_xor(nl, off, 64);

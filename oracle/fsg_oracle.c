@@ -144,9 +144,11 @@ static inline uint64_t transform(const props *P, uint64_t c, int m) {
  * Random bits: r[0:3]/r[4:7] sand vertical skip L/R, r[8]/r[9] MOVED on fall
  * L/R, r[10:11]/r[12:13] nudge L/R, r[14]/r[15] gas rise skip L/R,
  * r[16:19]/r[20:23] stopped sand unsticks TL/TR;
+ * r[24:27]/r[28:31] a SOUP cell facing away turns, top/bottom row;
  * s[0:4]/s[5:9] slide refused L/R, s[10] slide side choice, s[11:19] MOVED
  * cleared after a slide, s[20:21] slide nudge, s[22] gas diagonal choice,
- * s[23]/s[24] horizontal swap top/bottom row. */
+ * s[23]/s[24] gas spreads in the top/bottom row.
+ * MOVED is sand's sliding flag and a SOUP cell's heading. */
 static int block_rule(const props *P, const fsg_config *cfg, int it, uint32_t r, uint32_t s,
                       uint64_t q[4]) {
     const uint64_t o0 = q[0], o1 = q[1], o2 = q[2], o3 = q[3];
@@ -303,16 +305,33 @@ static int block_rule(const props *P, const fsg_config *cfg, int it, uint32_t r,
 
     /* --- H: horizontal moves, top row then bottom row -----------------
      * SOUP flows sideways into a lighter non-solid cell (the discrete SOUP
-     * rule, world.cpp:1839-1861) and GAS spreads into air (1862-1882), each
-     * with p=1/2 so that the side is unbiased. */
+     * rule, world.cpp:1839-1861) and GAS spreads into air (1862-1882).
+     * A SOUP cell keeps a heading in its MOVED bit (set: right, clear:
+     * left).  Facing the cell it can flow into, it moves; facing away, it
+     * turns and moves with p=1/16 (r[24:27] top row, r[28:31] bottom row).
+     * The reference's scan moves a liquid front every pass, toward the one
+     * side that is free (world.cpp:1839-1861 in bottom-up, left-to-right
+     * order); a Margolus pair seen twice in a row would undo a plain swap,
+     * and a fair coin per pair makes the front a random walk, so the heading
+     * is what carries a front forward.  GAS spreads with p=1/2
+     * (s[23]/s[24]). */
     for (int row = 0; row < 2; row++) {
         const int i = 2 * row, j = i + 1;
         const int ml = mid(q[i]), mr = mid(q[j]);
-        const int f = pass_rank(P, mr) < flow_rank(P, ml, it) ||
-                      pass_rank(P, ml) < flow_rank(P, mr, it) ||
-                      (gas_act(P, ml, it) && is_air(P, mr)) ||
-                      (gas_act(P, mr, it) && is_air(P, ml));
-        if (f && ((s >> (23 + row)) & 1u)) SWAP(q[i], q[j]);
+        const int fl = pass_rank(P, mr) < flow_rank(P, ml, it); /* left flows right */
+        const int fr = pass_rank(P, ml) < flow_rank(P, mr, it); /* right flows left */
+        if (fl || fr) {
+            const int facing = fl ? (q[i] & MOVED) != 0 : (q[j] & MOVED) == 0;
+            const int turn = ((r >> (24 + 4 * row)) & 15u) == 0;
+            if (facing || turn) {
+                if (fl) q[i] |= MOVED;
+                else q[j] &= ~MOVED;
+                SWAP(q[i], q[j]);
+            }
+        } else {
+            const int g = (gas_act(P, ml, it) && is_air(P, mr)) || (gas_act(P, mr, it) && is_air(P, ml));
+            if (g && ((s >> (23 + row)) & 1u)) SWAP(q[i], q[j]);
+        }
     }
     return CHANGED();
 #undef CHANGED
@@ -466,7 +485,7 @@ int orc_run(uint64_t *cells, int W, int H, const fsg_config *cfg, uint64_t seed,
 /* ----------------------------------------------------------------------- */
 /* A block can only change if it holds a pair of cells that some rule could
  * act on (vertical, diagonal or horizontal) or a single cell that can change
- * by itself (a set MOVED flag, burning wood).  The iteration gate is taken
+ * by itself (a sand cell's set MOVED flag, burning wood).  The iteration gate is taken
  * at it = 0, where every material with iterations > 0 is active. */
 static int pair_v(const props *P, const fsg_config *cfg, uint64_t t, uint64_t u) {
     const int mt = mid(t), mu = mid(u);
@@ -494,7 +513,7 @@ int orc_region_unstable(const uint64_t *cells, int W, int H, const fsg_config *c
     for (int y = y0; y < y1; y++) {
         for (int x = x0; x < x1; x++) {
             const uint64_t c = cells[(size_t)y * W + x];
-            if (c & MOVED) return 1;
+            if ((c & MOVED) && is_sand(&P, mid(c))) return 1; /* a SOUP cell's heading alone changes nothing */
             if (cfg->reactions && mid(c) == cfg->mat_wood && temp_of(c) > cfg->wood_burn_temp)
                 return 1;
             if (x + 1 < x1 && pair_h(&P, c, cells[(size_t)y * W + x + 1])) return 1;

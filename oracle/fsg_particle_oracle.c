@@ -101,6 +101,9 @@ int orc_particles_tick_zone(uint64_t *cells, int W, int H, const fsg_config *cfg
                 int ok = 0;
                 for (int s = 0; s < nspiral; s++) {
                     const int cx = (int)(c.x + (float)spiral[2 * s]), cy = (int)(c.y + (float)spiral[2 * s + 1]);
+                    /* the reference indexes real_tiles here without a bounds
+                     * check (world.cpp:2126); offsets outside the world are
+                     * skipped instead of read out of range */
                     if (cx < 0 || cy < 0 || cx >= W || cy >= H) continue;
                     const size_t ci = (size_t)cx + (size_t)cy * (size_t)W;
                     if (cell_type(cells, cfg, ci) == FSG_AIR) {
@@ -126,7 +129,7 @@ int orc_particles_tick_zone(uint64_t *cells, int W, int H, const fsg_config *cfg
         if (!done && c.lifetime > 0) c.lifetime--;
         p[out++] = c;
     }
-    /* erase_if(cells, y > height) (world.cpp:2183) */
+    /* erase_if(cells, y > height) (world.cpp:2190) */
     size_t m = 0;
     for (size_t i = 0; i < out; i++)
         if (!(p[i].y > (float)H)) p[m++] = p[i];

@@ -115,11 +115,44 @@ def test_block_slide_three_cycle(orc):
     assert [(v >> 16) & 0xFFFF for v in q] == [2, 4, 3, 1]
 
 
-def test_block_horizontal_flow_is_a_fair_coin(orc):
+def test_block_liquid_flows_along_its_heading(orc):
+    """A SOUP cell's MOVED bit is its heading (set: right).  Facing the air
+    beside it, it flows whatever the draw; facing away it turns only when
+    its row's 4 turn bits (r[24:27] top, r[28:31] bottom) are all zero, and
+    then carries the new heading."""
     cfg = orc.default_config()
-    q0 = [cell(STONE), cell(STONE), cell(WATER), cell(AIR)]
-    moved = [orc.block_rule(cfg, 0, 0, s << 24, q0)[1][3] & 0xFF for s in (0, 1)]
-    assert moved == [AIR, WATER]
+    NOTURN = 0x11000000  # one bit set in each row's turn field
+    # bottom row: water left, air right
+    right = [cell(STONE), cell(STONE), cell(WATER, moved=True), cell(AIR)]
+    _, q = orc.block_rule(cfg, 0, NOTURN, 0, right)
+    assert [v & 0xFF for v in q] == [STONE, STONE, AIR, WATER] and q[3] & 0x100
+    left = [cell(STONE), cell(STONE), cell(WATER), cell(AIR)]
+    ch, q = orc.block_rule(cfg, 0, NOTURN, 0xFFFFFFFF, left)
+    assert not ch  # heading left, away from the air: stays
+    _, q = orc.block_rule(cfg, 0, 0x01000000, 0, left)  # bottom row's turn bits zero
+    assert [v & 0xFF for v in q] == [STONE, STONE, AIR, WATER] and q[3] & 0x100  # turned right and moved
+    # top row: air left, water right heading left: flows; its heading stays left
+    top = [cell(AIR), cell(WATER), cell(STONE), cell(STONE)]
+    _, q = orc.block_rule(cfg, 0, NOTURN, 0, top)
+    assert [v & 0xFF for v in q] == [WATER, AIR, STONE, STONE] and not q[0] & 0x100
+    # gas still spreads with the s[23]/s[24] coins
+    gas = [cell(STONE), cell(STONE), cell(STEAM), cell(AIR)]
+    moved = [orc.block_rule(cfg, 0, 0, s << 24, gas)[1][3] & 0xFF for s in (0, 1)]
+    assert moved == [AIR, STEAM]
+
+
+def test_dam_break_front_advances(orc):
+    """A wide column of water released against the left wall of an empty
+    basin: the front carries its heading across the floor, so within 300
+    ticks water reaches beyond the middle of the basin; a fair coin per pair
+    (a random walk of the front) would still be near the dam."""
+    cfg = orc.default_config(temp_every=0)
+    W, H = 192, 96
+    g = box(W, H)
+    g[20:H - 1, 1:40] = cell(WATER)
+    orc.run(g, cfg, 1, 0, 300)
+    cols = np.nonzero((mats(g) == WATER).any(axis=0))[0]
+    assert cols.max() > W // 2
 
 
 # --- small worlds ---------------------------------------------------------------

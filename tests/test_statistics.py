@@ -15,6 +15,14 @@ W1 <= 0.02 H and pile heights within 15% on a poured-column pile after 2000
 ticks.  Each test also checks that the tolerance rejects a plausible wrong
 model: sand that only falls straight down (its column heights are those of
 the initial world), and sand that never comes unstuck (a steep tower).
+
+Liquids and gas: oil and water mixed in a basin layer the same way (oil
+above every water cell of its column in both, the mean depth gap within
+0.01 H); steam released at the floor of a closed box gathers under the lid
+the same way (W1 of the steam rows <= 0.01 H); a water column released in a
+basin ends flat in both (column heights within one cell), the deterministic
+tick about six times later: the reference's bottom-up scan lets a column of
+water fall by a cell in one pass, a Margolus phase moves each cell once.
 """
 import numpy as np
 import pytest
@@ -95,3 +103,76 @@ def test_reacting_world_conserves_like_sequential_tick(orc):
     # the reactions did fire in both
     assert nd[LAVA] < n0[LAVA] and nr[LAVA] < n0[LAVA]
     assert nd[STEAM] > 0 and nr[STEAM] > 0
+
+
+def basin(W, H):
+    c = np.full((H, W), fsg.make_cell(fsg.MAT_AIR), dtype=np.uint64)
+    c[H - 1, :] = fsg.make_cell(STONE)
+    c[:, 0] = fsg.make_cell(STONE)
+    c[:, W - 1] = fsg.make_cell(STONE)
+    return c
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_oil_and_water_layer_like_sequential_tick(orc, seed):
+    W, H = 128, 96
+    rng = np.random.default_rng(seed)
+    c0 = basin(W, H)
+    mix = rng.random((50, W - 2)) < 0.4
+    c0[H - 51:H - 1, 1:W - 1] = np.where(mix, fsg.make_cell(OIL), fsg.make_cell(WATER)).astype(np.uint64)
+    cfg = orc.default_config(temp_every=0)
+    det = orc.run(c0.copy(), cfg, seed, 0, 300)
+    ref = orc.refseq(c0.copy(), cfg, seed, 300, 8)
+
+    def gap(c):  # mean depth of water below mean depth of oil, in H
+        yw, _ = np.nonzero((c & 0xFF) == WATER)
+        yo, _ = np.nonzero((c & 0xFF) == OIL)
+        return (yw.mean() - yo.mean()) / H
+
+    def oil_on_top(c):  # oil cells above every water cell of their column
+        m = c & 0xFF
+        top = [(np.nonzero(m[:, x] == OIL)[0] < np.nonzero(m[:, x] == WATER)[0].min()).sum() for x in range(1, W - 1)]
+        return sum(top) / int((m == OIL).sum())
+
+    assert oil_on_top(det) == oil_on_top(ref) == 1.0
+    assert abs(gap(det) - gap(ref)) <= 0.01, (gap(det), gap(ref))
+    # the unsorted mix is rejected
+    assert abs(gap(c0) - gap(ref)) > 0.2 and oil_on_top(c0) < 0.1
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_steam_gathers_like_sequential_tick(orc, seed):
+    W, H = 128, 128
+    rng = np.random.default_rng(seed)
+    c0 = basin(W, H)
+    c0[0, :] = fsg.make_cell(STONE)
+    puffs = rng.random((20, W - 2)) < 0.5
+    c0[H - 21:H - 1, 1:W - 1] = np.where(puffs, fsg.make_cell(STEAM), fsg.make_cell(fsg.MAT_AIR)).astype(np.uint64)
+    cfg = orc.default_config(temp_every=0)
+    det = orc.run(c0.copy(), cfg, seed, 0, 300)
+    ref = orc.refseq(c0.copy(), cfg, seed, 300, 8)
+    rows = [np.nonzero((c & 0xFF) == STEAM)[0] for c in (c0, det, ref)]
+    assert len(rows[1]) == len(rows[2]) == len(rows[0])
+    assert w1(rows[1], rows[2], H) <= 0.01, w1(rows[1], rows[2], H)
+    assert w1(rows[0], rows[2], H) > 0.5  # steam left where it started is rejected
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_water_levels_like_sequential_tick(orc, seed):
+    W, H = 96, 48
+    c0 = basin(W, H)
+    c0[10:H - 1, 1:25] = fsg.make_cell(WATER)
+    cfg = orc.default_config(temp_every=0)
+    ref = orc.refseq(c0.copy(), cfg, seed, 1000, 8)
+    det = orc.run(c0.copy(), cfg, seed, 0, 500)
+    hr = column_heights(ref, WATER)[1:-1]
+    assert hr.max() - hr.min() <= 1  # the sequential tick is flat by 1000 ticks (by 500 in practice)
+    hd = column_heights(det, WATER)[1:-1]
+    assert hd.max() - hd.min() > 1   # the deterministic tick is not yet (the declared difference)
+    orc.run(det, cfg, seed, 500, 3500)
+    hd = column_heights(det, WATER)[1:-1]
+    assert hd.max() - hd.min() <= 1 and w1(hd, hr, H) <= 1.0 / H
+    # water that never flows sideways keeps the column: rejected
+    h0 = column_heights(c0, WATER)[1:-1]
+    assert w1(h0, hr, H) > 0.1
+
