@@ -563,7 +563,8 @@ def main():
     ap.add_argument("--side", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
-    ap.add_argument("--mode", default="all", choices=["all", "c3", "copy", "c5", "render", "islands", "pack", "mesh", "particles"],
+    ap.add_argument("--mode", default="all", choices=["all", "c1", "c2", "c3", "c4", "copy", "c5", "render", "islands", "pack",
+                                                      "mesh", "particles"],
                     help="all: the JSON line; c3: only the timed tick loop (for profiling); "
                          "copy: the HBM copy calibration kernel; render: only the texture pass")
     ap.add_argument("--copy-bytes", type=int, default=2 << 30)
@@ -585,6 +586,9 @@ def main():
         return
     if args.mode == "c5":
         print(json.dumps(bench_c5(fsg, rates=(args.rate,)) if args.rate > 0 else bench_c5(fsg)))
+        return
+    if args.mode in ("c1", "c2", "c4"):
+        print(json.dumps({"c1": bench_c1, "c2": bench_c2, "c4": bench_c4}[args.mode](fsg)))
         return
     if args.mode == "islands":
         print(json.dumps(bench_islands(fsg)))
