@@ -64,11 +64,12 @@ __device__ __forceinline__ int temp_of(const MatTab &M, const int32_t *addt, int
 // pair of cells of the LDS region (the chunk and its HALO ring) take part in
 // a rule under some random draw, can any cell change alone, or is any
 // temperature stencil of the chunk non-zero?
+template <int N>
 __device__ bool tile_unstable(const MatTab &M, const uint32_t (*L)[LW], const uint32_t *T0, const int32_t *addt,
                               bool temp_enabled) {
     bool u = false;
     constexpr uint32_t KV = K_SINK | K_GRISE | K_REACT, KD = K_SINK | K_GRISE;
-    for (int i = threadIdx.x; i < LW * LH; i += NT) {
+    for (int i = threadIdx.x; i < LW * LH; i += N) {
         const int ly = i / LW, lx = i - ly * LW;
         const uint32_t c = L[ly][lx];
         const uint32_t t = cid(c) << 4;
@@ -81,7 +82,7 @@ __device__ bool tile_unstable(const MatTab &M, const uint32_t (*L)[LW], const ui
         }
     }
     if (temp_enabled) {
-        for (int i = threadIdx.x; i < TW * TH; i += NT) {
+        for (int i = threadIdx.x; i < TW * TH; i += N) {
             const int ly = HALO + i / TW, lx = HALO + (i & (TW - 1));
             const uint32_t c = L[ly][lx];
             if (cid(c) == WALL) continue;
@@ -94,8 +95,10 @@ __device__ bool tile_unstable(const MatTab &M, const uint32_t (*L)[LW], const ui
     return u;
 }
 
-// Blocks a wave handles in one phase: at most ceil(36 * 36 / NT) rounds of 64.
-constexpr int QCAP = ((36 * 36 + NT - 1) / NT) * 64;
+// Blocks a wave handles in one phase: at most ceil(36 * 36 / N) rounds of 64,
+// for a workgroup of N threads.
+template <int N>
+constexpr int qcap() { return ((36 * 36 + N - 1) / N) * 64; }
 
 // One Margolus phase over the blocks whose cells the rest of the tick still
 // reads: after this phase the tick needs `need` more rings around the chunk
@@ -120,7 +123,7 @@ struct ZoneBlocks {
     uint32_t w, h;
 };
 
-template <bool TRACK, bool REACT, bool ZONE>
+template <int N, bool TRACK, bool REACT, bool ZONE>
 __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW], const uint32_t *T, int it,
                                               uint64_t key, int ox, int oy, int need, int gx0, int gy0,
                                               bool has_wood, uint16_t *queue, const uint32_t *perm,
@@ -144,9 +147,10 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
         return (uint32_t)(lx0 + 2 * bx - zb.x0) < zb.w && (uint32_t)(ly0 + 2 * by - zb.y0) < zb.h;
     };
     const int lane = threadIdx.x & 63;
+    constexpr int QCAP = qcap<N>();
     uint16_t *Q = queue + (threadIdx.x >> 6) * QCAP;
     int nplain = 0, nother = 0;  // wave-uniform
-    for (int b0 = threadIdx.x & ~63; b0 < nb; b0 += NT) {
+    for (int b0 = threadIdx.x & ~63; b0 < nb; b0 += N) {
         const int b = b0 + lane;
         bool act = false, plain = false;
         if (b < nb) {
@@ -254,20 +258,20 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
     return v;
 }
 
-// Final compact words of a thread's 2 x 8 output cells (columns lx, lx + 1,
-// rows ly0 .. ly0 + 7), with the temperature step applied when temp_now; the
+// Final compact words of a thread's 2 x ROWS output cells (columns lx, lx + 1,
+// rows ly0 .. ly0 + ROWS - 1), with the temperature step applied when temp_now; the
 // stencil slides down the strip.  With TCHG, returns whether a temperature
 // changed (only chunk skipping needs to know).  With ZONE, only cells of the
 // tick zone take the step: LDS rows [zr0, zr1), the thread's columns whose
 // bits (1: lx, 2: lx + 1) are set in zc.
-template <bool WALLS, bool TCHG = true, bool ZONE = false>
+template <int ROWS, bool WALLS, bool TCHG = true, bool ZONE = false>
 __device__ __forceinline__ bool final_words(const MatTab &M, const uint32_t (*L)[LW], const int32_t *addt,
-                                            bool temp_now, int lx, int ly0, uint32_t (&fw)[2 * (TH / 8)],
+                                            bool temp_now, int lx, int ly0, uint32_t (&fw)[2 * ROWS],
                                             int zr0 = 0, int zr1 = 0, uint32_t zc = 0) {
     bool tchg = false;
     if (!temp_now) {
 #pragma unroll
-        for (int r = 0; r < TH / 8; r++) {
+        for (int r = 0; r < ROWS; r++) {
             const uint2 c = *reinterpret_cast<const uint2 *>(&L[ly0 + r][lx]);
             fw[2 * r] = c.x;
             fw[2 * r + 1] = c.y;
@@ -281,21 +285,21 @@ __device__ __forceinline__ bool final_words(const MatTab &M, const uint32_t (*L)
         const uint32_t w0 = L[ly0][lx];
         uint32_t acc = 0;  // OR of the differences from w0; temperatures in bits 16-31
 #pragma unroll
-        for (int r = 0; r < TH / 8; r++) {
+        for (int r = 0; r < ROWS; r++) {
             const uint2 c = *reinterpret_cast<const uint2 *>(&L[ly0 + r][lx]);
             fw[2 * r] = c.x;
             fw[2 * r + 1] = c.y;
             acc |= (c.x ^ w0) | (c.y ^ w0) | (L[ly0 + r][lx - 1] ^ w0) | (L[ly0 + r][lx + 2] ^ w0);
         }
         const uint2 up = *reinterpret_cast<const uint2 *>(&L[ly0 - 1][lx]);
-        const uint2 dn = *reinterpret_cast<const uint2 *>(&L[ly0 + TH / 8][lx]);
+        const uint2 dn = *reinterpret_cast<const uint2 *>(&L[ly0 + ROWS][lx]);
         acc |= (up.x ^ w0) | (up.y ^ w0) | (dn.x ^ w0) | (dn.y ^ w0);
         if (__all((acc >> 16) == 0)) return false;
     }
     uint2 up = *reinterpret_cast<const uint2 *>(&L[ly0 - 1][lx]);
     uint2 cur = *reinterpret_cast<const uint2 *>(&L[ly0][lx]);
 #pragma unroll
-    for (int r = 0; r < TH / 8; r++) {
+    for (int r = 0; r < ROWS; r++) {
         const uint2 dn = *reinterpret_cast<const uint2 *>(&L[ly0 + r + 1][lx]);
         const uint32_t cl = L[ly0 + r][lx - 1], cr = L[ly0 + r][lx + 2];
         const int t0 = ctemp(cur.x), t1 = ctemp(cur.y);
@@ -321,16 +325,21 @@ __device__ __forceinline__ bool final_words(const MatTab &M, const uint32_t (*L)
 }
 
 // Region map of the load and of the colour plane.  The region is 72 rows of
-// 36 two-cell pairs.  Thread t < 252 owns pair column t % 36 of the rows
-// 1 + t / 36 + 7k (k < 10), which are rows 1 .. 70: every row a cell of the
+// 36 two-cell pairs.  At 256 threads, thread t < 252 owns pair column t % 36 of
+// the rows 1 + t / 36 + 7k (k < 10), which are rows 1 .. 70: every row a cell of the
 // chunk can come from in one tick (MAX_IT rows at most), so these are also
 // the colours the thread holds.  Threads t < 72 load the two edge rows 0
 // and 71 besides.  The rows of one thread are 7 apart, so its addresses
 // differ by constants: the address arithmetic is done once, not per pair.
-constexpr int PR = LW / 2;         // two-cell pairs per region row
-constexpr int RS = NT / PR;        // rows one sweep of the workgroup covers (7)
-constexpr int KC = (LH - 2) / RS;  // sweeps over rows 1 .. LH-2 (10)
-static_assert(RS * KC == LH - 2 && HALO - MAX_IT >= 1, "sweeps cover rows 1 .. LH-2 exactly");
+// (For N = 512 threads: 14 rows a sweep, 5 sweeps; for 1024: 28 rows a sweep,
+// and the third sweep covers the last 14 rows only.)
+constexpr int PR = LW / 2;                         // two-cell pairs per region row
+template <int N> constexpr int rows_per_sweep() { return N / PR; }                // 7 at 256 threads
+template <int N> constexpr int sweeps() {                                         // 10 at 256 threads
+    return (LH - 2 + rows_per_sweep<N>() - 1) / rows_per_sweep<N>();
+}
+template <int N> constexpr bool whole_sweeps() { return rows_per_sweep<N>() * sweeps<N>() == LH - 2; }
+static_assert(whole_sweeps<256>() && whole_sweeps<512>() && HALO - MAX_IT >= 1, "sweeps cover rows 1 .. LH-2 exactly");
 static_assert((uint64_t)FSG_MAX_SIDE * 8u * LH < (1ull << 32), "32-bit byte offsets within a region");
 
 // The tick zone against a workgroup's region [gx0, gx0 + LW) x [gy0, gy0 + LH):
@@ -374,10 +383,16 @@ __device__ __forceinline__ bool chunk_awake(const DevWorld &w, const uint32_t *f
 // SK == 3: skipping on, and the chunk is known to be awake (it was taken from
 // the awake list k_wake built).  ZN: the tick zone is tested (1), or the
 // world's zone covers it and no zone code is compiled in (0).
-template <int SK, int RE, int ZN>
+template <int SK, int RE, int ZN, int N = NT>
 __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, int tile, uint32_t (*L)[LW],
                                             uint32_t (*lut)[256], uint32_t *rgba, int32_t *addt,
                                             uint16_t *queue, uint32_t *red) {
+    static_assert(N == 256 || N == 512 || N == 1024, "the load map and the write-back strips");
+    constexpr int RS = rows_per_sweep<N>(), KC = sweeps<N>();
+    // sweep k of this thread's rows lies in rows 1 .. LH-2 (always, unless the
+    // last sweep is partial)
+    auto row_in = [&](int k) { return whole_sweeps<N>() || (int)(threadIdx.x / PR) + RS * k < LH - 2; };
+    constexpr int ROWS = TH * 32 / N;  // rows of a thread's write-back strip (8 at 256 threads)
     const int tid = threadIdx.x;
     const MatTab &M = w.mat;
     const bool react = RE == 2 ? M.reactions != 0 : RE == 1;
@@ -415,9 +430,10 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     // workgroup waits for one memory latency, not one per loop trip.
     const int ci = (int)M.cell_iter;
     uint32_t pv[MAX_IT];
+    const bool tab = N == 256 || tid < 256;  // one pair-table entry per thread
 #pragma unroll
     for (int k = 0; k < MAX_IT; k++)
-        if (k < ci) pv[k] = w.pair[k][tid];
+        if (k < ci && tab) pv[k] = w.pair[k][tid];
     uint32_t mrgba = 0;
     int32_t madd = 0;
     if (tid < 16) {
@@ -439,7 +455,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         if (mine) {
 #pragma unroll
             for (int k = 0; k < KC; k++)
-                v[k] = *reinterpret_cast<const uint4 *>(src + (off + (uint32_t)(RS * k) * rowb));
+                if (row_in(k)) v[k] = *reinterpret_cast<const uint4 *>(src + (off + (uint32_t)(RS * k) * rowb));
         }
         if (edge) ve = *reinterpret_cast<const uint4 *>(src + (uint32_t)ey * rowb + 16u * (uint32_t)ec2);
     } else {
@@ -453,7 +469,8 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         };
         if (mine) {
 #pragma unroll
-            for (int k = 0; k < KC; k++) v[k] = load(1 + rr + RS * k, c2);
+            for (int k = 0; k < KC; k++)
+                if (row_in(k)) v[k] = load(1 + rr + RS * k, c2);
         }
         if (edge) ve = load(ey, ec2);
     }
@@ -466,6 +483,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         const uint32_t orgx = ((uint32_t)(2 * c2) & 7u) << 9, oy = (uint32_t)(1 + rr) << 12;
 #pragma unroll
         for (int k = 0; k < KC; k++) {
+            if (!row_in(k)) continue;
             const uint32_t org = orgx | ((oy + ((uint32_t)(RS * k) << 12)) & 0x7000u);  // (row & 7) << 12
             *reinterpret_cast<uint2 *>(Lt + RS * k * LW) = make_uint2(v[k].x | org, v[k].z | org | (1u << 9));
             cola[k] = v[k].y;
@@ -479,14 +497,15 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     if (RE != 0 && react) {
         if (mine) {
 #pragma unroll
-            for (int k = 0; k < KC; k++) wood |= (v[k].x & 0xFFu) == M.wood || (v[k].z & 0xFFu) == M.wood;
+            for (int k = 0; k < KC; k++)
+                if (row_in(k)) wood |= (v[k].x & 0xFFu) == M.wood || (v[k].z & 0xFFu) == M.wood;
         }
         if (edge) wood |= (ve.x & 0xFFu) == M.wood || (ve.z & 0xFFu) == M.wood;
     }
 #pragma unroll
     for (int k = 0; k < MAX_IT; k++)
-        if (k < ci) lut[k][tid] = pv[k];
-    {
+        if (k < ci && tab) lut[k][tid] = pv[k];
+    if (tab) {
         // scatter offsets for mask y = tid: cell k goes to k ^ ((y >> 2k) & 3)
         uint32_t po = 0;
 #pragma unroll
@@ -510,7 +529,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         // the chunk did not change last tick: if nothing in its region can
         // change now it keeps its state, and both buffers already hold it
         // (the test ignores the tick zone: it may only wake too many)
-        const bool pot = __syncthreads_or(tile_unstable(M, L, lut[0], addt, M.temp_every != 0)) != 0;
+        const bool pot = __syncthreads_or(tile_unstable<N>(M, L, lut[0], addt, M.temp_every != 0)) != 0;
         if (!pot) {
             if (tid == 0) {
                 w.flags[tick & 1][tile] = 0;
@@ -528,43 +547,44 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
             const uint64_t key = splitmix64(seedmix ^ g);
             const int ox = kOX[g & 3], oy = kOY[g & 3];
             if (zt.blocks.on)  // on the zone's edge: one form for every setting (the bands may go unused)
-                bands |= run_phase<true, RE != 0, true>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood,
+                bands |= run_phase<N, true, RE != 0, true>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood,
                                                         queue, lut[MAX_IT], zt.blocks);
             else if (SK != 0 && skip)
-                bands |= run_phase<true, RE != 0, false>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood,
+                bands |= run_phase<N, true, RE != 0, false>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood,
                                                          queue, lut[MAX_IT], zt.blocks);
             else if (SK != 1 && SK != 3)
-                run_phase<false, RE != 0, false>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue,
+                run_phase<N, false, RE != 0, false>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue,
                                                  lut[MAX_IT], zt.blocks);
             __syncthreads();
         }
     }
 
-    // Write-back.  Thread = 2-column strip x 8 rows of the chunk.
+    // Write-back.  Thread = 2-column strip x ROWS rows of the chunk.
     const bool temp_now = !zt.dead && M.temp_every != 0 && (tick % M.temp_every) == M.temp_phase;
     const int j = tid & 31, rg = tid >> 5;
-    const int lx = HALO + 2 * j, ly0 = HALO + rg * (TH / 8);
-    uint32_t fw[2 * (TH / 8)];
+    const int lx = HALO + 2 * j, ly0 = HALO + rg * ROWS;
+    uint32_t fw[2 * ROWS];
     bool tchg;
     if (zt.blocks.on) {
         // on the zone's edge: cells outside it keep their temperature (the
         // wall-checking form serves interior chunks too)
         const int gx = gx0 + lx;
         const uint32_t zc = (gx >= w.zx0 && gx < w.zx1 ? 1u : 0u) | (gx + 1 >= w.zx0 && gx + 1 < w.zx1 ? 2u : 0u);
-        tchg = final_words<true, true, true>(M, L, addt, temp_now, lx, ly0, fw, w.zy0 - gy0, w.zy1 - gy0, zc);
+        tchg = final_words<ROWS, true, true, true>(M, L, addt, temp_now, lx, ly0, fw, w.zy0 - gy0, w.zy1 - gy0, zc);
     } else {
-        tchg = skip ? (interior ? final_words<false>(M, L, addt, temp_now, lx, ly0, fw)
-                                : final_words<true>(M, L, addt, temp_now, lx, ly0, fw))
-                    : (interior ? final_words<false, false>(M, L, addt, temp_now, lx, ly0, fw)
-                                : final_words<true, false>(M, L, addt, temp_now, lx, ly0, fw));
+        tchg = skip ? (interior ? final_words<ROWS, false>(M, L, addt, temp_now, lx, ly0, fw)
+                                : final_words<ROWS, true>(M, L, addt, temp_now, lx, ly0, fw))
+                    : (interior ? final_words<ROWS, false, false>(M, L, addt, temp_now, lx, ly0, fw)
+                                : final_words<ROWS, true, false>(M, L, addt, temp_now, lx, ly0, fw));
     }
-    if (skip && tchg) bands |= band_bits(2 * j, 2 * j + 1, rg * (TH / 8), rg * (TH / 8) + TH / 8 - 1);
+    if (skip && tchg) bands |= band_bits(2 * j, 2 * j + 1, rg * ROWS, rg * ROWS + ROWS - 1);
     __syncthreads();
     // the colour plane: each cell's colour at its origin (rows 1 .. LH-2)
     if (mine) {
         uint32_t *Lt = &L[1 + rr][2 * c2];
 #pragma unroll
         for (int k = 0; k < KC; k++) {
+            if (!row_in(k)) continue;
             Lt[RS * k * LW] = cola[k];
             Lt[RS * k * LW + 1] = colb[k];
         }
@@ -596,7 +616,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         const uint32_t rowb = (uint32_t)w.W * 8u;
         const uint32_t voff = (uint32_t)(ly0 - HALO) * rowb + 16u * (uint32_t)j;
 #pragma unroll
-        for (int r = 0; r < TH / 8; r++) {
+        for (int r = 0; r < ROWS; r++) {
             const uint32_t a = fw[2 * r], b = fw[2 * r + 1];
             uint32_t ca, cb;
             if (xf_on) {
@@ -610,7 +630,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         }
     } else {
 #pragma unroll
-        for (int r = 0; r < TH / 8; r++) {
+        for (int r = 0; r < ROWS; r++) {
             const int gy = gy0 + ly0 + r;
 #pragma unroll
             for (int c = 0; c < 2; c++) {
@@ -628,7 +648,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
         __syncthreads();
         if (tid == 0) {
             uint32_t all = 0;
-            for (int k = 0; k < NT / 64; k++) all |= red[k];
+            for (int k = 0; k < N / 64; k++) all |= red[k];
             w.flags[tick & 1][tile] = (zt.dead ? 0u : F_POT) | all;  // a dead chunk cannot change by itself
             atomicAdd(w.processed, 1ull);
         }
@@ -645,10 +665,31 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
     uint32_t(*L)[LW] = reinterpret_cast<uint32_t(*)[LW]>(Lmem);
     uint32_t *rgba = Lmem + LH * LW;
     __shared__ int32_t addt[16];
-    __shared__ uint16_t queue[(NT / 64) * QCAP];
+    __shared__ uint16_t queue[(NT / 64) * qcap<NT>()];
     __shared__ uint32_t red[NT / 64];
     tile_update<SK, RE, ZN>(w, w.tick + k, blockIdx.x, L, lut, rgba, addt, queue, red);
 }
+// Small worlds (at most one chunk per compute unit): the same update with
+// SMALL_NT (1024) threads per chunk.  A small world leaves compute units idle,
+// so a tick lasts one workgroup's latency; four times the waves per chunk cut
+// each wave's share of the phases and let four waves per SIMD hide each
+// other's LDS latency.
+template <int RE, int ZN>
+__global__ __launch_bounds__(SMALL_NT) void k_tick_small(DevWorld w, uint32_t k) {
+    __shared__ __align__(16) uint32_t Lmem[LH * LW + 16];  // the region, then the 16 base colours
+    __shared__ uint32_t lut[MAX_IT + 1][256];               // pair tables + scatter offsets
+    uint32_t(*L)[LW] = reinterpret_cast<uint32_t(*)[LW]>(Lmem);
+    uint32_t *rgba = Lmem + LH * LW;
+    __shared__ int32_t addt[16];
+    __shared__ uint16_t queue[(SMALL_NT / 64) * qcap<SMALL_NT>()];
+    __shared__ uint32_t red[SMALL_NT / 64];
+    tile_update<0, RE, ZN, SMALL_NT>(w, w.tick + k, blockIdx.x, L, lut, rgba, addt, queue, red);
+}
+template __global__ void k_tick_small<0, 0>(DevWorld, uint32_t);
+template __global__ void k_tick_small<1, 0>(DevWorld, uint32_t);
+template __global__ void k_tick_small<0, 1>(DevWorld, uint32_t);
+template __global__ void k_tick_small<1, 1>(DevWorld, uint32_t);
+
 template __global__ void k_tick_world<0, 0, 0>(DevWorld, uint32_t);
 template __global__ void k_tick_world<0, 1, 0>(DevWorld, uint32_t);
 template __global__ void k_tick_world<0, 0, 1>(DevWorld, uint32_t);
@@ -706,7 +747,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
     uint32_t(*L)[LW] = reinterpret_cast<uint32_t(*)[LW]>(Lmem);
     uint32_t *rgba = Lmem + LH * LW;
     __shared__ int32_t addt[16];
-    __shared__ uint16_t queue[(NT / 64) * QCAP];
+    __shared__ uint16_t queue[(NT / 64) * qcap<NT>()];
     __shared__ uint32_t red[NT / 64];
     const uint64_t tick = w.tick + k;
     if (blockIdx.x >= w.awake_n[tick & 1]) return;
@@ -726,7 +767,7 @@ __device__ __forceinline__ void tick_batch_chunk(const DevWorld *__restrict__ wo
     uint32_t(*L)[LW] = reinterpret_cast<uint32_t(*)[LW]>(Lmem);
     uint32_t *rgba = Lmem + LH * LW;
     __shared__ int32_t addt[16];
-    __shared__ uint16_t queue[(NT / 64) * QCAP];
+    __shared__ uint16_t queue[(NT / 64) * qcap<NT>()];
     __shared__ uint32_t red[NT / 64];
     const uint32_t gt = blockIdx.x;
     int lo = 0, hi = nworlds - 1;
