@@ -611,6 +611,29 @@ def batch_step_timed(ctx: Context, worlds, ticks: int) -> float:
     return ms.value
 
 
+def lz4_decode_blocks(ctx: Context, blocks, guard: int = 0, fill: int = 0xA5):
+    """Decodes raw LZ4 blocks [(bytes, capacity), ...] on the device with the
+    unpack's decode kernel (fsg_lz4_decode_blocks).  Each block's output is
+    followed by `guard` bytes; all start as `fill`.  Returns [(size or -1,
+    output bytes incl. the guard), ...]."""
+    n = len(blocks)
+    src = b"".join(b for b, _ in blocks)
+    so = np.zeros(max(n, 1), np.int64)
+    sb = np.array([len(b) for b, _ in blocks] or [0], np.int32)
+    cap = np.array([c for _, c in blocks] or [0], np.int32)
+    so[1:n] = np.cumsum(sb[:n])[:-1] if n > 1 else so[1:n]
+    oo = np.zeros(max(n, 1), np.int64)
+    oo[1:n] = np.cumsum(cap[:n].astype(np.int64) + guard)[:-1] if n > 1 else oo[1:n]
+    total = int(cap[:n].astype(np.int64).sum()) + guard * n
+    out = np.full(max(total, 1), fill, np.uint8)
+    res = np.zeros(max(n, 1), np.int32)
+    sbuf = np.frombuffer(src, np.uint8) if src else np.zeros(1, np.uint8)
+    check(lib().fsg_lz4_decode_blocks(ctx.handle, sbuf.ctypes.data, len(src), so.ctypes.data, sb.ctypes.data,
+                                      cap.ctypes.data, n, out.ctypes.data, total, oo.ctypes.data, res.ctypes.data),
+          "fsg_lz4_decode_blocks")
+    return [(int(res[k]), out[oo[k]:oo[k] + cap[k] + guard].tobytes()) for k in range(n)]
+
+
 def perworld_step_timed(ctx: Context, worlds, ticks: int) -> float:
     ms = C.c_double()
     check(lib().fsg_perworld_step_timed(ctx.handle, _handles(worlds), len(worlds), ticks, C.byref(ms)),

@@ -168,6 +168,7 @@ SIGNATURES = {
     "fsg_world_unpack": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, P, P, P, P]),
     "fsg_world_set_pack_map": (C.c_int, [P, C.POINTER(PackMap)]),
     "fsg_pack_map_default": (C.c_int, [C.POINTER(Config), C.POINTER(PackMap)]),
+    "fsg_lz4_decode_blocks": (C.c_int, [P, P, C.c_size_t, P, P, P, C.c_int32, P, C.c_size_t, P, P]),
     "fsg_world_set_cells": (C.c_int, [P, P, C.c_size_t]),
     "fsg_world_set_cells_noalloc": (C.c_int, [P, P, C.c_size_t]),
     "fsg_world_step": (C.c_int, [P, C.c_int]),

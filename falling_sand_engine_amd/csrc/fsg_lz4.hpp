@@ -23,6 +23,10 @@
 // the host for tests and in the decode kernel, whose Mem (Staged) keeps a
 // window of the block and the recent output in LDS and the lookahead in the
 // lanes' registers: one LDS read per token, its bytes read across lanes.
+// Runs of short sequences (a few literals and a short match, one per record
+// of a noisy chunk) are decoded a group at a time (Staged::group): the lanes
+// parse every candidate start of the lookahead at once and the group's
+// output bytes are produced together, one per lane.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -30,6 +34,9 @@
 namespace fsg_lz4 {
 
 constexpr int MINMATCH = 4, LASTLITERALS = 5, MFLIMIT = 12, RUN_MASK = 15;
+
+__host__ __device__ inline int popc64(uint64_t x) { return __builtin_popcountll(x); }
+__host__ __device__ inline int clz64(uint64_t x) { return __builtin_clzll(x); }  // x != 0
 
 // Lane `lane` of `nl` copies bytes lane, lane + nl, ... of a literal run.
 __host__ __device__ inline void copy_lane(uint8_t *d, const uint8_t *s, int len, int lane, int nl) {
@@ -67,8 +74,15 @@ __host__ __device__ inline void match_lane(uint8_t *d, int off, int len, int lan
 // register on the device), lane_value(v, k, row) is lane k's v in every lane
 // and own(v, lane, row) this lane's v (row: the bytes the lanes hold, which
 // the host reads instead of the lanes' registers); mod(i) is match table
-// entry i (LDS on the device, filled before the decode).
+// entry i (LDS on the device, filled before the decode).  Lane vectors
+// (type V: one 32-bit value per lane, a register on the device, an array on
+// the host) serve the group decode: make(f) is f(lane) in every lane,
+// get(v, lane) a lane's value inside such an f, lane_of(v, k) lane k's value
+// as a uniform, ballot(v) the mask of lanes whose v is nonzero, shfl(v, i)
+// lane i's v in each lane, below(m, lane) the number of bits of m below the
+// lane's.
 constexpr int IW = 1024, RING = 4096, NL = 64;
+constexpr int GMAX = NL / MINMATCH;  // sequences in one group at most
 
 // The match table: entry off * NL + j is j mod off for 1 <= off < NL and
 // j < NL (MODTAB bytes; entry i is modtab_entry(i)).  A short-offset match
@@ -86,6 +100,7 @@ struct Staged {
     uint8_t *dst;        // the output
     uint8_t *win;        // IW bytes, 16-byte aligned
     uint8_t *ring;       // RING bytes
+    uint32_t *gdesc;     // GMAX words: the sequences of a group
     int wbase;           // the window holds block bytes [wbase, wbase + IW)
     Lanes L;
 
@@ -179,14 +194,95 @@ struct Staged {
             if (g + NL < len) L.barrier();
         }
     }
+    // A group of short sequences from the lookahead (block byte ip = at0 on):
+    // sequences with no length bytes, all of whose bytes lie in the
+    // lookahead, with a match offset in [1, min(op, RING - NL)], while their
+    // output fits NL bytes.  Lane s parses the sequence that would start at
+    // lookahead byte s; the real chain from byte 0 is then followed in
+    // uniform registers (H marks each sequence's first output byte, S its
+    // token byte).  Output lane j finds its sequence by counting the bits of
+    // H up to j, and its byte: a literal from the window, a match byte
+    // written before the group from the ring, or one of the group's own
+    // bytes, a lane p < j (the match source, op - off + (m mod off) for the
+    // m-th match byte, lies before the sequence's match).  Pointer jumping
+    // over p resolves the chains of such lanes.  The caller guarantees
+    // ip + NL + 6 <= n and op + NL + MFLIMIT <= cap, so no sequence here is
+    // the last one (its literals end at most ip + NL - 2 <= n - 8 and its
+    // output at most op + NL <= cap - MFLIMIT) and every match ends before
+    // cap - LASTLITERALS.  Returns false, consuming nothing, when the first
+    // sequence is not a short one.
+    __host__ __device__ bool group(int &ip, int &op) {
+        const int base = at0 - wbase, op0 = op;
+        // lane s: the sequence's literals << 5 | s << 9 | offset << 16 (desc)
+        // and its output length (127 when not short) | next token byte << 7 (link)
+        const auto desc = L.make([&](int s) -> uint32_t {
+            const int ll = (int)(L.own(ahead, s, win + base) >> 4), k = s + 1 + ll < NL - 2 ? s + 1 + ll : NL - 2;
+            const int off = (int)win[base + k] | (int)win[base + k + 1] << 8;
+            return (uint32_t)ll << 5 | (uint32_t)s << 9 | (uint32_t)off << 16;
+        });
+        const auto link = L.make([&](int s) -> uint32_t {
+            const uint32_t t = L.own(ahead, s, win + base);
+            const int ll = (int)(t >> 4), mc = (int)(t & 15u);
+            const int off = (int)(L.get(desc, s) >> 16);
+            const bool ok = ll != RUN_MASK && mc != RUN_MASK && s + 3 + ll <= NL && off != 0 && off <= op0 &&
+                            off <= RING - NL;
+            return (ok ? (uint32_t)(ll + mc + MINMATCH) : 127u) | (uint32_t)(s + 3 + ll) << 7;
+        });
+        uint64_t H = 0, S = 0;
+        int at = 0, tot = 0;
+        uint32_t lk = L.lane_of(link, 0);
+        for (int t2 = (int)(lk & 127u); t2 <= NL; t2 = tot + (int)(lk & 127u)) {
+            H |= 1ull << tot;
+            S |= 1ull << at;
+            tot = t2;
+            at = (int)(lk >> 7);
+            lk = L.lane_of(link, at & (NL - 1));
+            lk = at < NL ? lk : 127u;  // the next token is not in the lookahead
+        }
+        if (tot == 0) return false;
+        L.barrier();
+        L.each([&](int lane) {
+            if ((S >> lane) & 1u) gdesc[L.below(S, lane)] = L.get(desc, lane);
+        });
+        L.barrier();
+        // output lane j: 256 | byte when known, else (group lane of its source) << 9
+        auto w = L.make([&](int j) -> uint32_t {
+            const uint64_t upto = H & (~0ull >> (63 - j));  // bit 0 is set
+            const uint32_t d = gdesc[popc64(upto) - 1];
+            const int o = 63 - clz64(upto), ll = (int)((d >> 5) & 15u), s = (int)((d >> 9) & 63u);
+            const int off = (int)(d >> 16), rel = j - o, m = rel - ll;
+            // m mod off for the match bytes (m < 15 + MINMATCH < 63): the table row of min(off, 63)
+            const int mm = (int)L.mod((off < NL - 1 ? off : NL - 1) * NL + (m & (NL - 1)));
+            const int src = op0 + o + ll - off + mm;
+            const bool lit = rel < ll;
+            // one read for every lane (in bounds for lanes past the group too)
+            const uint8_t *p = lit ? win + (base + s + 1 + rel) : ring + (src & (RING - 1));
+            const uint32_t v = *p;
+            return j >= tot ? 256u : !lit && src >= op0 ? (uint32_t)(src - op0) << 9 : 256u | v;
+        });
+        while (L.ballot(L.make([&](int j) { return (L.get(w, j) & 256u) ^ 256u; }))) {
+            const auto got = L.shfl(w, L.make([&](int j) { return L.get(w, j) >> 9; }));
+            w = L.make([&](int j) { return (L.get(w, j) & 256u) ? L.get(w, j) : L.get(got, j); });
+        }
+        L.barrier();
+        L.each([&](int j) {
+            if (j < tot) put(op0 + j, (uint8_t)L.get(w, j));
+        });
+        op = op0 + tot;
+        ip = at0 + at;
+        return true;
+    }
 };
 
 // Decodes the LZ4 block of n bytes into an output of cap bytes.  Each token
 // starts with M.look(ip), which makes the next NL block bytes readable as
 // M.at(k); length bytes and offsets beyond the lookahead come from M.byte(i).
-// Literals go through M.lit, matches through M.match.  Returns the decoded
-// size, or -1 for a block that lz4.c's rules reject.  Reads only block bytes
-// [0, n) and writes only output bytes [0, cap).
+// Literals go through M.lit, matches through M.match.  Away from the block's
+// ends (ip + NL + 6 <= n, op + NL + MFLIMIT <= cap), M.group(ip, op) may
+// decode a run of short sequences in one step and advance ip and op past
+// them; the rules below then hold for each of its sequences by construction.
+// Returns the decoded size, or -1 for a block that lz4.c's rules reject.
+// Reads only block bytes [0, n) and writes only output bytes [0, cap).
 template <class Mem>
 __host__ __device__ inline int decode_block(int n, int cap, Mem &M) {
     if (n <= 0 || cap < 0) return -1;
@@ -195,6 +291,7 @@ __host__ __device__ inline int decode_block(int n, int cap, Mem &M) {
     for (;;) {
         if (ip >= n) return -1;
         M.look(ip);
+        if (ip + NL + 6 <= n && op + NL + MFLIMIT <= cap && M.group(ip, op)) continue;
         auto next = [&](int &k) -> int {  // the byte at ip + k, then k + 1
             const int v = (int)(k < NL ? M.at(k) : M.byte(ip + k));
             k++;

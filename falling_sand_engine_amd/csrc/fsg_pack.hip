@@ -378,10 +378,11 @@ __global__ __launch_bounds__(256) void k_pack_assemble(const uint8_t *slots, con
     for (int k = t; k < b; k += 256) o[HDR + a + k] = sb[k];
 }
 
-// Decoder input: per block, its bytes in the image buffer; ok = 0 skips it.
+// Decoder input: per block, its bytes in[off, off + n) (n < 0: no block, the
+// result is -1) and its output out[dst, dst + cap).
 struct DecDesc {
-    int64_t off;
-    int32_t n, ok;
+    int64_t off, dst;
+    int32_t n, cap;
 };
 
 // The match table of the decode kernel (LDS, filled by each workgroup before
@@ -411,38 +412,52 @@ struct WaveLane {
     }
     __device__ uint32_t own(uint32_t v, int, const uint8_t *) const { return v; }
     __device__ uint32_t mod(int i) const { return dec_modtab[i]; }
+    // lane vectors are the lane's own register; called with every lane active
+    using V = uint32_t;
+    template <class F>
+    __device__ V make(F f) const {
+        return f(lane);
+    }
+    __device__ uint32_t get(V v, int) const { return v; }
+    __device__ uint32_t lane_of(V v, int k) const { return (uint32_t)__builtin_amdgcn_readlane((int)v, k); }
+    __device__ uint64_t ballot(V v) const { return __builtin_amdgcn_ballot_w64(v != 0u); }
+    __device__ V shfl(V v, V i) const { return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(i << 2), (int)v); }
+    __device__ uint32_t below(uint64_t m, int) const {
+        return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    }
 };
 
 // One wave per block, four blocks per workgroup (a workgroup per wave would
 // leave CUs short of waves: their workgroup slots run out first).  Wave
-// v = blockIdx.x * 4 + wave of the 2n takes chunk v's tiles-and-layer2 block
-// (384 KiB decoded) for v < n and chunk v - n's background block (64 KiB)
-// after: a workgroup holds blocks of one kind, so its waves end together
-// (a mixed one stays resident until its long blocks end), and the long
-// blocks start first.  Each wave keeps its own block window and output ring
-// in LDS and decodes alone (no workgroup barrier); status[2c + which] is 0
-// when the block decoded to its full size.
+// v = blockIdx.x * 4 + wave takes block desc[v]; the unpack lists every
+// chunk's tiles-and-layer2 block (384 KiB decoded) before the background
+// blocks (64 KiB), so a workgroup holds blocks of one kind and its waves end
+// together (a mixed one stays resident until its long blocks end), and the
+// long blocks start first.  Each wave keeps its own block window and output
+// ring in LDS and decodes alone (no workgroup barrier); result[v] is the
+// decoded size, or -1 for a block lz4.c rejects.
 constexpr int DW = 4;  // waves (blocks) per decode workgroup
 
 __global__ __launch_bounds__(64 * DW) __attribute__((amdgpu_waves_per_eu(6))) void k_pack_decode(
-    const uint8_t *in, const DecDesc *desc, uint8_t *raw, int32_t *status, int nblocks) {
+    const uint8_t *in, const DecDesc *desc, uint8_t *out, int32_t *result, int nblocks) {
     __shared__ __align__(16) uint8_t win[DW][fsg_lz4::IW];
     __shared__ uint8_t ring[DW][fsg_lz4::RING];
+    __shared__ uint32_t gdesc[DW][fsg_lz4::GMAX];
     for (int i = threadIdx.x; i < fsg_lz4::MODTAB; i += 64 * DW) dec_modtab[i] = fsg_lz4::modtab_entry(i);
     __syncthreads();
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int v = blockIdx.x * DW + wv, n = nblocks >> 1;
+    // the wave's index as a uniform, so that the block's descriptor and the
+    // whole parse (positions, lengths, limits) live in scalar registers
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    const int v = blockIdx.x * DW + wv;
     if (v >= nblocks) return;
-    const int which = v >= n ? 1 : 0, c = v - which * n, b = 2 * c + which;
-    const DecDesc d = desc[b];
-    const int cap = which ? NB : NA;
+    const DecDesc d = desc[v];
     int r = -1;
-    if (d.ok) {
-        fsg_lz4::Staged<WaveLane> M{in + d.off, d.n, raw + (size_t)c * RAW + (which ? NA : 0), win[wv], ring[wv],
-                                    -(1 << 30), WaveLane{lane}, 0u, 0};
-        r = fsg_lz4::decode_block(d.n, cap, M);
+    if (d.n >= 0) {
+        fsg_lz4::Staged<WaveLane> M{in + d.off, d.n, out + d.dst, win[wv], ring[wv], gdesc[wv], -(1 << 30),
+                                    WaveLane{lane}, 0u, 0};
+        r = fsg_lz4::decode_block(d.n, d.cap, M);
     }
-    if (lane == 0) status[b] = r == cap ? 0 : -1;
+    if (lane == 0) result[v] = r;
 }
 
 // Little-endian u32 at a 4-byte aligned staging address (records are 12 or
@@ -456,6 +471,7 @@ struct UnpackDst {
     uint64_t *l2;     // region layer2 out (null: not kept)
     uint32_t *bg;     // region background out (null: not kept)
     uint32_t moved;   // MOVED taken from pad byte 10 (else clear)
+    int n;            // chunks: status[c] and status[n + c] are chunk c's decoded block sizes
     int16_t from_file[256];  // material id of each file index, or -1
 };
 
@@ -466,7 +482,7 @@ __global__ __launch_bounds__(256) void k_pack_cells(const uint8_t *raw, const in
                                                     int32_t *chunk_status) {
     __shared__ int bad;
     const int c = blockIdx.x, t = threadIdx.x, cx = c % d.ncx, cy = c / d.ncx;
-    if (status[2 * c] != 0 || status[2 * c + 1] != 0) {
+    if (status[c] != NA || status[d.n + c] != NB) {
         if (t == 0) chunk_status[c] = FSG_ERR_INVALID;
         return;
     }
@@ -645,6 +661,7 @@ extern "C" int fsg_world_unpack(fsg_world *w, int32_t x0, int32_t y0, int32_t nc
     fsg_ctx *c = w->ctx;
     HIPCHK(hipSetDevice(c->device));
     // descriptors of the 2n blocks from the headers (chunk.cpp:83-145)
+    // chunk k's blocks: desc[k] (tiles and layer2, NA bytes) and desc[n + k] (background, NB)
     std::vector<DecDesc> desc(2 * n);
     std::vector<int8_t> ph(n, 0);
     const uint8_t *src = nullptr;
@@ -653,19 +670,19 @@ extern "C" int fsg_world_unpack(fsg_world *w, int32_t x0, int32_t y0, int32_t nc
         const uint8_t *p = static_cast<const uint8_t *>(images);
         for (size_t k = 0; k < n; k++) {
             const size_t len = image_bytes[k];
-            DecDesc a{(int64_t)in_total + HDR, 0, 0}, b{0, 0, 0};
+            DecDesc a{(int64_t)in_total + HDR, (int64_t)(k * RAW), -1, NA}, b{0, (int64_t)(k * RAW + NA), -1, NB};
             if (len >= (size_t)HDR) {
                 ph[k] = (int8_t)p[in_total];
                 const int32_t sa = rd_i32(p + in_total + 1), ca = rd_i32(p + in_total + 5);
                 const int32_t sb = rd_i32(p + in_total + 9), cb = rd_i32(p + in_total + 13);
                 if (sa == NA && sb == NB && ca >= 0 && cb >= 0 && (size_t)HDR + (size_t)ca + (size_t)cb <= len) {
                     a.n = ca;
-                    a.ok = 1;
-                    b = DecDesc{a.off + ca, cb, 1};
+                    b.off = a.off + ca;
+                    b.n = cb;
                 }
             }
-            desc[2 * k] = a;
-            desc[2 * k + 1] = b;
+            desc[k] = a;
+            desc[n + k] = b;
             in_total += len;
         }
         if (grow(c, &w->d_pack_in, &w->pack_in_cap, std::max<size_t>(in_total, 1))) return FSG_ERR_NOMEM;
@@ -675,8 +692,8 @@ extern "C" int fsg_world_unpack(fsg_world *w, int32_t x0, int32_t y0, int32_t nc
         int64_t o = 0;
         for (size_t k = 0; k < n; k++) {
             const int32_t ca = w->pack_sizes[2 * k], cb = w->pack_sizes[2 * k + 1];
-            desc[2 * k] = DecDesc{o + HDR, ca, 1};
-            desc[2 * k + 1] = DecDesc{o + HDR + ca, cb, 1};
+            desc[k] = DecDesc{o + HDR, (int64_t)(k * RAW), ca, NA};
+            desc[n + k] = DecDesc{o + HDR + ca, (int64_t)(k * RAW + NA), cb, NB};
             ph[k] = w->pack_phase[k];
             o += HDR + ca + cb;
         }
@@ -701,6 +718,7 @@ extern "C" int fsg_world_unpack(fsg_world *w, int32_t x0, int32_t y0, int32_t nc
     d.bg = background ? reinterpret_cast<uint32_t *>(dec + b_desc + b_st + b_l2) : nullptr;
     const fsg_pack_map &map = world_map(w);
     d.moved = (map.flags & FSG_PACK_NO_MOVED) ? 0u : 1u;
+    d.n = (int)n;
     std::memcpy(d.from_file, map.from_file, sizeof(d.from_file));
     HIPCHK(hipMemcpyAsync(d_desc, desc.data(), 2 * n * sizeof(DecDesc), hipMemcpyHostToDevice, c->stream));
     if (b_l2 + b_bg) HIPCHK(hipMemsetAsync(dec + b_desc + b_st, 0, b_l2 + b_bg, c->stream));  // failed chunks: 0
@@ -717,5 +735,46 @@ extern "C" int fsg_world_unpack(fsg_world *w, int32_t x0, int32_t y0, int32_t nc
     w->d.force_tick = w->d.tick;
     w->stamped = false;
     w->prev_valid = false;
+    return FSG_OK;
+}
+
+namespace {
+struct DeviceBuffer {  // freed on every return path
+    void *p = nullptr;
+    ~DeviceBuffer() {
+        if (p) (void)hipFree(p);
+    }
+};
+}  // namespace
+
+extern "C" int fsg_lz4_decode_blocks(fsg_ctx *c, const void *src, size_t src_len, const int64_t *src_off,
+                                     const int32_t *src_bytes, const int32_t *cap, int32_t n, void *out,
+                                     size_t out_len, const int64_t *out_off, int32_t *result) {
+    if (!c || n < 0) return FSG_ERR_INVALID;
+    if (n == 0) return FSG_OK;
+    if (!src || !src_off || !src_bytes || !cap || !out || !out_off || !result) return FSG_ERR_INVALID;
+    std::vector<DecDesc> desc((size_t)n);
+    for (int32_t k = 0; k < n; k++) {
+        if (src_off[k] < 0 || src_bytes[k] < 0 || (uint64_t)src_off[k] + (uint64_t)src_bytes[k] > src_len ||
+            cap[k] < 0 || out_off[k] < 0 || (uint64_t)out_off[k] + (uint64_t)cap[k] > out_len)
+            return FSG_ERR_INVALID;
+        desc[k] = DecDesc{src_off[k], out_off[k], src_bytes[k], cap[k]};
+    }
+    HIPCHK(hipSetDevice(c->device));
+    DeviceBuffer d_src, d_out, d_desc, d_res;
+    HIPCHK(hipMalloc(&d_src.p, std::max<size_t>(src_len, 1)));
+    HIPCHK(hipMalloc(&d_out.p, std::max<size_t>(out_len, 1)));
+    HIPCHK(hipMalloc(&d_desc.p, (size_t)n * sizeof(DecDesc)));
+    HIPCHK(hipMalloc(&d_res.p, (size_t)n * 4));
+    if (src_len) HIPCHK(hipMemcpyAsync(d_src.p, src, src_len, hipMemcpyHostToDevice, c->stream));
+    if (out_len) HIPCHK(hipMemcpyAsync(d_out.p, out, out_len, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_desc.p, desc.data(), (size_t)n * sizeof(DecDesc), hipMemcpyHostToDevice, c->stream));
+    k_pack_decode<<<(unsigned)((n + DW - 1) / DW), 64 * DW, 0, c->stream>>>(
+        static_cast<const uint8_t *>(d_src.p), static_cast<const DecDesc *>(d_desc.p),
+        static_cast<uint8_t *>(d_out.p), static_cast<int32_t *>(d_res.p), n);
+    HIPCHK(hipGetLastError());
+    if (out_len) HIPCHK(hipMemcpyAsync(out, d_out.p, out_len, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(result, d_res.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return FSG_OK;
 }

@@ -237,6 +237,17 @@ int fsg_world_set_pack_map(fsg_world *w, const fsg_pack_map *map);
 /* The default map of a config (identity on its n_materials ids). */
 int fsg_pack_map_default(const fsg_config *cfg, fsg_pack_map *map);
 
+/* Decodes n raw LZ4 blocks on the device with the decode kernel that
+ * fsg_world_unpack runs: block k is src[src_off[k], src_off[k] +
+ * src_bytes[k]) and decodes into out[out_off[k], out_off[k] + cap[k]);
+ * result[k] = the decoded size, or -1 for a block that LZ4_decompress_safe
+ * (lz4.c:1949-2360, as chunk.cpp:151 and 191 call it) rejects.  Bytes of out
+ * that no block writes keep their values.  A block check for callers and
+ * tests; ChunkRead's blocks go through fsg_world_unpack. */
+int fsg_lz4_decode_blocks(fsg_ctx *ctx, const void *src, size_t src_len, const int64_t *src_off,
+                          const int32_t *src_bytes, const int32_t *cap, int32_t n, void *out, size_t out_len,
+                          const int64_t *out_off, int32_t *result);
+
 /* setTile (world.cpp:1004-1008) batched: edits are applied in array order
  * before the next tick; out-of-bounds edits are ignored like setTile's. */
 typedef struct fsg_cell_edit {

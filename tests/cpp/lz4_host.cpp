@@ -3,8 +3,9 @@
 //   * fsg_test_lz4_decode: plain memory, each copy running the 64 lanes'
 //     shares one after another;
 //   * fsg_test_lz4_decode_staged: the kernel's staged access (block window
-//     and output ring, LDS on the device) with the 64 lanes taken in turn
-//     inside each step, as the device's lanes take each step together.
+//     and output ring, LDS on the device) and its group decode of short
+//     sequences, with the 64 lanes taken in turn inside each step, as the
+//     device's lanes take each step together.
 #include "fsg_lz4.hpp"
 
 namespace {
@@ -21,6 +22,28 @@ struct HostLanes {
     uint32_t lane_value(uint32_t, int k, const uint8_t *row) const { return row[k]; }
     uint32_t own(uint32_t, int lane, const uint8_t *row) const { return row[lane]; }
     uint32_t mod(int i) const { return fsg_lz4::modtab_entry(i); }
+    struct V {
+        uint32_t v[fsg_lz4::NL];
+    };
+    template <class F>
+    V make(F f) const {
+        V r;
+        for (int lane = 0; lane < fsg_lz4::NL; lane++) r.v[lane] = f(lane);
+        return r;
+    }
+    uint32_t get(const V &x, int lane) const { return x.v[lane]; }
+    uint32_t lane_of(const V &x, int k) const { return x.v[k]; }
+    uint64_t ballot(const V &x) const {
+        uint64_t m = 0;
+        for (int lane = 0; lane < fsg_lz4::NL; lane++) m |= (uint64_t)(x.v[lane] != 0) << lane;
+        return m;
+    }
+    V shfl(const V &x, const V &i) const {
+        V r;
+        for (int lane = 0; lane < fsg_lz4::NL; lane++) r.v[lane] = x.v[i.v[lane] & (fsg_lz4::NL - 1)];
+        return r;
+    }
+    uint32_t below(uint64_t m, int lane) const { return (uint32_t)fsg_lz4::popc64(m & ((1ull << lane) - 1)); }
 };
 
 struct AllLanes {
@@ -37,6 +60,7 @@ struct AllLanes {
     void match(int op, int off, int len) {
         for (int l = 0; l < fsg_lz4::NL; l++) fsg_lz4::match_lane(dst + op, off, len, l, fsg_lz4::NL);
     }
+    bool group(int &, int &) { return false; }  // one sequence at a time
 };
 
 }  // namespace
@@ -49,6 +73,7 @@ extern "C" int fsg_test_lz4_decode(const uint8_t *src, int n, uint8_t *dst, int 
 extern "C" int fsg_test_lz4_decode_staged(const uint8_t *src, int n, uint8_t *dst, int cap) {
     alignas(16) static uint8_t win[fsg_lz4::IW];
     static uint8_t ring[fsg_lz4::RING];
-    fsg_lz4::Staged<HostLanes> m{src, n, dst, win, ring, -(1 << 30), HostLanes{}, 0u, 0};
+    static uint32_t gdesc[fsg_lz4::GMAX];
+    fsg_lz4::Staged<HostLanes> m{src, n, dst, win, ring, gdesc, -(1 << 30), HostLanes{}, 0u, 0};
     return fsg_lz4::decode_block(n, cap, m);
 }

@@ -11,7 +11,7 @@ import pytest
 import falling_sand_engine_amd as fsg
 import packcases
 from falling_sand_engine_amd import _abi
-from test_pack import NMAT, load_golden
+from test_pack import NMAT, corrupt_cases, far_match_blocks, load_golden, sample_blocks, short_sequence_blocks
 
 pytestmark = pytest.mark.gpu
 
@@ -77,6 +77,27 @@ def test_pack_ticked_world_with_maps(ctx, orc, refmap):
     for k in range(8):
         assert got[k] == ref[k], k
     assert (st == 0).all() and np.array_equal(back, want) and np.array_equal(gl2, l2)
+
+
+def test_device_decoder_matches_oracle_on_raw_blocks(ctx, orc):
+    """The unpack's decode kernel on raw LZ4 blocks (fsg_lz4_decode_blocks):
+    the encoder's and the reference compressor's chunk blocks, far and
+    self-overlapping matches, blocks of short sequences (the kernel's group
+    decode) and corrupt variants of all of them, accepted and decoded exactly
+    when the oracle's decoder accepts them; nothing is written past a
+    block's capacity."""
+    import lz4ref
+    blocks = sample_blocks(lz4ref.lib() is not None) + far_match_blocks() + short_sequence_blocks()
+    cases = blocks + corrupt_cases(blocks, 2500, 12)
+    print("raw blocks", len(cases), flush=True)
+    got = fsg.lz4_decode_blocks(ctx, cases, guard=64)
+    accepted = 0
+    for k, ((src, cap), (r, out)) in enumerate(zip(cases, got)):
+        want = orc.lz4_decode(src, cap)
+        assert (out[:cap] if r == cap else None) == want, (k, len(src), cap, r)
+        assert out[cap:] == b"\xa5" * 64, k
+        accepted += want is not None
+    assert accepted > len(blocks)
 
 
 def test_unpack_golden_reference_images(ctx, orc):

@@ -344,13 +344,70 @@ def far_match_blocks():
     return out
 
 
+def short_sequence_blocks(count=48, seed=5):
+    """Random blocks made mostly of short sequences, as a noisy chunk's
+    records give: 0-14 literals and a match of 4-18 bytes at a small offset
+    (1-70), a record (12) or a row (1536) back, with now and then a long
+    literal run, a long match, a far offset, an offset 0 or one past the
+    output start.  Matches overlap themselves (offset below the length) and
+    read the sequences just before them; the blocks end with 12-20 literals
+    or, for some, with a shorter tail that lz4.c rejects."""
+    rng = np.random.default_rng(seed)
+    small = [1, 2, 3, 4, 5, 7, 8, 11, 12, 13, 16, 24, 31, 32, 33, 47, 63, 64, 65, 70]
+    out = []
+    for b in range(count):
+        target = int(rng.integers(200, 6000))
+        first = rng.integers(0, 256, int(rng.integers(1, 40))).astype(np.uint8).tobytes()
+        dec = bytearray(first)
+        blk = bytearray()
+        lit = first
+        while len(dec) < target:
+            r = rng.random()
+            off = (int(rng.choice(small)) if r < 0.8 else 12 if r < 0.9 else 1536 if r < 0.95
+                   else int(rng.integers(1, 5000)))
+            ml = int(rng.integers(4, 19)) if rng.random() < 0.93 else int(rng.integers(19, 300))
+            op = len(dec)
+            bad = rng.random()
+            if bad < 0.004:
+                off = 0
+            elif bad < 0.008:
+                off = op + 1
+            elif off > op:
+                off = 1 + int(rng.integers(0, op))
+            blk += lz4_seq(lit, off, ml)
+            for _ in range(ml):
+                dec.append(dec[-off] if 0 < off <= op else 0)
+            n_lit = int(rng.integers(0, 15)) if rng.random() < 0.95 else int(rng.integers(15, 80))
+            lit = rng.integers(0, 256, n_lit).astype(np.uint8).tobytes()
+            dec += lit
+        tail = rng.integers(0, 256, int(rng.integers(12, 21) if b % 6 else rng.integers(0, 12))).astype(np.uint8)
+        dec += tail.tobytes()
+        blk += lz4_seq(lit + tail.tobytes())
+        out.append((bytes(blk), len(dec)))
+    return out
+
+
+@needs_ref
+def test_short_sequence_blocks_are_lz4():
+    """The short-sequence blocks are read by LZ4_decompress_safe as by the
+    oracle's decoder: accepted or rejected alike, the same bytes."""
+    blocks = short_sequence_blocks()
+    results = [lz4ref.decompress_safe(src, cap) for src, cap in blocks]
+    assert sum(r == cap for (r, _), (_, cap) in zip(results, blocks)) >= len(blocks) // 2
+    for (src, cap), (r, ref) in zip(blocks, results):
+        mine = O.lz4_decode(src, cap)
+        assert (r == cap) == (mine is not None)
+        if mine is not None:
+            assert mine == ref
+
+
 def test_library_decoder_matches_oracle():
     """The decode kernel's parse (fsg_lz4.hpp) over plain memory and over the
     kernel's staged window and output ring, 64 lanes taken in turn, gives the
     oracle's result on good and corrupt blocks, writing nothing past the
     capacity (guard bytes after it stay intact)."""
     L = host_decoder()
-    blocks = sample_blocks(lz4ref.lib() is not None) + far_match_blocks()
+    blocks = sample_blocks(lz4ref.lib() is not None) + far_match_blocks() + short_sequence_blocks()
     for src, cap in blocks + corrupt_cases(blocks, 2500, 12):
         s = np.frombuffer(src, np.uint8) if src else np.zeros(1, np.uint8)
         want = O.lz4_decode(src, cap)
