@@ -69,7 +69,12 @@ __device__ bool tile_unstable(const MatTab &M, const uint32_t (*L)[LW], const ui
                               bool temp_enabled) {
     bool u = false;
     constexpr uint32_t KV = K_SINK | K_GRISE | K_REACT, KD = K_SINK | K_GRISE;
-    for (int i = threadIdx.x; i < LW * LH; i += N) {
+    // loops with a uniform counter (the wave's first index) and idle lanes
+    // in the last round, rather than a loop under exec masks
+    const int w0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u)), lane = threadIdx.x & 63;
+    for (int i0 = w0; i0 < LW * LH; i0 += N) {
+        const int i = i0 + lane;
+        if (i >= LW * LH) continue;
         const int ly = i / LW, lx = i - ly * LW;
         const uint32_t c = L[ly][lx];
         const uint32_t t = cid(c) << 4;
@@ -82,7 +87,9 @@ __device__ bool tile_unstable(const MatTab &M, const uint32_t (*L)[LW], const ui
         }
     }
     if (temp_enabled) {
-        for (int i = threadIdx.x; i < TW * TH; i += N) {
+        for (int i0 = w0; i0 < TW * TH; i0 += N) {
+            static_assert((TW * TH) % N == 0, "the chunk's cells fill whole rounds");
+            const int i = i0 + lane;
             const int ly = HALO + i / TW, lx = HALO + (i & (TW - 1));
             const uint32_t c = L[ly][lx];
             if (cid(c) == WALL) continue;
@@ -148,35 +155,33 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
     };
     const int lane = threadIdx.x & 63;
     constexpr int QCAP = qcap<N>();
-    uint16_t *Q = queue + (threadIdx.x >> 6) * QCAP;
+    // the wave's first block as a uniform: the compiler cannot tell that
+    // threadIdx.x & ~63 is one, and would run the loop under exec masks
+    const int wb0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
+    uint16_t *Q = queue + (wb0 >> 6) * QCAP;
     int nplain = 0, nother = 0;  // wave-uniform
-    for (int b0 = threadIdx.x & ~63; b0 < nb; b0 += N) {
+    for (int b0 = wb0; b0 < nb; b0 += N) {
         const int b = b0 + lane;
-        bool act = false, plain = false;
-        if (b < nb) {
-            const uint32_t *p0 = block_at(b);
-            const uint32_t q0 = p0[0], q1 = p0[1], q2 = p0[LW], q3 = p0[LW + 1];
-            const uint32_t i0 = q0 & 15u, i1 = q1 & 15u, i2 = q2 & 15u, i3 = q3 & 15u;
-            const uint32_t VL = T[(i0 << 4) | i2], VR = T[(i1 << 4) | i3];
-            const uint32_t DL = T[(i0 << 4) | i3], DR = T[(i1 << 4) | i2];
-            const uint32_t HT = T[(i0 << 4) | i1], HB = T[(i2 << 4) | i3];
-            act = !block_inert(VL, VR, DL, DR, HT, HB, (q0 >> 8) & 1u, (q1 >> 8) & 1u);
-            plain = ((((VL | VR) & (K_GRISE | K_REACT)) | ((DL | DR) & K_GRISE) | ((HT | HB) & K_HSWAP)) == 0);
-            if (REACT && has_wood && (wood_hot(M, q0) || wood_hot(M, q1) || wood_hot(M, q2) || wood_hot(M, q3))) {
-                act = true;
-                plain = false;
-            }
-            if (ZONE) act = act && in_zone(b);  // workgroups on the zone's edge only
-            plain = plain && act;
+        // lanes past the last block test the last one again and queue nothing
+        const uint32_t *p0 = block_at(b < nb ? b : nb - 1);
+        const uint32_t q0 = p0[0], q1 = p0[1], q2 = p0[LW], q3 = p0[LW + 1];
+        const uint32_t i0 = q0 & 15u, i1 = q1 & 15u, i2 = q2 & 15u, i3 = q3 & 15u;
+        const uint32_t VL = T[(i0 << 4) | i2], VR = T[(i1 << 4) | i3];
+        const uint32_t DL = T[(i0 << 4) | i3], DR = T[(i1 << 4) | i2];
+        const uint32_t HT = T[(i0 << 4) | i1], HB = T[(i2 << 4) | i3];
+        bool act = !block_inert(VL, VR, DL, DR, HT, HB, (q0 >> 8) & 1u, (q1 >> 8) & 1u);
+        bool plain = ((((VL | VR) & (K_GRISE | K_REACT)) | ((DL | DR) & K_GRISE) | ((HT | HB) & K_HSWAP)) == 0);
+        if (REACT && has_wood && (wood_hot(M, q0) || wood_hot(M, q1) || wood_hot(M, q2) || wood_hot(M, q3))) {
+            act = true;
+            plain = false;
         }
-        const uint64_t mp = __ballot(plain), mo = __ballot(act && !plain);
-        if (plain) {
-            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mp >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mp, 0u));
-            Q[nplain + (int)below] = (uint16_t)b;
-        } else if (act) {
-            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mo >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mo, 0u));
-            Q[QCAP - 1 - (nother + (int)below)] = (uint16_t)b;
-        }
+        act = act && b < nb;
+        if (ZONE) act = act && in_zone(b);  // workgroups on the zone's edge only
+        plain = plain && act;
+        const uint64_t mp = __builtin_amdgcn_ballot_w64(plain), mo = __builtin_amdgcn_ballot_w64(act && !plain);
+        const int bp = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mp >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mp, 0u));
+        const int bo = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mo >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mo, 0u));
+        if (act) Q[plain ? nplain + bp : QCAP - 1 - (nother + bo)] = (uint16_t)b;
         nplain += __popcll(mp);
         nother += __popcll(mo);
     }
@@ -206,7 +211,10 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
             p0[(po >> (8 * k)) & 0xFFu] = v;
         }
     };
-    for (int i = lane; i < nplain; i += 64) {
+    // uniform loops over the queues (rounds of 64 blocks)
+    for (int qi = 0; qi < nplain; qi += 64) {
+        const int i = qi + lane;
+        if (i >= nplain) continue;  // the last round's idle lanes (not a divergent exit)
         uint32_t *p0 = block_at(Q[i]);
         int lx, ly;
         pos(p0, lx, ly);
@@ -220,7 +228,9 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
         if (dec != 0) scatter(p0, q, dec);
         track(dec != 0, lx, ly);
     }
-    for (int i = lane; i < nother; i += 64) {
+    for (int qi = 0; qi < nother; qi += 64) {
+        const int i = qi + lane;
+        if (i >= nother) continue;
         uint32_t *p0 = block_at(Q[QCAP - 1 - i]);
         int lx, ly;
         pos(p0, lx, ly);
