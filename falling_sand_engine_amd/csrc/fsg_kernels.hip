@@ -665,6 +665,21 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     }
 }
 
+// Position in a row-major order of n chunks taken by workgroup i.  The
+// hardware deals workgroups to the eight XCDs in turn (i % 8), each with its
+// own L2.  In every run of 8g positions, XCD k takes the g consecutive ones
+// from kg on, so horizontally adjacent chunks, which share the 128-byte lines
+// of their halo columns, run back to back on one L2, while the grid still
+// sweeps the world in order (a tail shorter than 8g keeps its order).  With
+// g = tx / 8 for a world of tx chunks a row, the 16384^2 world fetches 1.01x
+// its algorithmic bytes instead of 1.26x.
+__device__ __forceinline__ int xcd_order(int i, int n, int g) {
+    const int run = 8 * g, full = n - n % run;
+    if (i >= full) return i;
+    const int b = i / run, r = i - b * run;  // r = 8 * (slot in XCD) + XCD
+    return b * run + (r & 7) * g + (r >> 3);
+}
+
 // Five workgroups per CU (at most 96 VGPRs, 28 KB of LDS each).  One
 // instantiation per (chunk skipping, reactions) setting of the world, so
 // the branches and paths a setting never takes are not compiled in.
@@ -677,7 +692,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
     __shared__ int32_t addt[16];
     __shared__ uint16_t queue[(NT / 64) * qcap<NT>()];
     __shared__ uint32_t red[NT / 64];
-    tile_update<SK, RE, ZN>(w, w.tick + k, blockIdx.x, L, lut, rgba, addt, queue, red);
+    tile_update<SK, RE, ZN>(w, w.tick + k, xcd_order(blockIdx.x, (int)gridDim.x, (w.tiles_x & 7) ? 8 : w.tiles_x >> 3), L, lut, rgba, addt, queue, red);
 }
 // Small worlds (at most one chunk per compute unit): the same update with
 // SMALL_NT (1024) threads per chunk.  A small world leaves compute units idle,
@@ -760,8 +775,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
     __shared__ uint16_t queue[(NT / 64) * qcap<NT>()];
     __shared__ uint32_t red[NT / 64];
     const uint64_t tick = w.tick + k;
-    if (blockIdx.x >= w.awake_n[tick & 1]) return;
-    tile_update<3, RE, ZN>(w, tick, (int)w.awake[blockIdx.x], L, lut, rgba, addt, queue, red);
+    const int n = (int)w.awake_n[tick & 1];
+    if ((int)blockIdx.x >= n) return;
+    // the list runs in row-major order within each reserving workgroup of k_wake
+    tile_update<3, RE, ZN>(w, tick, (int)w.awake[xcd_order(blockIdx.x, n, 8)], L, lut, rgba, addt, queue, red);
 }
 template __global__ void k_tick_list<0, 0>(DevWorld, uint32_t);
 template __global__ void k_tick_list<1, 0>(DevWorld, uint32_t);
