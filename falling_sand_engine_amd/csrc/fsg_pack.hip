@@ -140,6 +140,7 @@ __device__ int walk_row(int rows, int n, const Rec &rec, uint64_t (*stage)[CB], 
         }
         __syncthreads();
         if (!mine) continue;
+        sink.window(i0 + jb, stage[t], left);
         for (int jj = 0; jj < CB; jj++) {
             const int j = jb + jj, i = i0 + j;
             const uint64_t v = stage[t][jj];
@@ -179,6 +180,7 @@ __device__ int walk_row(int rows, int n, const Rec &rec, uint64_t (*stage)[CB], 
 struct Sizer {
     bool hm = false;
     int lead = 0, rest = 0;
+    __device__ void window(int, const uint64_t *, uint64_t) {}
     __device__ void match(int pend, int, int ml, int) {
         if (!hm) {
             hm = true;
@@ -228,18 +230,38 @@ __device__ void put_raw(const Out &o, int at, int p, int cnt, const Rec &rec) {
     }
 }
 
+// Record i from the thread's staged column block (records base .. base +
+// CB - 1, LDS) or the record just before it (a register) when it lies there,
+// else from the block's source.
+template <class Rec>
+struct Near {
+    const Rec *rec;
+    int base = 0;
+    const uint64_t *win = nullptr;
+    uint64_t prev = 0;
+    __device__ uint64_t operator()(int i) const {
+        const int d = i - base;
+        return win && d >= 0 && d < CB ? win[d] : win && d == -1 ? prev : (*rec)(i);
+    }
+};
+
 // Pass 2: writes the row's sequences.
 template <int R, class Rec>
 struct Writer {
     Out o;
     int oc, carry;
     bool first = true;
-    const Rec *rec;
+    Near<Rec> near;
+    __device__ void window(int base, const uint64_t *win, uint64_t prev) {
+        near.base = base;
+        near.win = win;
+        near.prev = prev;
+    }
     __device__ void match(int pend, int mstart, int ml, int off) {
         const int ll = first ? carry + pend : pend;
         o.put(oc++, (uint32_t)((min(ll, 15) << 4) | min(ml - 4, 15)));
         oc = put_len(o, oc, ll);
-        put_raw<R>(o, oc + (ll - pend), mstart - pend, pend, *rec);  // carried bytes come from earlier rows
+        put_raw<R>(o, oc + (ll - pend), mstart - pend, pend, near);  // carried bytes come from earlier rows
         oc += ll;
         o.put(oc++, (uint32_t)off & 0xFFu);
         o.put(oc++, (uint32_t)off >> 8);
@@ -300,7 +322,7 @@ __device__ void encode_block(int n, const Rec &rec, uint8_t *out, int cap, int32
         return;
     }
     const Out o{out, cap};
-    Writer<R, Rec> w{o, t < rows ? start[t] : 0, t < rows ? carry[t] : 0, true, &rec};
+    Writer<R, Rec> w{o, t < rows ? start[t] : 0, t < rows ? carry[t] : 0, true, Near<Rec>{&rec}};
     walk_row<R>(rows, n, rec, stage, w);
     if (t < rows) {
         const int row0 = t * S, tstart = row0 + S - pend;  // first trailing literal byte
@@ -310,7 +332,7 @@ __device__ void encode_block(int n, const Rec &rec, uint8_t *out, int cap, int32
             int oc = w.oc;
             o.put(oc++, (uint32_t)(min(llf, 15) << 4));
             oc = put_len(o, oc, llf);
-            put_raw<R>(o, oc + (llf - pend), tstart, pend, rec);
+            put_raw<R>(o, oc + (llf - pend), tstart, pend, w.near);  // the last staged block, or the source
         } else {
             // trailing literals belong to the first sequence of the next row with a match
             // (or to the last sequence)
@@ -318,7 +340,7 @@ __device__ void encode_block(int n, const Rec &rec, uint8_t *out, int cap, int32
             const int ll0 = carry[u] + lead[u];
             const int lit_out = start[u] + 1 + extlen(ll0);
             const int lit_raw = u * S - carry[u];
-            put_raw<R>(o, lit_out + (tstart - lit_raw), tstart, pend, rec);
+            put_raw<R>(o, lit_out + (tstart - lit_raw), tstart, pend, w.near);
         }
     }
     if (t == 0) *size_out = total;
