@@ -81,7 +81,7 @@ __host__ __device__ inline void match_lane(uint8_t *d, int off, int len, int lan
 // as a uniform, ballot(v) the mask of lanes whose v is nonzero, shfl(v, i)
 // lane i's v in each lane, below(m, lane) the number of bits of m below the
 // lane's.
-constexpr int IW = 1024, RING = 4096, NL = 64;
+constexpr int IW = 1024, RING = 2048, NL = 64;
 constexpr int GMAX = NL / MINMATCH;  // sequences in one group at most
 
 // The match table: entry off * NL + j is j mod off for 1 <= off < NL and

@@ -231,3 +231,29 @@ def test_pack_arguments(ctx, orc):
         assert min(sizes) > 17
         rc = fsg.lib().fsg_world_pack_download(w.handle, None, 0)
         assert rc == _abi.FSG_ERR_INVALID
+
+
+def test_lz4_decode_blocks_arguments(ctx):
+    """Blocks or outputs outside their buffers, negative sizes and a null
+    buffer are rejected before anything runs; no blocks is a no-op."""
+    import ctypes as C
+    L = fsg.lib()
+    src = np.zeros(32, np.uint8)
+    out = np.zeros(64, np.uint8)
+    res = np.zeros(1, np.int32)
+
+    def call(so, sb, cap, oo, out_len=64, src_ptr=src.ctypes.data, n=1):
+        a = [np.array([v], dt) for v, dt in ((so, np.int64), (sb, np.int32), (cap, np.int32), (oo, np.int64))]
+        return L.fsg_lz4_decode_blocks(ctx.handle, src_ptr, 32, a[0].ctypes.data, a[1].ctypes.data,
+                                       a[2].ctypes.data, n, out.ctypes.data, out_len, a[3].ctypes.data,
+                                       res.ctypes.data)
+    for bad in [(-1, 4, 8, 0), (30, 4, 8, 0), (0, -1, 8, 0), (0, 4, -1, 0), (0, 4, 60, 8), (0, 4, 8, -1)]:
+        assert call(*bad) == _abi.FSG_ERR_INVALID, bad
+    assert call(0, 4, 8, 0, src_ptr=None) == _abi.FSG_ERR_INVALID
+    assert call(0, 4, 8, 0, n=-1) == _abi.FSG_ERR_INVALID
+    assert call(0, 4, 8, 0, n=0) == _abi.FSG_OK
+    # a literal-only block of 12 bytes decodes in place; a block of 0 bytes is rejected
+    blk = bytes([0xC0]) + bytes(range(12))
+    (r, got), (r0, _) = fsg.lz4_decode_blocks(ctx, [(blk, 12), (b"", 4)], guard=4)
+    assert r == 12 and got == bytes(range(12)) + b"\xa5" * 4
+    assert r0 == -1
