@@ -173,6 +173,29 @@ void fill_pair_table(DevWorld &d) {
         for (uint32_t i = 0; i < 256; i++) d.pair[it][i] = (uint16_t)fsg::pair_code(d.mat, it, i >> 4, i & 15u);
 }
 
+// The block-class tables of the classification pass (DevWorld::ct), for a
+// world of at most 8 materials: the class of every block of 3-bit material
+// ids at every iteration, from the pair table.  bit 0: some pair relation
+// (K_SINK, K_GRISE, K_HSWAP, K_REACT); bit 1: plain (no gas or reaction in
+// the columns, no gas diagonal, no horizontal swap); bits 2-3: K_TSANDACT of
+// the top left / right cell.
+std::vector<uint32_t> class_tables(const DevWorld &d) {
+    std::vector<uint32_t> t((size_t)fsg::MAX_IT * 512, 0u);
+    for (int it = 0; it < fsg::MAX_IT; it++)
+        for (uint32_t i = 0; i < 4096; i++) {
+            const uint32_t i0 = i >> 9, i1 = (i >> 6) & 7u, i2 = (i >> 3) & 7u, i3 = i & 7u;
+            auto P = [&](uint32_t a, uint32_t b) { return (uint32_t)d.pair[it][a * 16 + b]; };
+            const uint32_t VL = P(i0, i2), VR = P(i1, i3), DL = P(i0, i3), DR = P(i1, i2), HT = P(i0, i1),
+                           HB = P(i2, i3);
+            const uint32_t rel = (VL | VR | DL | DR | HT | HB) & (fsg::K_SINK | fsg::K_GRISE | fsg::K_HSWAP | fsg::K_REACT);
+            const uint32_t other = ((VL | VR) & (fsg::K_GRISE | fsg::K_REACT)) | ((DL | DR) & fsg::K_GRISE) |
+                                   ((HT | HB) & fsg::K_HSWAP);
+            const uint32_t nib = (rel ? 1u : 0u) | (other ? 0u : 2u) | (((VL >> 8) & 1u) << 2) | (((VR >> 8) & 1u) << 3);
+            t[(size_t)it * 512 + (i >> 3)] |= nib << ((i & 7u) * 4);
+        }
+    return t;
+}
+
 bool cell_ok(const fsg_config &c, uint16_t mat) {
     return (mat & 0xFE00u) == 0 && (uint32_t)(mat & 0xFFu) < c.n_materials;
 }
@@ -414,7 +437,7 @@ extern "C" int fsg_world_create(fsg_ctx *c, int W, int H, uint64_t seed, const f
 
     const int tx = (W + fsg::TW - 1) / fsg::TW, ty = (H + fsg::TH - 1) / fsg::TH;
     const size_t cells = (size_t)W * H;
-    const size_t bytes = 2 * cells * 8 + 3 * (size_t)tx * ty * 4 + 16 + 18 * 8;
+    const size_t bytes = world_state_bytes(W, H, conf.n_materials);
     {
         std::lock_guard<std::mutex> lk(c->mu);
         if (c->budget && c->used + bytes > c->budget) return FSG_ERR_BUDGET;
@@ -429,6 +452,7 @@ extern "C" int fsg_world_create(fsg_ctx *c, int W, int H, uint64_t seed, const f
             dev_free(w->d.flags[1]);
             dev_free(w->d.awake);
             dev_free(w->d.awake_n);
+            dev_free(const_cast<uint32_t *>(w->d.ct));
             dev_free(w->d_stats);
             delete w;
         }
@@ -471,6 +495,15 @@ extern "C" int fsg_world_create(fsg_ctx *c, int W, int H, uint64_t seed, const f
     w->d.zy1 = H;
     w->d.mat = build_mattab(conf);
     fill_pair_table(w->d);
+    w->d.ct = nullptr;
+    if (conf.n_materials <= 8) {
+        w->ct_host = class_tables(w->d);  // kept until the world goes (the copy is asynchronous)
+        if (dev_alloc(c, &p, w->ct_host.size() * 4)) return fail(FSG_ERR_NOMEM);
+        w->d.ct = (const uint32_t *)p;
+        if (hipMemcpyAsync(p, w->ct_host.data(), w->ct_host.size() * 4, hipMemcpyHostToDevice, c->stream) !=
+            hipSuccess)
+            return fail(FSG_ERR_DEVICE);
+    }
     const uint64_t air = conf.mat_air;
     fsg::k_fill<<<grid_for(cells, 256), 256, 0, c->stream>>>(w->d.buf[0], cells, air);
     fsg::k_fill<<<grid_for(cells, 256), 256, 0, c->stream>>>(w->d.buf[1], cells, air);
@@ -513,6 +546,7 @@ extern "C" int fsg_world_destroy(fsg_world *w) {
     dev_free(w->d.flags[1]);
     dev_free(w->d.awake);
     dev_free(w->d.awake_n);
+    dev_free(const_cast<uint32_t *>(w->d.ct));
     dev_free(w->d_stats);
     dev_free(w->d_pixels);
     dev_free(w->d_pixbody);

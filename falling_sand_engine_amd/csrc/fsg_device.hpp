@@ -54,6 +54,13 @@ struct DevWorld {
     // chunks tick t updates (k_wake, k_tick_list)
     uint32_t *awake;
     uint32_t *awake_n;
+    // block-class tables of the classification pass, MAX_IT x 512 words
+    // (worlds of at most 8 materials; null otherwise): word i0<<6|i1<<3|i2 of
+    // table it holds, in nibble i3, the class of a block of material ids
+    // i0 i1 / i2 i3 at iteration it: bit 0 some pair relation (K_SINK,
+    // K_GRISE, K_HSWAP, K_REACT), bit 1 plain, bits 2-3 K_TSANDACT of the top
+    // left / right cell
+    const uint32_t *ct;
     MatTab mat;
     uint16_t pair[MAX_IT][256];  // pair_code(mat, it, t, u) at [it][t * 16 + u], built on the host
 };

@@ -77,6 +77,7 @@ struct fsg_world {
     // tick) or for any tick (list_any: both zero)
     bool list_any = true;
     uint64_t list_next = 0;
+    std::vector<uint32_t> ct_host;  // host copy of the class tables (DevWorld::ct)
     uint32_t *d_layer[3] = {nullptr, nullptr, nullptr};  // texture layers (fsg_world_render)
     unsigned long long *d_tally = nullptr;                 // render census replicas
     void *d_isl = nullptr;                                 // island check scratch
@@ -139,6 +140,14 @@ struct fsg_world {
 
 
 namespace fsg_impl {
+// Device bytes a world holds from fsg_world_create on (both state buffers,
+// chunk flags and awake list, counts, stats, and the class tables of a world
+// of at most 8 materials): what the context's HBM budget counts.
+inline size_t world_state_bytes(int W, int H, uint32_t n_materials) {
+    const size_t tiles = (size_t)((W + fsg::TW - 1) / fsg::TW) * (size_t)((H + fsg::TH - 1) / fsg::TH);
+    return 2 * (size_t)W * H * 8 + 3 * tiles * 4 + 16 + 18 * 8 +
+           (n_materials <= 8 ? (size_t)fsg::MAX_IT * 512 * 4 : 0);
+}
 int dev_alloc(fsg_ctx *c, void **p, size_t bytes);
 void dev_free(void *p);
 void trim_cache(fsg_ctx *c, size_t keep);  // hipFree cached blocks down to `keep` bytes

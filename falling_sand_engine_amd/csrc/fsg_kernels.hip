@@ -134,7 +134,7 @@ template <int N, bool TRACK, bool REACT, bool ZONE>
 __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW], const uint32_t *T, int it,
                                               uint64_t key, int ox, int oy, int need, int gx0, int gy0,
                                               bool has_wood, uint16_t *queue, const uint32_t *perm,
-                                              const ZoneBlocks &zb) {
+                                              const ZoneBlocks &zb, const uint32_t *ct) {
     // first/last block origin (chunk coordinates) with the phase's parity
     const int x0f = -need - ((need ^ ox) & 1), y0f = -need - ((need ^ oy) & 1);
     const int x0l = (TW - 1 + need) - (((TW - 1 + need) ^ ox) & 1);
@@ -165,12 +165,19 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
         // lanes past the last block test the last one again and queue nothing
         const uint32_t *p0 = block_at(b < nb ? b : nb - 1);
         const uint32_t q0 = p0[0], q1 = p0[1], q2 = p0[LW], q3 = p0[LW + 1];
-        const uint32_t i0 = q0 & 15u, i1 = q1 & 15u, i2 = q2 & 15u, i3 = q3 & 15u;
-        const uint32_t VL = T[(i0 << 4) | i2], VR = T[(i1 << 4) | i3];
-        const uint32_t DL = T[(i0 << 4) | i3], DR = T[(i1 << 4) | i2];
-        const uint32_t HT = T[(i0 << 4) | i1], HB = T[(i2 << 4) | i3];
-        bool act = !block_inert(VL, VR, DL, DR, HT, HB, (q0 >> 8) & 1u, (q1 >> 8) & 1u);
-        bool plain = ((((VL | VR) & (K_GRISE | K_REACT)) | ((DL | DR) & K_GRISE) | ((HT | HB) & K_HSWAP)) == 0);
+        bool act, plain;
+        if (ct) {  // one class-table read (ids < 8: at most 8 materials and no wall in the region)
+            const uint32_t nib = (ct[((q0 & 7u) << 6) | ((q1 & 7u) << 3) | (q2 & 7u)] >> ((q3 & 7u) << 2)) & 15u;
+            act = ((nib | ((nib >> 2) & (q0 >> 8)) | ((nib >> 3) & (q1 >> 8))) & 1u) != 0;
+            plain = (nib & 2u) != 0;
+        } else {
+            const uint32_t i0 = q0 & 15u, i1 = q1 & 15u, i2 = q2 & 15u, i3 = q3 & 15u;
+            const uint32_t VL = T[(i0 << 4) | i2], VR = T[(i1 << 4) | i3];
+            const uint32_t DL = T[(i0 << 4) | i3], DR = T[(i1 << 4) | i2];
+            const uint32_t HT = T[(i0 << 4) | i1], HB = T[(i2 << 4) | i3];
+            act = !block_inert(VL, VR, DL, DR, HT, HB, (q0 >> 8) & 1u, (q1 >> 8) & 1u);
+            plain = ((((VL | VR) & (K_GRISE | K_REACT)) | ((DL | DR) & K_GRISE) | ((HT | HB) & K_HSWAP)) == 0);
+        }
         if (REACT && has_wood && (wood_hot(M, q0) || wood_hot(M, q1) || wood_hot(M, q2) || wood_hot(M, q3))) {
             act = true;
             plain = false;
@@ -453,6 +460,12 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     // A chunk whose halo lies inside an even-width world takes the fast
     // paths: 16-byte loads/stores with no bounds or wall checks.
     const bool interior = gx0 >= 0 && gy0 >= 0 && gx0 + LW <= w.W && gy0 + LH <= w.H && (w.W & 1) == 0;
+    // the classification reads the world's class tables when it has them and
+    // the region holds no wall (its ids are then all below 8); the slot in
+    // LDS holds the table of the phase's iteration
+    const bool use_ct = w.ct != nullptr && interior;
+    uint2 ctv = make_uint2(0u, 0u);
+    if (use_ct && tab) ctv = reinterpret_cast<const uint2 *>(w.ct)[tid];
     const bool mine = tid < RS * PR, edge = tid < 2 * PR;
     const int rr = tid / PR, c2 = tid - rr * PR;                      // sweep rows 1 + rr + RS * k
     const int ey = tid < PR ? 0 : LH - 1, ec2 = tid < PR ? tid : tid - PR;  // edge row (edge threads)
@@ -524,6 +537,7 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
             po |= ((d & 1u) + (d >> 1) * (uint32_t)LW) << (8 * k);
         }
         lut[MAX_IT][tid] = po;
+        if (use_ct) reinterpret_cast<uint2 *>(lut[MAX_IT + 1])[tid] = ctv;
     }
     if (tid < 16) {
         rgba[tid] = mrgba;
@@ -552,20 +566,28 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     uint32_t bands = 0;
     if (!zt.dead) {
         const uint64_t seedmix = splitmix64(w.seed);
+        const uint32_t *ct = use_ct ? lut[MAX_IT + 1] : nullptr;
         for (int it = 0; it < ci; it++) {
             const uint64_t g = tick * (uint64_t)M.cell_iter + (uint64_t)it;
             const uint64_t key = splitmix64(seedmix ^ g);
             const int ox = kOX[g & 3], oy = kOY[g & 3];
+            // the next iteration's class table, loaded while this phase runs
+            const bool next_ct = use_ct && it + 1 < ci;
+            if (next_ct && tab) ctv = reinterpret_cast<const uint2 *>(w.ct + (it + 1) * 512)[tid];
             if (zt.blocks.on)  // on the zone's edge: one form for every setting (the bands may go unused)
                 bands |= run_phase<N, true, RE != 0, true>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood,
-                                                        queue, lut[MAX_IT], zt.blocks);
+                                                        queue, lut[MAX_IT], zt.blocks, ct);
             else if (SK != 0 && skip)
                 bands |= run_phase<N, true, RE != 0, false>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood,
-                                                         queue, lut[MAX_IT], zt.blocks);
+                                                         queue, lut[MAX_IT], zt.blocks, ct);
             else if (SK != 1 && SK != 3)
                 run_phase<N, false, RE != 0, false>(M, L, lut[it], it, key, ox, oy, ci - it, gx0, gy0, has_wood, queue,
-                                                 lut[MAX_IT], zt.blocks);
+                                                 lut[MAX_IT], zt.blocks, ct);
             __syncthreads();
+            if (next_ct) {  // every wave's classification of this phase is done
+                if (tab) reinterpret_cast<uint2 *>(lut[MAX_IT + 1])[tid] = ctv;
+                __syncthreads();
+            }
         }
     }
 
@@ -686,7 +708,7 @@ __device__ __forceinline__ int xcd_order(int i, int n, int g) {
 template <int SK, int RE, int ZN>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_tick_world(DevWorld w, uint32_t k) {
     __shared__ __align__(16) uint32_t Lmem[LH * LW + 16];  // the region, then the 16 base colours
-    __shared__ uint32_t lut[MAX_IT + 1][256];               // pair tables + scatter offsets
+    __shared__ uint32_t lut[MAX_IT + 3][256];  // pair tables, scatter offsets, the phase's class table
     uint32_t(*L)[LW] = reinterpret_cast<uint32_t(*)[LW]>(Lmem);
     uint32_t *rgba = Lmem + LH * LW;
     __shared__ int32_t addt[16];
@@ -702,7 +724,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
 template <int RE, int ZN>
 __global__ __launch_bounds__(SMALL_NT) void k_tick_small(DevWorld w, uint32_t k) {
     __shared__ __align__(16) uint32_t Lmem[LH * LW + 16];  // the region, then the 16 base colours
-    __shared__ uint32_t lut[MAX_IT + 1][256];               // pair tables + scatter offsets
+    __shared__ uint32_t lut[MAX_IT + 3][256];  // pair tables, scatter offsets, the phase's class table
     uint32_t(*L)[LW] = reinterpret_cast<uint32_t(*)[LW]>(Lmem);
     uint32_t *rgba = Lmem + LH * LW;
     __shared__ int32_t addt[16];
@@ -768,7 +790,7 @@ __global__ __launch_bounds__(256) void k_wake(DevWorld w, uint32_t k) {
 template <int RE, int ZN>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_tick_list(DevWorld w, uint32_t k) {
     __shared__ __align__(16) uint32_t Lmem[LH * LW + 16];  // the region, then the 16 base colours
-    __shared__ uint32_t lut[MAX_IT + 1][256];               // pair tables + scatter offsets
+    __shared__ uint32_t lut[MAX_IT + 3][256];  // pair tables, scatter offsets, the phase's class table
     uint32_t(*L)[LW] = reinterpret_cast<uint32_t(*)[LW]>(Lmem);
     uint32_t *rgba = Lmem + LH * LW;
     __shared__ int32_t addt[16];
@@ -790,7 +812,7 @@ template __global__ void k_tick_list<1, 1>(DevWorld, uint32_t);
 __device__ __forceinline__ void tick_batch_chunk(const DevWorld *__restrict__ worlds,
                                                  const uint32_t *__restrict__ tile_start, int nworlds, uint32_t k) {
     __shared__ __align__(16) uint32_t Lmem[LH * LW + 16];  // the region, then the 16 base colours
-    __shared__ uint32_t lut[MAX_IT + 1][256];               // pair tables + scatter offsets
+    __shared__ uint32_t lut[MAX_IT + 3][256];  // pair tables, scatter offsets, the phase's class table
     uint32_t(*L)[LW] = reinterpret_cast<uint32_t(*)[LW]>(Lmem);
     uint32_t *rgba = Lmem + LH * LW;
     __shared__ int32_t addt[16];

@@ -77,8 +77,9 @@ constexpr double kDt = 1.0 / 30.3;  // b2world->Step interval (world.cpp:2276 at
 constexpr double kGravity = 20.0;   // b2World gravity (0, 20), world.cpp:147-148
 
 uint64_t world_bytes(int W, int H) {
-    const uint64_t tiles = (uint64_t)((W + fsg::TW - 1) / fsg::TW) * ((H + fsg::TH - 1) / fsg::TH);
-    return 2ull * W * H * 8 + 2 * tiles * 4 + 18 * 8;
+    fsg_config cfg;
+    fsg_default_config(&cfg);  // served worlds take the default config
+    return fsg_impl::world_state_bytes(W, H, cfg.n_materials);
 }
 
 // Deterministic bodies of a served world: boxes 4-64 px and convex polygons

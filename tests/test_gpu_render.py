@@ -116,7 +116,8 @@ def test_layers_subset_and_census_edge_cases(ctx, orc):
 def test_layers_count_against_the_budget(orc):
     cfg = orc.default_config()
     W = H = 256
-    world_bytes = 2 * W * H * 8 + 2 * 16 * 4 + 18 * 8
+    # the world's device bytes: two states, chunk flags and awake list, counts, stats, class tables
+    world_bytes = 2 * W * H * 8 + 3 * 16 * 4 + 16 + 18 * 8 + 3 * 512 * 4
     with fsg.Context(0, hbm_budget=world_bytes + W * H * 4 + 1024) as c, fsg.World(c, W, H, 1, cfg) as w:
         w.upload(fsg.generate(1, W, H, 1))
         w.render(LAYER_CELLS)  # fits
