@@ -462,8 +462,11 @@ __device__ __forceinline__ void tile_update(const DevWorld &w, uint64_t tick, in
     const bool interior = gx0 >= 0 && gy0 >= 0 && gx0 + LW <= w.W && gy0 + LH <= w.H && (w.W & 1) == 0;
     // the classification reads the world's class tables when it has them and
     // the region holds no wall (its ids are then all below 8); the slot in
-    // LDS holds the table of the phase's iteration
-    const bool use_ct = w.ct != nullptr && interior;
+    // LDS holds the table of the phase's iteration.  Not in the 1024-thread
+    // form: a small world's tick is one chunk's latency, and the refill's
+    // barrier between phases cost it 3% (512^2 world, 12.8 -> 13.2 ms for
+    // 1000 ticks)
+    const bool use_ct = N == NT && w.ct != nullptr && interior;
     uint2 ctv = make_uint2(0u, 0u);
     if (use_ct && tab) ctv = reinterpret_cast<const uint2 *>(w.ct)[tid];
     const bool mine = tid < RS * PR, edge = tid < 2 * PR;
