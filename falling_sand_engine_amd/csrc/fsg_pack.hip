@@ -545,7 +545,8 @@ __global__ __launch_bounds__(256) void k_pack_cells(const uint8_t *raw, const in
         if (t == 0) chunk_status[c] = FSG_ERR_INVALID;
         return;
     }
-    for (int i = t; i < 2 * CW * CW; i += 256) {
+    const int nrec = d.l2 ? 2 * CW * CW : CW * CW;  // layer2 records are read only when kept
+    for (int i = t; i < nrec; i += 256) {
         const uint8_t *q = r + (size_t)i * RA;
         const uint32_t w2 = ld32(q + 8);  // temperature, pad byte 10, pad byte 11
         const uint64_t v = (uint64_t)(uint16_t)d.from_file[ld32(q)] | (d.moved ? (w2 >> 8) & 0x100u : 0u) |
