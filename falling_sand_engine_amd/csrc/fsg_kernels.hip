@@ -143,10 +143,17 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
     const int nb = nbx * nby;
     const uint32_t magic = (65536u + (uint32_t)nbx - 1u) / (uint32_t)nbx;  // b / nbx for b < 2^16/nbx
     const int lx0 = HALO + x0f, ly0 = HALO + y0f;
-    auto block_at = [&](int b) -> uint32_t * {
+    // LDS coordinates of block b's top-left cell, and that cell
+    auto block_xy = [&](int b, int &lx, int &ly) {
         const int by = (int)(__umul24((uint32_t)b, magic) >> 16);
         const int bx = b - (int)__umul24((uint32_t)by, (uint32_t)nbx);
-        return &L[0][0] + __umul24((uint32_t)(ly0 + 2 * by), (uint32_t)LW) + lx0 + 2 * bx;
+        lx = lx0 + 2 * bx;
+        ly = ly0 + 2 * by;
+    };
+    auto block_at = [&](int b) -> uint32_t * {
+        int lx, ly;
+        block_xy(b, lx, ly);
+        return &L[0][0] + __umul24((uint32_t)ly, (uint32_t)LW) + lx;
     };
     auto in_zone = [&](int b) -> bool {
         const int by = (int)(__umul24((uint32_t)b, magic) >> 16);
@@ -193,12 +200,6 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
         nother += __popcll(mo);
     }
     uint32_t bands = 0;
-    // block position from its LDS pointer (for the hash and the bands)
-    auto pos = [&](const uint32_t *p0, int &lx, int &ly) {
-        const int off = (int)(p0 - &L[0][0]);
-        ly = (int)(__umul24((uint32_t)off, 58255u) >> 22);  // off / LW for off < LW * LH
-        lx = off - ly * LW;
-    };
     auto track = [&](bool wrote, int lx, int ly) {
         if (TRACK && wrote) {
             const int cx = lx - HALO, cy = ly - HALO;  // chunk coordinates of the block
@@ -212,9 +213,8 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
         const uint32_t po = perm[dec & 0xFFu];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            uint32_t v = q[k];
-            if ((dec >> (8 + k)) & 1u) v |= (uint32_t)MOVED;
-            if ((dec >> (12 + k)) & 1u) v &= ~(uint32_t)MOVED;
+            // MOVED set on cell k (dec bit 8 + k), then cleared (bit 12 + k)
+            const uint32_t v = (q[k] | ((dec >> k) & (uint32_t)MOVED)) & ~((dec >> (k + 4)) & (uint32_t)MOVED);
             p0[(po >> (8 * k)) & 0xFFu] = v;
         }
     };
@@ -222,9 +222,9 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
     for (int qi = 0; qi < nplain; qi += 64) {
         const int i = qi + lane;
         if (i >= nplain) continue;  // the last round's idle lanes (not a divergent exit)
-        uint32_t *p0 = block_at(Q[i]);
         int lx, ly;
-        pos(p0, lx, ly);
+        block_xy(Q[i], lx, ly);
+        uint32_t *p0 = &L[0][0] + __umul24((uint32_t)ly, (uint32_t)LW) + lx;
         const uint32_t q[4] = {p0[0], p0[1], p0[LW], p0[LW + 1]};
         const uint32_t i0 = q[0] & 15u, i1 = q[1] & 15u, i2 = q[2] & 15u, i3 = q[3] & 15u;
         uint32_t r, s;
@@ -238,9 +238,9 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
     for (int qi = 0; qi < nother; qi += 64) {
         const int i = qi + lane;
         if (i >= nother) continue;
-        uint32_t *p0 = block_at(Q[QCAP - 1 - i]);
         int lx, ly;
-        pos(p0, lx, ly);
+        block_xy(Q[QCAP - 1 - i], lx, ly);
+        uint32_t *p0 = &L[0][0] + __umul24((uint32_t)ly, (uint32_t)LW) + lx;
         uint32_t q[4] = {p0[0], p0[1], p0[LW], p0[LW + 1]};
         const uint32_t i0 = q[0] & 15u, i1 = q[1] & 15u, i2 = q[2] & 15u, i3 = q[3] & 15u;
         const uint32_t VL = T[(i0 << 4) | i2], VR = T[(i1 << 4) | i3];
