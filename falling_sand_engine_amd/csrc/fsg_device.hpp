@@ -18,6 +18,7 @@ constexpr int LW = TW + 2 * HALO;    // LDS tile width  (72)
 constexpr int LH = TH + 2 * HALO;    // LDS tile height (72)
 constexpr int NT = 256;              // threads per workgroup (4 wave64)
 constexpr int SMALL_NT = 1024;       // threads per chunk of a world with at most one chunk per CU
+constexpr int QUARTER_NT = 1024;     // threads per quarter chunk of a world with at most a quarter chunk per CU
 constexpr int MAX_IT = 3;            // largest cell_iter the fused kernel supports
 constexpr uint64_t MOVED = 0x100;
 constexpr uint64_t WALL = 15;        // out-of-world cell: inert solid (getTile OOB, world.cpp:1000)

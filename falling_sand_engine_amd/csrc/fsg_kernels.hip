@@ -130,15 +130,19 @@ struct ZoneBlocks {
     uint32_t w, h;
 };
 
-template <int N, bool TRACK, bool REACT, bool ZONE>
+// EXT: the side of the square of chunk cells the tick computes, from chunk
+// coordinates (cx0, cy0): the whole chunk, or a quarter of it.
+template <int N, bool TRACK, bool REACT, bool ZONE, int EXT = TW>
 __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW], const uint32_t *T, int it,
                                               uint64_t key, int ox, int oy, int need, int gx0, int gy0,
                                               bool has_wood, uint16_t *queue, const uint32_t *perm,
-                                              const ZoneBlocks &zb, const uint32_t *ct) {
+                                              const ZoneBlocks &zb, const uint32_t *ct, int cx0 = 0, int cy0 = 0) {
+    static_assert(EXT == TW || !TRACK, "band tracking covers whole chunks");
     // first/last block origin (chunk coordinates) with the phase's parity
-    const int x0f = -need - ((need ^ ox) & 1), y0f = -need - ((need ^ oy) & 1);
-    const int x0l = (TW - 1 + need) - (((TW - 1 + need) ^ ox) & 1);
-    const int y0l = (TH - 1 + need) - (((TH - 1 + need) ^ oy) & 1);
+    const int xlo = cx0 - need, ylo = cy0 - need;
+    const int x0f = xlo - ((xlo ^ ox) & 1), y0f = ylo - ((ylo ^ oy) & 1);
+    const int x0l = (cx0 + EXT - 1 + need) - (((cx0 + EXT - 1 + need) ^ ox) & 1);
+    const int y0l = (cy0 + EXT - 1 + need) - (((cy0 + EXT - 1 + need) ^ oy) & 1);
     const int nbx = ((x0l - x0f) >> 1) + 1, nby = ((y0l - y0f) >> 1) + 1;
     const int nb = nbx * nby;
     const uint32_t magic = (65536u + (uint32_t)nbx - 1u) / (uint32_t)nbx;  // b / nbx for b < 2^16/nbx
@@ -735,6 +739,172 @@ __global__ __launch_bounds__(SMALL_NT) void k_tick_small(DevWorld w, uint32_t k)
     __shared__ uint32_t red[SMALL_NT / 64];
     tile_update<0, RE, ZN, SMALL_NT>(w, w.tick + k, blockIdx.x, L, lut, rgba, addt, queue, red);
 }
+// Smaller worlds still (at most a quarter chunk per compute unit): each
+// chunk as four 32x32 quarters, one workgroup of QUARTER_NT threads each,
+// with the HALO ring around the quarter.  Such a tick lasts one workgroup's
+// chain of dependent steps (stage, each phase's two passes, write-back);
+// a quarter's window is (32 + 2 HALO)^2 cells instead of (64 + 2 HALO)^2, so
+// each wave of it takes one round of blocks per pass (512^2 world: 9.7 us a
+// tick instead of 11.2 with one 1024-thread workgroup per chunk).  The window sits at its
+// place in the chunk's LDS layout, so block origins, hash positions, origin
+// residues and the phase schedule are the chunk update's, and the quarter's
+// cells come out the same: every cell of the quarter depends, over the
+// tick's phases, only on cells of its window (one ring per phase plus the
+// temperature stencil's).  No chunk skipping and no tick zone here.
+constexpr int QW = TW / 2, QH = TH / 2;                   // the quarter (32 x 32)
+constexpr int QLW = QW + 2 * HALO, QLH = QH + 2 * HALO;   // its window (40 x 40)
+static_assert(TW == 2 * QW && TH == 2 * QH && (QW & 1) == 0, "quarters of whole cell pairs");
+template <int RE, int N>
+__device__ __forceinline__ void quarter_update(const DevWorld &w, uint64_t tick, int tile, int quad,
+                                               uint32_t (*L)[LW], uint32_t (*lut)[256], uint32_t *rgba,
+                                               int32_t *addt, uint16_t *queue) {
+    static_assert(N == 256 || N == 512 || N == 1024, "one pair-table entry per thread, write-back strips");
+    const int tid = threadIdx.x;
+    const MatTab &M = w.mat;
+    const int tx = tile % w.tiles_x, ty = tile / w.tiles_x;
+    const int gx0 = tx * TW - HALO, gy0 = ty * TH - HALO;    // world coords of L[0][0], as for the chunk
+    const int cx0 = (quad & 1) * QW, cy0 = (quad >> 1) * QH;  // the quarter (chunk coords) = its window (LDS coords)
+    const uint64_t *__restrict__ in = w.buf[tick & 1];
+    uint64_t *__restrict__ out = w.buf[(tick + 1) & 1];
+    const int ci = (int)M.cell_iter;
+    uint32_t pv[MAX_IT];
+#pragma unroll
+    for (int k = 0; k < MAX_IT; k++)
+        if (k < ci && tid < 256) pv[k] = w.pair[k][tid];
+    uint32_t mrgba = 0;
+    int32_t madd = 0;
+    if (tid < 16) {
+        mrgba = M.rgba[tid];
+        madd = M.add_temp[tid];
+    }
+    // the window: QLH rows of QP cell pairs, world corner (wx0, wy0)
+    constexpr int QP = QLW / 2, NPAIR = QLH * QP, KQ = (NPAIR + N - 1) / N;
+    const int wx0 = gx0 + cx0, wy0 = gy0 + cy0;
+    const bool interior = wx0 >= 0 && wy0 >= 0 && wx0 + QLW <= w.W && wy0 + QLH <= w.H && (w.W & 1) == 0;
+    uint4 v[KQ];
+#pragma unroll
+    for (int k = 0; k < KQ; k++) {
+        const int i = tid + k * N;
+        if (i >= NPAIR) continue;
+        const int row = i / QP, p = i - row * QP;
+        const int gx = wx0 + 2 * p, gy = wy0 + row;
+        if (interior) {
+            v[k] = *reinterpret_cast<const uint4 *>(in + (size_t)gy * w.W + gx);
+        } else {
+            const bool rin = gy >= 0 && gy < w.H;
+            const int64_t rb = (int64_t)gy * w.W;
+            const uint64_t a = (rin && gx >= 0 && gx < w.W) ? in[rb + gx] : WALL;
+            const uint64_t b = (rin && gx + 1 >= 0 && gx + 1 < w.W) ? in[rb + gx + 1] : WALL;
+            v[k] = make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
+        }
+    }
+    // compact cells with their origin residues (LDS coordinates mod 8), the
+    // colours kept in registers until the write-back
+    uint32_t cola[KQ], colb[KQ];
+    bool wood = false;
+#pragma unroll
+    for (int k = 0; k < KQ; k++) {
+        const int i = tid + k * N;
+        if (i >= NPAIR) continue;
+        const int row = i / QP, p = i - row * QP;
+        const int lx = cx0 + 2 * p, ly = cy0 + row;
+        const uint32_t org = (((uint32_t)lx & 7u) << 9) | (((uint32_t)ly & 7u) << 12);
+        *reinterpret_cast<uint2 *>(&L[ly][lx]) = make_uint2(v[k].x | org, v[k].z | org | (1u << 9));
+        cola[k] = v[k].y;
+        colb[k] = v[k].w;
+        if (RE != 0) wood |= (v[k].x & 0xFFu) == M.wood || (v[k].z & 0xFFu) == M.wood;
+    }
+#pragma unroll
+    for (int k = 0; k < MAX_IT; k++)
+        if (k < ci && tid < 256) lut[k][tid] = pv[k];
+    if (tid < 256) {
+        // scatter offsets for mask y = tid: cell k goes to k ^ ((y >> 2k) & 3)
+        uint32_t po = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t d = (uint32_t)k ^ (((uint32_t)tid >> (2 * k)) & 3u);
+            po |= ((d & 1u) + (d >> 1) * (uint32_t)LW) << (8 * k);
+        }
+        lut[MAX_IT][tid] = po;
+    }
+    if (tid < 16) {
+        rgba[tid] = mrgba;
+        addt[tid] = madd;
+    }
+    bool has_wood = false;
+    if (RE == 0) __syncthreads();
+    else has_wood = __syncthreads_or(wood) != 0;
+
+    const uint64_t seedmix = splitmix64(w.seed);
+    for (int it = 0; it < ci; it++) {
+        const uint64_t g = tick * (uint64_t)M.cell_iter + (uint64_t)it;
+        const uint64_t key = splitmix64(seedmix ^ g);
+        run_phase<N, false, RE != 0, false, QW>(M, L, lut[it], it, key, kOX[g & 3], kOY[g & 3], ci - it, gx0, gy0,
+                                                has_wood, queue, lut[MAX_IT], ZoneBlocks{}, nullptr, cx0, cy0);
+        __syncthreads();
+    }
+
+    // Write-back.  Thread = 2-column strip x QROWS rows of the quarter (the
+    // first 512 threads, whole waves, at 1024).
+    constexpr int QSTRIPS = QH * (QW / 2);                      // 512 pair strips of one row
+    constexpr int QROWS = QSTRIPS > N ? QSTRIPS / N : 1;        // 2 at 256 threads, else 1
+    const bool writer = tid < QSTRIPS / QROWS;
+    const bool temp_now = M.temp_every != 0 && (tick % M.temp_every) == M.temp_phase;
+    const int j = tid & (QW / 2 - 1), rg = tid / (QW / 2);
+    const int lx = HALO + cx0 + 2 * j, ly0 = HALO + cy0 + rg * QROWS;
+    uint32_t fw[2 * QROWS];
+    if (writer) {
+        if (interior) final_words<QROWS, false, false>(M, L, addt, temp_now, lx, ly0, fw);
+        else final_words<QROWS, true, false>(M, L, addt, temp_now, lx, ly0, fw);
+    }
+    __syncthreads();
+    // the colour plane: each cell's colour at its origin
+#pragma unroll
+    for (int k = 0; k < KQ; k++) {
+        const int i = tid + k * N;
+        if (i >= NPAIR) continue;
+        const int row = i / QP, p = i - row * QP;
+        L[cy0 + row][cx0 + 2 * p] = cola[k];
+        L[cy0 + row][cx0 + 2 * p + 1] = colb[k];
+    }
+    __syncthreads();
+    const uint32_t *Lf = &L[0][0];
+    auto colour = [&](uint32_t wd, int x, int y) -> uint32_t {
+        if (RE != 0 && (wd & C_XF)) return rgba[wd & 15u];
+        const int dx = (int)((((wd >> 9) - (uint32_t)x) & 7u) ^ 4u) - 4;   // signed 3-bit differences
+        const int dy = (int)((((wd >> 12) - (uint32_t)y) & 7u) ^ 4u) - 4;
+        return Lf[(y + dy) * LW + x + dx];
+    };
+#pragma unroll
+    for (int r = 0; r < QROWS; r++) {
+        if (!writer) break;
+        const int ly = ly0 + r, gy = gy0 + ly;
+        const uint32_t a = fw[2 * r], b = fw[2 * r + 1];
+        const uint32_t ca = colour(a, lx, ly), cb = colour(b, lx + 1, ly);
+        const int gx = gx0 + lx;
+        if (interior) {
+            *reinterpret_cast<uint4 *>(out + (size_t)gy * w.W + gx) = make_uint4(a & C_KEEP, ca, b & C_KEEP, cb);
+        } else {
+            if (gy < w.H && gx < w.W) out[(size_t)gy * w.W + gx] = (uint64_t)(a & C_KEEP) | ((uint64_t)ca << 32);
+            if (gy < w.H && gx + 1 < w.W) out[(size_t)gy * w.W + gx + 1] = (uint64_t)(b & C_KEEP) | ((uint64_t)cb << 32);
+        }
+    }
+}
+
+template <int RE>
+__global__ __launch_bounds__(QUARTER_NT) void k_tick_quarter(DevWorld w, uint32_t k) {
+    __shared__ __align__(16) uint32_t Lmem[LH * LW + 16];  // the chunk's region layout, then the 16 base colours
+    __shared__ uint32_t lut[MAX_IT + 3][256];  // pair tables, scatter offsets
+    uint32_t(*L)[LW] = reinterpret_cast<uint32_t(*)[LW]>(Lmem);
+    uint32_t *rgba = Lmem + LH * LW;
+    __shared__ int32_t addt[16];
+    __shared__ uint16_t queue[(QUARTER_NT / 64) * qcap<QUARTER_NT>()];
+    quarter_update<RE, QUARTER_NT>(w, w.tick + k, (int)(blockIdx.x >> 2), (int)(blockIdx.x & 3u), L, lut, rgba, addt,
+                                   queue);
+}
+template __global__ void k_tick_quarter<0>(DevWorld, uint32_t);
+template __global__ void k_tick_quarter<1>(DevWorld, uint32_t);
+
 template __global__ void k_tick_small<0, 0>(DevWorld, uint32_t);
 template __global__ void k_tick_small<1, 0>(DevWorld, uint32_t);
 template __global__ void k_tick_small<0, 1>(DevWorld, uint32_t);

@@ -41,8 +41,10 @@ def test_small_worlds_random_mix(ctx, orc, W, H, seed):
     assert first_diff(got, ref) is None, first_diff(got, ref)
 
 
+# up to 64 chunks a world ticks as quarter chunks (4 workgroups per chunk),
+# up to 256 chunks as one 1024-thread workgroup per chunk (640x640: 100)
 @pytest.mark.parametrize("kind,W,H,seed,ticks", [(1, 512, 512, 1, 1000), (2, 256, 256, 2, 200),
-                                                 (3, 512, 256, 3, 100)])
+                                                 (3, 512, 256, 3, 100), (1, 640, 640, 5, 200)])
 def test_generated_worlds(ctx, orc, kind, W, H, seed, ticks):
     cells = fsg.generate(kind, W, H, seed)
     cfg = orc.default_config(reactions=0 if kind == 3 else 1)
