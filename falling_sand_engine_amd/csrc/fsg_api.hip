@@ -234,11 +234,16 @@ int launch_tick(fsg_world *w) {
         }
         w->list_any = false;
         w->list_next = w->d.tick + 1;
-    } else if (!zoned && 4 * tiles <= (unsigned)w->ctx->n_cu) {
+    } else if (4 * tiles <= (unsigned)w->ctx->n_cu) {
         // at most a quarter chunk per compute unit: four workgroups per chunk
         // (512^2 world: 9.7 us a tick instead of 11.2)
-        if (re) fsg::k_tick_quarter<1><<<4 * tiles, fsg::QUARTER_NT, 0, st>>>(w->d, 0u);
-        else fsg::k_tick_quarter<0><<<4 * tiles, fsg::QUARTER_NT, 0, st>>>(w->d, 0u);
+        if (zoned) {
+            if (re) fsg::k_tick_quarter<1, 1><<<4 * tiles, fsg::QUARTER_NT, 0, st>>>(w->d, 0u);
+            else fsg::k_tick_quarter<0, 1><<<4 * tiles, fsg::QUARTER_NT, 0, st>>>(w->d, 0u);
+        } else {
+            if (re) fsg::k_tick_quarter<1, 0><<<4 * tiles, fsg::QUARTER_NT, 0, st>>>(w->d, 0u);
+            else fsg::k_tick_quarter<0, 0><<<4 * tiles, fsg::QUARTER_NT, 0, st>>>(w->d, 0u);
+        }
     } else if (tiles <= (unsigned)w->ctx->n_cu) {
         // at most one chunk per compute unit: 1024 threads per chunk (a 512^2
         // world ticks in 11.4 us instead of 16.0; with more chunks than

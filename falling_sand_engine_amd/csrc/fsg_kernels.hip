@@ -750,11 +750,13 @@ __global__ __launch_bounds__(SMALL_NT) void k_tick_small(DevWorld w, uint32_t k)
 // residues and the phase schedule are the chunk update's, and the quarter's
 // cells come out the same: every cell of the quarter depends, over the
 // tick's phases, only on cells of its window (one ring per phase plus the
-// temperature stencil's).  No chunk skipping and no tick zone here.
+// temperature stencil's).  No chunk skipping here.  ZN: the tick zone is
+// tested, with the chunk's zone test (a window inside a dead region is dead;
+// on the zone's edge every block and cell is tested).
 constexpr int QW = TW / 2, QH = TH / 2;                   // the quarter (32 x 32)
 constexpr int QLW = QW + 2 * HALO, QLH = QH + 2 * HALO;   // its window (40 x 40)
 static_assert(TW == 2 * QW && TH == 2 * QH && (QW & 1) == 0, "quarters of whole cell pairs");
-template <int RE, int N>
+template <int RE, int ZN, int N>
 __device__ __forceinline__ void quarter_update(const DevWorld &w, uint64_t tick, int tile, int quad,
                                                uint32_t (*L)[LW], uint32_t (*lut)[256], uint32_t *rgba,
                                                int32_t *addt, uint16_t *queue) {
@@ -764,6 +766,8 @@ __device__ __forceinline__ void quarter_update(const DevWorld &w, uint64_t tick,
     const int tx = tile % w.tiles_x, ty = tile / w.tiles_x;
     const int gx0 = tx * TW - HALO, gy0 = ty * TH - HALO;    // world coords of L[0][0], as for the chunk
     const int cx0 = (quad & 1) * QW, cy0 = (quad >> 1) * QH;  // the quarter (chunk coords) = its window (LDS coords)
+    ZoneTest zt{};
+    if (ZN) zt = zone_test(w, gx0, gy0);
     const uint64_t *__restrict__ in = w.buf[tick & 1];
     uint64_t *__restrict__ out = w.buf[(tick + 1) & 1];
     const int ci = (int)M.cell_iter;
@@ -836,11 +840,15 @@ __device__ __forceinline__ void quarter_update(const DevWorld &w, uint64_t tick,
     else has_wood = __syncthreads_or(wood) != 0;
 
     const uint64_t seedmix = splitmix64(w.seed);
-    for (int it = 0; it < ci; it++) {
+    for (int it = 0; it < ci && !zt.dead; it++) {
         const uint64_t g = tick * (uint64_t)M.cell_iter + (uint64_t)it;
         const uint64_t key = splitmix64(seedmix ^ g);
-        run_phase<N, false, RE != 0, false, QW>(M, L, lut[it], it, key, kOX[g & 3], kOY[g & 3], ci - it, gx0, gy0,
-                                                has_wood, queue, lut[MAX_IT], ZoneBlocks{}, nullptr, cx0, cy0);
+        if (ZN && zt.blocks.on)
+            run_phase<N, false, RE != 0, true, QW>(M, L, lut[it], it, key, kOX[g & 3], kOY[g & 3], ci - it, gx0, gy0,
+                                                   has_wood, queue, lut[MAX_IT], zt.blocks, nullptr, cx0, cy0);
+        else
+            run_phase<N, false, RE != 0, false, QW>(M, L, lut[it], it, key, kOX[g & 3], kOY[g & 3], ci - it, gx0,
+                                                    gy0, has_wood, queue, lut[MAX_IT], zt.blocks, nullptr, cx0, cy0);
         __syncthreads();
     }
 
@@ -849,13 +857,21 @@ __device__ __forceinline__ void quarter_update(const DevWorld &w, uint64_t tick,
     constexpr int QSTRIPS = QH * (QW / 2);                      // 512 pair strips of one row
     constexpr int QROWS = QSTRIPS > N ? QSTRIPS / N : 1;        // 2 at 256 threads, else 1
     const bool writer = tid < QSTRIPS / QROWS;
-    const bool temp_now = M.temp_every != 0 && (tick % M.temp_every) == M.temp_phase;
+    const bool temp_now = !zt.dead && M.temp_every != 0 && (tick % M.temp_every) == M.temp_phase;
     const int j = tid & (QW / 2 - 1), rg = tid / (QW / 2);
     const int lx = HALO + cx0 + 2 * j, ly0 = HALO + cy0 + rg * QROWS;
     uint32_t fw[2 * QROWS];
     if (writer) {
-        if (interior) final_words<QROWS, false, false>(M, L, addt, temp_now, lx, ly0, fw);
-        else final_words<QROWS, true, false>(M, L, addt, temp_now, lx, ly0, fw);
+        if (ZN && zt.blocks.on) {
+            // on the zone's edge: cells outside it keep their temperature
+            const int gx = gx0 + lx;
+            const uint32_t zc = (gx >= w.zx0 && gx < w.zx1 ? 1u : 0u) | (gx + 1 >= w.zx0 && gx + 1 < w.zx1 ? 2u : 0u);
+            final_words<QROWS, true, false, true>(M, L, addt, temp_now, lx, ly0, fw, w.zy0 - gy0, w.zy1 - gy0, zc);
+        } else if (interior) {
+            final_words<QROWS, false, false>(M, L, addt, temp_now, lx, ly0, fw);
+        } else {
+            final_words<QROWS, true, false>(M, L, addt, temp_now, lx, ly0, fw);
+        }
     }
     __syncthreads();
     // the colour plane: each cell's colour at its origin
@@ -891,7 +907,7 @@ __device__ __forceinline__ void quarter_update(const DevWorld &w, uint64_t tick,
     }
 }
 
-template <int RE>
+template <int RE, int ZN>
 __global__ __launch_bounds__(QUARTER_NT) void k_tick_quarter(DevWorld w, uint32_t k) {
     __shared__ __align__(16) uint32_t Lmem[LH * LW + 16];  // the chunk's region layout, then the 16 base colours
     __shared__ uint32_t lut[MAX_IT + 3][256];  // pair tables, scatter offsets
@@ -899,11 +915,13 @@ __global__ __launch_bounds__(QUARTER_NT) void k_tick_quarter(DevWorld w, uint32_
     uint32_t *rgba = Lmem + LH * LW;
     __shared__ int32_t addt[16];
     __shared__ uint16_t queue[(QUARTER_NT / 64) * qcap<QUARTER_NT>()];
-    quarter_update<RE, QUARTER_NT>(w, w.tick + k, (int)(blockIdx.x >> 2), (int)(blockIdx.x & 3u), L, lut, rgba, addt,
+    quarter_update<RE, ZN, QUARTER_NT>(w, w.tick + k, (int)(blockIdx.x >> 2), (int)(blockIdx.x & 3u), L, lut, rgba, addt,
                                    queue);
 }
-template __global__ void k_tick_quarter<0>(DevWorld, uint32_t);
-template __global__ void k_tick_quarter<1>(DevWorld, uint32_t);
+template __global__ void k_tick_quarter<0, 0>(DevWorld, uint32_t);
+template __global__ void k_tick_quarter<1, 0>(DevWorld, uint32_t);
+template __global__ void k_tick_quarter<0, 1>(DevWorld, uint32_t);
+template __global__ void k_tick_quarter<1, 1>(DevWorld, uint32_t);
 
 template __global__ void k_tick_small<0, 0>(DevWorld, uint32_t);
 template __global__ void k_tick_small<1, 0>(DevWorld, uint32_t);

@@ -37,7 +37,7 @@ template <int SK, int RE, int ZN>
 __global__ void k_tick_world(DevWorld w, uint32_t k);
 template <int RE, int ZN>
 __global__ void k_tick_small(DevWorld w, uint32_t k);
-template <int RE>
+template <int RE, int ZN>
 __global__ void k_tick_quarter(DevWorld w, uint32_t k);
 __global__ void k_tick_batch(const DevWorld *__restrict__ worlds, const uint32_t *__restrict__ tile_start,
                              int nworlds, uint32_t k);

@@ -32,6 +32,22 @@ def test_zone_matches_oracle(ctx, orc, kind, zone, skip):
         assert np.array_equal(w.download(), ref)
 
 
+def test_zone_one_workgroup_per_chunk(ctx, orc):
+    # W x H above keeps to the quarter-chunk kernel (20 chunks); 640x640 has
+    # 100, one 1024-thread workgroup per chunk; the reference's zone
+    W2 = H2 = 640
+    seed = 9
+    cfg = orc.default_config()
+    cells = fsg.generate(2, W2, H2, seed)
+    zone = (128, 128, W2 - 256, H2 - 256)
+    ref = orc.run(cells.copy(), cfg, seed, 0, 40, zone=zone)
+    with fsg.World(ctx, W2, H2, seed, cfg) as w:
+        w.upload(cells)
+        w.tickZone = zone
+        w.tick(40)
+        assert np.array_equal(w.download(), ref)
+
+
 @pytest.mark.parametrize("skip", [0, 1])
 def test_zone_changes_mid_run(ctx, orc, skip):
     seed = 7
