@@ -255,12 +255,19 @@ int launch_tick(fsg_world *w) {
             if (re) fsg::k_tick_small<1, 0><<<tiles, fsg::SMALL_NT, 0, st>>>(w->d, 0u);
             else fsg::k_tick_small<0, 0><<<tiles, fsg::SMALL_NT, 0, st>>>(w->d, 0u);
         }
-    } else if (zoned) {
-        if (re) fsg::k_tick_world<0, 1, 1><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
-        else fsg::k_tick_world<0, 0, 1><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
     } else {
-        if (re) fsg::k_tick_world<0, 1, 0><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
-        else fsg::k_tick_world<0, 0, 0><<<tiles, fsg::NT, 0, st>>>(w->d, 0u);
+        // whole chunks in full rounds of resident workgroups (five per CU),
+        // the remainder as quarters
+        const unsigned slots = 5u * (unsigned)w->ctx->n_cu;
+        const unsigned nfull = tiles <= slots ? tiles : tiles / slots * slots;
+        const unsigned grid = nfull + 4u * (tiles - nfull);
+        if (zoned) {
+            if (re) fsg::k_tick_world<0, 1, 1><<<grid, fsg::NT, 0, st>>>(w->d, 0u, (int)nfull);
+            else fsg::k_tick_world<0, 0, 1><<<grid, fsg::NT, 0, st>>>(w->d, 0u, (int)nfull);
+        } else {
+            if (re) fsg::k_tick_world<0, 1, 0><<<grid, fsg::NT, 0, st>>>(w->d, 0u, (int)nfull);
+            else fsg::k_tick_world<0, 0, 0><<<grid, fsg::NT, 0, st>>>(w->d, 0u, (int)nfull);
+        }
     }
     HIPCHK(hipGetLastError());
     w->d.tick++;

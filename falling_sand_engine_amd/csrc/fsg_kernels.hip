@@ -709,20 +709,6 @@ __device__ __forceinline__ int xcd_order(int i, int n, int g) {
     return b * run + (r & 7) * g + (r >> 3);
 }
 
-// Five workgroups per CU (at most 96 VGPRs, 28 KB of LDS each).  One
-// instantiation per (chunk skipping, reactions) setting of the world, so
-// the branches and paths a setting never takes are not compiled in.
-template <int SK, int RE, int ZN>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_tick_world(DevWorld w, uint32_t k) {
-    __shared__ __align__(16) uint32_t Lmem[LH * LW + 16];  // the region, then the 16 base colours
-    __shared__ uint32_t lut[MAX_IT + 3][256];  // pair tables, scatter offsets, the phase's class table
-    uint32_t(*L)[LW] = reinterpret_cast<uint32_t(*)[LW]>(Lmem);
-    uint32_t *rgba = Lmem + LH * LW;
-    __shared__ int32_t addt[16];
-    __shared__ uint16_t queue[(NT / 64) * qcap<NT>()];
-    __shared__ uint32_t red[NT / 64];
-    tile_update<SK, RE, ZN>(w, w.tick + k, xcd_order(blockIdx.x, (int)gridDim.x, (w.tiles_x & 7) ? 8 : w.tiles_x >> 3), L, lut, rgba, addt, queue, red);
-}
 // Small worlds (at most one chunk per compute unit): the same update with
 // SMALL_NT (1024) threads per chunk.  A small world leaves compute units idle,
 // so a tick lasts one workgroup's latency; four times the waves per chunk cut
@@ -928,10 +914,37 @@ template __global__ void k_tick_small<1, 0>(DevWorld, uint32_t);
 template __global__ void k_tick_small<0, 1>(DevWorld, uint32_t);
 template __global__ void k_tick_small<1, 1>(DevWorld, uint32_t);
 
-template __global__ void k_tick_world<0, 0, 0>(DevWorld, uint32_t);
-template __global__ void k_tick_world<0, 1, 0>(DevWorld, uint32_t);
-template __global__ void k_tick_world<0, 0, 1>(DevWorld, uint32_t);
-template __global__ void k_tick_world<0, 1, 1>(DevWorld, uint32_t);
+// Five workgroups per CU (at most 96 VGPRs, 28 KB of LDS each).  One
+// instantiation per (chunk skipping, reactions) setting of the world, so
+// the branches and paths a setting never takes are not compiled in.
+// Workgroups [0, nfull) update whole chunks in the XCD-aware order; the
+// chunks from nfull on are updated as quarters (four workgroups each, the
+// grid's last), so the launch's last round of workgroups is short: a chunk
+// left over from the last full round of whole chunks would hold one
+// compute unit for a whole chunk's latency while the others idle.
+template <int SK, int RE, int ZN>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_tick_world(DevWorld w, uint32_t k,
+                                                                                            int nfull) {
+    __shared__ __align__(16) uint32_t Lmem[LH * LW + 16];  // the region, then the 16 base colours
+    __shared__ uint32_t lut[MAX_IT + 3][256];  // pair tables, scatter offsets, the phase's class table
+    uint32_t(*L)[LW] = reinterpret_cast<uint32_t(*)[LW]>(Lmem);
+    uint32_t *rgba = Lmem + LH * LW;
+    __shared__ int32_t addt[16];
+    __shared__ uint16_t queue[(NT / 64) * qcap<NT>()];
+    __shared__ uint32_t red[NT / 64];
+    const int i = (int)blockIdx.x;
+    if (i < nfull) {
+        tile_update<SK, RE, ZN>(w, w.tick + k, xcd_order(i, nfull, (w.tiles_x & 7) ? 8 : w.tiles_x >> 3), L, lut,
+                                rgba, addt, queue, red);
+    } else {
+        const int q = i - nfull;
+        quarter_update<RE, ZN, NT>(w, w.tick + k, nfull + (q >> 2), q & 3, L, lut, rgba, addt, queue);
+    }
+}
+template __global__ void k_tick_world<0, 0, 0>(DevWorld, uint32_t, int);
+template __global__ void k_tick_world<0, 1, 0>(DevWorld, uint32_t, int);
+template __global__ void k_tick_world<0, 0, 1>(DevWorld, uint32_t, int);
+template __global__ void k_tick_world<0, 1, 1>(DevWorld, uint32_t, int);
 
 // The awake chunks of tick t = w.tick + k, listed for k_tick_list: one thread
 // per chunk runs the wake test of the skip path (and writes the flags of the

@@ -34,7 +34,7 @@ struct StampArgs {
 };
 
 template <int SK, int RE, int ZN>
-__global__ void k_tick_world(DevWorld w, uint32_t k);
+__global__ void k_tick_world(DevWorld w, uint32_t k, int nfull);
 template <int RE, int ZN>
 __global__ void k_tick_small(DevWorld w, uint32_t k);
 template <int RE, int ZN>

@@ -55,6 +55,18 @@ def test_generated_worlds(ctx, orc, kind, W, H, seed, ticks):
     assert np.count_nonzero(fsg.cell_mat(got) == 2) == np.count_nonzero(fsg.cell_mat(cells) == 2)
 
 
+def test_last_round_as_quarters(ctx, orc):
+    # 36 x 36 = 1296 chunks: more than a round of resident workgroups (five per
+    # CU, 1280 on 256 CUs), so the chunks past the last full round tick as
+    # quarters in the same launch
+    W = H = 2304
+    cells = fsg.generate(2, W, H, 21)
+    cfg = orc.default_config()
+    ref = orc.run(cells.copy(), cfg, 21, 0, 12)
+    got = run_gpu(ctx, cells, cfg, 21, 12)
+    assert first_diff(got, ref) is None, first_diff(got, ref)
+
+
 def test_resume_at_tick(ctx, orc):
     cells = fsg.generate(2, 192, 128, 5)
     cfg = orc.default_config(temp_every=4, temp_phase=2)
