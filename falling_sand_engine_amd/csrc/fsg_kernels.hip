@@ -132,7 +132,11 @@ struct ZoneBlocks {
 
 // EXT: the side of the square of chunk cells the tick computes, from chunk
 // coordinates (cx0, cy0): the whole chunk, or a quarter of it.
-template <int N, bool TRACK, bool REACT, bool ZONE, int EXT = TW>
+// DIRECT: no classification; each thread decides one block at a time with
+// the general form (the same result for inert and plain blocks), for
+// workgroups with about as many threads as the phase has blocks, where the
+// classification pass only lengthens the chain of dependent steps.
+template <int N, bool TRACK, bool REACT, bool ZONE, int EXT = TW, bool DIRECT = false>
 __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW], const uint32_t *T, int it,
                                               uint64_t key, int ox, int oy, int need, int gx0, int gy0,
                                               bool has_wood, uint16_t *queue, const uint32_t *perm,
@@ -171,7 +175,7 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
     const int wb0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
     uint16_t *Q = queue + (wb0 >> 6) * QCAP;
     int nplain = 0, nother = 0;  // wave-uniform
-    for (int b0 = wb0; b0 < nb; b0 += N) {
+    for (int b0 = wb0; b0 < nb && !DIRECT; b0 += N) {
         const int b = b0 + lane;
         // lanes past the last block test the last one again and queue nothing
         const uint32_t *p0 = block_at(b < nb ? b : nb - 1);
@@ -239,11 +243,10 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
         if (dec != 0) scatter(p0, q, dec);
         track(dec != 0, lx, ly);
     }
-    for (int qi = 0; qi < nother; qi += 64) {
-        const int i = qi + lane;
-        if (i >= nother) continue;
+    // the general form of a block: reactions, gas and horizontal relations
+    auto general = [&](int b) {
         int lx, ly;
-        block_xy(Q[QCAP - 1 - i], lx, ly);
+        block_xy(b, lx, ly);
         uint32_t *p0 = &L[0][0] + __umul24((uint32_t)ly, (uint32_t)LW) + lx;
         uint32_t q[4] = {p0[0], p0[1], p0[LW], p0[LW + 1]};
         const uint32_t i0 = q[0] & 15u, i1 = q[1] & 15u, i2 = q[2] & 15u, i3 = q[3] & 15u;
@@ -270,6 +273,20 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
             wrote = dec != 0;
         }
         track(wrote, lx, ly);
+    };
+    if (DIRECT) {
+        for (int b0 = wb0; b0 < nb; b0 += N) {
+            const int b = b0 + lane;
+            if (b >= nb) continue;
+            if (ZONE && !in_zone(b)) continue;
+            general(b);
+        }
+        return bands;
+    }
+    for (int qi = 0; qi < nother; qi += 64) {
+        const int i = qi + lane;
+        if (i >= nother) continue;
+        general(Q[QCAP - 1 - i]);
     }
     return bands;
 }
@@ -829,12 +846,16 @@ __device__ __forceinline__ void quarter_update(const DevWorld &w, uint64_t tick,
     for (int it = 0; it < ci && !zt.dead; it++) {
         const uint64_t g = tick * (uint64_t)M.cell_iter + (uint64_t)it;
         const uint64_t key = splitmix64(seedmix ^ g);
+        // at 1024 threads a thread per block (at most 20 x 20 a phase): no classification
+        constexpr bool DIRECT = N >= 1024;
         if (ZN && zt.blocks.on)
-            run_phase<N, false, RE != 0, true, QW>(M, L, lut[it], it, key, kOX[g & 3], kOY[g & 3], ci - it, gx0, gy0,
-                                                   has_wood, queue, lut[MAX_IT], zt.blocks, nullptr, cx0, cy0);
+            run_phase<N, false, RE != 0, true, QW, DIRECT>(M, L, lut[it], it, key, kOX[g & 3], kOY[g & 3], ci - it,
+                                                           gx0, gy0, has_wood, queue, lut[MAX_IT], zt.blocks, nullptr,
+                                                           cx0, cy0);
         else
-            run_phase<N, false, RE != 0, false, QW>(M, L, lut[it], it, key, kOX[g & 3], kOY[g & 3], ci - it, gx0,
-                                                    gy0, has_wood, queue, lut[MAX_IT], zt.blocks, nullptr, cx0, cy0);
+            run_phase<N, false, RE != 0, false, QW, DIRECT>(M, L, lut[it], it, key, kOX[g & 3], kOY[g & 3], ci - it,
+                                                            gx0, gy0, has_wood, queue, lut[MAX_IT], zt.blocks, nullptr,
+                                                            cx0, cy0);
         __syncthreads();
     }
 
