@@ -759,10 +759,15 @@ __global__ __launch_bounds__(SMALL_NT) void k_tick_small(DevWorld w, uint32_t k)
 constexpr int QW = TW / 2, QH = TH / 2;                   // the quarter (32 x 32)
 constexpr int QLW = QW + 2 * HALO, QLH = QH + 2 * HALO;   // its window (40 x 40)
 static_assert(TW == 2 * QW && TH == 2 * QH && (QW & 1) == 0, "quarters of whole cell pairs");
-template <int RE, int ZN, int N>
+// SEP: the colours have an LDS array of their own (colw, QLW x QLH, the
+// window's cells), written at the stage, so the write-back needs no colour
+// plane and no barrier; otherwise they go through registers and the
+// region array after the phases (the chunk update's way, which keeps LDS
+// for occupancy).
+template <int RE, int ZN, int N, bool SEP = false>
 __device__ __forceinline__ void quarter_update(const DevWorld &w, uint64_t tick, int tile, int quad,
                                                uint32_t (*L)[LW], uint32_t (*lut)[256], uint32_t *rgba,
-                                               int32_t *addt, uint16_t *queue) {
+                                               int32_t *addt, uint16_t *queue, uint32_t *colw = nullptr) {
     static_assert(N == 256 || N == 512 || N == 1024, "one pair-table entry per thread, write-back strips");
     const int tid = threadIdx.x;
     const MatTab &M = w.mat;
@@ -805,8 +810,8 @@ __device__ __forceinline__ void quarter_update(const DevWorld &w, uint64_t tick,
             v[k] = make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
         }
     }
-    // compact cells with their origin residues (LDS coordinates mod 8), the
-    // colours kept in registers until the write-back
+    // compact cells with their origin residues (LDS coordinates mod 8); the
+    // colours go to colw (SEP) or stay in registers until the write-back
     uint32_t cola[KQ], colb[KQ];
     bool wood = false;
 #pragma unroll
@@ -817,8 +822,12 @@ __device__ __forceinline__ void quarter_update(const DevWorld &w, uint64_t tick,
         const int lx = cx0 + 2 * p, ly = cy0 + row;
         const uint32_t org = (((uint32_t)lx & 7u) << 9) | (((uint32_t)ly & 7u) << 12);
         *reinterpret_cast<uint2 *>(&L[ly][lx]) = make_uint2(v[k].x | org, v[k].z | org | (1u << 9));
-        cola[k] = v[k].y;
-        colb[k] = v[k].w;
+        if (SEP) {
+            *reinterpret_cast<uint2 *>(&colw[row * QLW + 2 * p]) = make_uint2(v[k].y, v[k].w);
+        } else {
+            cola[k] = v[k].y;
+            colb[k] = v[k].w;
+        }
         if (RE != 0) wood |= (v[k].x & 0xFFu) == M.wood || (v[k].z & 0xFFu) == M.wood;
     }
 #pragma unroll
@@ -880,22 +889,25 @@ __device__ __forceinline__ void quarter_update(const DevWorld &w, uint64_t tick,
             final_words<QROWS, true, false>(M, L, addt, temp_now, lx, ly0, fw);
         }
     }
-    __syncthreads();
-    // the colour plane: each cell's colour at its origin
+    if (!SEP) {
+        __syncthreads();
+        // the colour plane: each cell's colour at its origin
 #pragma unroll
-    for (int k = 0; k < KQ; k++) {
-        const int i = tid + k * N;
-        if (i >= NPAIR) continue;
-        const int row = i / QP, p = i - row * QP;
-        L[cy0 + row][cx0 + 2 * p] = cola[k];
-        L[cy0 + row][cx0 + 2 * p + 1] = colb[k];
+        for (int k = 0; k < KQ; k++) {
+            const int i = tid + k * N;
+            if (i >= NPAIR) continue;
+            const int row = i / QP, p = i - row * QP;
+            L[cy0 + row][cx0 + 2 * p] = cola[k];
+            L[cy0 + row][cx0 + 2 * p + 1] = colb[k];
+        }
+        __syncthreads();
     }
-    __syncthreads();
     const uint32_t *Lf = &L[0][0];
     auto colour = [&](uint32_t wd, int x, int y) -> uint32_t {
         if (RE != 0 && (wd & C_XF)) return rgba[wd & 15u];
         const int dx = (int)((((wd >> 9) - (uint32_t)x) & 7u) ^ 4u) - 4;   // signed 3-bit differences
         const int dy = (int)((((wd >> 12) - (uint32_t)y) & 7u) ^ 4u) - 4;
+        if (SEP) return colw[(y + dy - cy0) * QLW + (x + dx - cx0)];
         return Lf[(y + dy) * LW + x + dx];
     };
 #pragma unroll
@@ -922,8 +934,9 @@ __global__ __launch_bounds__(QUARTER_NT) void k_tick_quarter(DevWorld w, uint32_
     uint32_t *rgba = Lmem + LH * LW;
     __shared__ int32_t addt[16];
     __shared__ uint16_t queue[(QUARTER_NT / 64) * qcap<QUARTER_NT>()];
-    quarter_update<RE, ZN, QUARTER_NT>(w, w.tick + k, (int)(blockIdx.x >> 2), (int)(blockIdx.x & 3u), L, lut, rgba, addt,
-                                   queue);
+    __shared__ __align__(8) uint32_t colw[QLW * QLH];  // the window's colours
+    quarter_update<RE, ZN, QUARTER_NT, true>(w, w.tick + k, (int)(blockIdx.x >> 2), (int)(blockIdx.x & 3u), L, lut,
+                                             rgba, addt, queue, colw);
 }
 template __global__ void k_tick_quarter<0, 0>(DevWorld, uint32_t);
 template __global__ void k_tick_quarter<1, 0>(DevWorld, uint32_t);
