@@ -745,10 +745,12 @@ __global__ __launch_bounds__(SMALL_NT) void k_tick_small(DevWorld w, uint32_t k)
 // Smaller worlds still (at most a quarter chunk per compute unit): each
 // chunk as four 32x32 quarters, one workgroup of QUARTER_NT threads each,
 // with the HALO ring around the quarter.  Such a tick lasts one workgroup's
-// chain of dependent steps (stage, each phase's two passes, write-back);
-// a quarter's window is (32 + 2 HALO)^2 cells instead of (64 + 2 HALO)^2, so
-// each wave of it takes one round of blocks per pass (512^2 world: 9.7 us a
-// tick instead of 11.2 with one 1024-thread workgroup per chunk).  The window sits at its
+// chain of dependent steps (stage, the phases, write-back); a quarter's
+// window is (32 + 2 HALO)^2 cells instead of (64 + 2 HALO)^2, so at 1024
+// threads each thread decides one block per phase, with no classification
+// pass (512^2 world: 8.9 us a tick instead of 11.2 with one 1024-thread
+// workgroup per chunk).  The same update, at 256 threads and classified,
+// runs the last round's chunks of k_tick_world.  The window sits at its
 // place in the chunk's LDS layout, so block origins, hash positions, origin
 // residues and the phase schedule are the chunk update's, and the quarter's
 // cells come out the same: every cell of the quarter depends, over the
