@@ -25,7 +25,11 @@ def first_diff(a, b):
     return f"{len(idx)} cells differ; first at (x={x}, y={y}): gpu={int(a[y, x]):#x} cpu={int(b[y, x]):#x}"
 
 
-@pytest.mark.parametrize("W,H", [(8, 8), (64, 64), (70, 50), (130, 66), (128, 192), (65, 33), (201, 140)])
+# up to 64 chunks: the quarter-chunk kernel; 640x576 (90 chunks): one
+# 1024-thread workgroup per chunk; 1088x1088 (289 chunks, ids below 8): the
+# 256-thread chunk kernel with its class tables
+@pytest.mark.parametrize("W,H", [(8, 8), (64, 64), (70, 50), (130, 66), (128, 192), (65, 33), (201, 140),
+                                 (640, 576), (1088, 1088)])
 @pytest.mark.parametrize("seed", [0, 1, 7])
 def test_small_worlds_random_mix(ctx, orc, W, H, seed):
     rng = np.random.default_rng(seed * 1000 + W * 7 + H)
