@@ -71,6 +71,23 @@ def test_last_round_as_quarters(ctx, orc):
     assert first_diff(got, ref) is None, first_diff(got, ref)
 
 
+def test_last_round_as_quarters_random_mix(ctx, orc):
+    # every material, hot cells (burning wood, reactions), MOVED flags, in a
+    # world whose last 16 of 1296 chunks tick as 256-thread quarters
+    W = H = 2304
+    rng = np.random.default_rng(77)
+    mats = rng.choice([0, 1, 2, 3, 4, 5, 6, 7], size=(H, W), p=[.35, .1, .2, .12, .08, .07, .05, .03])
+    temps = rng.integers(0, 700 * 16, size=(H, W))
+    moved = rng.integers(0, 2, size=(H, W)) * (mats == 2)
+    cells = (mats.astype(np.uint64) | (moved.astype(np.uint64) << np.uint64(8)) |
+             ((temps.astype(np.uint64) & np.uint64(0xFFFF)) << np.uint64(16)) |
+             (rng.integers(0, 2**32, size=(H, W), dtype=np.uint64) << np.uint64(32)))
+    cfg = orc.default_config()
+    ref = orc.run(cells.copy(), cfg, 77, 0, 10)
+    got = run_gpu(ctx, cells, cfg, 77, 10)
+    assert first_diff(got, ref) is None, first_diff(got, ref)
+
+
 def test_resume_at_tick(ctx, orc):
     cells = fsg.generate(2, 192, 128, 5)
     cfg = orc.default_config(temp_every=4, temp_phase=2)
