@@ -973,8 +973,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
         tile_update<SK, RE, ZN>(w, w.tick + k, xcd_order(i, nfull, (w.tiles_x & 7) ? 8 : w.tiles_x >> 3), L, lut,
                                 rgba, addt, queue, red);
     } else {
-        const int q = i - nfull;
-        quarter_update<RE, ZN, NT>(w, w.tick + k, nfull + (q >> 2), q & 3, L, lut, rgba, addt, queue);
+        // the four quarters of a chunk on one XCD (workgroup i runs on XCD
+        // i % 8, and nfull is a multiple of 8), so they share their halo
+        // lines in one L2: in each run of 32 workgroups, those of one XCD
+        // take the four quarters of one chunk (a tail shorter than 32 keeps
+        // the plain order)
+        const int q = i - nfull, nq = (int)gridDim.x - nfull, full = nq & ~31;
+        const int c = q < full ? (q & 7) + ((q >> 5) << 3) : q >> 2;
+        const int quad = q < full ? (q >> 3) & 3 : q & 3;
+        quarter_update<RE, ZN, NT>(w, w.tick + k, nfull + c, quad, L, lut, rgba, addt, queue);
     }
 }
 template __global__ void k_tick_world<0, 0, 0>(DevWorld, uint32_t, int);
