@@ -251,11 +251,16 @@ __device__ __forceinline__ uint32_t run_phase(const MatTab &M, uint32_t (*L)[LW]
         uint32_t q[4] = {p0[0], p0[1], p0[LW], p0[LW + 1]};
         const uint32_t i0 = q[0] & 15u, i1 = q[1] & 15u, i2 = q[2] & 15u, i3 = q[3] & 15u;
         const uint32_t VL = T[(i0 << 4) | i2], VR = T[(i1 << 4) | i3];
-        uint32_t r, s;
-        block_hash(key, gx0 + lx, gy0 + ly, r, s);
         bool hot = false;
         if (REACT && has_wood)
             hot = wood_hot(M, q[0]) || wood_hot(M, q[1]) || wood_hot(M, q[2]) || wood_hot(M, q[3]);
+        // without classification: a wave whose blocks are all inert stops here
+        if (DIRECT && !__builtin_amdgcn_ballot_w64(
+                          hot || !block_inert(VL, VR, T[(i0 << 4) | i3], T[(i1 << 4) | i2], T[(i0 << 4) | i1],
+                                              T[(i2 << 4) | i3], (q[0] >> 8) & 1u, (q[1] >> 8) & 1u)))
+            return;
+        uint32_t r, s;
+        block_hash(key, gx0 + lx, gy0 + ly, r, s);
         bool wrote = false;
         if (REACT && (hot || ((VL | VR) & K_REACT))) {
             if (block_react(M, q)) {
